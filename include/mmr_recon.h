@@ -1,0 +1,429 @@
+/*
+ * mmr_recon.h — C ABI of the MI355X (gfx950) pixel-reconstruction library.
+ *
+ * The library reconstructs pictures from post-entropy buffers (coefficient
+ * levels, macroblock records, motion vectors, prediction weights) that a CPU
+ * entropy stage hands over, and composites / colour-converts the results. It
+ * is the device-side replacement for the reconstruction loops of MMRecode's
+ * codec crates; the entropy decoders stay on the CPU.
+ *
+ * Conventions follow crates/capi/include/mmrecode.h of MMRecode:
+ *   - int32_t status codes with the same values (crates/capi/include/mmrecode.h:24-32),
+ *     plus MMR_STATUS_BUSY for non-blocking back-pressure (DecodeExecutor::submit,
+ *     crates/playback/src/executor.rs:235-245);
+ *   - struct_size-prefixed descriptors (crates/capi/include/mmrecode.h:57-100);
+ *   - a thread-local last error (crates/capi/include/mmrecode.h:113-118);
+ *   - borrowed host inputs are only read during the call.
+ *
+ * Memory: every pointer inside a submit descriptor is a DEVICE pointer
+ * (HBM-resident), obtained from mmr_recon_device_alloc or any other HIP
+ * allocation on the context's device. Descriptors themselves are host memory
+ * and are copied during the submit call. Device buffers named by a submission
+ * must stay valid until its ticket completes.
+ */
+#ifndef MMR_RECON_H
+#define MMR_RECON_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) && __GNUC__ >= 4
+#define MMR_RECON_API __attribute__((visibility("default")))
+#else
+#define MMR_RECON_API
+#endif
+
+#define MMR_RECON_ABI_VERSION 1u
+
+#ifndef MMR_ABI_VERSION
+typedef int32_t mmr_status;
+#define MMR_STATUS_OK ((mmr_status)0)
+#define MMR_STATUS_INVALID_ARGUMENT ((mmr_status)1)
+#define MMR_STATUS_INVALID_DATA ((mmr_status)2)
+#define MMR_STATUS_UNSUPPORTED ((mmr_status)3)
+#define MMR_STATUS_INVALID_STATE ((mmr_status)4)
+#define MMR_STATUS_IO_ERROR ((mmr_status)5)
+#define MMR_STATUS_INTERNAL_ERROR ((mmr_status)100)
+#endif
+/* Queue full or HBM budget exhausted: retry later (no work was queued). */
+#define MMR_STATUS_BUSY ((mmr_status)6)
+
+#define MMR_RANGE_FULL 1u
+#define MMR_RANGE_LIMITED 2u
+#define MMR_MATRIX_BT601 1u
+#define MMR_MATRIX_BT709 2u
+
+/* Submission flags. */
+/* Plain path: one kernel launch per frame / picture, in submission order. */
+#define MMR_SUBMIT_PER_FRAME_LAUNCH 1u
+
+typedef struct mmr_recon_context mmr_recon_context;
+
+typedef struct mmr_recon_config {
+    size_t struct_size;
+    int32_t device_ordinal;      /* HIP device ordinal */
+    uint32_t queue_depth;        /* max in-flight tickets before MMR_STATUS_BUSY (0: 256) */
+    uint64_t hbm_budget_bytes;   /* admission budget of the serving loop (0: unlimited) */
+    uint32_t persistent_groups;  /* workgroups of a persistent launch (0: autotuned) */
+    uint32_t flags;              /* reserved, 0 */
+} mmr_recon_config;
+
+/* ------------------------------------------------------------------ core */
+
+/* Replaces: mmr_abi_version, crates/capi/include/mmrecode.h:102-103. */
+MMR_RECON_API uint32_t mmr_recon_abi_version(void);
+
+/* Replaces: mmr_last_error_message, crates/capi/include/mmrecode.h:113-118.
+ * Copies the calling thread's last diagnostic (NUL-terminated when capacity
+ * is nonzero) and returns the capacity required including the NUL. */
+MMR_RECON_API size_t mmr_recon_last_error_message(char *buffer, size_t capacity);
+
+/* New (no reference counterpart; the reference decoders are constructed per
+ * stream, crates/core/src/codec.rs:102-130): a context owns one device, its
+ * stream, descriptor arena and work-queue counters. */
+MMR_RECON_API mmr_status mmr_recon_context_create(const mmr_recon_config *config,
+                                                  mmr_recon_context **out_context);
+MMR_RECON_API void mmr_recon_context_destroy(mmr_recon_context *context);
+
+/* Device memory helpers for hosts without their own HIP allocator. */
+MMR_RECON_API mmr_status mmr_recon_device_alloc(mmr_recon_context *context, size_t bytes,
+                                                void **out_device_ptr);
+MMR_RECON_API mmr_status mmr_recon_device_free(mmr_recon_context *context, void *device_ptr);
+MMR_RECON_API mmr_status mmr_recon_copy_to_device(mmr_recon_context *context, void *device_dst,
+                                                  const void *host_src, size_t bytes);
+MMR_RECON_API mmr_status mmr_recon_copy_to_host(mmr_recon_context *context, void *host_dst,
+                                                const void *device_src, size_t bytes);
+
+/* Completion. Tickets are issued in increasing order per context. */
+MMR_RECON_API mmr_status mmr_recon_wait(mmr_recon_context *context, uint64_t ticket);
+MMR_RECON_API mmr_status mmr_recon_poll(mmr_recon_context *context, uint64_t ticket,
+                                        int32_t *out_done);
+MMR_RECON_API mmr_status mmr_recon_synchronize(mmr_recon_context *context);
+
+/* Device time (milliseconds, HIP events around the launches) of the last
+ * completed ticket's kernels, for roofline accounting. */
+MMR_RECON_API mmr_status mmr_recon_last_kernel_ms(mmr_recon_context *context, uint64_t ticket,
+                                                  double *out_ms);
+
+/* ----------------------------------------------------------------- MJPEG */
+
+/*
+ * One baseline JPEG frame after Huffman decoding.
+ * Replaces the per-block reconstruction of reconstruct(),
+ * crates/codecs/mjpeg/src/decoder.rs:386-399: dequantisation
+ * (decode_block :502-546), inverse_dct (crates/codecs/mjpeg/src/transform.rs:86-109)
+ * and write_block (decoder.rs:550-565).
+ *
+ * Block order is the reference's MCU walk (decoder.rs:346-401): MCUs in raster
+ * order; inside an interleaved MCU, components in order, then block rows, then
+ * block columns. A single-component frame is non-interleaved (one block per
+ * MCU, ceil(W/8) x ceil(H/8) MCUs).
+ */
+typedef struct mmr_recon_mjpeg_frame {
+    size_t struct_size;
+    uint32_t width;
+    uint32_t height;
+    uint32_t component_count;     /* 1..4 */
+    uint8_t h_sampling[4];
+    uint8_t v_sampling[4];
+    uint8_t quant_table[4];       /* per component: row of quant_tables */
+    uint32_t block_count;         /* must equal the MCU walk's block count */
+    const int16_t *levels;        /* [block_count][64], zigzag order; [0] = DC predictor value */
+    const uint16_t *quant_tables; /* [n][64], zigzag order (values_in_zigzag_order) */
+    uint8_t *planes[4];           /* component planes, stride = component width */
+    uint8_t *rgb;                 /* optional W*H*3 RGB8 (needs 3 components), NULL to skip */
+    uint32_t rgb_range;           /* MMR_RANGE_FULL (JPEG default) or MMR_RANGE_LIMITED */
+} mmr_recon_mjpeg_frame;
+
+MMR_RECON_API mmr_status mmr_recon_mjpeg_submit(mmr_recon_context *context,
+                                                const mmr_recon_mjpeg_frame *frames,
+                                                uint32_t frame_count, uint32_t flags,
+                                                uint64_t *out_ticket);
+
+/* ---------------------------------------------------------------- MPEG-2 */
+
+#define MMR_MPEG2_MB_INTRA 1u
+#define MMR_MPEG2_MB_FORWARD 2u
+#define MMR_MPEG2_MB_BACKWARD 4u
+#define MMR_MPEG2_MB_INTERLACED_DCT 8u
+#define MMR_MPEG2_MB_FIELD_MOTION 16u  /* field prediction: MMR_STATUS_UNSUPPORTED */
+
+/*
+ * One macroblock after VLC decoding, skip resolution and motion-vector
+ * prediction (crates/codecs/mpeg2/src/decoder.rs:340-679 produce these
+ * values; skipped macroblocks are explicit records with cbp 0).
+ */
+typedef struct mmr_recon_mpeg2_macroblock {
+    uint8_t flags;          /* MMR_MPEG2_MB_* */
+    uint8_t cbp;            /* coded_block_pattern, bit (5-b) = block b; ignored for intra */
+    uint8_t qscale;         /* quantiser_scale value (decode_quantiser_scale, decoder.rs:721-732) */
+    uint8_t reserved;
+    int16_t mv[2][2];       /* [forward, backward][x, y], half-sample units */
+    uint32_t block_offset;  /* index of the first coded block of this macroblock in levels */
+} mmr_recon_mpeg2_macroblock;
+
+#define MMR_MPEG2_PICTURE_I 1u
+#define MMR_MPEG2_PICTURE_P 2u
+#define MMR_MPEG2_PICTURE_B 3u
+
+/*
+ * One frame picture. Replaces Mpeg2PictureDecoder::decode_picture's
+ * reconstruction (crates/codecs/mpeg2/src/decoder.rs:263-304): intra and
+ * non-intra dequantisation with mismatch control (:734-846), inverse_dct
+ * (crates/codecs/mpeg2/src/transform.rs:90-110), half-sample prediction
+ * (decoder.rs:944-1046, :1150-1222) and put_block (:1231-1298).
+ *
+ * Planes are coded-size (FrameBuffer::new, decoder.rs:103-124): luma stride
+ * ceil16(width), chroma stride = luma stride / 2, heights ceil16(height) and half.
+ * Coded blocks of a macroblock are stored consecutively in block order 0..5.
+ */
+typedef struct mmr_recon_mpeg2_picture {
+    size_t struct_size;
+    uint32_t width;
+    uint32_t height;
+    uint32_t picture_type;          /* MMR_MPEG2_PICTURE_* */
+    uint32_t alternate_scan;        /* 0: ZIGZAG, 1: ALTERNATE_SCAN */
+    uint32_t intra_dc_precision;    /* 0..3 */
+    const uint8_t *matrices;        /* [4][64] natural order: intra, non-intra, chroma intra, chroma non-intra */
+    const mmr_recon_mpeg2_macroblock *macroblocks;  /* raster order, mb_width * mb_height */
+    const int16_t *levels;          /* [coded blocks][64] by scan position; intra [0] = DC predictor */
+    uint8_t *planes[3];             /* output coded planes */
+    const uint8_t *forward[3];      /* P: newer anchor, B: older anchor */
+    const uint8_t *backward[3];     /* B: newer anchor */
+    int32_t forward_source;         /* index in this submission producing forward[], or -1 */
+    int32_t backward_source;        /* index in this submission producing backward[], or -1 */
+} mmr_recon_mpeg2_picture;
+
+MMR_RECON_API mmr_status mmr_recon_mpeg2_submit(mmr_recon_context *context,
+                                                const mmr_recon_mpeg2_picture *pictures,
+                                                uint32_t picture_count, uint32_t flags,
+                                                uint64_t *out_ticket);
+
+/* -------------------------------------------------------------------- DV */
+
+#define MMR_DV_SYSTEM_525_60 0u  /* 720x480, 4:1:1, 10 DIF sequences */
+#define MMR_DV_SYSTEM_625_50 1u  /* 720x576, 4:2:0, 12 DIF sequences */
+
+/*
+ * One DV25 frame after VLC decoding and the three-level spill.
+ * Replaces decode_segment's reconstruction (crates/codecs/dv/src/decoder.rs:186-189):
+ * dequantisation (consume_ac :262-271, inverse_factor_for_encoding :290-310),
+ * inverse_88 / inverse_248 (:520-602) and place_macroblock / put_split_chroma
+ * (:432-505) at macroblock_coordinates (:395-422).
+ *
+ * Blocks are in DIF order: sequence, segment slot (27), macroblock (5), block (6).
+ * block_info bit 0 = dct_mode (2-4-8), bits 1-2 = class, bits 3-6 = quantisation number.
+ */
+typedef struct mmr_recon_dv_frame {
+    size_t struct_size;
+    uint32_t system;              /* MMR_DV_SYSTEM_* */
+    const int16_t *levels;        /* [blocks][64] by scan position; [0] = signed 9-bit DC */
+    const uint8_t *block_info;    /* [blocks] */
+    uint8_t *planes[3];           /* Y 720xH, Cb, Cr tightly packed */
+} mmr_recon_dv_frame;
+
+MMR_RECON_API mmr_status mmr_recon_dv_submit(mmr_recon_context *context,
+                                             const mmr_recon_dv_frame *frames,
+                                             uint32_t frame_count, uint32_t flags,
+                                             uint64_t *out_ticket);
+
+/* ----------------------------------------------------------------- H.264 */
+
+#define MMR_H264_MB_INTRA 1u          /* deblocking bS 3/4 only; no intra prediction */
+#define MMR_H264_MB_TRANSFORM_8X8 2u
+#define MMR_H264_MB_NO_DEBLOCK 4u     /* disable_deblocking_filter_idc == 1 for this MB's slice */
+#define MMR_H264_MB_SLICE_EDGE_OFF 8u /* disable_deblocking_filter_idc == 2: no filtering across slices */
+
+/* One macroblock (inter reconstruction and deblocking inputs). */
+typedef struct mmr_recon_h264_macroblock {
+    uint8_t flags;           /* MMR_H264_MB_* */
+    uint8_t qp;              /* luma QP after mb_qp_delta, 0..51 */
+    uint8_t cbp;             /* bits 0-3 luma 8x8 groups, bits 4-5 chroma (0, 1 = DC, 2 = DC+AC) */
+    uint8_t reserved;
+    uint16_t slice_id;
+    int8_t offset_a;         /* FilterOffsetA */
+    int8_t offset_b;         /* FilterOffsetB */
+    uint32_t coef_offset;    /* index of this MB's 384-level record in levels (if cbp != 0) */
+} mmr_recon_h264_macroblock;
+
+/* Motion of one 4x4 luma block (16 per macroblock, H.264 block-index order). */
+typedef struct mmr_recon_h264_block {
+    int16_t mv[2][2];        /* [L0, L1][x, y], quarter-sample luma units */
+    int8_t ref_idx[2];       /* -1: list unused */
+    uint8_t total_coeff;     /* nonzero coefficient count (CAVLC total_coeff / CABAC 8x8 count) */
+    uint8_t reserved;
+} mmr_recon_h264_block;
+
+/* Explicit single-list weight (PredictionWeight, crates/codecs/h264/src/decoder.rs:9884-9907). */
+typedef struct mmr_recon_h264_weight {
+    int16_t weight;
+    int16_t offset;
+    uint8_t log2_denom;
+    uint8_t reserved[3];
+} mmr_recon_h264_weight;
+
+/* Bi-prediction weight pair for (ref_idx L0, ref_idx L1). */
+typedef struct mmr_recon_h264_bi_weight {
+    int16_t w0;
+    int16_t w1;
+    int16_t o0;
+    int16_t o1;
+    uint8_t log2_denom;
+    uint8_t reserved[3];
+} mmr_recon_h264_bi_weight;
+
+#define MMR_H264_MAX_REFS 16u
+
+typedef struct mmr_recon_h264_reference {
+    const uint8_t *planes[3];  /* coded-size reference planes, same geometry as the picture */
+    int32_t poc;               /* picture order count (reference identity for bS) */
+    int32_t reserved;
+} mmr_recon_h264_reference;
+
+/* Level record of one coded MB (384 x int16):
+ *   transform 4x4: [0,256)  16 luma blocks x 16 levels, zig-zag scan, block-index order
+ *   transform 8x8: [0,256)  4 luma 8x8 blocks x 64 levels, 8x8 zig-zag scan (de-interleaved)
+ *   [256,264) chroma DC: Cb c0..c3, Cr c0..c3
+ *   [264,384) chroma AC: Cb blocks 0..3, Cr blocks 0..3, 15 levels each (scan index 1..15) */
+#define MMR_H264_LEVELS_PER_MB 384u
+
+#define MMR_H264_WEIGHTED_NONE 0u      /* default: average / copy */
+#define MMR_H264_WEIGHTED_TABLE 1u     /* explicit or implicit: use the weight tables */
+
+/*
+ * One inter frame picture. Replaces H264 inter reconstruction:
+ * predict_inter_partition / predict_bi_inter_partition
+ * (crates/codecs/h264/src/decoder.rs:11602-11656, :11901-12119) with luma_qpel,
+ * chroma_epel and the weights (:13619-13926), the residual paths
+ * add_luma_residual_blocks / _8x8_blocks / add_chroma_residual (:12630-12789),
+ * and deblock -> filter_picture (:12791-12848, crates/codecs/h264/src/deblock.rs:84-587).
+ */
+typedef struct mmr_recon_h264_picture {
+    size_t struct_size;
+    uint32_t width_mbs;
+    uint32_t height_mbs;
+    uint32_t weighted_mode;                     /* MMR_H264_WEIGHTED_* */
+    uint32_t deblock;                           /* 1: run the loop filter */
+    int32_t chroma_qp_offset[2];                /* Cb, Cr */
+    const mmr_recon_h264_macroblock *macroblocks;  /* raster order */
+    const mmr_recon_h264_block *blocks;         /* [mbs][16] */
+    const int16_t *levels;                      /* [coded mbs][384] */
+    const mmr_recon_h264_weight *weights;       /* [2][MAX_REFS][3] single-list (Y, Cb, Cr) */
+    const mmr_recon_h264_bi_weight *bi_weights; /* [MAX_REFS][MAX_REFS][3] */
+    const uint8_t *scaling_lists;               /* NULL: Flat_16; else [6][16] 4x4 then [2][64] 8x8, raster */
+    uint8_t *planes[3];                         /* output coded planes (stride 16*width_mbs, 8*width_mbs) */
+    uint32_t ref_count[2];
+    mmr_recon_h264_reference refs[2][MMR_H264_MAX_REFS];
+} mmr_recon_h264_picture;
+
+MMR_RECON_API mmr_status mmr_recon_h264_submit(mmr_recon_context *context,
+                                               const mmr_recon_h264_picture *pictures,
+                                               uint32_t picture_count, uint32_t flags,
+                                               uint64_t *out_ticket);
+
+/* Implicit bi-prediction weights (implicit_b_prediction_weights,
+ * crates/codecs/h264/src/decoder.rs:3577-3620) for one reference pair, host-side:
+ * writes the (w0, w1) used with log2_denom 5 and zero offsets. */
+MMR_RECON_API mmr_status mmr_recon_h264_implicit_weights(int32_t poc0, int32_t poc1,
+                                                         int32_t current_poc,
+                                                         int32_t long_term0, int32_t long_term1,
+                                                         int32_t *out_w0, int32_t *out_w1);
+
+/* ------------------------------------------------------- colour / blend */
+
+/*
+ * YCbCr -> RGB8 (packed, 3 bytes per pixel) at luma resolution.
+ * BT.601 replaces ycbcr_to_rgb + the chroma mapping of video_frame_image
+ * (crates/cli/src/terminal_preview.rs:7159-7238, :7308-7331).
+ * BT.709 has no counterpart in the reference; its integer form is defined here.
+ */
+typedef struct mmr_recon_yuv_to_rgb_job {
+    size_t struct_size;
+    uint32_t width;                 /* luma width = output width */
+    uint32_t height;
+    uint32_t chroma_width;
+    uint32_t chroma_height;
+    uint32_t luma_stride;
+    uint32_t chroma_stride;
+    uint32_t range;                 /* MMR_RANGE_* */
+    uint32_t matrix;                /* MMR_MATRIX_* */
+    const uint8_t *planes[3];
+    uint8_t *rgb;                   /* width*height*3 */
+    uint8_t *rgba;                  /* optional width*height*4 (alpha 255) instead of / besides rgb */
+} mmr_recon_yuv_to_rgb_job;
+
+MMR_RECON_API mmr_status mmr_recon_yuv_to_rgb_submit(mmr_recon_context *context,
+                                                     const mmr_recon_yuv_to_rgb_job *jobs,
+                                                     uint32_t job_count, uint32_t flags,
+                                                     uint64_t *out_ticket);
+
+#define MMR_BLEND_SOURCE_OVER 1u        /* straight-alpha RGBA8 (blend_rgba_pixel) */
+#define MMR_BLEND_PREMULTIPLIED_OVER 2u /* premultiplied RGBA8 (LinearPixel::over at 8 bits) */
+
+typedef struct mmr_recon_layer {
+    const uint8_t *rgba;   /* RGBA8 layer */
+    uint32_t width;
+    uint32_t height;
+    uint32_t stride;       /* bytes per row */
+    uint32_t x;
+    uint32_t y;
+    uint32_t op;           /* MMR_BLEND_* */
+    uint32_t opacity;      /* premultiplied op only: 0..255 */
+} mmr_recon_layer;
+
+#define MMR_RECON_MAX_LAYERS 8u
+
+/*
+ * Composite layers in order over an RGBA8 target in place.
+ * Replaces CpuCompositionBackend's RGBA path (crates/render/src/frame_graph.rs:307-355,
+ * blend_rgba8 / blend_rgba_pixel :504-545, :638-657); the premultiplied operator follows
+ * LinearPixel::over (crates/mmfx/src/render.rs:1826-1854) with 8-bit channels.
+ */
+typedef struct mmr_recon_composite_job {
+    size_t struct_size;
+    uint8_t *target;       /* RGBA8 */
+    uint32_t width;
+    uint32_t height;
+    uint32_t stride;
+    uint32_t layer_count;
+    mmr_recon_layer layers[MMR_RECON_MAX_LAYERS];
+} mmr_recon_composite_job;
+
+MMR_RECON_API mmr_status mmr_recon_composite_submit(mmr_recon_context *context,
+                                                    const mmr_recon_composite_job *jobs,
+                                                    uint32_t job_count, uint32_t flags,
+                                                    uint64_t *out_ticket);
+
+/*
+ * Blend a preconverted YUV 4:2:0 overlay with alpha into a YUV 4:2:0 target.
+ * Replaces blend_yuv420 / blend_channel (crates/render/src/frame_graph.rs:548-671).
+ */
+typedef struct mmr_recon_yuv_blend_job {
+    size_t struct_size;
+    uint8_t *target[3];
+    uint32_t target_width;
+    uint32_t target_height;
+    uint32_t target_stride[3];
+    const uint8_t *overlay[3];     /* Y (width x height), U, V (width/2 x height/2), range already chosen */
+    const uint8_t *rgba;           /* straight alpha in byte 3 of each RGBA pixel (width x height) */
+    const uint8_t *chroma_alpha;   /* 2x2-averaged alpha (width/2 x height/2) */
+    uint32_t width;
+    uint32_t height;
+    uint32_t x;
+    uint32_t y;
+} mmr_recon_yuv_blend_job;
+
+MMR_RECON_API mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context *context,
+                                                    const mmr_recon_yuv_blend_job *jobs,
+                                                    uint32_t job_count, uint32_t flags,
+                                                    uint64_t *out_ticket);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MMR_RECON_H */

@@ -1,0 +1,180 @@
+"""mmrecode_amd: MI355X (gfx950) pixel reconstruction behind the C ABI of include/mmr_recon.h.
+
+This module loads the in-tree libmmr_recon.so (built by mmrecode_amd/build.py) and wraps its
+context, device memory and submit / wait calls. There is no CPU fallback: if the library is
+missing or no HIP device is present, the calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libmmr_recon.so")
+_LIB = None
+
+
+class ReconError(RuntimeError):
+    """A non-OK mmr_status, with the library's last error message."""
+
+    def __init__(self, status, message):
+        super().__init__(f"mmr_status {status}: {message}")
+        self.status = status
+        self.message = message
+
+
+def load_library():
+    """Load libmmr_recon.so and declare every entry point of include/mmr_recon.h."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(f"{LIBRARY_PATH} is not built; run `python -m mmrecode_amd.build`")
+        lib = C.CDLL(LIBRARY_PATH)
+        for name, restype, argtypes in abi.ENTRY_POINTS:
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _LIB = lib
+    return _LIB
+
+
+def last_error():
+    lib = load_library()
+    n = lib.mmr_recon_last_error_message(None, 0)
+    buf = C.create_string_buffer(n)
+    lib.mmr_recon_last_error_message(buf, n)
+    return buf.value.decode(errors="replace")
+
+
+def check(status):
+    if status != abi.STATUS_OK:
+        raise ReconError(status, last_error())
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (freed on free() or context close)."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx = ctx
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def free(self):
+        if self.ptr:
+            self.ctx._free(self)
+
+    def __int__(self):
+        return self.ptr
+
+
+class Context:
+    """One mmr_recon_context: a device, its stream and in-flight tickets."""
+
+    def __init__(self, device=0, queue_depth=0, hbm_budget_bytes=0, persistent_groups=0):
+        self.lib = load_library()
+        cfg = abi.new(abi.Config, device_ordinal=device, queue_depth=queue_depth,
+                      hbm_budget_bytes=hbm_budget_bytes, persistent_groups=persistent_groups)
+        handle = C.c_void_p()
+        check(self.lib.mmr_recon_context_create(C.byref(cfg), C.byref(handle)))
+        self.handle = handle
+        self._buffers = {}
+
+    def close(self):
+        if self.handle:
+            for buf in list(self._buffers.values()):
+                self._free(buf)
+            self.lib.mmr_recon_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- memory
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        check(self.lib.mmr_recon_device_alloc(self.handle, max(int(nbytes), 1), C.byref(p)))
+        buf = DeviceBuffer(self, p.value, int(nbytes))
+        self._buffers[p.value] = buf
+        return buf
+
+    def _free(self, buf):
+        if buf.ptr in self._buffers:
+            del self._buffers[buf.ptr]
+            check(self.lib.mmr_recon_device_free(self.handle, C.c_void_p(buf.ptr)))
+        buf.ptr = None
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        buf = self.alloc(a.nbytes)
+        if a.nbytes:
+            check(self.lib.mmr_recon_copy_to_device(self.handle, C.c_void_p(buf.ptr), a.ctypes.data, a.nbytes))
+        return buf
+
+    def write(self, buf, array, offset=0):
+        a = np.ascontiguousarray(array)
+        check(self.lib.mmr_recon_copy_to_device(self.handle, C.c_void_p(buf.ptr + offset), a.ctypes.data, a.nbytes))
+
+    def download(self, buf, dtype=np.uint8, shape=None, offset=0, nbytes=None):
+        dtype = np.dtype(dtype)
+        n = buf.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(n // dtype.itemsize, dtype)
+        check(self.lib.mmr_recon_copy_to_host(self.handle, out.ctypes.data, C.c_void_p(buf.ptr + offset), out.nbytes))
+        return out if shape is None else out.reshape(shape)
+
+    # ---- submission
+    def _submit(self, fn, struct_type, items, flags):
+        arr = (struct_type * len(items))(*items)
+        ticket = C.c_uint64()
+        check(fn(self.handle, arr, len(items), flags, C.byref(ticket)))
+        return ticket.value
+
+    def submit_mjpeg(self, frames, flags=0):
+        return self._submit(self.lib.mmr_recon_mjpeg_submit, abi.MjpegFrame, frames, flags)
+
+    def submit_mpeg2(self, pictures, flags=0):
+        return self._submit(self.lib.mmr_recon_mpeg2_submit, abi.Mpeg2Picture, pictures, flags)
+
+    def submit_dv(self, frames, flags=0):
+        return self._submit(self.lib.mmr_recon_dv_submit, abi.DvFrame, frames, flags)
+
+    def submit_h264(self, pictures, flags=0):
+        return self._submit(self.lib.mmr_recon_h264_submit, abi.H264Picture, pictures, flags)
+
+    def submit_yuv_to_rgb(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_yuv_to_rgb_submit, abi.YuvToRgbJob, jobs, flags)
+
+    def submit_composite(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_composite_submit, abi.CompositeJob, jobs, flags)
+
+    def submit_yuv_blend(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_yuv_blend_submit, abi.YuvBlendJob, jobs, flags)
+
+    def wait(self, ticket):
+        check(self.lib.mmr_recon_wait(self.handle, ticket))
+
+    def poll(self, ticket):
+        done = C.c_int32()
+        check(self.lib.mmr_recon_poll(self.handle, ticket, C.byref(done)))
+        return bool(done.value)
+
+    def synchronize(self):
+        check(self.lib.mmr_recon_synchronize(self.handle))
+
+    def kernel_ms(self, ticket):
+        ms = C.c_double()
+        check(self.lib.mmr_recon_last_kernel_ms(self.handle, ticket, C.byref(ms)))
+        return ms.value
+
+
+def implicit_weights(poc0, poc1, current_poc, long_term0=False, long_term1=False):
+    """(w0, w1) of implicit bi-prediction, computed by the library's host code."""
+    lib = load_library()
+    w0, w1 = C.c_int32(), C.c_int32()
+    check(lib.mmr_recon_h264_implicit_weights(poc0, poc1, current_poc, int(long_term0), int(long_term1),
+                                              C.byref(w0), C.byref(w1)))
+    return w0.value, w1.value

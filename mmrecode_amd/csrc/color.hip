@@ -1,0 +1,195 @@
+// YCbCr -> RGB8, RGBA8 compositing and YUV 4:2:0 overlay blending.
+//
+// Reference: ycbcr_to_rgb + video_frame_image's chroma mapping
+// (crates/cli/src/terminal_preview.rs:7159-7238, :7308-7331); blend_rgba8 / blend_rgba_pixel
+// (crates/render/src/frame_graph.rs:504-545, :638-657); LinearPixel::over / multiply_channel /
+// add_saturating (crates/mmfx/src/render.rs:1826-1854, here with 8-bit channels); blend_yuv420 /
+// blend_channel (crates/render/src/frame_graph.rs:548-671).
+#include "device_common.h"
+#include "recon_internal.h"
+
+namespace mmr {
+
+MMR_DEV void ycbcr_to_rgb(int y, int cb, int cr, uint32_t range, uint32_t matrix, int rgb[3]) {
+    const int b = cb - 128, r = cr - 128;
+    int R, G, B;
+    if (range == MMR_RANGE_LIMITED) {
+        const int yy = max(y - 16, 0);
+        if (matrix == MMR_MATRIX_BT709) {
+            R = (298 * yy + 459 * r + 128) >> 8;
+            G = (298 * yy - 55 * b - 136 * r + 128) >> 8;
+            B = (298 * yy + 541 * b + 128) >> 8;
+        } else {
+            R = (298 * yy + 409 * r + 128) >> 8;
+            G = (298 * yy - 100 * b - 208 * r + 128) >> 8;
+            B = (298 * yy + 516 * b + 128) >> 8;
+        }
+    } else {
+        if (matrix == MMR_MATRIX_BT709) {
+            R = y + ((403 * r + 128) >> 8);
+            G = y - ((48 * b + 120 * r + 128) >> 8);
+            B = y + ((475 * b + 128) >> 8);
+        } else {
+            R = y + ((359 * r + 128) >> 8);
+            G = y - ((88 * b + 183 * r + 128) >> 8);
+            B = y + ((454 * b + 128) >> 8);
+        }
+    }
+    rgb[0] = clampi(R, 0, 255);
+    rgb[1] = clampi(G, 0, 255);
+    rgb[2] = clampi(B, 0, 255);
+}
+
+// One thread per 4 horizontally adjacent pixels (12 RGB bytes / 16 RGBA bytes).
+__global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restrict__ jobs, uint32_t job_count, uint32_t total_quads) {
+    const uint32_t gq = blockIdx.x * 256 + threadIdx.x;
+    if (gq >= total_quads) return;
+    const uint32_t ji = find_job(jobs, job_count, gq, [](const RgbJob& j) { return j.quad_begin; });
+    const RgbJob& j = jobs[ji];
+    const uint32_t q = gq - j.quad_begin;
+    const uint32_t y = q / j.quads_per_row, x0 = (q % j.quads_per_row) * 4;
+    uint32_t cy = (uint32_t)(((uint64_t)y * j.ch) / j.height);
+    cy = min(cy, j.ch - 1);
+    const uint8_t* Y = j.planes[0] + (size_t)y * j.luma_stride;
+    const uint8_t* U = j.planes[1] + (size_t)cy * j.chroma_stride;
+    const uint8_t* V = j.planes[2] + (size_t)cy * j.chroma_stride;
+    uint8_t px[16];
+    const int n = (int)min(4u, j.width - x0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i < n) {
+            const uint32_t x = x0 + i;
+            uint32_t cx = (uint32_t)(((uint64_t)x * j.cw) / j.width);
+            cx = min(cx, j.cw - 1);
+            int rgb[3];
+            ycbcr_to_rgb(Y[x], U[cx], V[cx], j.range, j.matrix, rgb);
+            px[i * 4 + 0] = (uint8_t)rgb[0];
+            px[i * 4 + 1] = (uint8_t)rgb[1];
+            px[i * 4 + 2] = (uint8_t)rgb[2];
+            px[i * 4 + 3] = 255;
+        }
+    }
+    const size_t p = (size_t)y * j.width + x0;
+    if (j.rgb) {
+        uint8_t* o = j.rgb + p * 3;
+        for (int i = 0; i < n; ++i) {
+            o[i * 3 + 0] = px[i * 4 + 0];
+            o[i * 3 + 1] = px[i * 4 + 1];
+            o[i * 3 + 2] = px[i * 4 + 2];
+        }
+    }
+    if (j.rgba) {
+        uint8_t* o = j.rgba + p * 4;
+        if (n == 4 && (((uintptr_t)o) & 15) == 0) {
+            *reinterpret_cast<uint4*>(o) = *reinterpret_cast<const uint4*>(px);
+        } else {
+            for (int i = 0; i < n * 4; ++i) o[i] = px[i];
+        }
+    }
+}
+
+MMR_DEV void blend_rgba_pixel(uint32_t d[4], const uint32_t s[4]) {
+    const uint32_t sa = s[3];
+    if (sa == 0) return;
+    if (sa == 255 || d[3] == 0) {
+        d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
+        return;
+    }
+    const uint32_t da = d[3], inverse = 255 - sa;
+    const uint32_t oa = sa + (da * inverse + 127) / 255;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const uint32_t pre = s[c] * sa + (d[c] * da * inverse + 127) / 255;
+        d[c] = (pre + oa / 2) / oa;
+    }
+    d[3] = oa;
+}
+
+MMR_DEV uint32_t mul255(uint32_t a, uint32_t b) { return (a * b + 127) / 255; }
+MMR_DEV uint32_t sat255(uint32_t v) { return v > 255 ? 255 : v; }
+
+MMR_DEV void premultiplied_over(uint32_t d[4], const uint32_t s[4], uint32_t opacity) {
+    const uint32_t sa = mul255(s[3], opacity), inverse = 255 - sa;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = sat255(mul255(s[c], opacity) + mul255(d[c], inverse));
+    d[3] = sat255(sa + mul255(d[3], inverse));
+}
+
+// One thread per target pixel; the layers covering the pixel are applied in order.
+__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, uint32_t job_count, uint32_t total_px) {
+    const uint32_t gp = blockIdx.x * 256 + threadIdx.x;
+    if (gp >= total_px) return;
+    const uint32_t ji = find_job(jobs, job_count, gp, [](const CompositeJobDev& j) { return j.px_begin; });
+    const CompositeJobDev& j = jobs[ji];
+    const uint32_t p = gp - j.px_begin;
+    const uint32_t y = p / j.width, x = p % j.width;
+    uint8_t* t = j.target + (size_t)y * j.stride + (size_t)x * 4;
+    const uint32_t tw = *reinterpret_cast<const uint32_t*>(t);
+    uint32_t d[4] = {tw & 255, (tw >> 8) & 255, (tw >> 16) & 255, tw >> 24};
+    for (uint32_t li = 0; li < j.layer_count; ++li) {
+        const mmr_recon_layer& L = j.layers[li];
+        if (x < L.x || y < L.y || x >= L.x + L.width || y >= L.y + L.height) continue;
+        const uint32_t sw = *reinterpret_cast<const uint32_t*>(L.rgba + (size_t)(y - L.y) * L.stride + (size_t)(x - L.x) * 4);
+        const uint32_t s[4] = {sw & 255, (sw >> 8) & 255, (sw >> 16) & 255, sw >> 24};
+        if (L.op == MMR_BLEND_PREMULTIPLIED_OVER) premultiplied_over(d, s, L.opacity);
+        else blend_rgba_pixel(d, s);
+    }
+    *reinterpret_cast<uint32_t*>(t) = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
+}
+
+MMR_DEV uint8_t blend_channel(uint32_t d, uint32_t s, uint32_t a) {
+    if (a == 0) return (uint8_t)d;
+    if (a == 255) return (uint8_t)s;
+    return (uint8_t)((s * a + d * (255 - a) + 127) / 255);
+}
+
+// grid.y = job; one thread per overlay luma pixel; the pixel at even (x, y) also blends the
+// co-located chroma sample.
+__global__ __launch_bounds__(256) void yuv_blend_kernel(const mmr_recon_yuv_blend_job* __restrict__ jobs) {
+    const mmr_recon_yuv_blend_job& j = jobs[blockIdx.y];
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= j.width * j.height) return;
+    const uint32_t y = p / j.width, x = p % j.width;
+    uint8_t* dy = j.target[0] + (size_t)(j.y + y) * j.target_stride[0] + j.x + x;
+    *dy = blend_channel(*dy, j.overlay[0][p], j.rgba[(size_t)p * 4 + 3]);
+    if (((x | y) & 1) == 0) {
+        const uint32_t cw = j.width / 2, s = (y / 2) * cw + x / 2;
+        const uint32_t a = j.chroma_alpha[s];
+        uint8_t* du = j.target[1] + (size_t)(j.y / 2 + y / 2) * j.target_stride[1] + j.x / 2 + x / 2;
+        uint8_t* dv = j.target[2] + (size_t)(j.y / 2 + y / 2) * j.target_stride[2] + j.x / 2 + x / 2;
+        *du = blend_channel(*du, j.overlay[1][s], a);
+        *dv = blend_channel(*dv, j.overlay[2][s], a);
+    }
+}
+
+__global__ void fill_zero_kernel(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s) {
+    if (total_quads == 0) return hipSuccess;
+    hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3((total_quads + 255) / 256), dim3(256), 0, s, d_jobs, job_count, total_quads);
+    return hipGetLastError();
+}
+
+hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s) {
+    if (total_px == 0) return hipSuccess;
+    hipLaunchKernelGGL(composite_kernel, dim3((total_px + 255) / 256), dim3(256), 0, s, d_jobs, job_count, total_px);
+    return hipGetLastError();
+}
+
+hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s) {
+    if (max_px == 0 || job_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(yuv_blend_kernel, dim3((max_px + 255) / 256, job_count), dim3(256), 0, s, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(uint32_t* d_words, uint32_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    uint32_t blocks = (count + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, s, d_words, count);
+    return hipGetLastError();
+}
+
+}  // namespace mmr

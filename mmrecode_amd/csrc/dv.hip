@@ -1,0 +1,243 @@
+// DV25 frame reconstruction: dequantisation, 8-8 / 2-4-8 direct-form f64 IDCTs, shuffled placement.
+//
+// Reference: consume_ac / inverse_factor_for_encoding (crates/codecs/dv/src/decoder.rs:262-310),
+// inverse_88 / inverse_248 (:520-602), place_macroblock / put_block / put_split_chroma (:432-505),
+// macroblock_coordinates (:395-422), tables (crates/codecs/dv/src/tables.rs:6-50).
+//
+// The cosine tables are computed on the host with the C library cos on exactly the reference's
+// argument expression ((2x+1) as f64 * u as f64 * PI / 16.0) and uploaded: the device never
+// evaluates cos itself, so the basis is the same double values the reference multiplies by.
+#include <math.h>
+
+#include "device_common.h"
+#include "recon_internal.h"
+
+namespace mmr {
+
+__constant__ double kDvCos16[64];   // [x*8+u] = cos((2x+1) u pi / 16)
+__constant__ double kDvCos8[32];    // [y*4+v] = cos((2y+1) v pi / 8)
+__constant__ double kDvCuCv[64];    // [v*8+u] = cu * cv
+
+__constant__ static const uint8_t kZigzag248[64] = {
+    0, 8, 1, 9, 16, 24, 2, 10, 17, 25, 32, 40, 48, 56, 33, 41, 18, 26, 3, 11, 4, 12, 19, 27, 34,
+    42, 49, 57, 50, 58, 35, 43, 20, 28, 5, 13, 6, 14, 21, 29, 36, 44, 51, 59, 52, 60, 37, 45, 22,
+    30, 7, 15, 23, 31, 38, 46, 53, 61, 54, 62, 39, 47, 55, 63};
+__constant__ static const uint8_t kQuantShifts[22][4] = {
+    {3, 3, 4, 4}, {3, 3, 4, 4}, {2, 3, 3, 4}, {2, 3, 3, 4}, {2, 2, 3, 3}, {2, 2, 3, 3},
+    {1, 2, 2, 3}, {1, 2, 2, 3}, {1, 1, 2, 2}, {1, 1, 2, 2}, {0, 1, 1, 2}, {0, 1, 1, 2},
+    {0, 0, 1, 1}, {0, 0, 1, 1}, {0, 0, 0, 1}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0},
+    {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+__constant__ static const uint8_t kQuantOffset[4] = {6, 3, 0, 1};
+__constant__ static const int32_t kIWeight88[64] = {
+    32768, 16705, 16705, 17734, 17032, 17734, 18205, 18081, 18081, 18205, 18725, 18562, 19195,
+    18562, 18725, 19266, 19091, 19705, 19705, 19091, 19266, 21407, 19643, 20267, 20228, 20267,
+    19643, 21407, 22725, 21826, 20853, 20806, 20806, 20853, 21826, 22725, 23170, 23170, 21407,
+    21400, 21407, 23170, 23170, 24598, 23786, 22018, 22018, 23786, 24598, 25251, 24465, 22654,
+    24465, 25251, 25972, 25172, 25172, 25972, 26722, 27969, 26722, 29692, 29692, 31521};
+__constant__ static const int32_t kIWeight248[64] = {
+    32768, 16384, 16705, 16705, 17734, 17734, 17734, 17734, 18081, 18081, 18725, 18725, 21407,
+    21407, 19091, 19091, 19195, 19195, 18205, 18205, 18725, 18725, 19705, 19705, 20267, 20267,
+    21826, 21826, 23170, 23170, 20806, 20806, 20267, 20267, 19266, 19266, 21407, 21407, 20853,
+    20853, 21400, 21400, 23786, 23786, 24465, 24465, 22018, 22018, 23170, 23170, 22725, 22725,
+    24598, 24598, 24465, 24465, 25172, 25172, 27969, 27969, 25972, 25972, 29692, 29692};
+
+constexpr double kFrac1Sqrt2 = 0.70710678118654752440;
+
+// macroblock_coordinates (decoder.rs:395-422): top-left of macroblock `mb` of segment (seq, slot),
+// in units of 8 samples.
+MMR_DEV void dv_mb_coordinates(uint32_t system, int seq, int slot, int mb, int* ox, int* oy) {
+    const int OFF[5] = {2, 6, 8, 0, 4};
+    const int SHUF3[5] = {18, 9, 27, 0, 36};
+    const int LSTART[5] = {9, 4, 13, 0, 18};
+    if (system == MMR_DV_SYSTEM_625_50) {
+        const int serpent1 = (slot % 6 < 3) ? (slot % 6) : (5 - slot % 6);
+        const int x = SHUF3[mb] + slot / 3;
+        const int y = serpent1 + ((seq + OFF[mb]) % 12) * 3;
+        *ox = x * 2;
+        *oy = y * 2;
+    } else {
+        const int row = (seq + OFF[mb]) % 10;
+        const int k = slot + ((mb == 1 || mb == 2) ? 3 : 0);
+        const int x = LSTART[mb] + k / 6;
+        const int serpent2 = (k % 12 < 6) ? (k % 12) : (11 - k % 12);
+        int y = serpent2 + row * 6;
+        if (x > 21) y = y * 2 - row * 6;
+        *ox = x * 4;
+        *oy = y;
+    }
+}
+
+__global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restrict__ jobs, uint32_t job_count,
+                                                           uint32_t total_blocks) {
+    __shared__ __align__(16) int32_t coef[kBlocksPerGroup][64];
+    __shared__ double s_cos16[64];
+    __shared__ double s_cucv[64];
+    const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    if (threadIdx.x < 64) {
+        s_cos16[threadIdx.x] = kDvCos16[threadIdx.x];
+        s_cucv[threadIdx.x] = kDvCuCv[threadIdx.x];
+    }
+    const uint32_t gb = blockIdx.x * kBlocksPerGroup + g;
+    const bool active = gb < total_blocks;
+    const uint32_t ji = active ? find_job(jobs, job_count, gb, [](const DvJob& j) { return j.block_begin; }) : 0;
+    const DvJob& job = jobs[ji];
+    const uint32_t blk = active ? gb - job.block_begin : 0;
+    const uint8_t info = active ? job.info[blk] : 0;
+    const int mode248 = info & 1, cls = (info >> 1) & 3, qno = (info >> 3) & 15;
+
+    if (active) {
+        const int4 raw = *reinterpret_cast<const int4*>(job.levels + (size_t)blk * 64 + lane * 8);
+        const int16_t* l8 = reinterpret_cast<const int16_t*>(&raw);
+        const uint8_t* scan = mode248 ? kZigzag248 : kZigzag;
+        const int32_t* weight = mode248 ? kIWeight248 : kIWeight88;
+        const int q = qno + kQuantOffset[cls];
+        const int class_shift = cls == 3;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int p = lane * 8 + i;
+            const int32_t level = l8[i];
+            int32_t v;
+            if (p == 0) {
+                v = level * 4 + 1024;  // dc * 4 + 1024 (decoder.rs:214)
+            } else if (level == 0) {
+                v = 0;
+            } else {
+                const int area = p <= 5 ? 0 : (p <= 20 ? 1 : (p <= 42 ? 2 : 3));
+                const int32_t factor = weight[p] << (kQuantShifts[q][area] + 1 + class_shift);
+                v = (level * factor + 8192) >> 14;
+            }
+            coef[g][scan[p]] = v;
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+
+    // Nonzero pattern of the block: bit v*8+u set when coefficient (v,u) != 0.
+    uint32_t rowbits = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rowbits |= (uint32_t)(coef[g][lane * 8 + u] != 0) << u;
+    const uint32_t lo = group8_or(lane < 4 ? rowbits << (lane * 8) : 0u);
+    const uint32_t hi = group8_or(lane >= 4 ? rowbits << ((lane - 4) * 8) : 0u);
+    const int x = lane;
+    uint8_t out[8];
+
+    if (!mode248) {
+        // inverse_88: for each (y, x) sum over (v, u) in ascending natural order of
+        // cu*cv*c*cos((2x+1)u pi/16)*cos((2y+1)v pi/16), then (sum / 4).round().clamp(0, 255).
+        // Zero coefficients contribute signed zeros and are skipped (bit-exact, see idct8x8_column).
+        double sum[8];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) sum[y] = 0.0;
+        uint64_t mask = ((uint64_t)hi << 32) | lo;
+        while (mask) {
+            const int p = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            const int v = p >> 3, u = p & 7;
+            double t = dmul(s_cucv[p], (double)coef[g][p]);
+            t = dmul(t, s_cos16[x * 8 + u]);
+#pragma unroll
+            for (int y = 0; y < 8; ++y) sum[y] = dadd(sum[y], dmul(t, s_cos16[y * 8 + v]));
+        }
+#pragma unroll
+        for (int y = 0; y < 8; ++y) out[y] = clamp_u8_d(round_half_away(sum[y] / 4.0));
+    } else {
+        // inverse_248: butterfly of row pairs, horizontal 8-point pass (/2), per-field vertical
+        // 4-point pass (/2), round, clamp.
+        double h[8];
+#pragma unroll
+        for (int row = 0; row < 8; ++row) {
+            const int pair = row >> 1, second = row & 1;
+            double s = 0.0;
+            for (int u = 0; u < 8; ++u) {
+                const double first = (double)coef[g][pair * 16 + u];
+                const double other = (double)coef[g][pair * 16 + 8 + u];
+                const double bfly = second ? first - other : first + other;
+                if (bfly != 0.0) {
+                    const double cu = u == 0 ? kFrac1Sqrt2 : 1.0;
+                    s = dadd(s, dmul(dmul(cu, bfly), s_cos16[x * 8 + u]));
+                }
+            }
+            h[row] = s / 2.0;
+        }
+#pragma unroll
+        for (int field = 0; field < 2; ++field)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                double s = 0.0;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double cv = v == 0 ? kFrac1Sqrt2 : 1.0;
+                    s = dadd(s, dmul(dmul(cv, h[v * 2 + field]), kDvCos8[y * 4 + v]));
+                }
+                out[y * 2 + field] = clamp_u8_d(round_half_away(s / 2.0));
+            }
+    }
+
+    // Placement (place_macroblock / put_split_chroma).
+    const uint32_t mb_index = blk / 6;
+    const int b = (int)(blk % 6);
+    const int seq = (int)(mb_index / 135), slot = (int)((mb_index % 135) / 5), mbs = (int)(mb_index % 5);
+    int mbx, mby;
+    dv_mb_coordinates(job.system, seq, slot, mbs, &mbx, &mby);
+    const int X = mbx * 8, Y = mby * 8;
+    const int width = 720;
+    if (job.system == MMR_DV_SYSTEM_625_50) {
+        if (b < 4) {
+            uint8_t* p = job.planes[0] + (size_t)(Y + (b >> 1) * 8) * width + X + (b & 1) * 8 + x;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+        } else {
+            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)(Y / 2) * 360 + X / 2 + x;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * 360] = out[r];
+        }
+    } else if (mbx < 88) {
+        if (b < 4) {
+            uint8_t* p = job.planes[0] + (size_t)Y * width + X + b * 8 + x;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+        } else {
+            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)Y * 180 + X / 4 + x;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * 180] = out[r];
+        }
+    } else {
+        if (b < 4) {
+            uint8_t* p = job.planes[0] + (size_t)(Y + (b >> 1) * 8) * width + X + (b & 1) * 8 + x;
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+        } else {
+            // Split chroma: columns 0..3 go to rows y..y+7, columns 4..7 to rows y+8..y+15.
+            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)(Y + (x >= 4 ? 8 : 0)) * 180 + X / 4 + (x & 3);
+#pragma unroll
+            for (int r = 0; r < 8; ++r) p[(size_t)r * 180] = out[r];
+        }
+    }
+}
+
+hipError_t dv_upload_tables(hipStream_t s) {
+    double cos16[64], cos8[32], cucv[64];
+    for (int x = 0; x < 8; ++x)
+        for (int u = 0; u < 8; ++u) cos16[x * 8 + u] = cos((double)(2 * x + 1) * (double)u * M_PI / 16.0);
+    for (int y = 0; y < 8; ++y)
+        for (int v = 0; v < 4; ++v) cos8[y * 4 + v] = cos((double)(2 * y + 1) * (double)v * M_PI / 8.0);
+    for (int v = 0; v < 8; ++v)
+        for (int u = 0; u < 8; ++u) {
+            const volatile double cu = u == 0 ? M_SQRT1_2 : 1.0, cv = v == 0 ? M_SQRT1_2 : 1.0;
+            cucv[v * 8 + u] = cu * cv;
+        }
+    hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCos16), cos16, sizeof cos16, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCos8), cos8, sizeof cos8, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCuCv), cucv, sizeof cucv, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
+hipError_t launch_dv(const DvJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s) {
+    if (total_blocks == 0) return hipSuccess;
+    const uint32_t groups = (total_blocks + kBlocksPerGroup - 1) / kBlocksPerGroup;
+    hipLaunchKernelGGL(dv_kernel, dim3(groups), dim3(kGroupThreads), 0, s, d_jobs, job_count, total_blocks);
+    return hipGetLastError();
+}
+
+}  // namespace mmr

@@ -1,0 +1,406 @@
+// H.264 inter reconstruction of frame pictures: quarter-sample luma / eighth-sample chroma
+// interpolation, explicit / implicit / default (bi-)prediction weights, 4x4 and 8x8 inverse
+// transforms with dequantisation, chroma DC Hadamard, residual add.
+//
+// Reference (crates/codecs/h264/src/decoder.rs): predict_inter_partition :11602-11656,
+// predict_bi_inter_partition :11901-12119, predict_weighted_luma/chroma_partition :13640-13768,
+// luma_qpel / half_* / chroma_epel / reference_sample / apply_(bi_)prediction_weight :13770-13926,
+// transform_residual_4x4 / _8x8 / inverse_transform_8 / level_scale_* :14185-14442,
+// transform_chroma_dc / chroma_qp :14102-14140, add_luma_residual_blocks / _8x8_blocks /
+// add_chroma_residual :12630-12789.
+//
+// One wave64 per macroblock, four macroblocks per 256-thread workgroup. Phases, separated by
+// workgroup barriers (every wave passes every barrier):
+//   1. prediction: lane l predicts row (l & 3) of luma 4x4 block (l >> 2) and one chroma sample of
+//      each of Cb / Cr for the same block, into LDS;
+//   2. residual front: 4x4 MBs: lanes 0-15 run a whole luma 4x4 block each; 8x8 MBs: all lanes
+//      dequantise the 256 luma levels into LDS; lanes 16/17 (4x4) or 0/1 (8x8) run the chroma DC;
+//   3. 8x8 row transforms (lanes 0-31) and chroma AC blocks (lanes 32-39);
+//   4. 8x8 column transforms;
+//   5. every lane adds residual to prediction for 4 luma + 2 chroma samples and stores them.
+// Transform arithmetic is 64-bit like the reference, so results match for every level value the
+// reference accepts.
+#include "device_common.h"
+#include "recon_internal.h"
+
+namespace mmr {
+
+__constant__ static const uint8_t kZz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};  // row*4+col
+__constant__ static const uint8_t kZz8[64] = {
+    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20,
+    13, 6, 7, 14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59,
+    52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+__constant__ static const int32_t kNorm4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
+__constant__ static const int32_t kNorm8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
+                                                 {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
+__constant__ static const int32_t kChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
+
+MMR_DEV int chroma_qp(int luma_qp, int offset) {
+    const int index = clampi(luma_qp + offset, 0, 51);
+    return index < 30 ? index : kChromaQp[index - 30];
+}
+
+MMR_DEV int32_t level_scale_4x4(int qp, int row, int col, const uint8_t* scaling) {
+    const int cat = ((row & 1) == 0 && (col & 1) == 0) ? 0 : (((row & 1) && (col & 1)) ? 1 : 2);
+    return kNorm4[qp % 6][cat] * (int32_t)scaling[row * 4 + col];
+}
+
+MMR_DEV int32_t level_scale_8x8(int qp, int row, int col, const uint8_t* scaling) {
+    const int rm = row & 3, cm = col & 3;
+    const bool re = (rm & 1) == 0, ce = (cm & 1) == 0;
+    int cat;
+    if (!re && !ce) cat = 1;
+    else if (re != ce) cat = (rm == 2 || cm == 2) ? 5 : 3;
+    else cat = (rm == 0 && cm == 0) ? 0 : ((rm == 2 && cm == 2) ? 2 : 4);
+    return kNorm8[qp % 6][cat] * (int32_t)scaling[row * 8 + col];
+}
+
+MMR_DEV int clip255(int v) { return clampi(v, 0, 255); }
+
+// transform_residual_4x4 (decoder.rs:14185-14233) on one lane; out in raster order.
+MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const uint8_t* scaling, int32_t out[16]) {
+    int64_t s[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int rc = kZz4[i];
+        const int64_t v = lv[i];
+        if (i == 0 && dc_scaled) {
+            s[rc] = v;
+        } else {
+            const int64_t scale = level_scale_4x4(qp, rc >> 2, rc & 3, scaling);
+            s[rc] = qp >= 24 ? (v * scale) << (qp / 6 - 4) : (v * scale + ((int64_t)1 << (3 - qp / 6))) >> (4 - qp / 6);
+        }
+    }
+    int64_t h[16];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int b = r * 4;
+        const int64_t e0 = s[b] + s[b + 2], e1 = s[b] - s[b + 2];
+        const int64_t e2 = (s[b + 1] >> 1) - s[b + 3], e3 = s[b + 1] + (s[b + 3] >> 1);
+        h[b] = e0 + e3;
+        h[b + 1] = e1 + e2;
+        h[b + 2] = e1 - e2;
+        h[b + 3] = e0 - e3;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int64_t g0 = h[c] + h[8 + c], g1 = h[c] - h[8 + c];
+        const int64_t g2 = (h[4 + c] >> 1) - h[12 + c], g3 = h[4 + c] + (h[12 + c] >> 1);
+        out[0 * 4 + c] = (int32_t)((g0 + g3 + 32) >> 6);
+        out[1 * 4 + c] = (int32_t)((g1 + g2 + 32) >> 6);
+        out[2 * 4 + c] = (int32_t)((g1 - g2 + 32) >> 6);
+        out[3 * 4 + c] = (int32_t)((g0 - g3 + 32) >> 6);
+    }
+}
+
+// inverse_transform_8 (decoder.rs:14280-14308).
+MMR_DEV void itrans8(const int64_t in[8], int64_t out[8]) {
+    const int64_t a0 = in[0] + in[4], a2 = in[0] - in[4];
+    const int64_t a4 = (in[2] >> 1) - in[6], a6 = in[2] + (in[6] >> 1);
+    const int64_t b0 = a0 + a6, b2 = a2 + a4, b4 = a2 - a4, b6 = a0 - a6;
+    const int64_t a1 = -in[3] + in[5] - in[7] - (in[7] >> 1);
+    const int64_t a3 = in[1] + in[7] - in[3] - (in[3] >> 1);
+    const int64_t a5 = -in[1] + in[7] + in[5] + (in[5] >> 1);
+    const int64_t a7 = in[3] + in[5] + in[1] + (in[1] >> 1);
+    const int64_t b1 = a1 + (a7 >> 2), b7 = a7 - (a1 >> 2), b3 = a3 + (a5 >> 2), b5 = (a3 >> 2) - a5;
+    out[0] = b0 + b7;
+    out[1] = b2 + b5;
+    out[2] = b4 + b3;
+    out[3] = b6 + b1;
+    out[4] = b6 - b1;
+    out[5] = b4 - b3;
+    out[6] = b2 - b5;
+    out[7] = b0 - b7;
+}
+
+MMR_DEV int div4(int v) { return v >> 2; }   // div_euclid(4) for two's complement
+MMR_DEV int div8(int v) { return v >> 3; }
+
+MMR_DEV uint32_t avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
+
+// Four consecutive luma samples (x_q4 + 4c, y_q4), c = 0..3, of luma_qpel (decoder.rs:13770-13829)
+// with reference_sample's edge clamp. The 6x9 integer window around the row is loaded once and
+// the half-sample terms are formed from registers exactly as the reference forms them.
+MMR_DEV void luma_qpel_row4(const uint8_t* __restrict__ p, int W, int H, int x_q4, int y_q4, uint32_t out[4]) {
+    const int x = div4(x_q4), y = div4(y_q4), fx = x_q4 & 3, fy = y_q4 & 3;
+    int cols[9], rows[6];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) cols[i] = clampi(x - 2 + i, 0, W - 1);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) rows[j] = clampi(y - 2 + j, 0, H - 1) * W;
+    if (fx == 0 && fy == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) out[c] = p[rows[2] + cols[c + 2]];
+        return;
+    }
+    int win[6][9];
+    if (fy == 0) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) win[2][i] = p[rows[2] + cols[i]];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int raw = win[2][c] - 5 * win[2][c + 1] + 20 * win[2][c + 2] + 20 * win[2][c + 3] - 5 * win[2][c + 4] + win[2][c + 5];
+            const uint32_t b = (uint32_t)clip255((raw + 16) >> 5);
+            out[c] = fx == 2 ? b : (fx == 1 ? avg2((uint32_t)win[2][c + 2], b) : avg2(b, (uint32_t)win[2][c + 3]));
+        }
+        return;
+    }
+    if (fx == 0) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+#pragma unroll
+            for (int i = 2; i < 6; ++i) win[j][i] = p[rows[j] + cols[i]];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int i = c + 2;
+            const int raw = win[0][i] - 5 * win[1][i] + 20 * win[2][i] + 20 * win[3][i] - 5 * win[4][i] + win[5][i];
+            const uint32_t h = (uint32_t)clip255((raw + 16) >> 5);
+            out[c] = fy == 2 ? h : (fy == 1 ? avg2((uint32_t)win[2][i], h) : avg2(h, (uint32_t)win[3][i]));
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) win[j][i] = p[rows[j] + cols[i]];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int rawh[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+            rawh[j] = win[j][c] - 5 * win[j][c + 1] + 20 * win[j][c + 2] + 20 * win[j][c + 3] - 5 * win[j][c + 4] + win[j][c + 5];
+        const uint32_t b0 = (uint32_t)clip255((rawh[2] + 16) >> 5);   // half_horizontal(x, y)
+        const uint32_t b1 = (uint32_t)clip255((rawh[3] + 16) >> 5);   // half_horizontal(x, y + 1)
+        const int i0 = c + 2, i1 = c + 3;
+        const uint32_t h0 = (uint32_t)clip255((win[0][i0] - 5 * win[1][i0] + 20 * win[2][i0] + 20 * win[3][i0] - 5 * win[4][i0] + win[5][i0] + 16) >> 5);
+        const uint32_t h1 = (uint32_t)clip255((win[0][i1] - 5 * win[1][i1] + 20 * win[2][i1] + 20 * win[3][i1] - 5 * win[4][i1] + win[5][i1] + 16) >> 5);
+        const uint32_t j = (uint32_t)clip255((rawh[0] - 5 * rawh[1] + 20 * rawh[2] + 20 * rawh[3] - 5 * rawh[4] + rawh[5] + 512) >> 10);
+        uint32_t r;
+        switch (fy * 4 + fx) {
+        case 5: r = avg2(b0, h0); break;
+        case 6: r = avg2(b0, j); break;
+        case 7: r = avg2(b0, h1); break;
+        case 9: r = avg2(h0, j); break;
+        case 10: r = j; break;
+        case 11: r = avg2(j, h1); break;
+        case 13: r = avg2(h0, b1); break;
+        case 14: r = avg2(j, b1); break;
+        default: r = avg2(h1, b1); break;  // case 15
+        }
+        out[c] = r;
+    }
+}
+
+// chroma_epel (decoder.rs:13851-13886).
+MMR_DEV uint32_t chroma_epel(const uint8_t* __restrict__ p, int W, int H, int x_q8, int y_q8) {
+    const int ix = div8(x_q8), iy = div8(y_q8), fx = x_q8 & 7, fy = y_q8 & 7;
+    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
+    const int y0 = clampi(iy, 0, H - 1) * W, y1 = clampi(iy + 1, 0, H - 1) * W;
+    const int A = p[y0 + x0], B = p[y0 + x1], C = p[y1 + x0], D = p[y1 + x1];
+    return (uint32_t)clip255(((8 - fx) * (8 - fy) * A + fx * (8 - fy) * B + (8 - fx) * fy * C + fx * fy * D + 32) >> 6);
+}
+
+// apply_prediction_weight (decoder.rs:13900-13908).
+MMR_DEV uint32_t weight1(uint32_t s, int d, int w, int o) {
+    const int weighted = w * (int)s;
+    const int scaled = d == 0 ? weighted : (weighted + (1 << (d - 1))) >> d;
+    return (uint32_t)clip255(scaled + o);
+}
+// apply_bi_prediction_weight (decoder.rs:13910-13926) / average_samples (:3573-3576).
+MMR_DEV uint32_t weight2(uint32_t s0, uint32_t s1, const mmr_recon_h264_bi_weight* bw, bool weighted) {
+    if (!weighted) return (s0 + s1 + 1) >> 1;
+    const int d = bw->log2_denom;
+    const int v = (bw->w0 * (int)s0 + bw->w1 * (int)s1 + (1 << d)) >> (d + 1);
+    return (uint32_t)clip255(v + ((bw->o0 + bw->o1 + 1) >> 1));
+}
+
+struct MbShared {
+    uint8_t pred_y[256];
+    uint8_t pred_c[2][64];
+    int32_t res_y[256];
+    int32_t res_c[2][64];
+    int64_t s8[4][64];
+    int32_t dc[2][4];
+};
+
+__global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict__ jobs, uint32_t job_count, uint32_t total_mbs,
+                                                       uint32_t* __restrict__ status) {
+    __shared__ MbShared sh[4];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    MbShared& S = sh[w];
+    const uint32_t gmb = blockIdx.x * 4 + w;
+    const bool active = gmb < total_mbs;
+    const uint32_t ji = active ? find_job(jobs, job_count, gmb, [](const H264Job& j) { return j.mb_begin; }) : 0;
+    const H264Job& job = jobs[ji];
+    const uint32_t mb = active ? gmb - job.mb_begin : 0;
+    const mmr_recon_h264_macroblock m = job.mbs[mb];
+    const int mbx = (int)(mb % job.width_mbs), mby = (int)(mb / job.width_mbs);
+    const int W = (int)job.width_mbs * 16, H = (int)job.height_mbs * 16, CW = W / 2, CH = H / 2;
+    const bool weighted = job.weighted == MMR_H264_WEIGHTED_TABLE;
+
+    // ---- phase 1: prediction
+    const int k = l >> 2;                               // H.264 block index
+    const int bx = ((k >> 2) & 1) * 2 + (k & 1), by = (k >> 3) * 2 + ((k >> 1) & 1);  // luma_block_position
+    if (active) {
+        const mmr_recon_h264_block blk = job.blocks[(size_t)mb * 16 + k];
+        int r0 = blk.ref_idx[0], r1 = blk.ref_idx[1];
+        // Intra macroblocks are not predicted on the device; out-of-range reference indices are
+        // invalid data. Both are reported through the status word and clamped so the kernel
+        // still completes without reading outside the reference list.
+        if (l == 0 && (m.flags & MMR_H264_MB_INTRA)) atomicOr(status, kDevStatusUnsupported);
+        if (r0 >= (int)job.ref_count[0] || r1 >= (int)job.ref_count[1] || (r0 < 0 && r1 < 0)) {
+            atomicOr(status, kDevStatusInvalidData);
+            r0 = job.ref_count[0] ? min(max(r0, 0), (int)job.ref_count[0] - 1) : -1;
+            r1 = (r0 < 0 && job.ref_count[1]) ? min(max(r1, 0), (int)job.ref_count[1] - 1) : -1;
+        }
+        const int ox = mbx * 16 + bx * 4, oy = mby * 16 + by * 4 + (l & 3);
+        const int cxs = mbx * 8 + bx * 2 + (l & 1), cys = mby * 8 + by * 2 + ((l >> 1) & 1);
+        uint32_t ys[4];
+        uint32_t cs[2];
+        if (r0 >= 0 && r1 >= 0) {
+            uint32_t a[4], b[4];
+            luma_qpel_row4(job.ref_planes[0][r0][0], W, H, ox * 4 + blk.mv[0][0], oy * 4 + blk.mv[0][1], a);
+            luma_qpel_row4(job.ref_planes[1][r1][0], W, H, ox * 4 + blk.mv[1][0], oy * 4 + blk.mv[1][1], b);
+            const mmr_recon_h264_bi_weight* bw = weighted ? job.bi_weights + ((size_t)r0 * MMR_H264_MAX_REFS + r1) * 3 : nullptr;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ys[c] = weight2(a[c], b[c], bw, weighted);
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const uint32_t s0 = chroma_epel(job.ref_planes[0][r0][1 + cc], CW, CH, cxs * 8 + blk.mv[0][0], cys * 8 + blk.mv[0][1]);
+                const uint32_t s1 = chroma_epel(job.ref_planes[1][r1][1 + cc], CW, CH, cxs * 8 + blk.mv[1][0], cys * 8 + blk.mv[1][1]);
+                cs[cc] = weight2(s0, s1, weighted ? bw + 1 + cc : nullptr, weighted);
+            }
+        } else {
+            const int list = r0 >= 0 ? 0 : 1, r = r0 >= 0 ? r0 : r1;
+            const int mvx = blk.mv[list][0], mvy = blk.mv[list][1];
+            uint32_t a[4];
+            luma_qpel_row4(job.ref_planes[list][r][0], W, H, ox * 4 + mvx, oy * 4 + mvy, a);
+            const mmr_recon_h264_weight* wt = weighted ? job.weights + ((size_t)list * MMR_H264_MAX_REFS + r) * 3 : nullptr;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ys[c] = wt ? weight1(a[c], wt[0].log2_denom, wt[0].weight, wt[0].offset) : a[c];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const uint32_t s = chroma_epel(job.ref_planes[list][r][1 + cc], CW, CH, cxs * 8 + mvx, cys * 8 + mvy);
+                cs[cc] = wt ? weight1(s, wt[1 + cc].log2_denom, wt[1 + cc].weight, wt[1 + cc].offset) : s;
+            }
+        }
+        const int py = by * 4 + (l & 3), px = bx * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) S.pred_y[py * 16 + px + c] = (uint8_t)ys[c];
+        const int cpy = by * 2 + ((l >> 1) & 1), cpx = bx * 2 + (l & 1);
+        S.pred_c[0][cpy * 8 + cpx] = (uint8_t)cs[0];
+        S.pred_c[1][cpy * 8 + cpx] = (uint8_t)cs[1];
+    }
+
+    const bool has_res = active && m.cbp != 0;
+    const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
+    const int16_t* lv = job.levels + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+    const int qp = m.qp;
+    const int cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both (decoder.rs:1921)
+
+    // ---- phase 2
+    if (has_res) {
+        if (!t8) {
+            if (l < 16) {
+                int32_t lev[16], res[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) lev[i] = lv[l * 16 + i];
+                residual_4x4(lev, qp, false, job.scaling + 3 * 16, res);
+                const int kbx = ((l >> 2) & 1) * 2 + (l & 1), kby = (l >> 3) * 2 + ((l >> 1) & 1);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) S.res_y[(kby * 4 + (i >> 2)) * 16 + kbx * 4 + (i & 3)] = res[i];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int idx = l * 4 + t, blk8 = idx >> 6, i = idx & 63, rc = kZz8[i];
+                const int64_t v = (int64_t)lv[idx] * level_scale_8x8(qp, rc >> 3, rc & 7, job.scaling + 96 + 64);
+                const int shift = qp / 6;
+                int64_t sv = shift >= 6 ? v << (shift - 6) : (v + ((int64_t)1 << (5 - shift))) >> (6 - shift);
+                if (rc == 0) sv += 32;
+                S.s8[blk8][rc] = sv;
+            }
+        }
+        const int dcl = t8 ? l : l - 16;
+        if (dcl == 0 || dcl == 1) {
+            // transform_chroma_dc (decoder.rs:14114-14140) with inter list 4 + component.
+            const int comp = dcl;
+            const int64_t c0 = lv[256 + comp * 4], c1 = lv[257 + comp * 4], c2 = lv[258 + comp * 4], c3 = lv[259 + comp * 4];
+            const int64_t f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
+            const int64_t scale = level_scale_4x4(cqp, 0, 0, job.scaling + (4 + comp) * 16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) S.dc[comp][i] = (int32_t)(((f[i] * scale) << (cqp / 6)) >> 5);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3
+    if (has_res) {
+        if (t8 && l < 32) {
+            const int blk8 = l >> 3, r = l & 7;
+            int64_t in[8], out[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][r * 8 + i];
+            itrans8(in, out);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) S.s8[blk8][r * 8 + i] = out[i];
+        }
+        if (l >= 32 && l < 40) {
+            const int comp = (l - 32) >> 2, b = (l - 32) & 3;
+            int32_t co[16], res[16];
+            co[0] = S.dc[comp][b];
+#pragma unroll
+            for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
+            residual_4x4(co, cqp, true, job.scaling + (4 + comp) * 16, res);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) S.res_c[comp][((b >> 1) * 4 + (i >> 2)) * 8 + (b & 1) * 4 + (i & 3)] = res[i];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 4
+    if (has_res && t8 && l < 32) {
+        const int blk8 = l >> 3, c = l & 7;
+        int64_t in[8], out[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][i * 8 + c];
+        itrans8(in, out);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            S.res_y[((blk8 >> 1) * 8 + i) * 16 + (blk8 & 1) * 8 + c] = (int32_t)(out[i] >> 6);
+    }
+    __syncthreads();
+
+    // ---- phase 5: residual add + store
+    if (!active) return;
+    {
+        const int row = l >> 2, c0 = (l & 3) * 4;
+        uint32_t word = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            int v = S.pred_y[row * 16 + c0 + c];
+            if (has_res) v = clip255(v + S.res_y[row * 16 + c0 + c]);
+            word |= (uint32_t)v << (8 * c);
+        }
+        *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)(mby * 16 + row) * W + mbx * 16 + c0) = word;
+    }
+    {
+        const int comp = l >> 5, row = (l >> 2) & 7, c0 = (l & 3) * 2;
+        uint32_t half = 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            int v = S.pred_c[comp][row * 8 + c0 + c];
+            if (has_res) v = clip255(v + S.res_c[comp][row * 8 + c0 + c]);
+            half |= (uint32_t)v << (8 * c);
+        }
+        *reinterpret_cast<uint16_t*>(job.planes[1 + comp] + (size_t)(mby * 8 + row) * CW + mbx * 8 + c0) = (uint16_t)half;
+    }
+}
+
+hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s) {
+    if (total_mbs == 0) return hipSuccess;
+    hipLaunchKernelGGL(h264_mb_kernel, dim3((total_mbs + 3) / 4), dim3(256), 0, s, d_jobs, job_count, total_mbs, d_status);
+    return hipGetLastError();
+}
+
+}  // namespace mmr

@@ -1,0 +1,811 @@
+// Host side of libmmr_recon: the C ABI of include/mmr_recon.h.
+//
+// A context owns one device and one HIP stream. Every submit call validates its descriptors,
+// packs them into device job records (recon_internal.h), uploads them with the work-queue
+// counters, launches the kernels stream-ordered and returns a ticket; completion is observed
+// through an event per ticket. Errors follow crates/capi: status codes plus a thread-local
+// message (crates/capi/src/lib.rs:46-48, :305-321). Kernels report unsupported or invalid records
+// found on the device through a per-ticket status word that mmr_recon_wait / mmr_recon_poll
+// return.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mmr_recon.h"
+#include "recon_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+mmr_status fail(mmr_status status, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+mmr_status hip_fail(hipError_t e, const char* what) {
+    return fail(MMR_STATUS_INTERNAL_ERROR, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct TicketRecord {
+    hipEvent_t start = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<char> host_jobs;      // kept alive until completion (async upload source)
+    uint32_t* host_status = nullptr;  // pinned; device status word copied here
+    bool finished = false;
+    mmr_status status = MMR_STATUS_OK;
+    double kernel_ms = -1.0;
+};
+
+constexpr uint32_t kStatusUnsupported = mmr::kDevStatusUnsupported;
+constexpr uint32_t kStatusInvalidData = mmr::kDevStatusInvalidData;
+
+}  // namespace
+
+struct mmr_recon_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t queue_depth = 256;
+    uint32_t cu_count = 256;
+    uint32_t persistent_groups = 0;
+    std::mutex mu;
+    uint64_t next_ticket = 1;
+    std::map<uint64_t, TicketRecord> tickets;
+    uint8_t* flat_scaling = nullptr;  // device Flat_16 scaling lists ([6][16] + [2][64])
+};
+
+namespace {
+
+// Retire finished tickets from the front so the map stays bounded.
+void reap(mmr_recon_context* ctx) {
+    while (!ctx->tickets.empty()) {
+        auto it = ctx->tickets.begin();
+        if (!it->second.finished) {
+            if (hipEventQuery(it->second.done) != hipSuccess) break;
+        }
+        TicketRecord& r = it->second;
+        if (r.start) (void)hipEventDestroy(r.start);
+        if (r.done) (void)hipEventDestroy(r.done);
+        if (r.host_status) (void)hipHostFree(r.host_status);
+        ctx->tickets.erase(it);
+    }
+}
+
+mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
+    if (!r.finished) {
+        hipError_t e = hipEventSynchronize(r.done);
+        if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.start, r.done) == hipSuccess) r.kernel_ms = ms;
+        const uint32_t st = r.host_status ? *r.host_status : 0u;
+        if (st & kStatusUnsupported) {
+            r.status = fail(MMR_STATUS_UNSUPPORTED, "a record of this submission uses a feature the device path does not reconstruct "
+                                                    "(MPEG-2 field prediction or H.264 intra macroblock); fall back to CPU reconstruction");
+        } else if (st & kStatusInvalidData) {
+            r.status = fail(MMR_STATUS_INVALID_DATA, "a record of this submission is invalid (reference index out of range)");
+        }
+        r.finished = true;
+    }
+    (void)ctx;
+    return r.status;
+}
+
+// A submission in flight: records the start event, owns the device allocation of job records +
+// counters + status word, and finishes by recording the completion event.
+struct Submission {
+    mmr_recon_context* ctx;
+    TicketRecord rec;
+    char* d_block = nullptr;
+    size_t jobs_bytes = 0;
+    size_t counter_words = 0;
+
+    explicit Submission(mmr_recon_context* c) : ctx(c) {}
+
+    mmr_status begin(const void* jobs, size_t bytes, size_t counters) {
+        if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
+        if (ctx->tickets.size() >= ctx->queue_depth)
+            return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
+        jobs_bytes = (bytes + 255) & ~size_t(255);
+        counter_words = counters + 1;  // + status word
+        rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
+        hipError_t e = hipEventCreate(&rec.start);
+        if (e == hipSuccess) e = hipEventCreate(&rec.done);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
+        if (e != hipSuccess) return hip_fail(e, "ticket setup");
+        *rec.host_status = 0;
+        e = hipMallocAsync((void**)&d_block, jobs_bytes + counter_words * sizeof(uint32_t), ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+        e = hipEventRecord(rec.start, ctx->stream);
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "job upload");
+        return MMR_STATUS_OK;
+    }
+    void* jobs_ptr() const { return d_block; }
+    uint32_t* counters_ptr() const { return reinterpret_cast<uint32_t*>(d_block + jobs_bytes); }
+    uint32_t* status_ptr() const { return counters_ptr() + (counter_words - 1); }
+
+    mmr_status end(uint64_t* out_ticket) {
+        hipError_t e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(rec.done, ctx->stream);
+        if (e == hipSuccess) e = hipFreeAsync(d_block, ctx->stream);
+        d_block = nullptr;
+        if (e != hipSuccess) return hip_fail(e, "submission end");
+        const uint64_t id = ctx->next_ticket++;
+        ctx->tickets.emplace(id, std::move(rec));
+        rec = TicketRecord();
+        if (out_ticket) *out_ticket = id;
+        return MMR_STATUS_OK;
+    }
+    ~Submission() {
+        if (d_block) (void)hipFreeAsync(d_block, ctx->stream);
+        if (rec.start) (void)hipEventDestroy(rec.start);
+        if (rec.done) (void)hipEventDestroy(rec.done);
+        if (rec.host_status) (void)hipHostFree(rec.host_status);
+    }
+};
+
+template <typename T>
+bool aligned(const T* p, size_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; }
+
+uint32_t groups_for(mmr_recon_context* ctx, uint32_t per_cu) {
+    return ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count * per_cu;
+}
+
+#define CHECK_CTX(ctx)                                                                 \
+    do {                                                                               \
+        if (!(ctx)) return fail(MMR_STATUS_INVALID_ARGUMENT, "context is NULL");       \
+    } while (0)
+
+#define LOCK_AND_SET_DEVICE(ctx)                                                       \
+    std::lock_guard<std::mutex> lock_((ctx)->mu);                                      \
+    do {                                                                               \
+        hipError_t e_ = hipSetDevice((ctx)->device);                                   \
+        if (e_ != hipSuccess) return hip_fail(e_, "hipSetDevice");                     \
+    } while (0)
+
+// ------------------------------------------------------------------ MJPEG helpers
+mmr_status build_mjpeg(const mmr_recon_mjpeg_frame& f, mmr::MjpegJob& j, uint32_t& rgb_quads) {
+    if (f.struct_size < sizeof(mmr_recon_mjpeg_frame)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG frame struct_size too small");
+    if (f.width == 0 || f.height == 0 || f.width > 65535 || f.height > 65535)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG frame size %ux%u is invalid", f.width, f.height);
+    if (f.component_count < 1 || f.component_count > 4)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG component count %u is invalid", f.component_count);
+    memset(&j, 0, sizeof j);
+    uint32_t hmax = 0, vmax = 0, per_mcu = 0;
+    for (uint32_t c = 0; c < f.component_count; ++c) {
+        if (f.h_sampling[c] < 1 || f.h_sampling[c] > 4 || f.v_sampling[c] < 1 || f.v_sampling[c] > 4)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG sampling factors of component %u are invalid", c);
+        if (f.quant_table[c] > 3) return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG quantization table index > 3");
+        if (!f.planes[c]) return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG plane %u is NULL", c);
+        hmax = std::max<uint32_t>(hmax, f.h_sampling[c]);
+        vmax = std::max<uint32_t>(vmax, f.v_sampling[c]);
+        per_mcu += f.h_sampling[c] * f.v_sampling[c];
+    }
+    const bool interleaved = f.component_count > 1;
+    if (interleaved && per_mcu > 10) return fail(MMR_STATUS_INVALID_DATA, "MJPEG MCU holds %u blocks (max 10)", per_mcu);
+    const uint32_t mcu_w = interleaved ? hmax * 8 : 8, mcu_h = interleaved ? vmax * 8 : 8;
+    const uint32_t cols = (f.width + mcu_w - 1) / mcu_w, rows = (f.height + mcu_h - 1) / mcu_h;
+    const uint64_t blocks = (uint64_t)cols * rows * (interleaved ? per_mcu : 1);
+    if (blocks != f.block_count)
+        return fail(MMR_STATUS_INVALID_DATA, "MJPEG block_count %u does not match the MCU walk (%llu)", f.block_count,
+                    (unsigned long long)blocks);
+    if (!f.levels || !f.quant_tables) return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG levels / quant_tables is NULL");
+    if (!aligned(f.levels, 16)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MJPEG levels must be 16-byte aligned");
+    j.levels = f.levels;
+    j.quant = f.quant_tables;
+    j.mcu_cols = cols;
+    j.blocks_per_mcu = interleaved ? per_mcu : 1;
+    j.block_count = f.block_count;
+    uint32_t k = 0;
+    for (uint32_t c = 0; c < f.component_count; ++c) {
+        j.planes[c] = f.planes[c];
+        j.plane_w[c] = (f.width * f.h_sampling[c] + hmax - 1) / hmax;
+        j.plane_h[c] = (f.height * f.v_sampling[c] + vmax - 1) / vmax;
+        j.comp_h[c] = interleaved ? f.h_sampling[c] : 1;
+        j.comp_v[c] = interleaved ? f.v_sampling[c] : 1;
+        j.qsel[c] = f.quant_table[c];
+        const uint32_t bh = interleaved ? f.v_sampling[c] : 1, bw = interleaved ? f.h_sampling[c] : 1;
+        for (uint32_t by = 0; by < bh; ++by)
+            for (uint32_t bx = 0; bx < bw; ++bx, ++k) {
+                j.blk_comp[k] = (uint8_t)c;
+                j.blk_bx[k] = (uint8_t)bx;
+                j.blk_by[k] = (uint8_t)by;
+            }
+    }
+    rgb_quads = 0;
+    if (f.rgb) {
+        if (f.component_count != 3) return fail(MMR_STATUS_UNSUPPORTED, "RGB output needs 3 components");
+        rgb_quads = ((f.width + 3) / 4) * f.height;
+    }
+    return MMR_STATUS_OK;
+}
+
+mmr::RgbJob rgb_job_of(const mmr_recon_mjpeg_frame& f, const mmr::MjpegJob& j) {
+    mmr::RgbJob r;
+    memset(&r, 0, sizeof r);
+    r.planes[0] = j.planes[0];
+    r.planes[1] = j.planes[1];
+    r.planes[2] = j.planes[2];
+    r.rgb = f.rgb;
+    r.width = f.width;
+    r.height = f.height;
+    r.cw = j.plane_w[1];
+    r.ch = j.plane_h[1];
+    r.luma_stride = j.plane_w[0];
+    r.chroma_stride = j.plane_w[1];
+    r.range = f.rgb_range ? f.rgb_range : MMR_RANGE_FULL;
+    r.matrix = MMR_MATRIX_BT601;
+    r.quads_per_row = (f.width + 3) / 4;
+    return r;
+}
+
+mmr_status check_rgb_job(const mmr_recon_yuv_to_rgb_job& s) {
+    if (s.struct_size < sizeof(mmr_recon_yuv_to_rgb_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB struct_size too small");
+    if (!s.width || !s.height || !s.chroma_width || !s.chroma_height)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB job has zero dimensions");
+    if (s.luma_stride < s.width || s.chroma_stride < s.chroma_width)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB stride smaller than width");
+    if (!s.planes[0] || !s.planes[1] || !s.planes[2] || (!s.rgb && !s.rgba))
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB job has a NULL plane or no output");
+    if (s.range != MMR_RANGE_FULL && s.range != MMR_RANGE_LIMITED) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB range invalid");
+    if (s.matrix != MMR_MATRIX_BT601 && s.matrix != MMR_MATRIX_BT709) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB matrix invalid");
+    return MMR_STATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t mmr_recon_abi_version(void) { return MMR_RECON_ABI_VERSION; }
+
+size_t mmr_recon_last_error_message(char* buffer, size_t capacity) {
+    const size_t need = g_last_error.size() + 1;
+    if (buffer && capacity) {
+        const size_t n = std::min(capacity - 1, g_last_error.size());
+        memcpy(buffer, g_last_error.data(), n);
+        buffer[n] = 0;
+    }
+    return need;
+}
+
+mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_context** out_context) {
+    if (!out_context) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_context is NULL");
+    *out_context = nullptr;
+    mmr_recon_config cfg;
+    memset(&cfg, 0, sizeof cfg);
+    if (config) {
+        if (config->struct_size < sizeof(mmr_recon_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "config struct_size too small");
+        cfg = *config;
+    }
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(MMR_STATUS_UNSUPPORTED, "no HIP device is available");
+    if (cfg.device_ordinal < 0 || cfg.device_ordinal >= count)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "device ordinal %d out of range (%d devices)", cfg.device_ordinal, count);
+    auto* ctx = new mmr_recon_context();
+    ctx->device = cfg.device_ordinal;
+    ctx->queue_depth = cfg.queue_depth ? cfg.queue_depth : 256;
+    ctx->persistent_groups = cfg.persistent_groups;
+    e = hipSetDevice(ctx->device);
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e == hipSuccess) ctx->cu_count = (uint32_t)prop.multiProcessorCount;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = mmr::dv_upload_tables(ctx->stream);
+    if (e == hipSuccess) e = hipMalloc((void**)&ctx->flat_scaling, 96 + 128);
+    if (e == hipSuccess) {
+        uint8_t flat[224];
+        memset(flat, 16, sizeof flat);
+        e = hipMemcpy(ctx->flat_scaling, flat, sizeof flat, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        mmr_status s = hip_fail(e, "context setup");
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+        delete ctx;
+        return s;
+    }
+    *out_context = ctx;
+    return MMR_STATUS_OK;
+}
+
+void mmr_recon_context_destroy(mmr_recon_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->tickets) {
+        if (kv.second.start) (void)hipEventDestroy(kv.second.start);
+        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+        if (kv.second.host_status) (void)hipHostFree(kv.second.host_status);
+    }
+    if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+mmr_status mmr_recon_device_alloc(mmr_recon_context* ctx, size_t bytes, void** out) {
+    CHECK_CTX(ctx);
+    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out pointer is NULL");
+    LOCK_AND_SET_DEVICE(ctx);
+    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipMalloc");
+}
+
+mmr_status mmr_recon_device_free(mmr_recon_context* ctx, void* p) {
+    CHECK_CTX(ctx);
+    LOCK_AND_SET_DEVICE(ctx);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && p) e = hipFree(p);
+    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipFree");
+}
+
+mmr_status mmr_recon_copy_to_device(mmr_recon_context* ctx, void* dst, const void* src, size_t bytes) {
+    CHECK_CTX(ctx);
+    if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
+    LOCK_AND_SET_DEVICE(ctx);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to device");
+}
+
+mmr_status mmr_recon_copy_to_host(mmr_recon_context* ctx, void* dst, const void* src, size_t bytes) {
+    CHECK_CTX(ctx);
+    if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
+    LOCK_AND_SET_DEVICE(ctx);
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to host");
+}
+
+mmr_status mmr_recon_wait(mmr_recon_context* ctx, uint64_t ticket) {
+    CHECK_CTX(ctx);
+    LOCK_AND_SET_DEVICE(ctx);
+    auto it = ctx->tickets.find(ticket);
+    if (it == ctx->tickets.end()) {
+        if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+        return MMR_STATUS_OK;  // already retired
+    }
+    mmr_status s = finish(ctx, it->second);
+    return s;
+}
+
+mmr_status mmr_recon_poll(mmr_recon_context* ctx, uint64_t ticket, int32_t* out_done) {
+    CHECK_CTX(ctx);
+    if (!out_done) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_done is NULL");
+    LOCK_AND_SET_DEVICE(ctx);
+    auto it = ctx->tickets.find(ticket);
+    if (it == ctx->tickets.end()) {
+        if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+        *out_done = 1;
+        return MMR_STATUS_OK;
+    }
+    hipError_t q = hipEventQuery(it->second.done);
+    if (q == hipErrorNotReady) {
+        *out_done = 0;
+        return MMR_STATUS_OK;
+    }
+    *out_done = 1;
+    return finish(ctx, it->second);
+}
+
+mmr_status mmr_recon_synchronize(mmr_recon_context* ctx) {
+    CHECK_CTX(ctx);
+    LOCK_AND_SET_DEVICE(ctx);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    mmr_status worst = MMR_STATUS_OK;
+    for (auto& kv : ctx->tickets) {
+        mmr_status s = finish(ctx, kv.second);
+        if (s != MMR_STATUS_OK && worst == MMR_STATUS_OK) worst = s;
+    }
+    reap(ctx);
+    return worst;
+}
+
+mmr_status mmr_recon_last_kernel_ms(mmr_recon_context* ctx, uint64_t ticket, double* out_ms) {
+    CHECK_CTX(ctx);
+    if (!out_ms) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_ms is NULL");
+    LOCK_AND_SET_DEVICE(ctx);
+    auto it = ctx->tickets.find(ticket);
+    if (it == ctx->tickets.end()) return fail(MMR_STATUS_INVALID_STATE, "ticket %llu is not tracked", (unsigned long long)ticket);
+    mmr_status s = finish(ctx, it->second);
+    *out_ms = it->second.kernel_ms;
+    return s;
+}
+
+// ------------------------------------------------------------------ MJPEG
+mmr_status mmr_recon_mjpeg_submit(mmr_recon_context* ctx, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
+                                  uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
+    std::vector<mmr::MjpegJob> jobs(count);
+    std::vector<mmr::RgbJob> rgb;
+    uint32_t total = 0, total_quads = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t quads = 0;
+        mmr_status s = build_mjpeg(frames[i], jobs[i], quads);
+        if (s) return s;
+        jobs[i].block_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total;
+        total += jobs[i].block_count;
+        if (quads) {
+            mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
+            r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
+            total_quads += quads;
+            rgb.push_back(r);
+        }
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    std::vector<char> blob(jobs.size() * sizeof(mmr::MjpegJob) + rgb.size() * sizeof(mmr::RgbJob));
+    memcpy(blob.data(), jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
+    if (!rgb.empty()) memcpy(blob.data() + jobs.size() * sizeof(mmr::MjpegJob), rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
+    Submission sub(ctx);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::MjpegJob*>(sub.jobs_ptr());
+    auto* dr = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + jobs.size() * sizeof(mmr::MjpegJob));
+    hipError_t e = hipSuccess;
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        size_t r = 0;
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            e = mmr::launch_mjpeg(dj + i, 1, jobs[i].block_count, ctx->stream);
+            if (e == hipSuccess && frames[i].rgb) {
+                e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, ctx->stream);
+                ++r;
+            }
+        }
+    } else {
+        e = mmr::launch_mjpeg(dj, count, total, ctx->stream);
+        if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
+    return sub.end(out_ticket);
+}
+
+// ------------------------------------------------------------------ MPEG-2
+mmr_status mmr_recon_mpeg2_submit(mmr_recon_context* ctx, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
+                                  uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::Mpeg2Job> jobs(count);
+    uint32_t tiles = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_mpeg2_picture& p = pics[i];
+        if (p.struct_size < sizeof(mmr_recon_mpeg2_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
+        if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u size %ux%u is invalid", i, p.width, p.height);
+        if (p.picture_type < MMR_MPEG2_PICTURE_I || p.picture_type > MMR_MPEG2_PICTURE_B)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u type %u is invalid", i, p.picture_type);
+        if (p.alternate_scan > 1 || p.intra_dc_precision > 3)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u scan / dc precision invalid", i);
+        if (!p.matrices || !p.macroblocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u has a NULL buffer", i);
+        if (!aligned(p.levels, 16) || !aligned(p.macroblocks, 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u: levels must be 16-byte aligned", i);
+        const bool needs_f = p.picture_type != MMR_MPEG2_PICTURE_I, needs_b = p.picture_type == MMR_MPEG2_PICTURE_B;
+        if ((needs_f && (!p.forward[0] || !p.forward[1] || !p.forward[2])) || (needs_b && (!p.backward[0] || !p.backward[1] || !p.backward[2])))
+            return fail(MMR_STATUS_INVALID_DATA, "missing %s MPEG-2 reference for picture %u", needs_b ? "forward/backward" : "forward", i);
+        if (!per_frame && (p.forward_source >= (int32_t)i || p.backward_source >= (int32_t)i || p.forward_source < -1 || p.backward_source < -1))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u depends on a picture that is not earlier in decode order", i);
+        mmr::Mpeg2Job& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.matrices = p.matrices;
+        j.mbs = p.macroblocks;
+        j.levels = p.levels;
+        for (int k = 0; k < 3; ++k) {
+            j.planes[k] = p.planes[k];
+            j.fwd[k] = p.forward[k] ? p.forward[k] : p.planes[k];
+            j.bwd[k] = p.backward[k] ? p.backward[k] : (p.forward[k] ? p.forward[k] : p.planes[k]);
+        }
+        j.fwd_src = per_frame ? -1 : p.forward_source;
+        j.bwd_src = per_frame ? -1 : p.backward_source;
+        j.mb_w = (p.width + 15) / 16;
+        j.mb_h = (p.height + 15) / 16;
+        j.alt_scan = p.alternate_scan;
+        j.dc_shift = 3 - p.intra_dc_precision;
+        j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
+        j.tile_begin = per_frame ? 0 : tiles;
+        tiles += j.tile_count;
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    const size_t counters = per_frame ? (size_t)count * 2 : (size_t)count + 1;
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::Mpeg2Job), counters);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    const uint32_t groups = groups_for(ctx, 8);
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_mpeg2(dj + i, 1, jobs[i].tile_count, sub.counters_ptr() + 2 * i, groups, sub.status_ptr(), ctx->stream);
+    } else {
+        e = mmr::launch_mpeg2(dj, count, tiles, sub.counters_ptr(), groups, sub.status_ptr(), ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "MPEG-2 launch");
+    return sub.end(out_ticket);
+}
+
+// ------------------------------------------------------------------ DV
+mmr_status mmr_recon_dv_submit(mmr_recon_context* ctx, const mmr_recon_dv_frame* frames, uint32_t count, uint32_t flags,
+                               uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::DvJob> jobs(count);
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_dv_frame& f = frames[i];
+        if (f.struct_size < sizeof(mmr_recon_dv_frame)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV struct_size too small");
+        if (f.system != MMR_DV_SYSTEM_525_60 && f.system != MMR_DV_SYSTEM_625_50)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u system %u is invalid", i, f.system);
+        if (!f.levels || !f.block_info || !f.planes[0] || !f.planes[1] || !f.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u has a NULL buffer", i);
+        if (!aligned(f.levels, 16)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV levels must be 16-byte aligned");
+        mmr::DvJob& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.levels = f.levels;
+        j.info = f.block_info;
+        for (int k = 0; k < 3; ++k) j.planes[k] = f.planes[k];
+        j.system = f.system;
+        j.block_count = (f.system == MMR_DV_SYSTEM_625_50 ? 12u : 10u) * 27u * 5u * 6u;
+        j.block_begin = per_frame ? 0 : total;
+        total += j.block_count;
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::DvJob), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::DvJob*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_dv(dj + i, 1, jobs[i].block_count, ctx->stream);
+    } else {
+        e = mmr::launch_dv(dj, count, total, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "DV launch");
+    return sub.end(out_ticket);
+}
+
+// ------------------------------------------------------------------ H.264
+mmr_status mmr_recon_h264_implicit_weights(int32_t poc0, int32_t poc1, int32_t cur, int32_t lt0, int32_t lt1, int32_t* w0,
+                                           int32_t* w1) {
+    // implicit_b_prediction_weights, crates/codecs/h264/src/decoder.rs:3577-3620.
+    if (!w0 || !w1) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL output");
+    int64_t td = std::clamp<int64_t>((int64_t)poc1 - poc0, -128, 127);
+    int32_t weight1 = 32;
+    if (!(lt0 || lt1 || td == 0)) {
+        const int64_t tb = std::clamp<int64_t>((int64_t)cur - poc0, -128, 127);
+        const int64_t tx = (16384 + (td < 0 ? -td : td) / 2) / td;
+        const int64_t scale = std::clamp<int64_t>((tb * tx + 32) >> 6, -1024, 1023);
+        const int64_t candidate = scale >> 2;
+        weight1 = (candidate >= -64 && candidate <= 128) ? (int32_t)candidate : 32;
+    }
+    *w0 = 64 - weight1;
+    *w1 = weight1;
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_h264_submit(mmr_recon_context* ctx, const mmr_recon_h264_picture* pics, uint32_t count, uint32_t flags,
+                                 uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::H264Job> jobs(count);
+    uint32_t mbs = 0, rows = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_h264_picture& p = pics[i];
+        if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
+        if (p.width_mbs == 0 || p.height_mbs == 0 || p.width_mbs > 1024 || p.height_mbs > 1024)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
+        if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u has a NULL buffer", i);
+        if (p.ref_count[0] > MMR_H264_MAX_REFS || p.ref_count[1] > MMR_H264_MAX_REFS || p.ref_count[0] + p.ref_count[1] == 0)
+            return fail(MMR_STATUS_INVALID_DATA, "H.264 picture %u reference counts invalid", i);
+        if (p.weighted_mode > MMR_H264_WEIGHTED_TABLE) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 weighted_mode invalid");
+        if (p.weighted_mode == MMR_H264_WEIGHTED_TABLE && (!p.weights || !p.bi_weights))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
+        if (!aligned(p.planes[0], 4) || !aligned(p.planes[1], 4) || !aligned(p.planes[2], 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 planes must be 4-byte aligned");
+        mmr::H264Job& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.mbs = p.macroblocks;
+        j.blocks = p.blocks;
+        j.levels = p.levels;
+        j.weights = p.weights;
+        j.bi_weights = p.bi_weights;
+        j.scaling = p.scaling_lists ? p.scaling_lists : ctx->flat_scaling;
+        for (int k = 0; k < 3; ++k) j.planes[k] = p.planes[k];
+        for (int list = 0; list < 2; ++list) {
+            j.ref_count[list] = p.ref_count[list];
+            for (uint32_t r = 0; r < MMR_H264_MAX_REFS; ++r) {
+                const uint32_t src = r < p.ref_count[list] ? r : 0;
+                const mmr_recon_h264_reference& ref = p.refs[list][src];
+                for (int k = 0; k < 3; ++k) {
+                    const uint8_t* plane = p.ref_count[list] ? ref.planes[k] : p.refs[1 - list][0].planes[k];
+                    if (!plane) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: reference plane is NULL", i);
+                    j.ref_planes[list][r][k] = plane;
+                }
+                j.ref_poc[list][r] = p.ref_count[list] ? ref.poc : 0;
+            }
+        }
+        j.width_mbs = p.width_mbs;
+        j.height_mbs = p.height_mbs;
+        j.weighted = p.weighted_mode;
+        j.deblock = p.deblock ? 1u : 0u;
+        j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
+        j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
+        j.mb_begin = per_frame ? 0 : mbs;
+        j.row_begin = per_frame ? 0 : rows;
+        mbs += p.width_mbs * p.height_mbs;
+        rows += j.deblock ? p.height_mbs : 0;
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    size_t counters = 0;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
+    } else {
+        counters = 1 + rows;
+    }
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::H264Job), counters);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::H264Job*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    const uint32_t groups = groups_for(ctx, 16);
+    if (per_frame) {
+        uint32_t* c = sub.counters_ptr();
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            e = mmr::launch_h264_mc(dj + i, 1, pics[i].width_mbs * pics[i].height_mbs, sub.status_ptr(), ctx->stream);
+            if (e == hipSuccess && jobs[i].deblock) {
+                e = mmr::launch_h264_deblock(dj + i, 1, pics[i].height_mbs, c, 1 + pics[i].height_mbs, groups, ctx->stream);
+                c += 1 + pics[i].height_mbs;
+            } else {
+                c += 1;
+            }
+        }
+    } else {
+        e = mmr::launch_h264_mc(dj, count, mbs, sub.status_ptr(), ctx->stream);
+        if (e == hipSuccess && rows) e = mmr::launch_h264_deblock(dj, count, rows, sub.counters_ptr(), 1 + rows, groups, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "H.264 launch");
+    return sub.end(out_ticket);
+}
+
+// ------------------------------------------------------------------ colour / blend
+mmr_status mmr_recon_yuv_to_rgb_submit(mmr_recon_context* ctx, const mmr_recon_yuv_to_rgb_job* in, uint32_t count, uint32_t flags,
+                                       uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::RgbJob> jobs(count);
+    uint32_t quads = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        mmr_status s = check_rgb_job(in[i]);
+        if (s) return s;
+        mmr::RgbJob& r = jobs[i];
+        memset(&r, 0, sizeof r);
+        for (int k = 0; k < 3; ++k) r.planes[k] = in[i].planes[k];
+        r.rgb = in[i].rgb;
+        r.rgba = in[i].rgba;
+        r.width = in[i].width;
+        r.height = in[i].height;
+        r.cw = in[i].chroma_width;
+        r.ch = in[i].chroma_height;
+        r.luma_stride = in[i].luma_stride;
+        r.chroma_stride = in[i].chroma_stride;
+        r.range = in[i].range;
+        r.matrix = in[i].matrix;
+        r.quads_per_row = (in[i].width + 3) / 4;
+        r.quad_begin = per_frame ? 0 : quads;
+        quads += r.quads_per_row * r.height;
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::RgbJob), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::RgbJob*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_yuv_to_rgb(dj + i, 1, jobs[i].quads_per_row * jobs[i].height, ctx->stream);
+    } else {
+        e = mmr::launch_yuv_to_rgb(dj, count, quads, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "YUV->RGB launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status mmr_recon_composite_submit(mmr_recon_context* ctx, const mmr_recon_composite_job* in, uint32_t count, uint32_t flags,
+                                      uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::CompositeJobDev> jobs(count);
+    uint32_t px = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_composite_job& c = in[i];
+        if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
+        if (!c.target || !c.width || !c.height || c.stride < c.width * 4 || c.layer_count > MMR_RECON_MAX_LAYERS)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
+        if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
+        for (uint32_t l = 0; l < c.layer_count; ++l) {
+            const mmr_recon_layer& L = c.layers[l];
+            if (!L.rgba || L.stride < L.width * 4 || L.stride % 4 || !aligned(L.rgba, 4))
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u layer %u storage is invalid", i, l);
+            if ((uint64_t)L.x + L.width > c.width || (uint64_t)L.y + L.height > c.height)
+                return fail(MMR_STATUS_INVALID_DATA, "RGBA overlay storage or placement is invalid (job %u layer %u)", i, l);
+            if (L.op != MMR_BLEND_SOURCE_OVER && L.op != MMR_BLEND_PREMULTIPLIED_OVER)
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite operator %u invalid", L.op);
+            if (L.opacity > 255) return fail(MMR_STATUS_INVALID_ARGUMENT, "opacity must be 0..255");
+        }
+        mmr::CompositeJobDev& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.target = c.target;
+        j.width = c.width;
+        j.height = c.height;
+        j.stride = c.stride;
+        j.layer_count = c.layer_count;
+        for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
+        j.px_begin = per_frame ? 0 : px;
+        px += c.width * c.height;
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::CompositeJobDev), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::CompositeJobDev*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_composite(dj + i, 1, jobs[i].width * jobs[i].height, ctx->stream);
+    } else {
+        e = mmr::launch_composite(dj, count, px, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "composite launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
+                                      uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    uint32_t max_px = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_yuv_blend_job& j = in[i];
+        if (j.struct_size < sizeof(mmr_recon_yuv_blend_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend struct_size too small");
+        if ((j.width | j.height | j.x | j.y) & 1 || (uint64_t)j.x + j.width > j.target_width || (uint64_t)j.y + j.height > j.target_height)
+            return fail(MMR_STATUS_INVALID_DATA, "YUV overlay storage or placement is invalid (job %u)", i);
+        for (int k = 0; k < 3; ++k)
+            if (!j.target[k] || !j.overlay[k]) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has a NULL plane", i);
+        if (!j.rgba || !j.chroma_alpha) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has no alpha", i);
+        max_px = std::max(max_px, j.width * j.height);
+    }
+    LOCK_AND_SET_DEVICE(ctx);
+    Submission sub(ctx);
+    mmr_status s = sub.begin(in, sizeof(mmr_recon_yuv_blend_job) * count, 0);
+    if (s) return s;
+    hipError_t e = hipSuccess;
+    auto* dj = reinterpret_cast<const mmr_recon_yuv_blend_job*>(sub.jobs_ptr());
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_yuv_blend(dj + i, 1, in[i].width * in[i].height, ctx->stream);
+    } else {
+        e = mmr::launch_yuv_blend(dj, count, max_px, ctx->stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "YUV blend launch");
+    return sub.end(out_ticket);
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// Internal job descriptors (device-resident, built by the host code of recon_host.cpp) and the
+// kernel launchers. Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mmr_recon.h"
+
+namespace mmr {
+
+// ---------------------------------------------------------------- MJPEG
+struct MjpegJob {
+    const int16_t* levels;
+    const uint16_t* quant;        // [n][64] zigzag
+    uint8_t* planes[4];
+    uint32_t plane_w[4];
+    uint32_t plane_h[4];
+    uint32_t mcu_cols;
+    uint32_t blocks_per_mcu;      // 1 when not interleaved
+    uint32_t block_begin;         // first global block index of this frame
+    uint32_t block_count;
+    uint8_t blk_comp[16];         // per block-in-MCU: component, column, row
+    uint8_t blk_bx[16];
+    uint8_t blk_by[16];
+    uint8_t comp_h[4];            // blocks per MCU row / column of each component (1,1 when not interleaved)
+    uint8_t comp_v[4];
+    uint8_t qsel[4];
+};
+
+// ---------------------------------------------------------------- MPEG-2
+struct Mpeg2Job {
+    const uint8_t* matrices;      // [4][64] natural
+    const mmr_recon_mpeg2_macroblock* mbs;
+    const int16_t* levels;
+    uint8_t* planes[3];
+    const uint8_t* fwd[3];
+    const uint8_t* bwd[3];
+    int32_t fwd_src;
+    int32_t bwd_src;
+    uint32_t mb_w;
+    uint32_t mb_h;
+    uint32_t alt_scan;
+    uint32_t dc_shift;            // 3 - intra_dc_precision
+    uint32_t tile_begin;          // first work-queue tile of this picture
+    uint32_t tile_count;
+};
+
+// ---------------------------------------------------------------- DV
+struct DvJob {
+    const int16_t* levels;
+    const uint8_t* info;
+    uint8_t* planes[3];
+    uint32_t system;
+    uint32_t block_begin;
+    uint32_t block_count;
+    uint32_t pad;
+};
+
+// ---------------------------------------------------------------- H.264
+struct H264Job {
+    const mmr_recon_h264_macroblock* mbs;
+    const mmr_recon_h264_block* blocks;
+    const int16_t* levels;
+    const mmr_recon_h264_weight* weights;
+    const mmr_recon_h264_bi_weight* bi_weights;
+    const uint8_t* scaling;       // never NULL on device (flat table substituted)
+    uint8_t* planes[3];
+    const uint8_t* ref_planes[2][MMR_H264_MAX_REFS][3];
+    int32_t ref_poc[2][MMR_H264_MAX_REFS];
+    uint32_t ref_count[2];
+    uint32_t width_mbs;
+    uint32_t height_mbs;
+    uint32_t weighted;
+    uint32_t deblock;
+    int32_t chroma_qp_offset[2];
+    uint32_t mb_begin;            // first global macroblock index of this picture
+    uint32_t row_begin;           // first global macroblock-row ticket of this picture
+};
+
+// ---------------------------------------------------------------- colour / blend
+struct RgbJob {
+    const uint8_t* planes[3];
+    uint8_t* rgb;
+    uint8_t* rgba;
+    uint32_t width, height, cw, ch, luma_stride, chroma_stride, range, matrix;
+    uint32_t quad_begin;          // first global 4-pixel quad of this job
+    uint32_t quads_per_row;
+};
+
+struct CompositeJobDev {
+    uint8_t* target;
+    uint32_t width, height, stride, layer_count;
+    uint32_t px_begin;
+    uint32_t pad;
+    mmr_recon_layer layers[MMR_RECON_MAX_LAYERS];
+};
+
+// Kernel launchers (stream-ordered). Each returns hipSuccess or the launch error.
+hipError_t launch_mjpeg(const MjpegJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s);
+hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_tiles, uint32_t* d_counters,
+                        uint32_t groups, uint32_t* d_status, hipStream_t s);
+hipError_t launch_dv(const DvJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s);
+hipError_t dv_upload_tables(hipStream_t s);
+hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s);
+hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
+                               uint32_t counter_words, uint32_t groups, hipStream_t s);
+hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
+hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
+hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
+hipError_t launch_fill_zero(uint32_t* d_words, uint32_t count, hipStream_t s);
+
+// Bits of the per-submission device status word.
+constexpr uint32_t kDevStatusUnsupported = 1u;
+constexpr uint32_t kDevStatusInvalidData = 2u;
+
+// Binary search over ascending begin offsets: index of the last job whose begin <= key.
+template <typename Job, typename Get>
+__device__ __forceinline__ uint32_t find_job(const Job* jobs, uint32_t count, uint32_t key, Get get) {
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (get(jobs[mid]) <= key) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+}  // namespace mmr

@@ -1,0 +1,90 @@
+/*
+ * Oracle: a scalar C restatement of MMRecode's reconstruction functions.
+ *
+ * TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
+ * calls this code; tests, the smoke check and the CPU-baseline leg of the
+ * benchmark use it as the checker (and as the timed CPU baseline).
+ *
+ * Every function follows the operation order, integer widths, rounding and
+ * non-fused f64 arithmetic of the Rust function it cites (paths relative to
+ * the MMRecode repository). Descriptors use the layouts of include/mmr_recon.h
+ * but all pointers are HOST pointers here.
+ *
+ * Build: oracle/Makefile (gcc -O2 -ffp-contract=off, no -ffast-math).
+ */
+#ifndef MMR_ORACLE_H
+#define MMR_ORACLE_H
+
+#include <stdint.h>
+#include "../include/mmr_recon.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* MJPEG (crates/codecs/mjpeg/src) */
+void oracle_mjpeg_inverse_dct(const int32_t coefficients[64], uint8_t out[64]);
+void oracle_mjpeg_dequantize(const int16_t levels[64], const uint16_t quant_zigzag[64],
+                             int32_t coefficients[64]);
+uint32_t oracle_mjpeg_block_count(const mmr_recon_mjpeg_frame *frame);
+int oracle_mjpeg_reconstruct(const mmr_recon_mjpeg_frame *frame);
+
+/* MPEG-2 (crates/codecs/mpeg2/src) */
+void oracle_mpeg2_inverse_dct(const int32_t coefficients[64], int16_t out[64]);
+void oracle_mpeg2_dequantize_intra(const int16_t levels[64], int component, int qscale,
+                                   int intra_dc_precision, int alternate_scan,
+                                   const uint8_t matrix[64], int32_t coefficients[64]);
+void oracle_mpeg2_dequantize_non_intra(const int16_t levels[64], int qscale, int alternate_scan,
+                                       const uint8_t matrix[64], int32_t coefficients[64]);
+uint8_t oracle_mpeg2_halfpel_sample(const uint8_t *reference, int stride, int height, int x,
+                                    int y, int mv_x, int mv_y);
+int oracle_mpeg2_reconstruct(const mmr_recon_mpeg2_picture *picture);
+
+/* DV (crates/codecs/dv/src) */
+void oracle_dv_inverse_88(const int32_t coefficients[64], uint8_t out[64]);
+void oracle_dv_inverse_248(const int32_t coefficients[64], uint8_t out[64]);
+int32_t oracle_dv_inverse_factor(int quantization, int class_number, int dct_mode, int position);
+void oracle_dv_dequantize(const int16_t levels[64], uint8_t block_info, int32_t coefficients[64]);
+void oracle_dv_macroblock_coordinates(uint32_t system, int sequence, int slot, int out_xy[10]);
+uint32_t oracle_dv_block_count(uint32_t system);
+int oracle_dv_reconstruct(const mmr_recon_dv_frame *frame);
+const double *oracle_dv_cos16_table(void); /* [x*8+u] = cos((2x+1)u pi/16) */
+const double *oracle_dv_cos8_table(void);  /* [y*4+v] = cos((2y+1)v pi/8) */
+
+/* H.264 (crates/codecs/h264/src/decoder.rs, deblock.rs) */
+int oracle_h264_transform_residual_4x4(const int32_t levels[16], int qp, int dc_already_scaled,
+                                       const uint8_t scaling[16], int32_t out[16]);
+int oracle_h264_transform_residual_8x8(const int32_t levels[64], int qp,
+                                       const uint8_t scaling[64], int32_t out[64]);
+int oracle_h264_transform_chroma_dc(const int32_t levels[4], int qp, const uint8_t scaling[16],
+                                    int32_t out[4]);
+int oracle_h264_chroma_qp(int luma_qp, int offset);
+uint8_t oracle_h264_luma_qpel(const uint8_t *plane, int width, int height, int x_q4, int y_q4);
+uint8_t oracle_h264_chroma_epel(const uint8_t *plane, int width, int height, int x_q8, int y_q8);
+uint8_t oracle_h264_apply_weight(uint8_t sample, int log2_denom, int weight, int offset);
+uint8_t oracle_h264_apply_bi_weight(uint8_t s0, uint8_t s1, int log2_denom, int w0, int w1,
+                                    int o0, int o1, int explicit_weights);
+void oracle_h264_implicit_weights(int32_t poc0, int32_t poc1, int32_t current_poc, int lt0,
+                                  int lt1, int32_t *w0, int32_t *w1);
+int oracle_h264_reconstruct(const mmr_recon_h264_picture *picture);
+int oracle_h264_deblock(const mmr_recon_h264_picture *picture);
+void oracle_h264_filter_samples(uint8_t *plane, const int p_idx[4], const int q_idx[4], int qp,
+                                int bs, int chroma, int offset_a, int offset_b);
+
+/* colour + blend (crates/cli/src/terminal_preview.rs, crates/render/src/frame_graph.rs,
+ * crates/mmfx/src/render.rs) */
+void oracle_ycbcr_to_rgb(uint8_t y, uint8_t cb, uint8_t cr, uint32_t range, uint32_t matrix,
+                         uint8_t rgb[3]);
+int oracle_yuv_to_rgb(const mmr_recon_yuv_to_rgb_job *job);
+void oracle_blend_rgba_pixel(uint8_t destination[4], const uint8_t source[4]);
+void oracle_premultiplied_over_pixel(uint8_t destination[4], const uint8_t source[4],
+                                     uint32_t opacity);
+uint8_t oracle_blend_channel(uint8_t destination, uint8_t source, uint8_t alpha);
+int oracle_composite(const mmr_recon_composite_job *job);
+int oracle_yuv_blend(const mmr_recon_yuv_blend_job *job);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
