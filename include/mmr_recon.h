@@ -422,6 +422,91 @@ MMR_RECON_API mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context *context,
                                                     uint32_t job_count, uint32_t flags,
                                                     uint64_t *out_ticket);
 
+/* --------------------------------------------------------------- serving */
+
+/*
+ * Continuous-batching request loop. Replaces the bounded worker pool + request / event pattern
+ * of the playback sources (DecodeExecutor, crates/playback/src/executor.rs:40-293;
+ * Mpeg2PlaybackSource / H264PlaybackSource request handling, crates/playback/src/mpeg2.rs:151-184,
+ * crates/playback/src/h264.rs:273-317) for device reconstruction.
+ *
+ * A request is a window of frames of one codec on one stream, optionally followed per frame by
+ * a YUV->RGB conversion and an RGBA composite. Requests are admitted while the sum of admitted
+ * requests' hbm_bytes stays within the budget; a scheduler thread keeps up to `slots` batches in
+ * flight on separate HIP streams and fills each new batch with the next frames of every admitted
+ * request (continuous batching), so requests that arrive while others run join the next batch.
+ * Frames of one request complete in order; batches that carry frames of the same request are
+ * ordered on the device by events.
+ */
+typedef struct mmr_recon_server mmr_recon_server;
+
+#define MMR_CODEC_MJPEG 1u
+#define MMR_CODEC_MPEG2 2u
+#define MMR_CODEC_DV 3u
+#define MMR_CODEC_H264 4u
+
+typedef struct mmr_recon_server_config {
+    size_t struct_size;
+    uint64_t hbm_budget_bytes;     /* admission budget (0: context's budget or unlimited) */
+    uint32_t slots;                /* batches in flight (HIP streams), 0: 4 */
+    uint32_t max_batch_frames;     /* frames per batch and codec, 0: 64 */
+    uint32_t max_frames_per_request_per_batch; /* 0: 8 */
+    uint32_t max_pending;          /* queued (not yet admitted) requests before BUSY, 0: 4096 */
+} mmr_recon_server_config;
+
+typedef struct mmr_recon_request {
+    size_t struct_size;
+    uint32_t codec;                 /* MMR_CODEC_* */
+    uint32_t stream_id;             /* requests of one stream (nonzero id) run in enqueue order:
+                                       their frames may use references left resident by earlier
+                                       requests of the stream; 0 = independent request */
+    uint32_t frame_count;
+    const void *frames;             /* frame_count codec descriptors (mmr_recon_*_frame / _picture);
+                                       MPEG-2 forward_source / backward_source index this array */
+    const mmr_recon_yuv_to_rgb_job *convert;     /* NULL or frame_count jobs */
+    const mmr_recon_composite_job *composite;    /* NULL or frame_count jobs (after convert) */
+    uint64_t hbm_bytes;             /* device memory the request holds while admitted */
+    uint64_t user_tag;
+} mmr_recon_request;
+
+typedef struct mmr_recon_frame_completion {
+    uint64_t request_id;
+    uint64_t user_tag;
+    uint32_t frame_index;
+    mmr_status status;
+    double latency_ms;              /* enqueue -> observed device completion of this frame */
+} mmr_recon_frame_completion;
+
+typedef struct mmr_recon_server_stats {
+    size_t struct_size;
+    uint64_t requests_enqueued;
+    uint64_t requests_completed;
+    uint64_t frames_completed;
+    uint64_t batches_launched;
+    uint64_t admitted_bytes;        /* currently admitted */
+    uint64_t peak_admitted_bytes;
+    uint32_t pending;               /* waiting for admission */
+    uint32_t active;                /* admitted, not complete */
+} mmr_recon_server_stats;
+
+/* New: no counterpart in the reference (its executor runs decode closures on CPU threads). */
+MMR_RECON_API mmr_status mmr_recon_server_create(mmr_recon_context *context,
+                                                 const mmr_recon_server_config *config,
+                                                 mmr_recon_server **out_server);
+/* Non-blocking; copies the descriptors. MMR_STATUS_BUSY when max_pending requests wait. */
+MMR_RECON_API mmr_status mmr_recon_server_enqueue(mmr_recon_server *server,
+                                                  const mmr_recon_request *request,
+                                                  uint64_t *out_request_id);
+/* Moves up to capacity frame completions (in completion order) into out. */
+MMR_RECON_API mmr_status mmr_recon_server_poll(mmr_recon_server *server,
+                                               mmr_recon_frame_completion *out, uint32_t capacity,
+                                               uint32_t *out_count);
+/* Blocks until every enqueued request has completed. */
+MMR_RECON_API mmr_status mmr_recon_server_drain(mmr_recon_server *server);
+MMR_RECON_API mmr_status mmr_recon_server_stats_get(mmr_recon_server *server,
+                                                    mmr_recon_server_stats *out);
+MMR_RECON_API void mmr_recon_server_destroy(mmr_recon_server *server);
+
 #ifdef __cplusplus
 }
 #endif

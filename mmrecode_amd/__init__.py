@@ -171,6 +171,56 @@ class Context:
         return ms.value
 
 
+class Server:
+    """Continuous-batching request loop of a Context (mmr_recon_server_*)."""
+
+    STRUCTS = {abi.CODEC_MJPEG: abi.MjpegFrame, abi.CODEC_MPEG2: abi.Mpeg2Picture, abi.CODEC_DV: abi.DvFrame,
+               abi.CODEC_H264: abi.H264Picture}
+
+    def __init__(self, ctx, hbm_budget_bytes=0, slots=0, max_batch_frames=0, max_frames_per_request_per_batch=0,
+                 max_pending=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = abi.new(abi.ServerConfig, hbm_budget_bytes=hbm_budget_bytes, slots=slots,
+                      max_batch_frames=max_batch_frames,
+                      max_frames_per_request_per_batch=max_frames_per_request_per_batch, max_pending=max_pending)
+        h = C.c_void_p()
+        check(self.lib.mmr_recon_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+
+    def enqueue(self, codec, frames, convert=None, composite=None, hbm_bytes=0, tag=0, stream_id=0):
+        """Enqueue one request; returns its id (raises ReconError, status BUSY when the queue is full)."""
+        n = len(frames)
+        farr = (self.STRUCTS[codec] * n)(*frames)
+        carr = (abi.YuvToRgbJob * n)(*convert) if convert else None
+        parr = (abi.CompositeJob * n)(*composite) if composite else None
+        req = abi.new(abi.Request, codec=codec, stream_id=stream_id, frame_count=n, frames=C.addressof(farr),
+                      convert=C.addressof(carr) if carr is not None else None,
+                      composite=C.addressof(parr) if parr is not None else None, hbm_bytes=hbm_bytes, user_tag=tag)
+        rid = C.c_uint64()
+        check(self.lib.mmr_recon_server_enqueue(self.handle, C.byref(req), C.byref(rid)))
+        return rid.value
+
+    def poll(self, capacity=4096):
+        out = (abi.FrameCompletion * capacity)()
+        n = C.c_uint32()
+        check(self.lib.mmr_recon_server_poll(self.handle, out, capacity, C.byref(n)))
+        return [(c.request_id, c.user_tag, c.frame_index, c.status, c.latency_ms) for c in out[: n.value]]
+
+    def drain(self):
+        check(self.lib.mmr_recon_server_drain(self.handle))
+
+    def stats(self):
+        st = abi.new(abi.ServerStats)
+        check(self.lib.mmr_recon_server_stats_get(self.handle, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in abi.ServerStats._fields_ if f != "struct_size"}
+
+    def close(self):
+        if self.handle:
+            self.lib.mmr_recon_server_destroy(self.handle)
+            self.handle = None
+
+
 def implicit_weights(poc0, poc1, current_poc, long_term0=False, long_term1=False):
     """(w0, w1) of implicit bi-prediction, computed by the library's host code."""
     lib = load_library()

@@ -250,6 +250,61 @@ class YuvBlendJob(C.Structure):
     ]
 
 
+CODEC_MJPEG = 1
+CODEC_MPEG2 = 2
+CODEC_DV = 3
+CODEC_H264 = 4
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("hbm_budget_bytes", C.c_uint64),
+        ("slots", C.c_uint32),
+        ("max_batch_frames", C.c_uint32),
+        ("max_frames_per_request_per_batch", C.c_uint32),
+        ("max_pending", C.c_uint32),
+    ]
+
+
+class Request(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("codec", C.c_uint32),
+        ("stream_id", C.c_uint32),
+        ("frame_count", C.c_uint32),
+        ("frames", vp),
+        ("convert", vp),
+        ("composite", vp),
+        ("hbm_bytes", C.c_uint64),
+        ("user_tag", C.c_uint64),
+    ]
+
+
+class FrameCompletion(C.Structure):
+    _fields_ = [
+        ("request_id", C.c_uint64),
+        ("user_tag", C.c_uint64),
+        ("frame_index", C.c_uint32),
+        ("status", C.c_int32),
+        ("latency_ms", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("requests_enqueued", C.c_uint64),
+        ("requests_completed", C.c_uint64),
+        ("frames_completed", C.c_uint64),
+        ("batches_launched", C.c_uint64),
+        ("admitted_bytes", C.c_uint64),
+        ("peak_admitted_bytes", C.c_uint64),
+        ("pending", C.c_uint32),
+        ("active", C.c_uint32),
+    ]
+
+
 def new(struct_type, **fields):
     """Zero-initialised struct with struct_size set (if it has one) and fields assigned."""
     s = struct_type()
@@ -290,4 +345,11 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(CompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_yuv_blend_submit", C.c_int32,
      [C.c_void_p, C.POINTER(YuvBlendJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_server_create", C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    ("mmr_recon_server_enqueue", C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    ("mmr_recon_server_poll", C.c_int32,
+     [C.c_void_p, C.POINTER(FrameCompletion), C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("mmr_recon_server_drain", C.c_int32, [C.c_void_p]),
+    ("mmr_recon_server_stats_get", C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    ("mmr_recon_server_destroy", None, [C.c_void_p]),
 ]

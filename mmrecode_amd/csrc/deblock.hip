@@ -1,21 +1,30 @@
-// H.264 loop filter over frame pictures as an edge-order wavefront.
+// H.264 loop filter over frame pictures as an edge-order wavefront inside one workgroup.
 //
 // Reference: filter_picture / derive_boundary_strengths / boundary_strength / motion_pairs_match /
 // filter_luma_macroblock / filter_chroma_macroblock / filter_vertical/horizontal_edge /
 // filter_samples / strong_filter / normal_filter / ALPHA / BETA / TC0
 // (crates/codecs/h264/src/deblock.rs:3-587).
 //
-// The reference filters macroblocks in raster order; macroblock (x, y) reads and writes samples
-// of (x-1, y) and (x, y-1), and (x+1, y-1) writes samples that (x, y) reads, so (x, y) may run once
-// (x-1, y) and (x+1, y-1) are done: a wavefront over t = x + 2y. Each wave owns one macroblock row
-// and walks it left to right; before macroblock x it waits until the row above has published
-// progress >= min(x + 2, width). Rows are handed out by a device ticket counter in order, so the
-// row a wave waits on was taken earlier by a running wave (no deadlock for any residency).
+// Ordering. The reference filters macroblocks in raster order. Macroblock (x, y) reads and writes
+// samples of (x-1, y) and (x, y-1), and (x+1, y-1) writes samples that (x, y) reads, so (x, y) may
+// run once (x-1, y) and (x+1, y-1) are done: a wavefront over t = x + 2y.
 //
-// Per macroblock the wave stages a 20x20 luma and two 12x12 chroma windows (4 samples of left /
-// top context) in LDS, computes the 32 boundary strengths from metadata, filters vertical edges
-// (one lane per row: 16 luma + 8 Cb + 8 Cr), then horizontal edges (one lane per column), and
-// writes the window back. Sample order inside a lane follows the reference exactly.
+// Mapping. One 1024-thread workgroup filters one picture at a time (pictures are taken from a
+// device counter). Wave w owns macroblock rows w, w+16, w+32, ... and walks each row left to right
+// with a private LDS window (20x20 luma, 2 x 12x12 chroma: 4 samples of left / top context). The
+// left context stays in the window (shifted by one macroblock per step); the top context comes
+// from the wave of the row above through an LDS ring: after its step x that wave stores the bottom
+// four sample rows of macroblock x (and completes those of x-1, whose right columns step x
+// modified) and publishes progress x+1; the row below reads slot x once progress >= x+2. All
+// hand-over is LDS + workgroup-scope atomics, so no cross-workgroup synchronisation or cache
+// maintenance is needed. A writer waits before reusing a ring slot until the reader has consumed
+// it (progress of the row below >= x - ring + 2). Waits between adjacent rows cannot form a cycle
+// (ring >= 3); the longer cycle through a wave's two rows is bounded by the ring size (kDbRing).
+//
+// Write-back. Each row writes its final samples once: luma rows y*16-3 .. y*16+12 (chroma
+// y*8-3 .. y*8+4) of columns x*16-4 .. x*16+11 after step x (the rows below are handed to and
+// written by the next row, which modifies them; the last row / column write the rest). The
+// samples a step writes are not touched by any other row afterwards.
 #include "device_common.h"
 #include "recon_internal.h"
 
@@ -31,11 +40,27 @@ __constant__ static const int kTc0[3][52] = {
     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 23, 25}};
 __constant__ static const int kDbChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
 
+constexpr int kDbWaves = 16;
+// Ring slots per wave. A row can finish at most kDbRing-1 macroblocks ahead of the row below, so
+// while wave w+1 still finishes row y-15 (it owns rows y-15 and y+1), row y-15 can advance at most
+// 16 * (kDbRing - 1) macroblocks; that must cover the picture width for the chain of waits to end:
+// width_mbs <= 16 * (kDbRing - 1) = 496 (checked by the host).
+constexpr int kDbRing = 32;
+constexpr int kDbMaxRows = 256;
+
 MMR_DEV int db_chroma_qp(int luma_qp, int offset) {
     const int index = clampi(luma_qp + offset, 0, 51);
     return index < 30 ? index : kDbChromaQp[index - 30];
 }
 MMR_DEV int iabs(int v) { return v < 0 ? -v : v; }
+
+// LDS ordering between the lanes of one wave (the wave issues LDS operations in order; this keeps
+// the compiler from moving accesses across the point and waits for the outstanding ones).
+MMR_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on LDS samples:
 // p_i at s[-(i+1)*step], q_i at s[i*step].
@@ -114,10 +139,10 @@ MMR_DEV bool ref_match(const RefMotion& a, const RefMotion& b) { return a.has_re
 MMR_DEV bool motion_match(const RefMotion& a, const RefMotion& b) { return ref_match(a, b) && vec_match(a, b); }
 MMR_DEV bool indexed_match(const RefMotion& a, const RefMotion& b) { return a.index == b.index && vec_match(a, b); }
 
-// boundary_strength + motion_pairs_match (deblock.rs:391-439).
-MMR_DEV int boundary_strength(const H264Job& job, uint32_t pm, int pb, uint32_t qm, int qb, bool mb_edge) {
-    if ((job.mbs[pm].flags | job.mbs[qm].flags) & MMR_H264_MB_INTRA) return mb_edge ? 4 : 3;
-    const mmr_recon_h264_block P = job.blocks[(size_t)pm * 16 + pb], Q = job.blocks[(size_t)qm * 16 + qb];
+// boundary_strength + motion_pairs_match (deblock.rs:391-439) from prefetched records.
+MMR_DEV int boundary_strength(const H264Job& job, uint8_t p_flags, uint8_t q_flags, const mmr_recon_h264_block& P,
+                              const mmr_recon_h264_block& Q, bool mb_edge) {
+    if ((p_flags | q_flags) & MMR_H264_MB_INTRA) return mb_edge ? 4 : 3;
     if (P.total_coeff != 0 || Q.total_coeff != 0) return 2;
     const RefMotion l0 = motion_of(job, P, 0), l1 = motion_of(job, P, 1), r0 = motion_of(job, Q, 0), r1 = motion_of(job, Q, 1);
     bool match;
@@ -129,151 +154,271 @@ MMR_DEV int boundary_strength(const H264Job& job, uint32_t pm, int pb, uint32_t 
 
 MMR_DEV int luma_block_index(int x, int y) { return (y / 2) * 8 + (x / 2) * 4 + (y % 2) * 2 + x % 2; }
 
-MMR_DEV bool edge_enabled(const H264Job& job, uint32_t pm, uint32_t qm) {
-    const mmr_recon_h264_macroblock q = job.mbs[qm];
-    return !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || job.mbs[pm].slice_id == q.slice_id;
+struct DbWave {
+    uint8_t ly[20 * 20];               // rows y*16-4 .., cols x*16-4 ..
+    uint8_t lc[2][12 * 12];            // rows y*8-4 .., cols x*8-4 ..
+    uint8_t ring_y[kDbRing][4 * 16];   // bottom 4 luma rows of a macroblock, for the row below
+    uint8_t ring_c[kDbRing][2][4 * 8];
+    int bs_v[4][4];
+    int bs_h[4][4];
+};
+
+// Per-lane prefetch of one macroblock step: own samples and the records its strengths need.
+struct DbPrefetch {
+    uint32_t own_y;        // luma dword (row l>>2, cols 4*(l&3)) of the macroblock
+    uint32_t own_c;        // chroma dword, lanes < 32 (comp l>>4, row (l>>1)&7, cols 4*(l&1))
+    mmr_recon_h264_block P, Q;
+    uint8_t p_flags, q_flags;
+    bool edge_on;
+};
+
+MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPrefetch& f) {
+    const int W = MW * 16, CW = MW * 8;
+    f.own_y = *reinterpret_cast<const uint32_t*>(job.planes[0] + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+    if (l < 32) {
+        const int comp = l >> 4, row = (l >> 1) & 7;
+        f.own_c = *reinterpret_cast<const uint32_t*>(job.planes[1 + comp] + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
+    }
+    const uint32_t a = (uint32_t)(y * MW + x);
+    f.edge_on = false;
+    if (l < 32) {
+        // lanes 0-15: vertical edge e = (l >> 2), segment l & 3; lanes 16-31: horizontal
+        const bool vertical = l < 16;
+        const int e = (l & 15) >> 2, sgm = l & 3;
+        const bool external = e == 0;
+        bool on = true;
+        uint32_t pm = a;
+        if (external) {
+            if (vertical) {
+                on = x > 0;
+                pm = on ? a - 1 : a;
+            } else {
+                on = y > 0;
+                pm = on ? a - MW : a;
+            }
+            if (on) {
+                const mmr_recon_h264_macroblock q = job.mbs[a];
+                on = !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || job.mbs[pm].slice_id == q.slice_id;   // external_edge_enabled
+            }
+        }
+        f.edge_on = on;
+        const int pb = vertical ? luma_block_index(external ? 3 : e - 1, sgm) : luma_block_index(sgm, external ? 3 : e - 1);
+        const int qb = vertical ? luma_block_index(e, sgm) : luma_block_index(sgm, e);
+        f.P = job.blocks[(size_t)pm * 16 + pb];
+        f.Q = job.blocks[(size_t)a * 16 + qb];
+        f.p_flags = job.mbs[pm].flags;
+        f.q_flags = job.mbs[a].flags;
+    }
 }
 
-__global__ __launch_bounds__(64) void h264_deblock_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
-                                                          uint32_t total_rows, uint32_t* __restrict__ counters) {
-    __shared__ __align__(16) uint8_t ly[20 * 20];
-    __shared__ __align__(16) uint8_t lc[2][12 * 12];
-    __shared__ int bs_v[4][4], bs_h[4][4];
-    __shared__ uint32_t s_ticket;
-    const int l = threadIdx.x;
-    uint32_t* ticket_counter = counters;
-    uint32_t* progress = counters + 1;
-    for (;;) {
-        if (l == 0) s_ticket = fetch_add_relaxed(ticket_counter, 1);
-        __syncthreads();
-        const uint32_t ticket = s_ticket;
-        __syncthreads();
-        if (ticket >= total_rows) break;
-        const uint32_t ji = find_job(jobs, job_count, ticket, [](const H264Job& j) { return j.row_begin; });
-        const H264Job& job = jobs[ji];
-        const int y = (int)(ticket - job.row_begin);
-        const int MW = (int)job.width_mbs;
-        const int W = MW * 16, CW = MW * 8;
-        for (int x = 0; x < MW; ++x) {
-            if (y > 0) {
-                const uint32_t need = (uint32_t)min(x + 2, MW);
-                while (load_acquire(&progress[ticket - 1]) < need) __builtin_amdgcn_s_sleep(1);
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            }
-            const uint32_t a = (uint32_t)(y * MW + x);
-            const mmr_recon_h264_macroblock m = job.mbs[a];
-            if (!(m.flags & MMR_H264_MB_NO_DEBLOCK)) {
-                const bool left_on = x > 0 && edge_enabled(job, a - 1, a);
-                const bool top_on = y > 0 && edge_enabled(job, a - MW, a);
-                // derive_boundary_strengths (deblock.rs:144-200)
-                if (l < 32) {
-                    const int e = (l & 15) >> 2, sgm = l & 3;
-                    int bs = 0;
-                    if (l < 16) {
-                        if (e != 0 || left_on)
-                            bs = boundary_strength(job, e == 0 ? a - 1 : a, luma_block_index(e == 0 ? 3 : e - 1, sgm), a,
-                                                   luma_block_index(e == 0 ? 0 : e, sgm), e == 0);
-                        bs_v[e][sgm] = bs;
-                    } else {
-                        if (e != 0 || top_on)
-                            bs = boundary_strength(job, e == 0 ? a - MW : a, luma_block_index(sgm, e == 0 ? 3 : e - 1), a,
-                                                   luma_block_index(sgm, e == 0 ? 0 : e), e == 0);
-                        bs_h[e][sgm] = bs;
-                    }
-                }
-                // Stage windows: luma rows y*16-4.., cols x*16-4..; chroma rows y*8-4.., cols x*8-4..
-                const int lx0 = x * 16 - 4, ly0 = y * 16 - 4, cx0 = x * 8 - 4, cy0 = y * 8 - 4;
-                for (int i = l; i < 20 * 5; i += 64) {
-                    const int r = i / 5, c4 = (i % 5) * 4;
-                    const int gy = ly0 + r, gx = lx0 + c4;
-                    if (gy >= 0 && gx >= 0)
-                        *reinterpret_cast<uint32_t*>(&ly[r * 20 + c4]) = *reinterpret_cast<const uint32_t*>(job.planes[0] + (size_t)gy * W + gx);
-                }
-                for (int i = l; i < 2 * 12 * 3; i += 64) {
-                    const int comp = i / 36, r = (i % 36) / 3, c4 = (i % 3) * 4;
-                    const int gy = cy0 + r, gx = cx0 + c4;
-                    if (gy >= 0 && gx >= 0)
-                        *reinterpret_cast<uint32_t*>(&lc[comp][r * 12 + c4]) =
-                            *reinterpret_cast<const uint32_t*>(job.planes[1 + comp] + (size_t)gy * CW + gx);
-                }
-                __syncthreads();
-                const int oa = m.offset_a, ob = m.offset_b;
-                const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
-                const int qp_left = x > 0 ? job.mbs[a - 1].qp : 0, qp_top = y > 0 ? job.mbs[a - MW].qp : 0;
-                // Vertical edges: lanes 0-15 luma rows, 16-23 Cb rows, 24-31 Cr rows.
-                if (l < 16) {
-                    for (int e = 0; e < 4; ++e) {
-                        if (e == 0 && !left_on) continue;
-                        if (t8 && (e == 1 || e == 3)) continue;
-                        const int bs = bs_v[e][l >> 2];
-                        if (bs == 0) continue;
-                        const int qp = ((e == 0 ? qp_left : m.qp) + m.qp + 1) >> 1;
-                        filter_samples(&ly[(4 + l) * 20 + 4 + e * 4], 1, qp, bs, false, oa, ob);
-                    }
-                } else if (l < 32) {
-                    const int comp = (l - 16) >> 3, row = (l - 16) & 7;
-                    const int off = job.chroma_qp_offset[comp];
-                    for (int e = 0; e < 2; ++e) {
-                        if (e == 0 && !left_on) continue;
-                        const int bs = bs_v[e * 2][row >> 1];
-                        if (bs == 0) continue;
-                        const int qp = (db_chroma_qp(e == 0 ? qp_left : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                        filter_samples(&lc[comp][(4 + row) * 12 + 4 + e * 4], 1, qp, bs, true, oa, ob);
-                    }
-                }
-                __syncthreads();
-                // Horizontal edges: lanes 0-15 luma columns, 16-23 Cb columns, 24-31 Cr columns.
-                if (l < 16) {
-                    for (int e = 0; e < 4; ++e) {
-                        if (e == 0 && !top_on) continue;
-                        if (t8 && (e == 1 || e == 3)) continue;
-                        const int bs = bs_h[e][l >> 2];
-                        if (bs == 0) continue;
-                        const int qp = ((e == 0 ? qp_top : m.qp) + m.qp + 1) >> 1;
-                        filter_samples(&ly[(4 + e * 4) * 20 + 4 + l], 20, qp, bs, false, oa, ob);
-                    }
-                } else if (l < 32) {
-                    const int comp = (l - 16) >> 3, col = (l - 16) & 7;
-                    const int off = job.chroma_qp_offset[comp];
-                    for (int e = 0; e < 2; ++e) {
-                        if (e == 0 && !top_on) continue;
-                        const int bs = bs_h[e * 2][col >> 1];
-                        if (bs == 0) continue;
-                        const int qp = (db_chroma_qp(e == 0 ? qp_top : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                        filter_samples(&lc[comp][(4 + e * 4) * 12 + 4 + col], 12, qp, bs, true, oa, ob);
-                    }
-                }
-                __syncthreads();
-                // Write back rows y*16-3.. (dword-aligned from column x*16-4; no other wave writes
-                // those samples while this macroblock is in flight).
-                for (int i = l; i < 20 * 5; i += 64) {
-                    const int r = i / 5, c4 = (i % 5) * 4;
-                    const int gy = ly0 + r, gx = lx0 + c4;
-                    if (r >= 1 && gy >= 0 && gx >= 0)
-                        *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&ly[r * 20 + c4]);
-                }
-                for (int i = l; i < 2 * 12 * 3; i += 64) {
-                    const int comp = i / 36, r = (i % 36) / 3, c4 = (i % 3) * 4;
-                    const int gy = cy0 + r, gx = cx0 + c4;
-                    if (r >= 1 && gy >= 0 && gx >= 0)
-                        *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)gy * CW + gx) =
-                            *reinterpret_cast<const uint32_t*>(&lc[comp][r * 12 + c4]);
-                }
-            }
-            // Publish macroblock x of this row.
-            __threadfence();
-            __syncthreads();
-            if (l == 0) add_release(&progress[ticket], 1);
+// Bounded wait on a workgroup progress counter: gives up (and reports) after about a second, so a
+// protocol error surfaces as a status instead of a hung kernel.
+constexpr uint32_t kDbSpinLimit = 1u << 25;
+MMR_DEV bool db_wait(uint32_t* counter, uint32_t need, uint32_t* status) {
+    uint32_t spins = 0;
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+        if (++spins > kDbSpinLimit) {
+            atomicOr(status, kDevStatusTimeout);
+            return false;
         }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status) {
+    DbWave& S = waves[w];
+    const DbWave& Above = waves[(w + kDbWaves - 1) % kDbWaves];
+    const int MW = (int)job.width_mbs, MH = (int)job.height_mbs;
+    const int W = MW * 16, CW = MW * 8;
+    const bool last_row = y == MH - 1;
+    DbPrefetch cur, nxt;
+    db_prefetch(job, MW, 0, y, l, cur);
+    for (int x = 0; x < MW; ++x) {
+        const uint32_t a = (uint32_t)(y * MW + x);
+        const mmr_recon_h264_macroblock m = job.mbs[a];
+        const bool filter = !(m.flags & MMR_H264_MB_NO_DEBLOCK);
+        // own samples into the window (rows 4.., cols 4..)
+        *reinterpret_cast<uint32_t*>(&S.ly[(4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]) = cur.own_y;
+        if (l < 32) {
+            const int comp = l >> 4, row = (l >> 1) & 7;
+            *reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + row) * 12 + 4 + 4 * (l & 1)]) = cur.own_c;
+        }
+        // derive_boundary_strengths (deblock.rs:144-200)
+        if (l < 32) {
+            const int e = (l & 15) >> 2, sgm = l & 3;
+            const int bs = cur.edge_on ? boundary_strength(job, cur.p_flags, cur.q_flags, cur.P, cur.Q, e == 0) : 0;
+            if (l < 16) S.bs_v[e][sgm] = bs;
+            else S.bs_h[e][sgm] = bs;
+        }
+        if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
+        const int qp_left = x > 0 ? job.mbs[a - 1].qp : 0, qp_top = y > 0 ? job.mbs[a - MW].qp : 0;
+        // top context from the row above (slot x), after it finished macroblock x + 1
+        if (y > 0) {
+            if (!db_wait(&prog[y - 1], (uint32_t)min(x + 2, MW), status)) return;
+            const int slot = x % kDbRing;
+            if (l < 16) {
+                *reinterpret_cast<uint32_t*>(&S.ly[(l >> 2) * 20 + 4 + 4 * (l & 3)]) =
+                    *reinterpret_cast<const uint32_t*>(&Above.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]);
+            } else if (l < 32) {
+                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
+                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12 + 4 + 4 * (k & 1)]) =
+                    *reinterpret_cast<const uint32_t*>(&Above.ring_c[slot][comp][row * 8 + 4 * (k & 1)]);
+            }
+        }
+        wave_sync();
+        if (filter) {
+            const int oa = m.offset_a, ob = m.offset_b;
+            const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
+            // vertical edges: lanes 0-15 luma rows, 16-23 Cb rows, 24-31 Cr rows
+            if (l < 16) {
+                for (int e = 0; e < 4; ++e) {
+                    if (t8 && (e == 1 || e == 3)) continue;
+                    const int bs = S.bs_v[e][l >> 2];
+                    if (bs == 0) continue;
+                    const int qp = ((e == 0 ? qp_left : m.qp) + m.qp + 1) >> 1;
+                    filter_samples(&S.ly[(4 + l) * 20 + 4 + e * 4], 1, qp, bs, false, oa, ob);
+                }
+            } else if (l < 32) {
+                const int comp = (l - 16) >> 3, row = (l - 16) & 7;
+                const int off = job.chroma_qp_offset[comp];
+                for (int e = 0; e < 2; ++e) {
+                    const int bs = S.bs_v[e * 2][row >> 1];
+                    if (bs == 0) continue;
+                    const int qp = (db_chroma_qp(e == 0 ? qp_left : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
+                    filter_samples(&S.lc[comp][(4 + row) * 12 + 4 + e * 4], 1, qp, bs, true, oa, ob);
+                }
+            }
+            wave_sync();
+            // horizontal edges: lanes 0-15 luma columns, 16-23 Cb columns, 24-31 Cr columns
+            if (l < 16) {
+                for (int e = 0; e < 4; ++e) {
+                    if (t8 && (e == 1 || e == 3)) continue;
+                    const int bs = S.bs_h[e][l >> 2];
+                    if (bs == 0) continue;
+                    const int qp = ((e == 0 ? qp_top : m.qp) + m.qp + 1) >> 1;
+                    filter_samples(&S.ly[(4 + e * 4) * 20 + 4 + l], 20, qp, bs, false, oa, ob);
+                }
+            } else if (l < 32) {
+                const int comp = (l - 16) >> 3, col = (l - 16) & 7;
+                const int off = job.chroma_qp_offset[comp];
+                for (int e = 0; e < 2; ++e) {
+                    const int bs = S.bs_h[e * 2][col >> 1];
+                    if (bs == 0) continue;
+                    const int qp = (db_chroma_qp(e == 0 ? qp_top : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
+                    filter_samples(&S.lc[comp][(4 + e * 4) * 12 + 4 + col], 12, qp, bs, true, oa, ob);
+                }
+            }
+            wave_sync();
+        }
+        // hand the bottom rows to the row below: slot x (all columns) and the right columns of
+        // slot x - 1, which this step's left edge modified
+        if (!last_row) {
+            const int free_at = x - kDbRing + 2;
+            if (free_at > 0 && !db_wait(&prog[y + 1], (uint32_t)free_at, status)) return;
+            const int slot = x % kDbRing, prev = (x + kDbRing - 1) % kDbRing;
+            if (l < 16) {
+                *reinterpret_cast<uint32_t*>(&S.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]) =
+                    *reinterpret_cast<const uint32_t*>(&S.ly[(16 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]);
+            } else if (l < 32) {
+                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
+                *reinterpret_cast<uint32_t*>(&S.ring_c[slot][comp][row * 8 + 4 * (k & 1)]) =
+                    *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * 12 + 4 + 4 * (k & 1)]);
+            } else if (x > 0 && l < 36) {
+                const int row = l - 32;
+                *reinterpret_cast<uint32_t*>(&S.ring_y[prev][row * 16 + 12]) = *reinterpret_cast<const uint32_t*>(&S.ly[(16 + row) * 20]);
+            } else if (x > 0 && l >= 40 && l < 48) {
+                const int k = l - 40, comp = k >> 2, row = k & 3;
+                *reinterpret_cast<uint32_t*>(&S.ring_c[prev][comp][row * 8 + 4]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * 12]);
+            }
+        }
+        // write back the samples that are now final
+        {
+            // luma: rows y*16-3 .. y*16+12 (window rows 1..16), columns x*16-4 .. x*16+11 (window cols 0..15)
+            const int r = 1 + (l >> 2), c4 = 4 * (l & 3);
+            const int gy = y * 16 - 4 + r, gx = x * 16 - 4 + c4;
+            if (gy >= 0 && gx >= 0)
+                *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&S.ly[r * 20 + c4]);
+            if (last_row && l < 12) {   // rows y*16+13 .. +15 (window rows 17..19)
+                const int r2 = 17 + (l >> 2), gx2 = x * 16 - 4 + c4;
+                if (gx2 >= 0)
+                    *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)(y * 16 - 4 + r2) * W + gx2) =
+                        *reinterpret_cast<const uint32_t*>(&S.ly[r2 * 20 + c4]);
+            }
+            if (x == MW - 1 && l < 19) {   // last column: columns x*16+12 .. +15 (window col 16)
+                const int r3 = 1 + l;
+                const int gy3 = y * 16 - 4 + r3;
+                if (gy3 >= 0 && (r3 <= 16 || last_row))
+                    *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)gy3 * W + x * 16 + 12) =
+                        *reinterpret_cast<const uint32_t*>(&S.ly[r3 * 20 + 16]);
+            }
+            // chroma: rows y*8-3 .. y*8+4 (window rows 1..8), columns x*8-4 .. x*8+3 (window cols 0..7)
+            if (l < 32) {
+                const int comp = l >> 4, rr = 1 + ((l >> 1) & 7), cc = 4 * (l & 1);
+                const int cy = y * 8 - 4 + rr, cx = x * 8 - 4 + cc;
+                if (cy >= 0 && cx >= 0)
+                    *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)cy * CW + cx) =
+                        *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
+            }
+            // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
+            // columns x*8+4 .. +7 (window col 8, 2 x 11 rows)
+            const int extra_rows = last_row ? 12 : 0, extra_cols = x == MW - 1 ? 22 : 0;
+            for (int i = l; i < extra_rows + extra_cols; i += 64) {
+                if (i < extra_rows) {
+                    const int comp = i / 6, rr = 9 + (i % 6) / 2, cc = 4 * (i & 1);
+                    const int cx = x * 8 - 4 + cc;
+                    if (cx >= 0)
+                        *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)(y * 8 - 4 + rr) * CW + cx) =
+                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
+                } else {
+                    const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
+                    const int cy = y * 8 - 4 + rr;
+                    if (cy >= 0 && (rr <= 8 || last_row))
+                        *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)cy * CW + x * 8 + 4) =
+                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + 8]);
+                }
+            }
+        }
+        wave_sync();
+        if (l == 0) __hip_atomic_store(&prog[y], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // shift the window left by one macroblock (its right 4 columns become the left context)
+        if (l < 20) {
+            *reinterpret_cast<uint32_t*>(&S.ly[l * 20]) = *reinterpret_cast<const uint32_t*>(&S.ly[l * 20 + 16]);
+        } else if (l < 44) {
+            const int k = l - 20, comp = k / 12, row = k % 12;
+            *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][row * 12 + 8]);
+        }
+        wave_sync();
+        cur = nxt;
+    }
+}
+
+__global__ __launch_bounds__(1024) void h264_deblock_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
+                                                            uint32_t* __restrict__ counters, uint32_t* __restrict__ status) {
+    __shared__ __align__(16) DbWave waves[kDbWaves];
+    __shared__ uint32_t prog[kDbMaxRows];
+    __shared__ uint32_t s_pic;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    for (;;) {
+        if (threadIdx.x == 0) s_pic = fetch_add_relaxed(&counters[0], 1);
+        for (int i = threadIdx.x; i < kDbMaxRows; i += blockDim.x) prog[i] = 0;
+        __syncthreads();
+        const uint32_t pic = s_pic;
+        if (pic >= job_count) break;
+        const H264Job& job = jobs[pic];
+        if (job.deblock)
+            for (int y = w; y < (int)job.height_mbs; y += kDbWaves) db_row(job, y, w, l, waves, prog, status);
+        __syncthreads();
     }
 }
 
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
-                               uint32_t counter_words, uint32_t groups, hipStream_t s) {
+                               uint32_t counter_words, uint32_t groups, uint32_t* d_status, hipStream_t s) {
     (void)counter_words;
-    if (total_rows == 0) return hipSuccess;
-    if (groups > total_rows) groups = total_rows;
-    hipLaunchKernelGGL(h264_deblock_kernel, dim3(groups), dim3(64), 0, s, d_jobs, job_count, total_rows, d_counters);
+    (void)total_rows;
+    if (job_count == 0) return hipSuccess;
+    if (groups > job_count) groups = job_count;
+    hipLaunchKernelGGL(h264_deblock_kernel, dim3(groups), dim3(kDbWaves * 64), 0, s, d_jobs, job_count, d_counters, d_status);
     return hipGetLastError();
 }
 

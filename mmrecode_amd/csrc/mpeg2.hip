@@ -9,11 +9,13 @@
 //
 // Work decomposition: one task per (picture, macroblock, block 0..5); every task writes exactly
 // the samples of its block (prediction, residual, or both), so tasks of a picture are
-// independent. Tasks are grouped in tiles of 32 (8 lanes per block, 256 threads). Workgroups take
-// tiles from a device counter in submission (decode) order; a tile of a picture first waits until
-// the pictures that produce its forward / backward references have completed all their tiles.
-// Every waited-on tile was taken earlier by a running workgroup that only waits on even earlier
-// tiles, so the queue cannot deadlock regardless of how many workgroups are resident.
+// independent. Tasks are grouped in tiles of 32 (8 lanes per block, 256 threads) and persistent
+// workgroups take tiles from a device counter. Pictures that depend on each other (the anchor
+// chain I -> P -> P, B after both anchors) are launched in dependency levels by the host: every
+// picture of one launch only reads pictures completed by earlier launches, so the kernel needs no
+// cross-workgroup synchronisation and no cache maintenance beyond the launch boundary (on MI355X
+// the eight XCDs have separate L2 caches; agent-scope release / acquire inside a launch costs an
+// L2 write-back / invalidate per tile).
 #include "device_common.h"
 #include "recon_internal.h"
 
@@ -48,19 +50,6 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
         if (tile >= total_tiles) break;
         const uint32_t ji = find_job(jobs, job_count, tile, [](const Mpeg2Job& j) { return j.tile_begin; });
         const Mpeg2Job& job = jobs[ji];
-        if (threadIdx.x == 0) {
-            // Anchor-chain dependencies: wait for the producing pictures of this batch.
-            if (job.fwd_src >= 0) {
-                const uint32_t need = jobs[job.fwd_src].tile_count;
-                while (load_acquire(&counters[1 + job.fwd_src]) < need) __builtin_amdgcn_s_sleep(2);
-            }
-            if (job.bwd_src >= 0) {
-                const uint32_t need = jobs[job.bwd_src].tile_count;
-                while (load_acquire(&counters[1 + job.bwd_src]) < need) __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 
         const uint32_t mbs = job.mb_w * job.mb_h;
         const uint32_t task = (tile - job.tile_begin) * kBlocksPerGroup + g;
@@ -175,10 +164,7 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
                 }
             }
         }
-        // Publish this tile: every thread's stores complete at agent scope, then one counter bump.
-        __threadfence();
-        __syncthreads();
-        if (threadIdx.x == 0) add_release(&counters[1 + ji], 1);
+        __syncthreads();   // s_tile is rewritten by the next iteration
     }
 }
 

@@ -41,8 +41,9 @@ mmr_status hip_fail(hipError_t e, const char* what) {
 }
 
 struct TicketRecord {
-    hipEvent_t start = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t start = nullptr;       // before the first kernel (after the job upload)
+    hipEvent_t kernels_done = nullptr;  // after the last kernel
+    hipEvent_t done = nullptr;        // after the status word is copied back
     std::vector<char> host_jobs;      // kept alive until completion (async upload source)
     uint32_t* host_status = nullptr;  // pinned; device status word copied here
     bool finished = false;
@@ -78,6 +79,7 @@ void reap(mmr_recon_context* ctx) {
         }
         TicketRecord& r = it->second;
         if (r.start) (void)hipEventDestroy(r.start);
+        if (r.kernels_done) (void)hipEventDestroy(r.kernels_done);
         if (r.done) (void)hipEventDestroy(r.done);
         if (r.host_status) (void)hipHostFree(r.host_status);
         ctx->tickets.erase(it);
@@ -89,9 +91,11 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
         hipError_t e = hipEventSynchronize(r.done);
         if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, r.start, r.done) == hipSuccess) r.kernel_ms = ms;
+        if (hipEventElapsedTime(&ms, r.start, r.kernels_done) == hipSuccess) r.kernel_ms = ms;
         const uint32_t st = r.host_status ? *r.host_status : 0u;
-        if (st & kStatusUnsupported) {
+        if (st & mmr::kDevStatusTimeout) {
+            r.status = fail(MMR_STATUS_INTERNAL_ERROR, "a device-side wait timed out (internal error); outputs are invalid");
+        } else if (st & kStatusUnsupported) {
             r.status = fail(MMR_STATUS_UNSUPPORTED, "a record of this submission uses a feature the device path does not reconstruct "
                                                     "(MPEG-2 field prediction or H.264 intra macroblock); fall back to CPU reconstruction");
         } else if (st & kStatusInvalidData) {
@@ -107,12 +111,13 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
 // counters + status word, and finishes by recording the completion event.
 struct Submission {
     mmr_recon_context* ctx;
+    hipStream_t stream;
     TicketRecord rec;
     char* d_block = nullptr;
     size_t jobs_bytes = 0;
     size_t counter_words = 0;
 
-    explicit Submission(mmr_recon_context* c) : ctx(c) {}
+    Submission(mmr_recon_context* c, hipStream_t s) : ctx(c), stream(s) {}
 
     mmr_status begin(const void* jobs, size_t bytes, size_t counters) {
         if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
@@ -122,15 +127,16 @@ struct Submission {
         counter_words = counters + 1;  // + status word
         rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
         hipError_t e = hipEventCreate(&rec.start);
+        if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
         if (e == hipSuccess) e = hipEventCreate(&rec.done);
         if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
         if (e != hipSuccess) return hip_fail(e, "ticket setup");
         *rec.host_status = 0;
-        e = hipMallocAsync((void**)&d_block, jobs_bytes + counter_words * sizeof(uint32_t), ctx->stream);
+        e = hipMallocAsync((void**)&d_block, jobs_bytes + counter_words * sizeof(uint32_t), stream);
         if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
-        e = hipEventRecord(rec.start, ctx->stream);
-        if (e == hipSuccess && bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
-        if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, ctx->stream);
+        if (bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, stream);
+        if (e == hipSuccess) e = hipEventRecord(rec.start, stream);
         if (e != hipSuccess) return hip_fail(e, "job upload");
         return MMR_STATUS_OK;
     }
@@ -139,9 +145,10 @@ struct Submission {
     uint32_t* status_ptr() const { return counters_ptr() + (counter_words - 1); }
 
     mmr_status end(uint64_t* out_ticket) {
-        hipError_t e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
-        if (e == hipSuccess) e = hipEventRecord(rec.done, ctx->stream);
-        if (e == hipSuccess) e = hipFreeAsync(d_block, ctx->stream);
+        hipError_t e = hipEventRecord(rec.kernels_done, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipEventRecord(rec.done, stream);
+        if (e == hipSuccess) e = hipFreeAsync(d_block, stream);
         d_block = nullptr;
         if (e != hipSuccess) return hip_fail(e, "submission end");
         const uint64_t id = ctx->next_ticket++;
@@ -151,8 +158,9 @@ struct Submission {
         return MMR_STATUS_OK;
     }
     ~Submission() {
-        if (d_block) (void)hipFreeAsync(d_block, ctx->stream);
+        if (d_block) (void)hipFreeAsync(d_block, stream);
         if (rec.start) (void)hipEventDestroy(rec.start);
+        if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
         if (rec.done) (void)hipEventDestroy(rec.done);
         if (rec.host_status) (void)hipHostFree(rec.host_status);
     }
@@ -268,6 +276,393 @@ mmr_status check_rgb_job(const mmr_recon_yuv_to_rgb_job& s) {
 
 }  // namespace
 
+namespace mmr_host {
+
+mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
+                                  uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
+    std::vector<mmr::MjpegJob> jobs(count);
+    std::vector<mmr::RgbJob> rgb;
+    uint32_t total = 0, total_quads = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t quads = 0;
+        mmr_status s = build_mjpeg(frames[i], jobs[i], quads);
+        if (s) return s;
+        jobs[i].block_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total;
+        total += jobs[i].block_count;
+        if (quads) {
+            mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
+            r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
+            total_quads += quads;
+            rgb.push_back(r);
+        }
+    }
+    std::vector<char> blob(jobs.size() * sizeof(mmr::MjpegJob) + rgb.size() * sizeof(mmr::RgbJob));
+    memcpy(blob.data(), jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
+    if (!rgb.empty()) memcpy(blob.data() + jobs.size() * sizeof(mmr::MjpegJob), rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::MjpegJob*>(sub.jobs_ptr());
+    auto* dr = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + jobs.size() * sizeof(mmr::MjpegJob));
+    hipError_t e = hipSuccess;
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        size_t r = 0;
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            e = mmr::launch_mjpeg(dj + i, 1, jobs[i].block_count, stream);
+            if (e == hipSuccess && frames[i].rgb) {
+                e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, stream);
+                ++r;
+            }
+        }
+    } else {
+        e = mmr::launch_mjpeg(dj, count, total, stream);
+        if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
+                                  uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::Mpeg2Job> jobs(count);
+    uint32_t tiles = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_mpeg2_picture& p = pics[i];
+        if (p.struct_size < sizeof(mmr_recon_mpeg2_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
+        if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u size %ux%u is invalid", i, p.width, p.height);
+        if (p.picture_type < MMR_MPEG2_PICTURE_I || p.picture_type > MMR_MPEG2_PICTURE_B)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u type %u is invalid", i, p.picture_type);
+        if (p.alternate_scan > 1 || p.intra_dc_precision > 3)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u scan / dc precision invalid", i);
+        if (!p.matrices || !p.macroblocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u has a NULL buffer", i);
+        if (!aligned(p.levels, 16) || !aligned(p.macroblocks, 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u: levels must be 16-byte aligned", i);
+        const bool needs_f = p.picture_type != MMR_MPEG2_PICTURE_I, needs_b = p.picture_type == MMR_MPEG2_PICTURE_B;
+        if ((needs_f && (!p.forward[0] || !p.forward[1] || !p.forward[2])) || (needs_b && (!p.backward[0] || !p.backward[1] || !p.backward[2])))
+            return fail(MMR_STATUS_INVALID_DATA, "missing %s MPEG-2 reference for picture %u", needs_b ? "forward/backward" : "forward", i);
+        if (!per_frame && (p.forward_source >= (int32_t)i || p.backward_source >= (int32_t)i || p.forward_source < -1 || p.backward_source < -1))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u depends on a picture that is not earlier in decode order", i);
+        mmr::Mpeg2Job& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.matrices = p.matrices;
+        j.mbs = p.macroblocks;
+        j.levels = p.levels;
+        for (int k = 0; k < 3; ++k) {
+            j.planes[k] = p.planes[k];
+            j.fwd[k] = p.forward[k] ? p.forward[k] : p.planes[k];
+            j.bwd[k] = p.backward[k] ? p.backward[k] : (p.forward[k] ? p.forward[k] : p.planes[k]);
+        }
+        j.fwd_src = per_frame ? -1 : p.forward_source;
+        j.bwd_src = per_frame ? -1 : p.backward_source;
+        j.mb_w = (p.width + 15) / 16;
+        j.mb_h = (p.height + 15) / 16;
+        j.alt_scan = p.alternate_scan;
+        j.dc_shift = 3 - p.intra_dc_precision;
+        j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
+        j.tile_begin = per_frame ? 0 : tiles;
+        tiles += j.tile_count;
+    }
+    // Dependency levels: a picture runs one launch after the latest picture it references.
+    std::vector<uint32_t> level(count, 0);
+    uint32_t levels = 1;
+    if (!per_frame) {
+        for (uint32_t i = 0; i < count; ++i) {
+            if (pics[i].forward_source >= 0) level[i] = std::max(level[i], level[pics[i].forward_source] + 1);
+            if (pics[i].backward_source >= 0) level[i] = std::max(level[i], level[pics[i].backward_source] + 1);
+            levels = std::max(levels, level[i] + 1);
+        }
+    }
+    // Jobs grouped by level (stable), tile offsets restart at 0 in every level.
+    std::vector<mmr::Mpeg2Job> ordered;
+    std::vector<uint32_t> level_first(levels + 1, 0), level_tiles(levels, 0);
+    if (per_frame) {
+        ordered = jobs;
+    } else {
+        for (uint32_t lv = 0; lv < levels; ++lv) {
+            level_first[lv] = (uint32_t)ordered.size();
+            for (uint32_t i = 0; i < count; ++i)
+                if (level[i] == lv) {
+                    mmr::Mpeg2Job j = jobs[i];
+                    j.tile_begin = level_tiles[lv];
+                    level_tiles[lv] += j.tile_count;
+                    ordered.push_back(j);
+                }
+        }
+        level_first[levels] = (uint32_t)ordered.size();
+    }
+    Submission sub(ctx, stream);
+    const size_t counters = per_frame ? (size_t)count : (size_t)levels;
+    mmr_status s = sub.begin(ordered.data(), ordered.size() * sizeof(mmr::Mpeg2Job), counters);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    const uint32_t groups = groups_for(ctx, 8);
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_mpeg2(dj + i, 1, jobs[i].tile_count, sub.counters_ptr() + i, groups, sub.status_ptr(), stream);
+    } else {
+        for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
+            e = mmr::launch_mpeg2(dj + level_first[lv], level_first[lv + 1] - level_first[lv], level_tiles[lv],
+                                  sub.counters_ptr() + lv, groups, sub.status_ptr(), stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "MPEG-2 launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_dv_frame* frames, uint32_t count, uint32_t flags,
+                               uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::DvJob> jobs(count);
+    uint32_t total = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_dv_frame& f = frames[i];
+        if (f.struct_size < sizeof(mmr_recon_dv_frame)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV struct_size too small");
+        if (f.system != MMR_DV_SYSTEM_525_60 && f.system != MMR_DV_SYSTEM_625_50)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u system %u is invalid", i, f.system);
+        if (!f.levels || !f.block_info || !f.planes[0] || !f.planes[1] || !f.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u has a NULL buffer", i);
+        if (!aligned(f.levels, 16)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV levels must be 16-byte aligned");
+        mmr::DvJob& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.levels = f.levels;
+        j.info = f.block_info;
+        for (int k = 0; k < 3; ++k) j.planes[k] = f.planes[k];
+        j.system = f.system;
+        j.block_count = (f.system == MMR_DV_SYSTEM_625_50 ? 12u : 10u) * 27u * 5u * 6u;
+        j.block_begin = per_frame ? 0 : total;
+        total += j.block_count;
+    }
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::DvJob), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::DvJob*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_dv(dj + i, 1, jobs[i].block_count, stream);
+    } else {
+        e = mmr::launch_dv(dj, count, total, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "DV launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_h264_picture* pics, uint32_t count, uint32_t flags,
+                                 uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::H264Job> jobs(count);
+    uint32_t mbs = 0, rows = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_h264_picture& p = pics[i];
+        if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
+        if (p.width_mbs == 0 || p.height_mbs == 0 || p.width_mbs > 1024 || p.height_mbs > 1024)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
+        if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u has a NULL buffer", i);
+        if (p.ref_count[0] > MMR_H264_MAX_REFS || p.ref_count[1] > MMR_H264_MAX_REFS || p.ref_count[0] + p.ref_count[1] == 0)
+            return fail(MMR_STATUS_INVALID_DATA, "H.264 picture %u reference counts invalid", i);
+        if (p.weighted_mode > MMR_H264_WEIGHTED_TABLE) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 weighted_mode invalid");
+        if (p.weighted_mode == MMR_H264_WEIGHTED_TABLE && (!p.weights || !p.bi_weights))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
+        if (!aligned(p.planes[0], 4) || !aligned(p.planes[1], 4) || !aligned(p.planes[2], 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 planes must be 4-byte aligned");
+        if (p.deblock && (p.width_mbs > 496 || p.height_mbs > 256))
+            return fail(MMR_STATUS_UNSUPPORTED, "H.264 deblocking supports up to 496x256 macroblocks (picture %u)", i);
+        mmr::H264Job& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.mbs = p.macroblocks;
+        j.blocks = p.blocks;
+        j.levels = p.levels;
+        j.weights = p.weights;
+        j.bi_weights = p.bi_weights;
+        j.scaling = p.scaling_lists ? p.scaling_lists : ctx->flat_scaling;
+        for (int k = 0; k < 3; ++k) j.planes[k] = p.planes[k];
+        for (int list = 0; list < 2; ++list) {
+            j.ref_count[list] = p.ref_count[list];
+            for (uint32_t r = 0; r < MMR_H264_MAX_REFS; ++r) {
+                const uint32_t src = r < p.ref_count[list] ? r : 0;
+                const mmr_recon_h264_reference& ref = p.refs[list][src];
+                for (int k = 0; k < 3; ++k) {
+                    const uint8_t* plane = p.ref_count[list] ? ref.planes[k] : p.refs[1 - list][0].planes[k];
+                    if (!plane) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: reference plane is NULL", i);
+                    j.ref_planes[list][r][k] = plane;
+                }
+                j.ref_poc[list][r] = p.ref_count[list] ? ref.poc : 0;
+            }
+        }
+        j.width_mbs = p.width_mbs;
+        j.height_mbs = p.height_mbs;
+        j.weighted = p.weighted_mode;
+        j.deblock = p.deblock ? 1u : 0u;
+        j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
+        j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
+        j.mb_begin = per_frame ? 0 : mbs;
+        j.row_begin = per_frame ? 0 : rows;
+        mbs += p.width_mbs * p.height_mbs;
+        rows += j.deblock ? p.height_mbs : 0;
+    }
+    Submission sub(ctx, stream);
+    size_t counters = 0;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
+    } else {
+        counters = 1 + rows;
+    }
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::H264Job), counters);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::H264Job*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count;
+    if (per_frame) {
+        uint32_t* c = sub.counters_ptr();
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            e = mmr::launch_h264_mc(dj + i, 1, pics[i].width_mbs * pics[i].height_mbs, sub.status_ptr(), stream);
+            if (e == hipSuccess && jobs[i].deblock) {
+                e = mmr::launch_h264_deblock(dj + i, 1, pics[i].height_mbs, c, 1 + pics[i].height_mbs, groups, sub.status_ptr(), stream);
+                c += 1 + pics[i].height_mbs;
+            } else {
+                c += 1;
+            }
+        }
+    } else {
+        e = mmr::launch_h264_mc(dj, count, mbs, sub.status_ptr(), stream);
+        if (e == hipSuccess && rows) e = mmr::launch_h264_deblock(dj, count, rows, sub.counters_ptr(), 1 + rows, groups, sub.status_ptr(), stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "H.264 launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_yuv_to_rgb_job* in, uint32_t count, uint32_t flags,
+                                       uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::RgbJob> jobs(count);
+    uint32_t quads = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        mmr_status s = check_rgb_job(in[i]);
+        if (s) return s;
+        mmr::RgbJob& r = jobs[i];
+        memset(&r, 0, sizeof r);
+        for (int k = 0; k < 3; ++k) r.planes[k] = in[i].planes[k];
+        r.rgb = in[i].rgb;
+        r.rgba = in[i].rgba;
+        r.width = in[i].width;
+        r.height = in[i].height;
+        r.cw = in[i].chroma_width;
+        r.ch = in[i].chroma_height;
+        r.luma_stride = in[i].luma_stride;
+        r.chroma_stride = in[i].chroma_stride;
+        r.range = in[i].range;
+        r.matrix = in[i].matrix;
+        r.quads_per_row = (in[i].width + 3) / 4;
+        r.quad_begin = per_frame ? 0 : quads;
+        quads += r.quads_per_row * r.height;
+    }
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::RgbJob), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::RgbJob*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_yuv_to_rgb(dj + i, 1, jobs[i].quads_per_row * jobs[i].height, stream);
+    } else {
+        e = mmr::launch_yuv_to_rgb(dj, count, quads, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "YUV->RGB launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_composite_job* in, uint32_t count, uint32_t flags,
+                                      uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::CompositeJobDev> jobs(count);
+    uint32_t px = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_composite_job& c = in[i];
+        if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
+        if (!c.target || !c.width || !c.height || c.stride < c.width * 4 || c.layer_count > MMR_RECON_MAX_LAYERS)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
+        if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
+        for (uint32_t l = 0; l < c.layer_count; ++l) {
+            const mmr_recon_layer& L = c.layers[l];
+            if (!L.rgba || L.stride < L.width * 4 || L.stride % 4 || !aligned(L.rgba, 4))
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u layer %u storage is invalid", i, l);
+            if ((uint64_t)L.x + L.width > c.width || (uint64_t)L.y + L.height > c.height)
+                return fail(MMR_STATUS_INVALID_DATA, "RGBA overlay storage or placement is invalid (job %u layer %u)", i, l);
+            if (L.op != MMR_BLEND_SOURCE_OVER && L.op != MMR_BLEND_PREMULTIPLIED_OVER)
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite operator %u invalid", L.op);
+            if (L.opacity > 255) return fail(MMR_STATUS_INVALID_ARGUMENT, "opacity must be 0..255");
+        }
+        mmr::CompositeJobDev& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.target = c.target;
+        j.width = c.width;
+        j.height = c.height;
+        j.stride = c.stride;
+        j.layer_count = c.layer_count;
+        for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
+        j.px_begin = per_frame ? 0 : px;
+        px += c.width * c.height;
+    }
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::CompositeJobDev), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::CompositeJobDev*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_composite(dj + i, 1, jobs[i].width * jobs[i].height, stream);
+    } else {
+        e = mmr::launch_composite(dj, count, px, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "composite launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status yuv_blend_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
+                                      uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    uint32_t max_px = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_yuv_blend_job& j = in[i];
+        if (j.struct_size < sizeof(mmr_recon_yuv_blend_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend struct_size too small");
+        if ((j.width | j.height | j.x | j.y) & 1 || (uint64_t)j.x + j.width > j.target_width || (uint64_t)j.y + j.height > j.target_height)
+            return fail(MMR_STATUS_INVALID_DATA, "YUV overlay storage or placement is invalid (job %u)", i);
+        for (int k = 0; k < 3; ++k)
+            if (!j.target[k] || !j.overlay[k]) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has a NULL plane", i);
+        if (!j.rgba || !j.chroma_alpha) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has no alpha", i);
+        max_px = std::max(max_px, j.width * j.height);
+    }
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(in, sizeof(mmr_recon_yuv_blend_job) * count, 0);
+    if (s) return s;
+    hipError_t e = hipSuccess;
+    auto* dj = reinterpret_cast<const mmr_recon_yuv_blend_job*>(sub.jobs_ptr());
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_yuv_blend(dj + i, 1, in[i].width * in[i].height, stream);
+    } else {
+        e = mmr::launch_yuv_blend(dj, count, max_px, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "YUV blend launch");
+    return sub.end(out_ticket);
+}
+
+}  // namespace mmr_host
+
 extern "C" {
 
 uint32_t mmr_recon_abi_version(void) { return MMR_RECON_ABI_VERSION; }
@@ -329,6 +724,7 @@ void mmr_recon_context_destroy(mmr_recon_context* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& kv : ctx->tickets) {
         if (kv.second.start) (void)hipEventDestroy(kv.second.start);
+        if (kv.second.kernels_done) (void)hipEventDestroy(kv.second.kernels_done);
         if (kv.second.done) (void)hipEventDestroy(kv.second.done);
         if (kv.second.host_status) (void)hipHostFree(kv.second.host_status);
     }
@@ -431,153 +827,24 @@ mmr_status mmr_recon_last_kernel_ms(mmr_recon_context* ctx, uint64_t ticket, dou
 mmr_status mmr_recon_mjpeg_submit(mmr_recon_context* ctx, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
                                   uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
-    std::vector<mmr::MjpegJob> jobs(count);
-    std::vector<mmr::RgbJob> rgb;
-    uint32_t total = 0, total_quads = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        uint32_t quads = 0;
-        mmr_status s = build_mjpeg(frames[i], jobs[i], quads);
-        if (s) return s;
-        jobs[i].block_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total;
-        total += jobs[i].block_count;
-        if (quads) {
-            mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
-            r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
-            total_quads += quads;
-            rgb.push_back(r);
-        }
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    std::vector<char> blob(jobs.size() * sizeof(mmr::MjpegJob) + rgb.size() * sizeof(mmr::RgbJob));
-    memcpy(blob.data(), jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
-    if (!rgb.empty()) memcpy(blob.data() + jobs.size() * sizeof(mmr::MjpegJob), rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
-    Submission sub(ctx);
-    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::MjpegJob*>(sub.jobs_ptr());
-    auto* dr = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + jobs.size() * sizeof(mmr::MjpegJob));
-    hipError_t e = hipSuccess;
-    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
-        size_t r = 0;
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_mjpeg(dj + i, 1, jobs[i].block_count, ctx->stream);
-            if (e == hipSuccess && frames[i].rgb) {
-                e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, ctx->stream);
-                ++r;
-            }
-        }
-    } else {
-        e = mmr::launch_mjpeg(dj, count, total, ctx->stream);
-        if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
-    return sub.end(out_ticket);
+    return mmr_host::mjpeg_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
 }
 
 // ------------------------------------------------------------------ MPEG-2
 mmr_status mmr_recon_mpeg2_submit(mmr_recon_context* ctx, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
                                   uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
-    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
-    std::vector<mmr::Mpeg2Job> jobs(count);
-    uint32_t tiles = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_mpeg2_picture& p = pics[i];
-        if (p.struct_size < sizeof(mmr_recon_mpeg2_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
-        if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u size %ux%u is invalid", i, p.width, p.height);
-        if (p.picture_type < MMR_MPEG2_PICTURE_I || p.picture_type > MMR_MPEG2_PICTURE_B)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u type %u is invalid", i, p.picture_type);
-        if (p.alternate_scan > 1 || p.intra_dc_precision > 3)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u scan / dc precision invalid", i);
-        if (!p.matrices || !p.macroblocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u has a NULL buffer", i);
-        if (!aligned(p.levels, 16) || !aligned(p.macroblocks, 4))
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u: levels must be 16-byte aligned", i);
-        const bool needs_f = p.picture_type != MMR_MPEG2_PICTURE_I, needs_b = p.picture_type == MMR_MPEG2_PICTURE_B;
-        if ((needs_f && (!p.forward[0] || !p.forward[1] || !p.forward[2])) || (needs_b && (!p.backward[0] || !p.backward[1] || !p.backward[2])))
-            return fail(MMR_STATUS_INVALID_DATA, "missing %s MPEG-2 reference for picture %u", needs_b ? "forward/backward" : "forward", i);
-        if (!per_frame && (p.forward_source >= (int32_t)i || p.backward_source >= (int32_t)i || p.forward_source < -1 || p.backward_source < -1))
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u depends on a picture that is not earlier in decode order", i);
-        mmr::Mpeg2Job& j = jobs[i];
-        memset(&j, 0, sizeof j);
-        j.matrices = p.matrices;
-        j.mbs = p.macroblocks;
-        j.levels = p.levels;
-        for (int k = 0; k < 3; ++k) {
-            j.planes[k] = p.planes[k];
-            j.fwd[k] = p.forward[k] ? p.forward[k] : p.planes[k];
-            j.bwd[k] = p.backward[k] ? p.backward[k] : (p.forward[k] ? p.forward[k] : p.planes[k]);
-        }
-        j.fwd_src = per_frame ? -1 : p.forward_source;
-        j.bwd_src = per_frame ? -1 : p.backward_source;
-        j.mb_w = (p.width + 15) / 16;
-        j.mb_h = (p.height + 15) / 16;
-        j.alt_scan = p.alternate_scan;
-        j.dc_shift = 3 - p.intra_dc_precision;
-        j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
-        j.tile_begin = per_frame ? 0 : tiles;
-        tiles += j.tile_count;
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    const size_t counters = per_frame ? (size_t)count * 2 : (size_t)count + 1;
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::Mpeg2Job), counters);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>(sub.jobs_ptr());
-    hipError_t e = hipSuccess;
-    const uint32_t groups = groups_for(ctx, 8);
-    if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_mpeg2(dj + i, 1, jobs[i].tile_count, sub.counters_ptr() + 2 * i, groups, sub.status_ptr(), ctx->stream);
-    } else {
-        e = mmr::launch_mpeg2(dj, count, tiles, sub.counters_ptr(), groups, sub.status_ptr(), ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "MPEG-2 launch");
-    return sub.end(out_ticket);
+    return mmr_host::mpeg2_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
 }
 
 // ------------------------------------------------------------------ DV
 mmr_status mmr_recon_dv_submit(mmr_recon_context* ctx, const mmr_recon_dv_frame* frames, uint32_t count, uint32_t flags,
                                uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
-    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
-    std::vector<mmr::DvJob> jobs(count);
-    uint32_t total = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_dv_frame& f = frames[i];
-        if (f.struct_size < sizeof(mmr_recon_dv_frame)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV struct_size too small");
-        if (f.system != MMR_DV_SYSTEM_525_60 && f.system != MMR_DV_SYSTEM_625_50)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u system %u is invalid", i, f.system);
-        if (!f.levels || !f.block_info || !f.planes[0] || !f.planes[1] || !f.planes[2])
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "DV frame %u has a NULL buffer", i);
-        if (!aligned(f.levels, 16)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV levels must be 16-byte aligned");
-        mmr::DvJob& j = jobs[i];
-        memset(&j, 0, sizeof j);
-        j.levels = f.levels;
-        j.info = f.block_info;
-        for (int k = 0; k < 3; ++k) j.planes[k] = f.planes[k];
-        j.system = f.system;
-        j.block_count = (f.system == MMR_DV_SYSTEM_625_50 ? 12u : 10u) * 27u * 5u * 6u;
-        j.block_begin = per_frame ? 0 : total;
-        total += j.block_count;
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::DvJob), 0);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::DvJob*>(sub.jobs_ptr());
-    hipError_t e = hipSuccess;
-    if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_dv(dj + i, 1, jobs[i].block_count, ctx->stream);
-    } else {
-        e = mmr::launch_dv(dj, count, total, ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "DV launch");
-    return sub.end(out_ticket);
+    return mmr_host::dv_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
 }
 
 // ------------------------------------------------------------------ H.264
@@ -602,210 +869,453 @@ mmr_status mmr_recon_h264_implicit_weights(int32_t poc0, int32_t poc1, int32_t c
 mmr_status mmr_recon_h264_submit(mmr_recon_context* ctx, const mmr_recon_h264_picture* pics, uint32_t count, uint32_t flags,
                                  uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
-    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
-    std::vector<mmr::H264Job> jobs(count);
-    uint32_t mbs = 0, rows = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_h264_picture& p = pics[i];
-        if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
-        if (p.width_mbs == 0 || p.height_mbs == 0 || p.width_mbs > 1024 || p.height_mbs > 1024)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
-        if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u has a NULL buffer", i);
-        if (p.ref_count[0] > MMR_H264_MAX_REFS || p.ref_count[1] > MMR_H264_MAX_REFS || p.ref_count[0] + p.ref_count[1] == 0)
-            return fail(MMR_STATUS_INVALID_DATA, "H.264 picture %u reference counts invalid", i);
-        if (p.weighted_mode > MMR_H264_WEIGHTED_TABLE) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 weighted_mode invalid");
-        if (p.weighted_mode == MMR_H264_WEIGHTED_TABLE && (!p.weights || !p.bi_weights))
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
-        if (!aligned(p.planes[0], 4) || !aligned(p.planes[1], 4) || !aligned(p.planes[2], 4))
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 planes must be 4-byte aligned");
-        mmr::H264Job& j = jobs[i];
-        memset(&j, 0, sizeof j);
-        j.mbs = p.macroblocks;
-        j.blocks = p.blocks;
-        j.levels = p.levels;
-        j.weights = p.weights;
-        j.bi_weights = p.bi_weights;
-        j.scaling = p.scaling_lists ? p.scaling_lists : ctx->flat_scaling;
-        for (int k = 0; k < 3; ++k) j.planes[k] = p.planes[k];
-        for (int list = 0; list < 2; ++list) {
-            j.ref_count[list] = p.ref_count[list];
-            for (uint32_t r = 0; r < MMR_H264_MAX_REFS; ++r) {
-                const uint32_t src = r < p.ref_count[list] ? r : 0;
-                const mmr_recon_h264_reference& ref = p.refs[list][src];
-                for (int k = 0; k < 3; ++k) {
-                    const uint8_t* plane = p.ref_count[list] ? ref.planes[k] : p.refs[1 - list][0].planes[k];
-                    if (!plane) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: reference plane is NULL", i);
-                    j.ref_planes[list][r][k] = plane;
-                }
-                j.ref_poc[list][r] = p.ref_count[list] ? ref.poc : 0;
-            }
-        }
-        j.width_mbs = p.width_mbs;
-        j.height_mbs = p.height_mbs;
-        j.weighted = p.weighted_mode;
-        j.deblock = p.deblock ? 1u : 0u;
-        j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
-        j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
-        j.mb_begin = per_frame ? 0 : mbs;
-        j.row_begin = per_frame ? 0 : rows;
-        mbs += p.width_mbs * p.height_mbs;
-        rows += j.deblock ? p.height_mbs : 0;
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    size_t counters = 0;
-    if (per_frame) {
-        for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
-    } else {
-        counters = 1 + rows;
-    }
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::H264Job), counters);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::H264Job*>(sub.jobs_ptr());
-    hipError_t e = hipSuccess;
-    const uint32_t groups = groups_for(ctx, 16);
-    if (per_frame) {
-        uint32_t* c = sub.counters_ptr();
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_h264_mc(dj + i, 1, pics[i].width_mbs * pics[i].height_mbs, sub.status_ptr(), ctx->stream);
-            if (e == hipSuccess && jobs[i].deblock) {
-                e = mmr::launch_h264_deblock(dj + i, 1, pics[i].height_mbs, c, 1 + pics[i].height_mbs, groups, ctx->stream);
-                c += 1 + pics[i].height_mbs;
-            } else {
-                c += 1;
-            }
-        }
-    } else {
-        e = mmr::launch_h264_mc(dj, count, mbs, sub.status_ptr(), ctx->stream);
-        if (e == hipSuccess && rows) e = mmr::launch_h264_deblock(dj, count, rows, sub.counters_ptr(), 1 + rows, groups, ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "H.264 launch");
-    return sub.end(out_ticket);
+    return mmr_host::h264_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
 }
 
 // ------------------------------------------------------------------ colour / blend
 mmr_status mmr_recon_yuv_to_rgb_submit(mmr_recon_context* ctx, const mmr_recon_yuv_to_rgb_job* in, uint32_t count, uint32_t flags,
                                        uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
-    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
-    std::vector<mmr::RgbJob> jobs(count);
-    uint32_t quads = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        mmr_status s = check_rgb_job(in[i]);
-        if (s) return s;
-        mmr::RgbJob& r = jobs[i];
-        memset(&r, 0, sizeof r);
-        for (int k = 0; k < 3; ++k) r.planes[k] = in[i].planes[k];
-        r.rgb = in[i].rgb;
-        r.rgba = in[i].rgba;
-        r.width = in[i].width;
-        r.height = in[i].height;
-        r.cw = in[i].chroma_width;
-        r.ch = in[i].chroma_height;
-        r.luma_stride = in[i].luma_stride;
-        r.chroma_stride = in[i].chroma_stride;
-        r.range = in[i].range;
-        r.matrix = in[i].matrix;
-        r.quads_per_row = (in[i].width + 3) / 4;
-        r.quad_begin = per_frame ? 0 : quads;
-        quads += r.quads_per_row * r.height;
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::RgbJob), 0);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::RgbJob*>(sub.jobs_ptr());
-    hipError_t e = hipSuccess;
-    if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_yuv_to_rgb(dj + i, 1, jobs[i].quads_per_row * jobs[i].height, ctx->stream);
-    } else {
-        e = mmr::launch_yuv_to_rgb(dj, count, quads, ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "YUV->RGB launch");
-    return sub.end(out_ticket);
+    return mmr_host::yuv_to_rgb_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
 }
 
 mmr_status mmr_recon_composite_submit(mmr_recon_context* ctx, const mmr_recon_composite_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
-    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
-    std::vector<mmr::CompositeJobDev> jobs(count);
-    uint32_t px = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_composite_job& c = in[i];
-        if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
-        if (!c.target || !c.width || !c.height || c.stride < c.width * 4 || c.layer_count > MMR_RECON_MAX_LAYERS)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
-        if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
-        for (uint32_t l = 0; l < c.layer_count; ++l) {
-            const mmr_recon_layer& L = c.layers[l];
-            if (!L.rgba || L.stride < L.width * 4 || L.stride % 4 || !aligned(L.rgba, 4))
-                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u layer %u storage is invalid", i, l);
-            if ((uint64_t)L.x + L.width > c.width || (uint64_t)L.y + L.height > c.height)
-                return fail(MMR_STATUS_INVALID_DATA, "RGBA overlay storage or placement is invalid (job %u layer %u)", i, l);
-            if (L.op != MMR_BLEND_SOURCE_OVER && L.op != MMR_BLEND_PREMULTIPLIED_OVER)
-                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite operator %u invalid", L.op);
-            if (L.opacity > 255) return fail(MMR_STATUS_INVALID_ARGUMENT, "opacity must be 0..255");
-        }
-        mmr::CompositeJobDev& j = jobs[i];
-        memset(&j, 0, sizeof j);
-        j.target = c.target;
-        j.width = c.width;
-        j.height = c.height;
-        j.stride = c.stride;
-        j.layer_count = c.layer_count;
-        for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
-        j.px_begin = per_frame ? 0 : px;
-        px += c.width * c.height;
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::CompositeJobDev), 0);
-    if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::CompositeJobDev*>(sub.jobs_ptr());
-    hipError_t e = hipSuccess;
-    if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_composite(dj + i, 1, jobs[i].width * jobs[i].height, ctx->stream);
-    } else {
-        e = mmr::launch_composite(dj, count, px, ctx->stream);
-    }
-    if (e != hipSuccess) return hip_fail(e, "composite launch");
-    return sub.end(out_ticket);
+    return mmr_host::composite_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
 }
 
 mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
     CHECK_CTX(ctx);
-    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
-    uint32_t max_px = 0;
-    for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_yuv_blend_job& j = in[i];
-        if (j.struct_size < sizeof(mmr_recon_yuv_blend_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend struct_size too small");
-        if ((j.width | j.height | j.x | j.y) & 1 || (uint64_t)j.x + j.width > j.target_width || (uint64_t)j.y + j.height > j.target_height)
-            return fail(MMR_STATUS_INVALID_DATA, "YUV overlay storage or placement is invalid (job %u)", i);
-        for (int k = 0; k < 3; ++k)
-            if (!j.target[k] || !j.overlay[k]) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has a NULL plane", i);
-        if (!j.rgba || !j.chroma_alpha) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV blend job %u has no alpha", i);
-        max_px = std::max(max_px, j.width * j.height);
-    }
     LOCK_AND_SET_DEVICE(ctx);
-    Submission sub(ctx);
-    mmr_status s = sub.begin(in, sizeof(mmr_recon_yuv_blend_job) * count, 0);
-    if (s) return s;
-    hipError_t e = hipSuccess;
-    auto* dj = reinterpret_cast<const mmr_recon_yuv_blend_job*>(sub.jobs_ptr());
-    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_yuv_blend(dj + i, 1, in[i].width * in[i].height, ctx->stream);
-    } else {
-        e = mmr::launch_yuv_blend(dj, count, max_px, ctx->stream);
+    return mmr_host::yuv_blend_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+}
+
+}  // extern "C"
+
+// ====================================================================== serving
+//
+// Continuous batching. A scheduler thread owns `slots` HIP streams. Each iteration it
+//   1. retires batches whose completion event has fired (frame completions with latency
+//      enqueue -> observed completion; requests that finish release their admitted bytes),
+//   2. admits pending requests in FIFO order while admitted bytes + request bytes <= budget
+//      (a request larger than the whole budget is admitted alone),
+//   3. while a slot is free and admitted requests have frames left, launches one batch: up to
+//      max_frames_per_request_per_batch next frames of every admitted request (round robin),
+//      at most max_batch_frames per codec; per codec one batched submit, then the batch's
+//      YUV->RGB and composite jobs, all on the slot's stream, then the batch event.
+// Frames of one request, and requests of one stream, are ordered: a batch waits (device-side,
+// hipStreamWaitEvent) on the previous batch that carried frames of the same request or stream.
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+
+namespace mmr_host {
+
+using Clock = std::chrono::steady_clock;
+
+struct BatchEvent {
+    hipEvent_t ev = nullptr;
+    ~BatchEvent() {
+        if (ev) (void)hipEventDestroy(ev);
     }
-    if (e != hipSuccess) return hip_fail(e, "YUV blend launch");
-    return sub.end(out_ticket);
+};
+
+size_t codec_struct_size(uint32_t codec) {
+    switch (codec) {
+    case MMR_CODEC_MJPEG: return sizeof(mmr_recon_mjpeg_frame);
+    case MMR_CODEC_MPEG2: return sizeof(mmr_recon_mpeg2_picture);
+    case MMR_CODEC_DV: return sizeof(mmr_recon_dv_frame);
+    case MMR_CODEC_H264: return sizeof(mmr_recon_h264_picture);
+    default: return 0;
+    }
+}
+
+struct ServerRequest {
+    uint64_t id = 0, tag = 0, hbm = 0;
+    uint32_t codec = 0, stream_id = 0, frame_count = 0;
+    std::vector<char> frames;
+    std::vector<mmr_recon_yuv_to_rgb_job> convert;
+    std::vector<mmr_recon_composite_job> composite;
+    uint32_t next = 0;   // next frame to place in a batch
+    uint32_t done = 0;   // frames completed
+    mmr_status status = MMR_STATUS_OK;
+    Clock::time_point t0;
+    std::shared_ptr<BatchEvent> last;   // last launched batch carrying frames of this request
+    const char* frame_ptr(uint32_t i) const { return frames.data() + (size_t)i * codec_struct_size(codec); }
+};
+
+struct InflightBatch {
+    std::shared_ptr<BatchEvent> ev;
+    int slot = -1;
+    std::vector<std::pair<ServerRequest*, uint32_t>> frames;
+    std::vector<uint64_t> tickets;
+};
+
+}  // namespace mmr_host
+
+struct mmr_recon_server {
+    mmr_recon_context* ctx = nullptr;
+    mmr_recon_server_config cfg{};
+    std::vector<hipStream_t> streams;
+    std::vector<bool> slot_busy;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::condition_variable idle_cv;
+    std::deque<std::unique_ptr<mmr_host::ServerRequest>> pending;
+    std::list<std::unique_ptr<mmr_host::ServerRequest>> active;
+    std::deque<mmr_host::InflightBatch> inflight;
+    std::deque<mmr_recon_frame_completion> completions;
+    std::unordered_map<uint32_t, std::shared_ptr<mmr_host::BatchEvent>> stream_last;   // per stream id
+    mmr_recon_server_stats stats{};
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::thread thread;
+};
+
+namespace mmr_host {
+
+mmr_status launch_batch(mmr_recon_server* s, int slot, InflightBatch& b, std::vector<ServerRequest*>& reqs_in_batch) {
+    mmr_recon_context* ctx = s->ctx;
+    hipStream_t st = s->streams[slot];
+    // Device-side ordering after earlier batches of the same requests.
+    std::vector<hipEvent_t> waits;
+    for (ServerRequest* r : reqs_in_batch)
+        if (r->last && r->last->ev) waits.push_back(r->last->ev);
+    std::sort(waits.begin(), waits.end());
+    waits.erase(std::unique(waits.begin(), waits.end()), waits.end());
+    for (hipEvent_t e : waits) {
+        hipError_t err = hipStreamWaitEvent(st, e, 0);
+        if (err != hipSuccess) return hip_fail(err, "hipStreamWaitEvent");
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const uint32_t codecs[4] = {MMR_CODEC_MJPEG, MMR_CODEC_MPEG2, MMR_CODEC_DV, MMR_CODEC_H264};
+    std::vector<mmr_recon_yuv_to_rgb_job> conv;
+    std::vector<mmr_recon_composite_job> comp;
+    for (uint32_t codec : codecs) {
+        std::vector<std::pair<ServerRequest*, uint32_t>> items;
+        for (auto& f : b.frames)
+            if (f.first->codec == codec) items.push_back(f);
+        if (items.empty()) continue;
+        const size_t sz = codec_struct_size(codec);
+        std::vector<char> blob(items.size() * sz);
+        for (size_t i = 0; i < items.size(); ++i) memcpy(blob.data() + i * sz, items[i].first->frame_ptr(items[i].second), sz);
+        if (codec == MMR_CODEC_MPEG2) {
+            // Translate in-request reference indices to positions in this batch (-1 when the
+            // producing picture ran in an earlier batch, which this stream already waited for).
+            auto* pics = reinterpret_cast<mmr_recon_mpeg2_picture*>(blob.data());
+            for (size_t i = 0; i < items.size(); ++i) {
+                for (int32_t* src : {&pics[i].forward_source, &pics[i].backward_source}) {
+                    int32_t pos = -1;
+                    if (*src >= 0)
+                        for (size_t k = 0; k < i; ++k)
+                            if (items[k].first == items[i].first && items[k].second == (uint32_t)*src) pos = (int32_t)k;
+                    *src = pos;
+                }
+            }
+        }
+        uint64_t t = 0;
+        mmr_status rc;
+        switch (codec) {
+        case MMR_CODEC_MJPEG: rc = mjpeg_submit_on(ctx, st, reinterpret_cast<mmr_recon_mjpeg_frame*>(blob.data()), (uint32_t)items.size(), 0, &t); break;
+        case MMR_CODEC_MPEG2: rc = mpeg2_submit_on(ctx, st, reinterpret_cast<mmr_recon_mpeg2_picture*>(blob.data()), (uint32_t)items.size(), 0, &t); break;
+        case MMR_CODEC_DV: rc = dv_submit_on(ctx, st, reinterpret_cast<mmr_recon_dv_frame*>(blob.data()), (uint32_t)items.size(), 0, &t); break;
+        default: rc = h264_submit_on(ctx, st, reinterpret_cast<mmr_recon_h264_picture*>(blob.data()), (uint32_t)items.size(), 0, &t); break;
+        }
+        if (rc) return rc;
+        b.tickets.push_back(t);
+    }
+    for (auto& f : b.frames) {
+        if (!f.first->convert.empty()) conv.push_back(f.first->convert[f.second]);
+        if (!f.first->composite.empty()) comp.push_back(f.first->composite[f.second]);
+    }
+    uint64_t t = 0;
+    if (!conv.empty()) {
+        mmr_status rc = yuv_to_rgb_submit_on(ctx, st, conv.data(), (uint32_t)conv.size(), 0, &t);
+        if (rc) return rc;
+        b.tickets.push_back(t);
+    }
+    if (!comp.empty()) {
+        mmr_status rc = composite_submit_on(ctx, st, comp.data(), (uint32_t)comp.size(), 0, &t);
+        if (rc) return rc;
+        b.tickets.push_back(t);
+    }
+    b.ev = std::make_shared<BatchEvent>();
+    e = hipEventCreateWithFlags(&b.ev->ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventRecord(b.ev->ev, st);
+    if (e != hipSuccess) return hip_fail(e, "batch event");
+    return MMR_STATUS_OK;
+}
+
+void server_loop(mmr_recon_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    size_t rr = 0;
+    for (;;) {
+        bool progressed = false;
+        // 1. retire completed batches (in launch order per slot; any order across slots)
+        for (auto it = s->inflight.begin(); it != s->inflight.end();) {
+            if (hipEventQuery(it->ev->ev) != hipSuccess) {
+                ++it;
+                continue;
+            }
+            const auto now = Clock::now();
+            mmr_status st = MMR_STATUS_OK;
+            {
+                std::lock_guard<std::mutex> clk(s->ctx->mu);
+                for (uint64_t t : it->tickets) {
+                    auto tr = s->ctx->tickets.find(t);
+                    if (tr != s->ctx->tickets.end()) {
+                        mmr_status x = finish(s->ctx, tr->second);
+                        if (x && !st) st = x;
+                    }
+                }
+                reap(s->ctx);
+            }
+            for (auto& f : it->frames) {
+                ServerRequest* r = f.first;
+                if (st && !r->status) r->status = st;
+                mmr_recon_frame_completion c;
+                c.request_id = r->id;
+                c.user_tag = r->tag;
+                c.frame_index = f.second;
+                c.status = st;
+                c.latency_ms = std::chrono::duration<double, std::milli>(now - r->t0).count();
+                s->completions.push_back(c);
+                s->stats.frames_completed++;
+                r->done++;
+            }
+            s->slot_busy[it->slot] = false;
+            it = s->inflight.erase(it);
+            progressed = true;
+        }
+        for (auto it = s->active.begin(); it != s->active.end();) {
+            if ((*it)->done == (*it)->frame_count) {
+                s->stats.admitted_bytes -= (*it)->hbm;
+                s->stats.requests_completed++;
+                it = s->active.erase(it);
+                progressed = true;
+            } else {
+                ++it;
+            }
+        }
+        // 2. admission by HBM budget (FIFO)
+        while (!s->pending.empty()) {
+            auto& r = s->pending.front();
+            const uint64_t budget = s->cfg.hbm_budget_bytes;
+            if (budget && !s->active.empty() && s->stats.admitted_bytes + r->hbm > budget) break;
+            s->stats.admitted_bytes += r->hbm;
+            s->stats.peak_admitted_bytes = std::max(s->stats.peak_admitted_bytes, s->stats.admitted_bytes);
+            s->active.push_back(std::move(r));
+            s->pending.pop_front();
+            progressed = true;
+        }
+        // 3. launch batches while slots are free
+        for (;;) {
+            int slot = -1;
+            for (size_t i = 0; i < s->slot_busy.size(); ++i)
+                if (!s->slot_busy[i]) { slot = (int)i; break; }
+            if (slot < 0) break;
+            InflightBatch b;
+            b.slot = slot;
+            std::vector<ServerRequest*> reqs;
+            uint32_t per_codec[5] = {0, 0, 0, 0, 0};
+            const size_t n = s->active.size();
+            // A stream's requests are batched in enqueue order: only its earliest request with
+            // frames left is eligible.
+            std::unordered_map<uint32_t, uint64_t> first_open;
+            for (auto& ar : s->active)
+                if (ar->stream_id && ar->next < ar->frame_count) {
+                    auto f = first_open.find(ar->stream_id);
+                    if (f == first_open.end() || ar->id < f->second) first_open[ar->stream_id] = ar->id;
+                }
+            auto start = s->active.begin();
+            for (size_t k = 0; k < (rr % (n ? n : 1)); ++k) ++start;
+            auto it = start;
+            for (size_t visited = 0; visited < n; ++visited) {
+                if (it == s->active.end()) it = s->active.begin();
+                ServerRequest* r = it->get();
+                ++it;
+                if (r->next >= r->frame_count) continue;
+                if (r->stream_id && first_open[r->stream_id] != r->id) continue;
+                uint32_t take = 0;
+                while (r->next < r->frame_count && take < s->cfg.max_frames_per_request_per_batch &&
+                       per_codec[r->codec] < s->cfg.max_batch_frames) {
+                    b.frames.push_back({r, r->next});
+                    r->next++;
+                    take++;
+                    per_codec[r->codec]++;
+                }
+                if (take) reqs.push_back(r);
+            }
+            if (b.frames.empty()) break;
+            rr++;
+            // A request's first batch also waits for the last batch of its stream's previous request.
+            for (ServerRequest* r : reqs)
+                if (r->stream_id && !r->last) {
+                    auto sl = s->stream_last.find(r->stream_id);
+                    if (sl != s->stream_last.end()) r->last = sl->second;
+                }
+            mmr_status rc = launch_batch(s, slot, b, reqs);
+            if (rc) {
+                for (auto& f : b.frames) {
+                    mmr_recon_frame_completion c{f.first->id, f.first->tag, f.second, rc, 0.0};
+                    s->completions.push_back(c);
+                    f.first->done++;
+                    if (!f.first->status) f.first->status = rc;
+                }
+                continue;
+            }
+            for (ServerRequest* r : reqs) {
+                r->last = b.ev;
+                if (r->stream_id) s->stream_last[r->stream_id] = b.ev;
+            }
+            s->slot_busy[slot] = true;
+            s->stats.batches_launched++;
+            s->inflight.push_back(std::move(b));
+            progressed = true;
+        }
+        s->stats.pending = (uint32_t)s->pending.size();
+        s->stats.active = (uint32_t)s->active.size();
+        if (s->pending.empty() && s->active.empty() && s->inflight.empty()) {
+            s->idle_cv.notify_all();
+            if (s->stop) break;
+            s->cv.wait(lk);
+            continue;
+        }
+        if (!progressed) {
+            if (s->inflight.empty()) s->cv.wait_for(lk, std::chrono::milliseconds(1));
+            else s->cv.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+}
+
+}  // namespace mmr_host
+
+extern "C" {
+
+mmr_status mmr_recon_server_create(mmr_recon_context* ctx, const mmr_recon_server_config* config, mmr_recon_server** out) {
+    CHECK_CTX(ctx);
+    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_server is NULL");
+    *out = nullptr;
+    mmr_recon_server_config cfg{};
+    if (config) {
+        if (config->struct_size < sizeof(mmr_recon_server_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "server config struct_size too small");
+        cfg = *config;
+    }
+    if (!cfg.slots) cfg.slots = 4;
+    if (cfg.slots > 32) return fail(MMR_STATUS_INVALID_ARGUMENT, "at most 32 slots");
+    if (!cfg.max_batch_frames) cfg.max_batch_frames = 64;
+    if (!cfg.max_frames_per_request_per_batch) cfg.max_frames_per_request_per_batch = 8;
+    if (!cfg.max_pending) cfg.max_pending = 4096;
+    auto* s = new mmr_recon_server();
+    s->ctx = ctx;
+    s->cfg = cfg;
+    hipError_t e = hipSetDevice(ctx->device);
+    for (uint32_t i = 0; i < cfg.slots && e == hipSuccess; ++i) {
+        hipStream_t st = nullptr;
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
+        if (e == hipSuccess) s->streams.push_back(st);
+    }
+    if (e != hipSuccess) {
+        for (auto st : s->streams) (void)hipStreamDestroy(st);
+        delete s;
+        return hip_fail(e, "server streams");
+    }
+    s->slot_busy.assign(cfg.slots, false);
+    s->stats.struct_size = sizeof(mmr_recon_server_stats);
+    s->thread = std::thread(mmr_host::server_loop, s);
+    *out = s;
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_server_enqueue(mmr_recon_server* s, const mmr_recon_request* req, uint64_t* out_id) {
+    if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
+    if (!req || req->struct_size < sizeof(mmr_recon_request)) return fail(MMR_STATUS_INVALID_ARGUMENT, "request invalid");
+    const size_t sz = mmr_host::codec_struct_size(req->codec);
+    if (!sz) return fail(MMR_STATUS_INVALID_ARGUMENT, "request codec %u invalid", req->codec);
+    if (!req->frame_count || !req->frames) return fail(MMR_STATUS_INVALID_ARGUMENT, "request has no frames");
+    auto r = std::make_unique<mmr_host::ServerRequest>();
+    r->codec = req->codec;
+    r->stream_id = req->stream_id;
+    r->frame_count = req->frame_count;
+    r->hbm = req->hbm_bytes;
+    r->tag = req->user_tag;
+    r->frames.assign((const char*)req->frames, (const char*)req->frames + sz * req->frame_count);
+    if (req->codec == MMR_CODEC_MPEG2) {
+        auto* pics = reinterpret_cast<const mmr_recon_mpeg2_picture*>(req->frames);
+        for (uint32_t i = 0; i < req->frame_count; ++i)
+            if (pics[i].forward_source >= (int32_t)i || pics[i].backward_source >= (int32_t)i)
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "request picture %u references a later picture", i);
+    }
+    if (req->convert) r->convert.assign(req->convert, req->convert + req->frame_count);
+    if (req->composite) r->composite.assign(req->composite, req->composite + req->frame_count);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return fail(MMR_STATUS_INVALID_STATE, "server is shutting down");
+        if (s->pending.size() >= s->cfg.max_pending) return fail(MMR_STATUS_BUSY, "server queue full (%u pending)", s->cfg.max_pending);
+        r->id = s->next_id++;
+        r->t0 = mmr_host::Clock::now();
+        if (out_id) *out_id = r->id;
+        s->pending.push_back(std::move(r));
+        s->stats.requests_enqueued++;
+    }
+    s->cv.notify_one();
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_server_poll(mmr_recon_server* s, mmr_recon_frame_completion* out, uint32_t capacity, uint32_t* out_count) {
+    if (!s || !out_count || (capacity && !out)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid poll arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint32_t n = 0;
+    while (n < capacity && !s->completions.empty()) {
+        out[n++] = s->completions.front();
+        s->completions.pop_front();
+    }
+    *out_count = n;
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_server_drain(mmr_recon_server* s) {
+    if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv.notify_all();
+    s->idle_cv.wait(lk, [s] { return s->pending.empty() && s->active.empty() && s->inflight.empty(); });
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_server_stats_get(mmr_recon_server* s, mmr_recon_server_stats* out) {
+    if (!s || !out || out->struct_size < sizeof(mmr_recon_server_stats)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid stats arguments");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    out->struct_size = sizeof(mmr_recon_server_stats);
+    return MMR_STATUS_OK;
+}
+
+void mmr_recon_server_destroy(mmr_recon_server* s) {
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->pending.clear();
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    s->thread.join();
+    for (auto st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        (void)hipStreamDestroy(st);
+    }
+    delete s;
 }
 
 }  // extern "C"

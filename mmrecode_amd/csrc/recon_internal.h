@@ -104,7 +104,7 @@ hipError_t launch_dv(const DvJob* d_jobs, uint32_t job_count, uint32_t total_blo
 hipError_t dv_upload_tables(hipStream_t s);
 hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
-                               uint32_t counter_words, uint32_t groups, hipStream_t s);
+                               uint32_t counter_words, uint32_t groups, uint32_t* d_status, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
@@ -113,6 +113,7 @@ hipError_t launch_fill_zero(uint32_t* d_words, uint32_t count, hipStream_t s);
 // Bits of the per-submission device status word.
 constexpr uint32_t kDevStatusUnsupported = 1u;
 constexpr uint32_t kDevStatusInvalidData = 2u;
+constexpr uint32_t kDevStatusTimeout = 4u;   // a bounded device wait gave up (internal error)
 
 // Binary search over ascending begin offsets: index of the last job whose begin <= key.
 template <typename Job, typename Get>

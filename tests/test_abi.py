@@ -33,6 +33,10 @@ STRUCTS = {
     "mmr_recon_layer": abi.Layer,
     "mmr_recon_composite_job": abi.CompositeJob,
     "mmr_recon_yuv_blend_job": abi.YuvBlendJob,
+    "mmr_recon_server_config": abi.ServerConfig,
+    "mmr_recon_request": abi.Request,
+    "mmr_recon_frame_completion": abi.FrameCompletion,
+    "mmr_recon_server_stats": abi.ServerStats,
 }
 
 needs_cc = pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is not available")
