@@ -8,6 +8,43 @@
 
 #define MMR_DEV __device__ __forceinline__
 
+// Pointers read from job records are generic to the compiler, which then emits flat accesses
+// (they wait on the LDS counter too). All buffers the kernels touch through them are device
+// memory, so they are cast to the global address space at the point of use.
+#define MMR_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+MMR_DEV const MMR_GLOBAL T* as_global(const T* p) { return (const MMR_GLOBAL T*)p; }
+template <typename T>
+MMR_DEV MMR_GLOBAL T* as_global(T* p) { return (MMR_GLOBAL T*)p; }
+
+// 16- and 8-byte global loads (clang vector types: HIP's int4 / uint2 cannot be copied out of an
+// address-space-qualified lvalue).
+typedef int32_t mmr_v4i32 __attribute__((ext_vector_type(4)));
+typedef uint32_t mmr_v2u32 __attribute__((ext_vector_type(2)));
+typedef int16_t mmr_v2i16 __attribute__((ext_vector_type(2)));
+MMR_DEV int4 load_global_int4(const void* p) {
+    const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(as_global(p));
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+MMR_DEV uint2 load_global_uint2(const void* p) {
+    const mmr_v2u32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v2u32*>(as_global(p));
+    return make_uint2(v.x, v.y);
+}
+
+// Element i (0..2, per lane) of a three-pointer array that is uniform across the workgroup, as a
+// select between scalar values (a plain p[i] is a per-lane load from the job record).
+template <typename T>
+MMR_DEV T* pick3(T* const* p, int i) {
+    uint64_t v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t x = reinterpret_cast<uint64_t>(p[k]);
+        v[k] = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+               (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x);
+    }
+    return reinterpret_cast<T*>(i == 0 ? v[0] : (i == 1 ? v[1] : v[2]));
+}
+
 // Non-fused f64 arithmetic. The reference evaluates every product and sum with its own rounding
 // (Rust never contracts a*b+c into an FMA); these builtins keep the compiler from fusing too.
 MMR_DEV double dmul(double a, double b) { return __dmul_rn(a, b); }
@@ -39,43 +76,99 @@ __constant__ static const uint8_t kAlternateScan[64] = {
     27, 34, 42, 50, 58, 35, 43, 51, 59, 20, 28, 5, 13, 6, 14, 21, 29, 36, 44, 52, 60, 37, 45, 53,
     61, 22, 30, 7, 15, 23, 31, 38, 46, 54, 62, 39, 47, 55, 63};
 
-// Blocks handled by one 256-thread workgroup in the 8-lanes-per-block kernels.
+// Blocks handled by one 256-thread workgroup in the 8-lanes-per-block kernels. Work is split into
+// tiles of 32 blocks that never straddle two frames; the host uploads a tile -> job table with the
+// jobs, so a workgroup finds its (uniform) job with one scalar load.
 constexpr int kBlocksPerGroup = 32;
 constexpr int kGroupThreads = 256;
 
-// Separable f64 IDCT of one 8x8 block, evaluated by the 8 lanes of a block group: lane x returns
-// the column-x sums (before level shift / rounding) for rows y = 0..7.
-//   row pass:    tmp[v][x] = sum_u M[x][u] * c[v][u]   (u ascending, from +0.0)
-//   column pass: out[y][x] = sum_v M[y][v] * tmp[v][x] (v ascending, from +0.0)
-// Rows of zero coefficients are skipped: their products are +-0.0 and adding a signed zero to an
-// accumulator that started at +0.0 leaves it unchanged (the accumulator is never -0.0), so the
-// result equals the dense evaluation bit for bit.
-MMR_DEV void idct8x8_column(const int32_t* __restrict__ coef, int x, uint32_t row_mask, double out[8]) {
-    double mx[8];
+// Store 8 bytes (little-endian in v) at p: one 8-byte store when aligned, else two 4-byte stores
+// (p must be 4-byte aligned), else bytes.
+MMR_DEV void store_row8(uint8_t* p, uint64_t v) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    MMR_GLOBAL uint8_t* q = as_global(p);
+    if ((a & 7) == 0) {
+        *reinterpret_cast<MMR_GLOBAL uint64_t*>(q) = v;
+    } else if ((a & 3) == 0) {
+        reinterpret_cast<MMR_GLOBAL uint32_t*>(q)[0] = (uint32_t)v;
+        reinterpret_cast<MMR_GLOBAL uint32_t*>(q)[1] = (uint32_t)(v >> 32);
+    } else {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) mx[u] = kIdctBasis[x * 8 + u];
-    double tmp[8];
+        for (int i = 0; i < 8; ++i) q[i] = (uint8_t)(v >> (8 * i));
+    }
+}
+
+// Orders LDS accesses between the lanes of one wave (the 8 lanes of a block group always share a
+// wave): a compiler-level fence around a wave barrier; the hardware executes a wave's LDS
+// operations in order, so no wait instruction is needed.
+MMR_DEV void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Separable f64 IDCT of one 8x8 block by the 8 lanes of its group, in two passes with an LDS
+// transpose between them (kIdctTmpStride doubles of LDS per group; the 64-byte pad after the 8x8
+// doubles puts consecutive groups 16 banks apart):
+//   row pass    (lane v, natural row v):  tmp[v][x] = sum_u M[x][u] * c[v][u]   (u ascending)
+//   column pass (lane x):                 out[y][x] = sum_v M[y][v] * tmp[v][x] (v ascending)
+// Every product and every sum is rounded on its own (dmul / dadd), in the reference's order. M is
+// uniform in both passes (scalar loads). The reference starts each sum from +0.0; starting from
+// the first product instead can only turn a +0.0 result into -0.0 when every term is zero, and
+// such a value leaves the later sums unchanged and rounds to the integer 0, so the integer
+// outputs are identical. Natural rows whose coefficients are all zero in every block of the wave
+// are skipped in the column pass (their tmp values are zero, so their terms are signed zeros); the
+// skip mask is wave-uniform, so the skip is a scalar branch.
+constexpr int kIdctTmpStride = 72;
+
+MMR_DEV void idct8x8_row_pass(const int32_t c[8], double* __restrict__ tmp_row) {
+    double cd[8];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-        tmp[v] = 0.0;
-        if (row_mask & (1u << v)) {
-            const int4 lo = *reinterpret_cast<const int4*>(coef + v * 8);
-            const int4 hi = *reinterpret_cast<const int4*>(coef + v * 8 + 4);
-            const int32_t c[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-            double t = 0.0;
+    for (int u = 0; u < 8; ++u) cd[u] = (double)c[u];
+    double t[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) t = dadd(t, dmul(mx[u], (double)c[u]));
-            tmp[v] = t;
-        }
+    for (int x = 0; x < 8; ++x) {
+        double s = dmul(kIdctBasis[x * 8], cd[0]);
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = dadd(s, dmul(kIdctBasis[x * 8 + u], cd[u]));
+        t[x] = s;
     }
 #pragma unroll
-    for (int y = 0; y < 8; ++y) {
-        double s = 0.0;
+    for (int x = 0; x < 8; x += 2) *reinterpret_cast<double2*>(tmp_row + x) = make_double2(t[x], t[x + 1]);
+}
+
+// Bit v set when natural row v is nonzero in any block of the wave; `row_nonzero` is the calling
+// lane's own row (v = lane & 7). Row 0 is always included (it starts every column sum).
+MMR_DEV uint32_t wave_nonzero_rows(bool row_nonzero) {
+    uint64_t b = __ballot(row_nonzero);
+    b |= b >> 32;
+    b |= b >> 16;
+    b |= b >> 8;
+    return ((uint32_t)b & 0xffu) | 1u;
+}
+
+MMR_DEV void idct8x8_column_pass(const double* __restrict__ tmp, int x, uint32_t rows, double out[8]) {
+    const double c0 = tmp[x];
 #pragma unroll
-        for (int v = 0; v < 8; ++v)
-            if (row_mask & (1u << v)) s = dadd(s, dmul(kIdctBasis[y * 8 + v], tmp[v]));
-        out[y] = s;
+    for (int y = 0; y < 8; ++y) out[y] = dmul(kIdctBasis[y * 8], c0);
+    // Remaining rows in ascending order; `rows` is wave-uniform, so this is a scalar loop.
+    uint32_t rest = rows & ~1u;
+    while (rest) {
+        const int v = __builtin_ctz(rest);
+        rest &= rest - 1;
+        const double cv = tmp[v * 8 + x];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) out[y] = dadd(out[y], dmul(kIdctBasis[y * 8 + v], cv));
     }
+}
+
+// round() (half away from zero) of a value with |v| < 2^31, as an int: trunc, then the exact
+// fractional part v - trunc(v) decides the step away from zero.
+MMR_DEV int round_half_away_i32(double v) {
+    const double t = trunc(v);
+    const int i = (int)t;
+    const double frac = dadd(v, -t);
+    return fabs(frac) >= 0.5 ? (v < 0.0 ? i - 1 : i + 1) : i;
 }
 
 MMR_DEV double round_half_away(double v) { return round(v); }

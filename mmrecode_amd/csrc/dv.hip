@@ -67,9 +67,10 @@ MMR_DEV void dv_mb_coordinates(uint32_t system, int seq, int slot, int mb, int* 
     }
 }
 
-__global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restrict__ jobs, uint32_t job_count,
-                                                           uint32_t total_blocks) {
+__global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restrict__ jobs,
+                                                           const uint32_t* __restrict__ tile_job, uint32_t tile_base) {
     __shared__ __align__(16) int32_t coef[kBlocksPerGroup][64];
+    __shared__ __align__(16) uint8_t pix[kBlocksPerGroup][64];
     __shared__ double s_cos16[64];
     __shared__ double s_cucv[64];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
@@ -77,16 +78,15 @@ __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restri
         s_cos16[threadIdx.x] = kDvCos16[threadIdx.x];
         s_cucv[threadIdx.x] = kDvCuCv[threadIdx.x];
     }
-    const uint32_t gb = blockIdx.x * kBlocksPerGroup + g;
-    const bool active = gb < total_blocks;
-    const uint32_t ji = active ? find_job(jobs, job_count, gb, [](const DvJob& j) { return j.block_begin; }) : 0;
-    const DvJob& job = jobs[ji];
-    const uint32_t blk = active ? gb - job.block_begin : 0;
-    const uint8_t info = active ? job.info[blk] : 0;
+    const uint32_t tile = tile_base + blockIdx.x;
+    const DvJob& job = jobs[tile_job[tile]];
+    const uint32_t blk = (tile - job.tile_begin) * kBlocksPerGroup + g;
+    const bool active = blk < job.block_count;
+    const uint8_t info = active ? as_global(job.info)[blk] : 0;
     const int mode248 = info & 1, cls = (info >> 1) & 3, qno = (info >> 3) & 15;
 
     if (active) {
-        const int4 raw = *reinterpret_cast<const int4*>(job.levels + (size_t)blk * 64 + lane * 8);
+        const int4 raw = load_global_int4(job.levels + (size_t)blk * 64 + lane * 8);
         const int16_t* l8 = reinterpret_cast<const int16_t*>(&raw);
         const uint8_t* scan = mode248 ? kZigzag248 : kZigzag;
         const int32_t* weight = mode248 ? kIWeight248 : kIWeight88;
@@ -110,7 +110,6 @@ __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restri
         }
     }
     __syncthreads();
-    if (!active) return;
 
     // Nonzero pattern of the block: bit v*8+u set when coefficient (v,u) != 0.
     uint32_t rowbits = 0;
@@ -173,7 +172,12 @@ __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restri
             }
     }
 
-    // Placement (place_macroblock / put_split_chroma).
+    // Column x of the block -> LDS, then lane r stores row r (place_macroblock / put_split_chroma).
+#pragma unroll
+    for (int r = 0; r < 8; ++r) pix[g][r * 8 + x] = out[r];
+    wave_lds_sync();
+    if (!active) return;
+    const uint64_t row = *reinterpret_cast<const uint64_t*>(&pix[g][lane * 8]);
     const uint32_t mb_index = blk / 6;
     const int b = (int)(blk % 6);
     const int seq = (int)(mb_index / 135), slot = (int)((mb_index % 135) / 5), mbs = (int)(mb_index % 5);
@@ -181,36 +185,27 @@ __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restri
     dv_mb_coordinates(job.system, seq, slot, mbs, &mbx, &mby);
     const int X = mbx * 8, Y = mby * 8;
     const int width = 720;
+    const int r = lane;
     if (job.system == MMR_DV_SYSTEM_625_50) {
         if (b < 4) {
-            uint8_t* p = job.planes[0] + (size_t)(Y + (b >> 1) * 8) * width + X + (b & 1) * 8 + x;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+            store_row8(job.planes[0] + (size_t)(Y + (b >> 1) * 8 + r) * width + X + (b & 1) * 8, row);
         } else {
-            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)(Y / 2) * 360 + X / 2 + x;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * 360] = out[r];
+            store_row8(job.planes[b == 4 ? 2 : 1] + (size_t)(Y / 2 + r) * 360 + X / 2, row);
         }
     } else if (mbx < 88) {
         if (b < 4) {
-            uint8_t* p = job.planes[0] + (size_t)Y * width + X + b * 8 + x;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+            store_row8(job.planes[0] + (size_t)(Y + r) * width + X + b * 8, row);
         } else {
-            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)Y * 180 + X / 4 + x;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * 180] = out[r];
+            store_row8(job.planes[b == 4 ? 2 : 1] + (size_t)(Y + r) * 180 + X / 4, row);
         }
     } else {
         if (b < 4) {
-            uint8_t* p = job.planes[0] + (size_t)(Y + (b >> 1) * 8) * width + X + (b & 1) * 8 + x;
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * width] = out[r];
+            store_row8(job.planes[0] + (size_t)(Y + (b >> 1) * 8 + r) * width + X + (b & 1) * 8, row);
         } else {
             // Split chroma: columns 0..3 go to rows y..y+7, columns 4..7 to rows y+8..y+15.
-            uint8_t* p = job.planes[b == 4 ? 2 : 1] + (size_t)(Y + (x >= 4 ? 8 : 0)) * 180 + X / 4 + (x & 3);
-#pragma unroll
-            for (int r = 0; r < 8; ++r) p[(size_t)r * 180] = out[r];
+            MMR_GLOBAL uint8_t* p = as_global(job.planes[b == 4 ? 2 : 1]) + (size_t)(Y + r) * 180 + X / 4;
+            *reinterpret_cast<MMR_GLOBAL uint32_t*>(p) = (uint32_t)row;
+            *reinterpret_cast<MMR_GLOBAL uint32_t*>(p + 8 * 180) = (uint32_t)(row >> 32);
         }
     }
 }
@@ -233,10 +228,9 @@ hipError_t dv_upload_tables(hipStream_t s) {
     return e;
 }
 
-hipError_t launch_dv(const DvJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s) {
-    if (total_blocks == 0) return hipSuccess;
-    const uint32_t groups = (total_blocks + kBlocksPerGroup - 1) / kBlocksPerGroup;
-    hipLaunchKernelGGL(dv_kernel, dim3(groups), dim3(kGroupThreads), 0, s, d_jobs, job_count, total_blocks);
+hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    hipLaunchKernelGGL(dv_kernel, dim3(tiles), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base);
     return hipGetLastError();
 }
 

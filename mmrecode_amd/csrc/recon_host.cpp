@@ -173,6 +173,22 @@ uint32_t groups_for(mmr_recon_context* ctx, uint32_t per_cu) {
     return ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count * per_cu;
 }
 
+// Appends the tiles of one job (ceil(blocks / 32) entries holding `job`) to a tile -> job table
+// and returns the index of its first tile.
+uint32_t add_tiles(std::vector<uint32_t>& table, uint32_t job, uint32_t blocks) {
+    const uint32_t first = (uint32_t)table.size();
+    table.insert(table.end(), (blocks + 31) / 32, job);
+    return first;
+}
+
+// Appends `bytes` at a 16-byte aligned offset of `blob` and returns that offset.
+size_t blob_append(std::vector<char>& blob, const void* data, size_t bytes) {
+    const size_t off = (blob.size() + 15) & ~size_t(15);
+    blob.resize(off + bytes);
+    if (bytes) memcpy(blob.data() + off, data, bytes);
+    return off;
+}
+
 #define CHECK_CTX(ctx)                                                                 \
     do {                                                                               \
         if (!(ctx)) return fail(MMR_STATUS_INVALID_ARGUMENT, "context is NULL");       \
@@ -284,13 +300,13 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     std::vector<mmr::MjpegJob> jobs(count);
     std::vector<mmr::RgbJob> rgb;
-    uint32_t total = 0, total_quads = 0;
+    std::vector<uint32_t> tiles;
+    uint32_t total_quads = 0;
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t quads = 0;
         mmr_status s = build_mjpeg(frames[i], jobs[i], quads);
         if (s) return s;
-        jobs[i].block_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total;
-        total += jobs[i].block_count;
+        jobs[i].tile_begin = add_tiles(tiles, i, jobs[i].block_count);
         if (quads) {
             mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
             r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
@@ -298,26 +314,29 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
             rgb.push_back(r);
         }
     }
-    std::vector<char> blob(jobs.size() * sizeof(mmr::MjpegJob) + rgb.size() * sizeof(mmr::RgbJob));
-    memcpy(blob.data(), jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
-    if (!rgb.empty()) memcpy(blob.data() + jobs.size() * sizeof(mmr::MjpegJob), rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
+    const size_t rgb_off = blob_append(blob, rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
     Submission sub(ctx, stream);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::MjpegJob*>(sub.jobs_ptr());
-    auto* dr = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + jobs.size() * sizeof(mmr::MjpegJob));
+    const char* base = (const char*)sub.jobs_ptr();
+    auto* dj = reinterpret_cast<const mmr::MjpegJob*>(base + jobs_off);
+    auto* dr = reinterpret_cast<const mmr::RgbJob*>(base + rgb_off);
+    auto* dt = reinterpret_cast<const uint32_t*>(base + tiles_off);
     hipError_t e = hipSuccess;
     if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
         size_t r = 0;
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_mjpeg(dj + i, 1, jobs[i].block_count, stream);
+            e = mmr::launch_mjpeg(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + 31) / 32, stream);
             if (e == hipSuccess && frames[i].rgb) {
                 e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, stream);
                 ++r;
             }
         }
     } else {
-        e = mmr::launch_mjpeg(dj, count, total, stream);
+        e = mmr::launch_mjpeg(dj, dt, 0, (uint32_t)tiles.size(), stream);
         if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, stream);
     }
     if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
@@ -330,7 +349,6 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::Mpeg2Job> jobs(count);
-    uint32_t tiles = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_mpeg2_picture& p = pics[i];
         if (p.struct_size < sizeof(mmr_recon_mpeg2_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
@@ -366,8 +384,6 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         j.alt_scan = p.alternate_scan;
         j.dc_shift = 3 - p.intra_dc_precision;
         j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
-        j.tile_begin = per_frame ? 0 : tiles;
-        tiles += j.tile_count;
     }
     // Dependency levels: a picture runs one launch after the latest picture it references.
     std::vector<uint32_t> level(count, 0);
@@ -379,38 +395,41 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
             levels = std::max(levels, level[i] + 1);
         }
     }
-    // Jobs grouped by level (stable), tile offsets restart at 0 in every level.
+    // Jobs grouped by level (stable); the tiles of one level are contiguous in the tile table.
     std::vector<mmr::Mpeg2Job> ordered;
-    std::vector<uint32_t> level_first(levels + 1, 0), level_tiles(levels, 0);
+    std::vector<uint32_t> tile_table;
+    std::vector<uint32_t> level_tile_first(levels + 1, 0);
     if (per_frame) {
         ordered = jobs;
+        for (uint32_t i = 0; i < count; ++i) ordered[i].tile_begin = add_tiles(tile_table, i, ordered[i].tile_count * 32);
     } else {
         for (uint32_t lv = 0; lv < levels; ++lv) {
-            level_first[lv] = (uint32_t)ordered.size();
+            level_tile_first[lv] = (uint32_t)tile_table.size();
             for (uint32_t i = 0; i < count; ++i)
                 if (level[i] == lv) {
                     mmr::Mpeg2Job j = jobs[i];
-                    j.tile_begin = level_tiles[lv];
-                    level_tiles[lv] += j.tile_count;
+                    j.tile_begin = add_tiles(tile_table, (uint32_t)ordered.size(), j.tile_count * 32);
                     ordered.push_back(j);
                 }
         }
-        level_first[levels] = (uint32_t)ordered.size();
+        level_tile_first[levels] = (uint32_t)tile_table.size();
     }
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, ordered.data(), ordered.size() * sizeof(mmr::Mpeg2Job));
+    const size_t tiles_off = blob_append(blob, tile_table.data(), tile_table.size() * sizeof(uint32_t));
     Submission sub(ctx, stream);
-    const size_t counters = per_frame ? (size_t)count : (size_t)levels;
-    mmr_status s = sub.begin(ordered.data(), ordered.size() * sizeof(mmr::Mpeg2Job), counters);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>(sub.jobs_ptr());
+    auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
-    const uint32_t groups = groups_for(ctx, 8);
     if (per_frame) {
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_mpeg2(dj + i, 1, jobs[i].tile_count, sub.counters_ptr() + i, groups, sub.status_ptr(), stream);
+            e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, sub.status_ptr(), stream);
     } else {
         for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
-            e = mmr::launch_mpeg2(dj + level_first[lv], level_first[lv + 1] - level_first[lv], level_tiles[lv],
-                                  sub.counters_ptr() + lv, groups, sub.status_ptr(), stream);
+            e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv],
+                                  sub.status_ptr(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "MPEG-2 launch");
     return sub.end(out_ticket);
@@ -422,7 +441,7 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::DvJob> jobs(count);
-    uint32_t total = 0;
+    std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_dv_frame& f = frames[i];
         if (f.struct_size < sizeof(mmr_recon_dv_frame)) return fail(MMR_STATUS_INVALID_ARGUMENT, "DV struct_size too small");
@@ -438,18 +457,22 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
         for (int k = 0; k < 3; ++k) j.planes[k] = f.planes[k];
         j.system = f.system;
         j.block_count = (f.system == MMR_DV_SYSTEM_625_50 ? 12u : 10u) * 27u * 5u * 6u;
-        j.block_begin = per_frame ? 0 : total;
-        total += j.block_count;
+        j.tile_begin = add_tiles(tiles, i, j.block_count);
     }
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::DvJob));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
     Submission sub(ctx, stream);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::DvJob), 0);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::DvJob*>(sub.jobs_ptr());
+    auto* dj = reinterpret_cast<const mmr::DvJob*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_dv(dj + i, 1, jobs[i].block_count, stream);
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_dv(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + 31) / 32, stream);
     } else {
-        e = mmr::launch_dv(dj, count, total, stream);
+        e = mmr::launch_dv(dj, dt, 0, (uint32_t)tiles.size(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "DV launch");
     return sub.end(out_ticket);

@@ -18,7 +18,7 @@ struct MjpegJob {
     uint32_t plane_h[4];
     uint32_t mcu_cols;
     uint32_t blocks_per_mcu;      // 1 when not interleaved
-    uint32_t block_begin;         // first global block index of this frame
+    uint32_t tile_begin;          // first entry of this frame in the launch's tile -> job table
     uint32_t block_count;
     uint8_t blk_comp[16];         // per block-in-MCU: component, column, row
     uint8_t blk_bx[16];
@@ -42,7 +42,7 @@ struct Mpeg2Job {
     uint32_t mb_h;
     uint32_t alt_scan;
     uint32_t dc_shift;            // 3 - intra_dc_precision
-    uint32_t tile_begin;          // first work-queue tile of this picture
+    uint32_t tile_begin;          // first entry of this picture in the submission's tile -> job table
     uint32_t tile_count;
 };
 
@@ -52,7 +52,7 @@ struct DvJob {
     const uint8_t* info;
     uint8_t* planes[3];
     uint32_t system;
-    uint32_t block_begin;
+    uint32_t tile_begin;          // first entry of this frame in the submission's tile -> job table
     uint32_t block_count;
     uint32_t pad;
 };
@@ -97,10 +97,12 @@ struct CompositeJobDev {
 };
 
 // Kernel launchers (stream-ordered). Each returns hipSuccess or the launch error.
-hipError_t launch_mjpeg(const MjpegJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s);
-hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_tiles, uint32_t* d_counters,
-                        uint32_t groups, uint32_t* d_status, hipStream_t s);
-hipError_t launch_dv(const DvJob* d_jobs, uint32_t job_count, uint32_t total_blocks, hipStream_t s);
+// The block kernels (MJPEG, MPEG-2, DV) run one workgroup per 32-block tile: workgroup i takes
+// tile tile_base + i, whose job index is d_tile_job[tile]. Tiles never straddle jobs.
+hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
+hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                        uint32_t* d_status, hipStream_t s);
+hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t dv_upload_tables(hipStream_t s);
 hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
