@@ -277,6 +277,256 @@ class H264Workload:
         return mp / dt, len(picks), f"{len(picks)} 1920x1088 pictures, one picture per thread, {dt:.2f}s"
 
 
+# ------------------------------------------------------------------ served mixed workload
+class ServedMix:
+    """Open-loop served workload: Poisson request arrivals over 64 streams (16 per codec), each
+    request a window of 1-30 consecutive frames of its stream, every frame followed by BT.709
+    limited-range YUV->RGBA and a 3-layer premultiplied alpha-over composite at the output size.
+    Inputs are pooled per codec and resident in HBM; every stream owns a ring of 32 output slots, so
+    MPEG-2 streams keep their anchors resident across requests, and H.264 streams reference 4
+    resident pictures per resolution."""
+
+    RING = 32
+    MAX_WINDOW = 30
+    H264_SIZES = ((80, 45), (120, 68), (240, 135))
+
+    def __init__(self, ctx, seed=2024, streams=64, hbm_budget=96 << 30):
+        self.ctx, self.seed, self.hbm_budget = ctx, seed, hbm_budget
+        t = time.time()
+        self.pool = {abi.CODEC_MJPEG: [wl.mjpeg_frame(seed, 1000 + i, rgb=False) for i in range(30)],
+                     abi.CODEC_DV: [wl.dv_frame(seed, 1000 + i, system=i % 2) for i in range(30)],
+                     abi.CODEC_MPEG2: wl.mpeg2_gop(seed, 99)}
+        self.h264_pool = {sz: [wl.h264_picture(seed, 100 + i, *sz) for i in range(4)] for sz in self.H264_SIZES}
+        self.h264_refs = {}
+        for sz in self.H264_SIZES:
+            refs = [dict(zip(("y", "cb", "cr"), wl.h264_reference_planes(seed, sz[0], sz[1], i))) for i in range(4)]
+            self.h264_refs[sz] = [{n: ctx.upload(a).ptr for n, a in r.items()} for r in refs]
+        # pooled inputs, uploaded once: id(unit) -> name -> device pointer
+        self.inputs = {}
+        for units in list(self.pool.values()) + list(self.h264_pool.values()):
+            for u in units:
+                self.inputs[id(u)] = {k: ctx.upload(v).ptr for k, v in u.inputs.items()}
+        self.layers = {}
+        self.streams = []
+        for sid in range(streams):
+            codec = (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)[sid % 4]
+            size = self.H264_SIZES[(sid // 4) % 3] if codec == abi.CODEC_H264 else None
+            self.streams.append({"id": sid + 1, "codec": codec, "size": size, "pos": 0, "ring": {}})
+        log(f"served mix: pools generated and uploaded in {time.time() - t:.1f}s")
+
+    # ---- geometry
+    def _units_of(self, st):
+        return self.h264_pool[st["size"]] if st["codec"] == abi.CODEC_H264 else self.pool[st["codec"]]
+
+    @staticmethod
+    def geometry(codec, u):
+        """(width, height, chroma_w, chroma_h, luma_stride, chroma_stride, plane names) of the output."""
+        if codec == abi.CODEC_MJPEG:
+            w, h = u.meta["width"], u.meta["height"]
+            return w, h, (w + 1) // 2, (h + 1) // 2, w, (w + 1) // 2, ("plane0", "plane1", "plane2")
+        if codec == abi.CODEC_DV:
+            h = u.meta["height"]
+            cw, ch = (360, 288) if u.meta["system"] == abi.DV_SYSTEM_625_50 else (180, 480)
+            return 720, h, cw, ch, 720, cw, ("y", "cb", "cr")
+        if codec == abi.CODEC_MPEG2:
+            w, h = u.meta["width"], u.meta["height"]
+            W = -(-w // 16) * 16
+            return w, h, (w + 1) // 2, (h + 1) // 2, W, W // 2, ("y", "cb", "cr")
+        W, H = u.meta["width_mbs"] * 16, u.meta["height_mbs"] * 16
+        return W, H, W // 2, H // 2, W, W // 2, ("y", "cb", "cr")
+
+    def _layers(self, w, h):
+        if (w, h) not in self.layers:
+            rng = np.random.Generator(np.random.PCG64([self.seed, 77, w, h]))
+            bufs = []
+            for _ in range(3):
+                px = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+                px[..., :3] = (px[..., :3].astype(np.uint32) * px[..., 3:4] // 255).astype(np.uint8)
+                bufs.append(self.ctx.upload(px))
+            self.layers[(w, h)] = bufs
+        return self.layers[(w, h)]
+
+    def _slot(self, st, u, k):
+        """Output buffers of ring slot k of stream st, allocated on first use at the largest size any
+        frame of the stream's pool needs (DV streams mix 720x576 4:2:0 and 720x480 4:1:1 frames)."""
+        if k not in st["ring"]:
+            units = self._units_of(st)
+            need = {}
+            for v in units:
+                for n, (shape, dt, _) in v.outputs.items():
+                    need[n] = max(need.get(n, 0), int(np.prod(shape)) * np.dtype(dt).itemsize)
+                w, h = self.geometry(st["codec"], v)[:2]
+                need["rgba"] = max(need.get("rgba", 0), w * h * 4)
+            st["ring"][k] = {n: self.ctx.alloc(b) for n, b in need.items()}
+        slot = st["ring"][k]
+        for n, (shape, dt, _) in u.outputs.items():
+            assert slot[n].nbytes >= int(np.prod(shape)) * np.dtype(dt).itemsize, (n, shape)
+        w, h = self.geometry(st["codec"], u)[:2]
+        assert slot["rgba"].nbytes >= w * h * 4
+        return slot
+
+    def frame(self, st, p, start):
+        """Descriptor, convert job, composite job, bytes and Mpix of picture p of stream st; `start`
+        is the stream position of the request's first frame."""
+        codec, units = st["codec"], self._units_of(st)
+        u = units[p % len(units)]
+        slot = self._slot(st, u, p % self.RING)
+        ptrs = dict(self.inputs[id(u)])
+        ptrs.update({n: b.ptr for n, b in slot.items() if n != "rgba"})
+        if codec == abi.CODEC_MPEG2:
+            ptrs.update({"forward_source": -1, "backward_source": -1})
+            gop0 = p - p % len(units)
+            for tag, src in (("fwd", "forward_source"), ("bwd", "backward_source")):
+                ref = u.meta[tag]
+                if ref is None:
+                    continue
+                rp = gop0 + ref
+                if rp >= start:
+                    ptrs[src] = rp - start
+                rs = self._slot(st, units[ref], rp % self.RING)
+                for n in ("y", "cb", "cr"):
+                    ptrs[f"{tag}_{n}"] = rs[n].ptr
+        elif codec == abi.CODEC_H264:
+            ptrs.update(wl.h264_ref_ptrs(u, self.h264_refs[st["size"]], "dev"))
+        desc = u.fill(ptrs)
+        w, h, cw, ch, ls, cs, names = self.geometry(codec, u)
+        conv = abi.new(abi.YuvToRgbJob, width=w, height=h, chroma_width=cw, chroma_height=ch, luma_stride=ls,
+                       chroma_stride=cs, range=abi.RANGE_LIMITED, matrix=abi.MATRIX_BT709, rgba=slot["rgba"].ptr)
+        for k, n in enumerate(names):
+            conv.planes[k] = slot[n].ptr
+        comp = abi.new(abi.CompositeJob, target=slot["rgba"].ptr, width=w, height=h, stride=w * 4, layer_count=3)
+        for i, b in enumerate(self._layers(w, h)):
+            comp.layers[i] = abi.Layer(rgba=b.ptr, width=w, height=h, stride=w * 4, x=0, y=0,
+                                       op=abi.BLEND_PREMULTIPLIED_OVER, opacity=255)
+        nbytes = u.input_bytes() + u.output_bytes() + w * h * 4
+        return desc, conv, comp, nbytes, metrics.mpix(w, h)
+
+    def trace(self, rate, seconds, rng):
+        """Arrival times and requests: (t, stream, frames)."""
+        out, t = [], 0.0
+        by_codec = {c: [s for s in self.streams if s["codec"] == c] for c in Server_CODECS}
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t >= seconds:
+                break
+            codec = Server_CODECS[rng.integers(0, 4)]
+            st = by_codec[codec][rng.integers(0, len(by_codec[codec]))]
+            out.append((t, st, int(rng.integers(1, self.MAX_WINDOW + 1))))
+        return out
+
+    def build(self, trace):
+        """Descriptors of every request of a trace (stream positions advance in arrival order)."""
+        reqs = []
+        for t, st, n in trace:
+            start = st["pos"]
+            frames = [self.frame(st, start + i, start) for i in range(n)]
+            st["pos"] += n
+            struct = Server.STRUCTS[st["codec"]]
+            farr = (struct * n)(*[f[0] for f in frames])
+            carr = (abi.YuvToRgbJob * n)(*[f[1] for f in frames])
+            parr = (abi.CompositeJob * n)(*[f[2] for f in frames])
+            req = abi.new(abi.Request, codec=st["codec"], stream_id=st["id"], frame_count=n,
+                          frames=C.addressof(farr), convert=C.addressof(carr), composite=C.addressof(parr),
+                          hbm_bytes=sum(f[3] for f in frames), user_tag=len(reqs))
+            reqs.append({"t": t, "req": req, "keep": (farr, carr, parr), "codec": st["codec"], "n": n,
+                         "mpix": [f[4] for f in frames], "frames": farr, "convert": carr, "composite": parr})
+        return reqs
+
+    def run_served(self, reqs, slots=4):
+        ctx, lib = self.ctx, self.ctx.lib
+        server = mmrecode_amd.Server(ctx, hbm_budget_bytes=self.hbm_budget, slots=slots, max_batch_frames=64,
+                                     max_frames_per_request_per_batch=8)
+        lat, mpix, status_bad = [], 0.0, 0
+        buf = (abi.FrameCompletion * 8192)()
+        n_out = C.c_uint32()
+
+        def poll():
+            nonlocal mpix, status_bad
+            mmrecode_amd.check(lib.mmr_recon_server_poll(server.handle, buf, 8192, C.byref(n_out)))
+            for c in buf[: n_out.value]:
+                if c.status != abi.STATUS_OK:
+                    status_bad += 1
+                lat.append(c.latency_ms)
+                mpix += reqs[c.user_tag]["mpix"][c.frame_index]
+            return n_out.value
+
+        rid = C.c_uint64()
+        t0 = time.perf_counter()
+        try:
+            for r in reqs:
+                while True:
+                    now = time.perf_counter() - t0
+                    if now >= r["t"]:
+                        break
+                    if not poll():
+                        time.sleep(min(2e-4, r["t"] - now))
+                mmrecode_amd.check(lib.mmr_recon_server_enqueue(server.handle, C.byref(r["req"]), C.byref(rid)))
+            server.drain()
+            while poll():
+                pass
+            elapsed = time.perf_counter() - t0
+            stats = server.stats()
+        finally:
+            server.close()
+        return {"mpix_per_s": round(mpix / elapsed, 1), "frames": len(lat), "bad_status": status_bad,
+                "p50_frame_ms": round(percentile(lat, 50), 3), "p99_frame_ms": round(percentile(lat, 99), 3),
+                "wall_s": round(elapsed, 3), "batches": stats["batches_launched"],
+                "peak_admitted_gb": round(stats["peak_admitted_bytes"] / 1e9, 2)}
+
+    def run_plain(self, reqs, limit_s):
+        """The same trace through the plain path: one synchronous launch per frame (decode, then
+        convert, then composite), requests in arrival order; stops after limit_s of wall time."""
+        ctx = self.ctx
+        submit = {abi.CODEC_MJPEG: ctx.submit_mjpeg, abi.CODEC_MPEG2: ctx.submit_mpeg2, abi.CODEC_DV: ctx.submit_dv,
+                  abi.CODEC_H264: ctx.submit_h264}
+        lat, mpix, done_frames, total = [], 0.0, 0, sum(r["n"] for r in reqs)
+        flags = abi.SUBMIT_PER_FRAME_LAUNCH
+        t0 = time.perf_counter()
+        for r in reqs:
+            now = time.perf_counter() - t0
+            if now > limit_s:
+                break
+            if now < r["t"]:
+                time.sleep(r["t"] - now)
+            for i in range(r["n"]):
+                ctx.wait(submit[r["codec"]]([r["frames"][i]], flags))
+                ctx.wait(ctx.submit_yuv_to_rgb([r["convert"][i]], flags))
+                ctx.wait(ctx.submit_composite([r["composite"][i]], flags))
+                lat.append((time.perf_counter() - t0 - r["t"]) * 1e3)
+                mpix += r["mpix"][i]
+                done_frames += 1
+        elapsed = time.perf_counter() - t0
+        return {"mpix_per_s": round(mpix / elapsed, 1), "frames": done_frames, "frames_offered": total,
+                "p50_frame_ms": round(percentile(lat, 50), 3) if lat else None,
+                "p99_frame_ms": round(percentile(lat, 99), 3) if lat else None, "wall_s": round(elapsed, 3)}
+
+
+Server_CODECS = (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)
+Server = mmrecode_amd.Server
+
+
+def run_served_sweep(ctx, rates, seconds, plain_limit):
+    mix = ServedMix(ctx)
+    rng = np.random.default_rng(4242)
+    out = []
+    for rate in rates:
+        reqs = mix.build(mix.trace(rate, seconds, rng))
+        served = mix.run_served(reqs)
+        log(f"served {rate} req/s: {served}")
+        entry = {"rate_req_s": rate, "requests": len(reqs), "served": served}
+        if plain_limit > 0:
+            entry["plain"] = mix.run_plain(reqs, plain_limit)
+            log(f"plain  {rate} req/s: {entry['plain']}")
+            if entry["plain"]["mpix_per_s"]:
+                entry["served_vs_plain"] = round(served["mpix_per_s"] / entry["plain"]["mpix_per_s"], 2)
+        out.append(entry)
+    return {"config": "Poisson arrivals, 64 streams (16 per codec: MJPEG 640x480, MPEG-2 1920x1080 GOP-15, "
+                      "DV 720x576/720x480, H.264 1280x720/1920x1088/3840x2160), windows of 1-30 frames, each frame "
+                      "+ BT.709 limited YUV->RGBA + 3-layer premultiplied alpha-over, 96 GB admission budget, "
+                      "resident references (configs[4])",
+            "seconds_per_rate": seconds, "rates": out}
+
+
 WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload}
 
 
@@ -289,6 +539,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-picture-per-launch leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--served", action="store_true", help="also run the served mixed-workload sweep")
+    ap.add_argument("--served-rates", default="50,200,500,1000,2000")
+    ap.add_argument("--served-seconds", type=float, default=2.0)
+    ap.add_argument("--plain-seconds", type=float, default=4.0,
+                    help="wall-time cap of each plain replay of a served trace (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -336,6 +591,11 @@ def main():
         cpu = {"value": round(v, 3), "unit": "Mpix/s", "cores": cores, "kind": "port", "sample": sample}
         log(f"cpu baseline: {v:.2f} Mpix/s on {cores} threads")
 
+    served = None
+    if args.served:
+        rates = [float(r) for r in args.served_rates.split(",") if r]
+        served = run_served_sweep(ctx, rates, args.served_seconds, args.plain_seconds)
+
     achieved = w.bytes / (k_ms * 1e-3) / 1e9 if k_ms else None
     result = {
         "metric": "reconstructed Mpix/s batched-served vs plain GPU vs CPU; p50/p99 frame latency",
@@ -362,6 +622,7 @@ def main():
                      "traffic": None, "kernel": w.kernel, "kernel_ms": k_ms,
                      "algorithmic_bytes_per_launch": int(w.bytes)},
         "cpu_baseline": cpu,
+        "served": served,
     }
     if w.f64_ops and k_ms:
         result["roofline"]["fp64_dense_tflops_equiv"] = round(w.f64_ops / (k_ms * 1e-3) / 1e12, 3)
