@@ -9,17 +9,21 @@
 // samples of (x-1, y) and (x, y-1), and (x+1, y-1) writes samples that (x, y) reads, so (x, y) may
 // run once (x-1, y) and (x+1, y-1) are done: a wavefront over t = x + 2y.
 //
-// Mapping. One 1024-thread workgroup filters one picture at a time (pictures are taken from a
-// device counter). Wave w owns macroblock rows w, w+16, w+32, ... and walks each row left to right
-// with a private LDS window (20x20 luma, 2 x 12x12 chroma: 4 samples of left / top context). The
-// left context stays in the window (shifted by one macroblock per step); the top context comes
-// from the wave of the row above through an LDS ring: after its step x that wave stores the bottom
-// four sample rows of macroblock x (and completes those of x-1, whose right columns step x
-// modified) and publishes progress x+1; the row below reads slot x once progress >= x+2. All
-// hand-over is LDS + workgroup-scope atomics, so no cross-workgroup synchronisation or cache
-// maintenance is needed. A writer waits before reusing a ring slot until the reader has consumed
-// it (progress of the row below >= x - ring + 2). Waits between adjacent rows cannot form a cycle
-// (ring >= 3); the longer cycle through a wave's two rows is bounded by the ring size (kDbRing).
+// Mapping. Pictures are cut into bands of 16 macroblock rows; a 1024-thread workgroup filters one
+// band at a time (band tickets from a device counter, in picture and band order) and wave w owns
+// row w of the band, walking it left to right with a private LDS window (20x20 luma, 2 x 12x12
+// chroma: 4 samples of left / top context). The left context stays in the window (shifted by one
+// macroblock per step). Inside a band the top context comes from the wave of the row above
+// through an LDS ring: after its step x that wave stores the bottom four sample rows of macroblock
+// x (and completes those of x-1, whose right columns step x modified) and publishes progress x+1;
+// the row below reads slot x once progress >= x+2. A writer waits before reusing a ring slot until
+// the reader has consumed it (progress of the row below >= x - ring + 2); waits between adjacent
+// rows cannot form a cycle (ring >= 4). Across bands the hand-over goes through the picture in
+// global memory: the band's last row writes its bottom rows as well and publishes its progress
+// with an agent-scope release every kDbPublish macroblocks; the next band's first row waits for
+// progress >= x+2 with relaxed loads, then an agent-scope acquire, and reads the four rows above
+// from the picture. Tickets are taken in order, so the band a workgroup waits for is held by a
+// running workgroup (or done), and no band waits for a later one.
 //
 // Write-back. Each row writes its final samples once: luma rows y*16-3 .. y*16+12 (chroma
 // y*8-3 .. y*8+4) of columns x*16-4 .. x*16+11 after step x (the rows below are handed to and
@@ -41,12 +45,11 @@ __constant__ static const int kTc0[3][52] = {
 __constant__ static const int kDbChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
 
 constexpr int kDbWaves = 16;
-// Ring slots per wave. A row can finish at most kDbRing-1 macroblocks ahead of the row below, so
-// while wave w+1 still finishes row y-15 (it owns rows y-15 and y+1), row y-15 can advance at most
-// 16 * (kDbRing - 1) macroblocks; that must cover the picture width for the chain of waits to end:
-// width_mbs <= 16 * (kDbRing - 1) = 496 (checked by the host).
+// Ring slots per wave: a row runs at most kDbRing-1 macroblocks ahead of the row below.
 constexpr int kDbRing = 32;
-constexpr int kDbMaxRows = 256;
+// Macroblocks between two progress publications of a band's last row.
+constexpr int kDbPublish = 8;
+static_assert(kDbWaves == (int)kH264DeblockBand, "one band row per wave");
 
 MMR_DEV int db_chroma_qp(int luma_qp, int offset) {
     const int index = clampi(luma_qp + offset, 0, 51);
@@ -226,12 +229,36 @@ MMR_DEV bool db_wait(uint32_t* counter, uint32_t need, uint32_t* status) {
     return true;
 }
 
-MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status) {
+// Wait for the previous band's published progress (another workgroup): relaxed agent-scope loads,
+// then one agent-scope acquire, so the rows that band wrote before its release are visible. `seen`
+// keeps the last value observed (and acquired).
+MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
+    if (seen >= need) return true;
+    uint32_t spins = 0;
+    for (;;) {
+        seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (seen >= need) break;
+        if (++spins > kDbSpinLimit) {
+            atomicOr(status, kDevStatusTimeout);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+// One macroblock row. band_first: the top context comes from the picture (gprev: the previous
+// band's progress); band_last: no row below in this band (gself: progress for the next band, or
+// null for the picture's last row) -- the row writes its bottom sample rows itself.
+MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status,
+                    bool band_first, bool band_last, uint32_t* gprev, uint32_t* gself) {
     DbWave& S = waves[w];
     const DbWave& Above = waves[(w + kDbWaves - 1) % kDbWaves];
     const int MW = (int)job.width_mbs, MH = (int)job.height_mbs;
     const int W = MW * 16, CW = MW * 8;
-    const bool last_row = y == MH - 1;
+    const bool last_row = band_last;   // writes its own bottom rows
+    uint32_t seen = 0;
     DbPrefetch cur, nxt;
     db_prefetch(job, MW, 0, y, l, cur);
     for (int x = 0; x < MW; ++x) {
@@ -254,8 +281,18 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
         if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
         const int qp_left = x > 0 ? job.mbs[a - 1].qp : 0, qp_top = y > 0 ? job.mbs[a - MW].qp : 0;
         // top context from the row above (slot x), after it finished macroblock x + 1
-        if (y > 0) {
-            if (!db_wait(&prog[y - 1], (uint32_t)min(x + 2, MW), status)) return;
+        if (y > 0 && band_first) {
+            if (!db_wait_band(gprev, (uint32_t)min(x + 2, MW), seen, status)) return;
+            if (l < 16) {
+                *reinterpret_cast<uint32_t*>(&S.ly[(l >> 2) * 20 + 4 + 4 * (l & 3)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
+                    as_global(job.planes[0]) + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+            } else if (l < 32) {
+                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
+                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12 + 4 + 4 * (k & 1)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
+                    as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
+            }
+        } else if (y > 0) {
+            if (!db_wait(&prog[w - 1], (uint32_t)min(x + 2, MW), status)) return;
             const int slot = x % kDbRing;
             if (l < 16) {
                 *reinterpret_cast<uint32_t*>(&S.ly[(l >> 2) * 20 + 4 + 4 * (l & 3)]) =
@@ -313,9 +350,9 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
         }
         // hand the bottom rows to the row below: slot x (all columns) and the right columns of
         // slot x - 1, which this step's left edge modified
-        if (!last_row) {
+        if (!band_last) {
             const int free_at = x - kDbRing + 2;
-            if (free_at > 0 && !db_wait(&prog[y + 1], (uint32_t)free_at, status)) return;
+            if (free_at > 0 && !db_wait(&prog[w + 1], (uint32_t)free_at, status)) return;
             const int slot = x % kDbRing, prev = (x + kDbRing - 1) % kDbRing;
             if (l < 16) {
                 *reinterpret_cast<uint32_t*>(&S.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]) =
@@ -380,7 +417,9 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
             }
         }
         wave_sync();
-        if (l == 0) __hip_atomic_store(&prog[y], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (gself && ((x + 1) % kDbPublish == 0 || x == MW - 1))
+            __hip_atomic_store(gself, (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         // shift the window left by one macroblock (its right 4 columns become the left context)
         if (l < 20) {
             *reinterpret_cast<uint32_t*>(&S.ly[l * 20]) = *reinterpret_cast<const uint32_t*>(&S.ly[l * 20 + 16]);
@@ -394,31 +433,37 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
 }
 
 __global__ __launch_bounds__(1024) void h264_deblock_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
-                                                            uint32_t* __restrict__ counters, uint32_t* __restrict__ status) {
+                                                            uint32_t total_bands, uint32_t* __restrict__ counters,
+                                                            uint32_t* __restrict__ status) {
     __shared__ __align__(16) DbWave waves[kDbWaves];
-    __shared__ uint32_t prog[kDbMaxRows];
-    __shared__ uint32_t s_pic;
+    __shared__ uint32_t prog[kDbWaves];
+    __shared__ uint32_t s_ticket;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     for (;;) {
-        if (threadIdx.x == 0) s_pic = fetch_add_relaxed(&counters[0], 1);
-        for (int i = threadIdx.x; i < kDbMaxRows; i += blockDim.x) prog[i] = 0;
+        if (threadIdx.x == 0) s_ticket = fetch_add_relaxed(&counters[0], 1);
+        if (threadIdx.x < kDbWaves) prog[threadIdx.x] = 0;
         __syncthreads();
-        const uint32_t pic = s_pic;
-        if (pic >= job_count) break;
-        const H264Job& job = jobs[pic];
-        if (job.deblock)
-            for (int y = w; y < (int)job.height_mbs; y += kDbWaves) db_row(job, y, w, l, waves, prog, status);
+        const uint32_t t = s_ticket;
+        if (t >= total_bands) break;
+        const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.band_begin; })];
+        const uint32_t band = t - job.band_begin;
+        const int y = (int)(band * kDbWaves) + w;
+        if (job.deblock && y < (int)job.height_mbs) {
+            const bool first = w == 0, last = w == kDbWaves - 1 || y == (int)job.height_mbs - 1;
+            uint32_t* gprev = first && band > 0 ? &counters[1 + t - 1] : nullptr;
+            uint32_t* gself = last && y < (int)job.height_mbs - 1 ? &counters[1 + t] : nullptr;
+            db_row(job, y, w, l, waves, prog, status, first, last, gprev, gself);
+        }
         __syncthreads();
     }
 }
 
-hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
-                               uint32_t counter_words, uint32_t groups, uint32_t* d_status, hipStream_t s) {
-    (void)counter_words;
-    (void)total_rows;
-    if (job_count == 0) return hipSuccess;
-    if (groups > job_count) groups = job_count;
-    hipLaunchKernelGGL(h264_deblock_kernel, dim3(groups), dim3(kDbWaves * 64), 0, s, d_jobs, job_count, d_counters, d_status);
+hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
+                               uint32_t groups, uint32_t* d_status, hipStream_t s) {
+    if (job_count == 0 || total_bands == 0) return hipSuccess;
+    if (groups > total_bands) groups = total_bands;
+    hipLaunchKernelGGL(h264_deblock_kernel, dim3(groups), dim3(kDbWaves * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
+                       d_status);
     return hipGetLastError();
 }
 

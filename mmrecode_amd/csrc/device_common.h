@@ -162,13 +162,13 @@ MMR_DEV void idct8x8_column_pass(const double* __restrict__ tmp, int x, uint32_t
     }
 }
 
-// round() (half away from zero) of a value with |v| < 2^31, as an int: trunc, then the exact
-// fractional part v - trunc(v) decides the step away from zero.
+// round() (half away from zero) of a value with |v| < 2^31, as an int: v + copysign(h, v) with
+// h the largest double below 0.5, truncated. For |v| < 2^52 the sum rounds to at least n + 1 exactly
+// when |v| >= n + 0.5 and stays below n + 1 otherwise (h is 2^-54 short of 0.5, less than half a
+// unit in the last place of any sum that could cross n + 1), so the truncation is the rounded value.
 MMR_DEV int round_half_away_i32(double v) {
-    const double t = trunc(v);
-    const int i = (int)t;
-    const double frac = dadd(v, -t);
-    return fabs(frac) >= 0.5 ? (v < 0.0 ? i - 1 : i + 1) : i;
+    const double h = __builtin_copysign(0.49999999999999994, v);
+    return (int)dadd(v, h);
 }
 
 MMR_DEV double round_half_away(double v) { return round(v); }

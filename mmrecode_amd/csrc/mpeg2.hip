@@ -16,31 +16,30 @@
 // synchronisation (on MI355X the eight XCDs have separate L2 caches; ordering inside a launch
 // would cost an L2 write-back / invalidate per tile).
 //
-// Per block, lane j issues its loads first (levels of scan positions 8j..8j+7, the quantiser
-// matrix row of natural row j, and the reference samples of block row j: 9 per row and reference,
-// as three aligned dwords when the row segment lies inside the picture). The levels go to natural
-// order in LDS, lane v dequantises natural row v and runs the IDCT row pass, lane x the column
-// pass, the residual goes back to LDS as rows, and lane j forms prediction row j with packed-byte
-// averages, adds residual row j and stores the 8 samples with one 8-byte store. The 8 lanes of a
-// block share a wave, so the LDS exchanges need no workgroup barrier.
+// Per block, lane j issues its loads first (levels of scan positions 8j..8j+7 and the reference
+// samples of block row j: 9 per row and reference, as three aligned dwords when the row segment
+// lies inside the picture). Lane j dequantises its scan positions against a per-wave copy of the
+// picture's quantiser matrices in scan order and scatters them to natural order in LDS; lane v
+// then runs the IDCT row pass on natural row v, lane x the column pass, the residual goes back to
+// LDS as rows, and lane j forms prediction row j with packed-byte averages, adds residual row j and
+// stores the 8 samples with one 8-byte store. The 8 lanes of a block share a wave, so the LDS
+// exchanges need no workgroup barrier.
 #include "device_common.h"
 #include "recon_internal.h"
 
 namespace mmr {
 
-MMR_DEV int div_euclid2(int v) { return v >= 0 ? (v >> 1) : -((-v + 1) >> 1); }
+// Byte-wise rounded averages with v_lerp_u8: lerp(a, b, r) = (a + b + r) >> 1 in every byte, r the
+// low bit of each byte of the third operand.
+constexpr uint32_t kOnes = 0x01010101u;
+MMR_DEV uint32_t avg2_u8x4(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, kOnes); }
 
-constexpr uint64_t kLo7 = 0xfefefefefefefefeull;
-constexpr uint64_t kEven = 0x00ff00ff00ff00ffull;
-
-// (a + b + 1) >> 1 in each byte: a + b = 2(a|b) - (a^b).
-MMR_DEV uint64_t avg2_u8x8(uint64_t a, uint64_t b) { return (a | b) - (((a ^ b) & kLo7) >> 1); }
-
-// (a + b + c + d + 2) >> 2 in each byte, with the even and odd bytes widened to 16-bit fields.
-MMR_DEV uint64_t avg4_u8x8(uint64_t a, uint64_t b, uint64_t c, uint64_t d) {
-    const uint64_t se = (a & kEven) + (b & kEven) + (c & kEven) + (d & kEven) + 0x0002000200020002ull;
-    const uint64_t so = ((a >> 8) & kEven) + ((b >> 8) & kEven) + ((c >> 8) & kEven) + ((d >> 8) & kEven) + 0x0002000200020002ull;
-    return ((se >> 2) & kEven) | (((so >> 2) & kEven) << 8);
+// (a + b + c + d + 2) >> 2 in every byte: with p = (a + b + 1) >> 1 and q = (c + d + 1) >> 1, the
+// result is (p + q + r) >> 1 where r = 1 exactly when a + b and c + d are both even (the two
+// round-ups then did not happen and the final one must).
+MMR_DEV uint32_t avg4_u8x4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    const uint32_t r = ~((a ^ b) | (c ^ d)) & kOnes;
+    return __builtin_amdgcn_lerp(avg2_u8x4(a, b), avg2_u8x4(c, d), r);
 }
 
 // Samples x..x+8 of a reference row (the 9 a half-sample row needs), as three dwords w0..w2 whose
@@ -74,14 +73,13 @@ MMR_DEV RefRow gather_ref_row(const MMR_GLOBAL uint8_t* row, int stride, int x) 
     return r;
 }
 
-// (samples 0..7, samples 1..8) of a row whose sample 0 is byte o of w0.
-MMR_DEV void ref_row_bytes(const RefRow& r, uint32_t o, uint64_t& a, uint64_t& b) {
-    const uint64_t w01 = ((uint64_t)r.w1 << 32) | r.w0;
-    const uint64_t w12 = ((uint64_t)r.w2 << 32) | r.w1;
-    const uint64_t lo = (uint32_t)(w01 >> (8 * o)) | ((uint64_t)(uint32_t)(w12 >> (8 * o)) << 32);
-    const uint64_t hi = (r.w2 >> (8 * o)) & 0xffu;
-    a = lo;
-    b = (lo >> 8) | (hi << 56);
+// Samples 0..7 (a0, a1) and 1..8 (b0, b1) of a row whose sample 0 is byte o of w0.
+MMR_DEV void ref_row_bytes(const RefRow& r, uint32_t o, uint32_t& a0, uint32_t& a1, uint32_t& b0, uint32_t& b1) {
+    a0 = __builtin_amdgcn_alignbyte(r.w1, r.w0, o);
+    a1 = __builtin_amdgcn_alignbyte(r.w2, r.w1, o);
+    const uint32_t s8 = __builtin_amdgcn_alignbyte(r.w2, r.w2, o);   // byte 0: sample 8
+    b0 = __builtin_amdgcn_alignbyte(a1, a0, 1);
+    b1 = __builtin_amdgcn_alignbyte(s8, a1, 1);
 }
 
 struct RefFetch {
@@ -92,26 +90,33 @@ struct RefFetch {
     bool fast, fx, fy;
 };
 
-// Issues the loads of block row `yy` (8 samples from x0) predicted with half-sample vector (mvx, mvy).
-MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int height, int x0, int yy, int mvx, int mvy) {
+// Issues the loads of block row `yy` (8 samples from x0) predicted with half-sample vector
+// (mvx, mvy); halfpel_sample's integer part is floor(v / 2) = v >> 1. With use == false (the
+// direction is not predicted) both dword triples come from the start of the plane. `aligned`: the
+// plane pointers of the picture are 4-byte aligned (strides are multiples of 8).
+MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int height, int x0, int yy, int mvx, int mvy, bool use,
+                           bool aligned) {
     RefFetch f;
-    const int bx = x0 + div_euclid2(mvx), by = yy + div_euclid2(mvy);
+    const int bx = x0 + (mvx >> 1), by = yy + (mvy >> 1);
     f.fx = (mvx & 1) != 0;
     f.fy = (mvy & 1) != 0;
     f.x = bx;
     f.stride = stride;
-    f.fast = bx >= 0 && bx + 8 < stride && ((reinterpret_cast<uintptr_t>(ref) | (uintptr_t)stride) & 3) == 0;
+    f.fast = aligned && bx >= 0 && bx + 8 < stride;
+    const uint32_t o0 = __umul24((uint32_t)clampi(by, 0, height - 1), (uint32_t)stride);
+    const uint32_t o1 = __umul24((uint32_t)clampi(by + (f.fy ? 1 : 0), 0, height - 1), (uint32_t)stride);
     const MMR_GLOBAL uint8_t* base = as_global(ref);
-    f.row0 = base + (size_t)clampi(by, 0, height - 1) * stride;
-    f.row1 = f.fy ? base + (size_t)clampi(by + 1, 0, height - 1) * stride : f.row0;
-    const int off = f.fast ? (bx & ~3) : 0;
-    f.r0 = load_ref_dwords((f.fast ? f.row0 : base) + off);
-    f.r1 = load_ref_dwords((f.fast ? f.row1 : base) + off);
+    f.row0 = base + o0;
+    f.row1 = base + o1;
+    const uint32_t xo = (uint32_t)bx & ~3u;
+    const bool load_rows = use && f.fast;
+    f.r0 = load_ref_dwords(base + (load_rows ? o0 + xo : 0u));
+    f.r1 = load_ref_dwords(base + (load_rows ? o1 + xo : 0u));
     return f;
 }
 
-// halfpel_sample (decoder.rs:1181-1206) for the 8 samples of the fetched row.
-MMR_DEV uint64_t halfpel_row(const RefFetch& f) {
+// halfpel_sample (decoder.rs:1181-1206) for the 8 samples of the fetched row, as two dwords.
+MMR_DEV uint2 halfpel_row(const RefFetch& f) {
     RefRow r0 = f.r0, r1 = f.r1;
     uint32_t o = (uint32_t)(f.x & 3);
     if (!f.fast) {
@@ -119,37 +124,103 @@ MMR_DEV uint64_t halfpel_row(const RefFetch& f) {
         r1 = gather_ref_row(f.row1, f.stride, f.x);
         o = 0;
     }
-    uint64_t a, b;
-    ref_row_bytes(r0, o, a, b);
-    if (!f.fy) return f.fx ? avg2_u8x8(a, b) : a;
-    uint64_t c, d;
-    ref_row_bytes(r1, o, c, d);
-    return f.fx ? avg4_u8x8(a, b, c, d) : avg2_u8x8(a, c);
+    uint32_t a0, a1, b0, b1;
+    ref_row_bytes(r0, o, a0, a1, b0, b1);
+    if (!f.fy) return f.fx ? make_uint2(avg2_u8x4(a0, b0), avg2_u8x4(a1, b1)) : make_uint2(a0, a1);
+    uint32_t c0, c1, d0, d1;
+    ref_row_bytes(r1, o, c0, c1, d0, d1);
+    if (!f.fx) return make_uint2(avg2_u8x4(a0, c0), avg2_u8x4(a1, c1));
+    return make_uint2(avg4_u8x4(a0, b0, c0, d0), avg4_u8x4(a1, b1, c1, d1));
+}
+
+// Block order of one picture: coded blocks first (in macroblock order), then the blocks without
+// residual, so that tiles of coded blocks run the dequantisation and IDCT on every lane and tiles
+// without residual skip them. One 1024-thread workgroup per picture: each thread takes a run of
+// consecutive macroblocks, counts their coded blocks, the workgroup scans the counts, and every
+// thread writes the positions of its blocks. An entry is mb_y << 13 | mb_x << 3 | block. Field
+// prediction (not reconstructed on the device) is reported here.
+constexpr int kOrderThreads = 1024;
+
+__global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2Job* __restrict__ jobs, uint32_t* __restrict__ status) {
+    __shared__ uint32_t s_scan[kOrderThreads];
+    __shared__ uint32_t s_total;
+    const Mpeg2Job& job = jobs[blockIdx.x];
+    const uint32_t mbs = job.mb_w * job.mb_h;
+    const uint32_t per = (mbs + kOrderThreads - 1) / kOrderThreads;
+    const uint32_t first = min(mbs, threadIdx.x * per), last = min(mbs, first + per);
+    // First dword of every macroblock record (flags, cbp, qscale), loaded 8 at a time so the loads
+    // of a chunk are in flight together.
+    const MMR_GLOBAL uint32_t* rec = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs));
+    constexpr uint32_t kRecWords = sizeof(mmr_recon_mpeg2_macroblock) / 4;
+    auto coded_mask = [](uint32_t w) -> uint32_t {   // bit (5 - b) set when block b is coded
+        return (w & MMR_MPEG2_MB_INTRA) ? 0x3fu : ((w >> 8) & 0x3fu);
+    };
+    uint32_t count = 0;
+    bool field = false;
+    for (uint32_t i0 = first; i0 < last; i0 += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = i0 + k < last ? rec[(size_t)(i0 + k) * kRecWords] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            count += (uint32_t)__popc(coded_mask(w[k]));
+            field |= (w[k] & MMR_MPEG2_MB_FIELD_MOTION) != 0;
+        }
+    }
+    if (field) atomicOr(status, kDevStatusUnsupported);
+    // exclusive scan of the counts (Hillis-Steele over 1024 entries)
+    s_scan[threadIdx.x] = count;
+    __syncthreads();
+    for (uint32_t d = 1; d < kOrderThreads; d <<= 1) {
+        const uint32_t v = threadIdx.x >= d ? s_scan[threadIdx.x - d] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (threadIdx.x == kOrderThreads - 1) s_total = s_scan[threadIdx.x];
+    __syncthreads();
+    const uint32_t total_coded = s_total;
+    uint32_t coded_pos = s_scan[threadIdx.x] - count;
+    MMR_GLOBAL uint32_t* order = as_global(job.order);
+    for (uint32_t i0 = first; i0 < last; i0 += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = i0 + k < last ? rec[(size_t)(i0 + k) * kRecWords] : 0u;
+        for (uint32_t k = 0; k < 8 && i0 + k < last; ++k) {
+            const uint32_t i = i0 + k, cbp = coded_mask(w[k]);
+            const uint32_t mb_y = i / job.mb_w, mb_x = i - mb_y * job.mb_w;
+#pragma unroll
+            for (uint32_t b = 0; b < 6; ++b) {
+                const uint32_t entry = (mb_y << 13) | (mb_x << 3) | b;
+                if ((cbp >> (5 - b)) & 1u) {
+                    order[coded_pos++] = entry;
+                } else {
+                    // blocks before this one minus coded blocks before it
+                    order[total_coded + (i * 6 + b) - coded_pos] = entry;
+                }
+            }
+        }
+    }
 }
 
 __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
                                                               const uint32_t* __restrict__ tile_job, uint32_t tile_base,
                                                               uint32_t* __restrict__ status) {
+    // Per block group: natural-order coefficients, then the IDCT row-pass output, then the residual
+    // rows, one after the other in the same LDS area.
     __shared__ __align__(16) double s_tmp[kBlocksPerGroup * kIdctTmpStride];
-    __shared__ __align__(16) int16_t s_blk[kBlocksPerGroup][64];   // levels in natural order, then residual rows
+    __shared__ __align__(16) uint8_t s_wscan[kGroupThreads / 64][256];
     const uint32_t tile = tile_base + blockIdx.x;
     const Mpeg2Job& job = jobs[tile_job[tile]];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const uint32_t mbs = job.mb_w * job.mb_h;
-    // Task (macroblock, block) of this group: the tile's first task is split on the scalar unit,
-    // the per-group offset (0..36) with small-constant arithmetic.
-    const uint32_t task0 = (tile - job.tile_begin) * kBlocksPerGroup;
-    const uint32_t mb0 = task0 / 6, t = task0 % 6 + (uint32_t)g;
-    const uint32_t dmb = (t * 43u) >> 8;   // t / 6 for t < 128
-    const uint32_t task = task0 + (uint32_t)g;
-    const bool active = task < mbs * 6;
-    const uint32_t mb = active ? mb0 + dmb : 0;
-    const int b = active ? (int)(t - dmb * 6) : 0;
-    uint32_t mb_x = mb0 % job.mb_w + (active ? dmb : 0), mb_y = mb0 / job.mb_w;
-    while (mb_x >= job.mb_w) {   // at most 6 steps (a tile spans at most 6 macroblocks)
-        mb_x -= job.mb_w;
-        ++mb_y;
-    }
+    // Block of this group, from the picture's coded-first block order.
+    const uint32_t pos = (tile - job.tile_begin) * kBlocksPerGroup + (uint32_t)g;
+    const bool active = pos < mbs * 6;
+    const uint32_t entry = as_global(job.order)[active ? pos : 0];
+    const uint32_t mb_x = (entry >> 3) & 0x3ffu, mb_y = entry >> 13;
+    const int b = (int)(entry & 7u);
+    const uint32_t mb = __umul24(mb_y, job.mb_w) + mb_x;
     mmr_recon_mpeg2_macroblock m;
     {
         static_assert(sizeof(mmr_recon_mpeg2_macroblock) == 16, "macroblock record is 4 dwords");
@@ -159,23 +230,28 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     }
     const bool intra = (m.flags & MMR_MPEG2_MB_INTRA) != 0;
     const bool coded = active && (intra || ((m.cbp >> (5 - b)) & 1));
-    int16_t* blk = s_blk[g];
     double* tmp = s_tmp + g * kIdctTmpStride;
-    // Field prediction is not reconstructed on the device: flag the submission (the caller falls
-    // back to CPU reconstruction) and predict as frame motion so the kernel still completes.
-    if (active && b == 0 && (m.flags & MMR_MPEG2_MB_FIELD_MOTION) && lane == 0) atomicOr(status, kDevStatusUnsupported);
+    // Field prediction (reported by the order pass) is predicted as frame motion here, so the
+    // kernel still completes.
 
+    // Quantiser matrices in scan order, one copy per wave: wscan[m][p] = W_m[scan[p]] (all blocks
+    // of a workgroup belong to one picture).
+    uint8_t* wscan = s_wscan[threadIdx.x >> 6];
+    {
+        const int p = threadIdx.x & 63;
+        const uint32_t sp = (job.alt_scan ? kAlternateScan : kZigzag)[p];
+        const MMR_GLOBAL uint8_t* Wg = as_global(job.matrices);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) wscan[mi * 64 + p] = Wg[mi * 64 + sp];
+    }
     // Loads, in the order they are consumed: levels (scan positions 8*lane..), the scan positions,
-    // the quantiser matrix row of natural row `lane`, then the reference rows (consumed after the
-    // IDCT). They are unconditional (uncoded blocks read the 256-byte matrix block instead of
-    // levels), so no branch joins force an early wait.
+    // then the reference rows (consumed after the IDCT). They are unconditional (uncoded blocks
+    // read the 256-byte matrix block instead of levels), so no branch joins force an early wait.
     const int component = b < 4 ? 0 : 1;
     const uint32_t index = m.block_offset + (intra ? (uint32_t)b : (uint32_t)__popc((m.cbp & 0x3fu) >> (6 - b)));
     const int16_t* lv = coded ? job.levels + (size_t)index * 64 : reinterpret_cast<const int16_t*>(job.matrices);
     const int4 raw = load_global_int4(lv + lane * 8);
     const uint2 sraw = *reinterpret_cast<const uint2*>((job.alt_scan ? kAlternateScan : kZigzag) + lane * 8);
-    const uint8_t* W = job.matrices + 64 * (intra ? (component ? 2 : 0) : (component ? 3 : 1));
-    const uint2 wrow = load_global_uint2(W + lane * 8);
 
     // Geometry of block b and of the row this lane predicts / stores.
     const bool interlaced = (m.flags & MMR_MPEG2_MB_INTERLACED_DCT) != 0;
@@ -197,57 +273,59 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     if (b >= 4) {  // chroma_motion: v / 2 truncating toward zero
         fmx /= 2; fmy /= 2; bmx /= 2; bmy /= 2;
     }
-    // Unused directions fetch row 0 of their plane (a dummy that hits one cache line).
     const uint8_t* fwd = pick3(job.fwd, plane);
     const uint8_t* bwd = pick3(job.bwd, plane);
-    const RefFetch ff = fetch_ref(fwd, stride, height, x0, has_f ? yy : 0, has_f ? fmx : -2 * x0, has_f ? fmy : 0);
-    const RefFetch fb = fetch_ref(bwd, stride, height, x0, has_b ? yy : 0, has_b ? bmx : -2 * x0, has_b ? bmy : 0);
+    const RefFetch ff = fetch_ref(fwd, stride, height, x0, yy, fmx, fmy, has_f, job.refs_aligned != 0);
+    const RefFetch fb = fetch_ref(bwd, stride, height, x0, yy, bmx, bmy, has_b, job.refs_aligned != 0);
 
-    // Levels to natural order.
+    // Dequantisation in scan order (decode_intra_block / decode_non_intra_block): branch-free, with
+    // 24-bit multiplies. |level| < 2^16, W * qscale < 2^16, so every product is below 2^32 and
+    // equals the reference's i32 product for all inputs where that one does not overflow.
+    wave_lds_sync();   // wscan
+    int32_t* coef = reinterpret_cast<int32_t*>(tmp);   // natural-order coefficients, then tmp, then residual
     if (coded) {
-        const uint8_t* s8 = reinterpret_cast<const uint8_t*>(&sraw);
+        const uint2 wrow = *reinterpret_cast<const uint2*>(wscan + 64 * (intra ? (component ? 2 : 0) : (component ? 3 : 1)) + lane * 8);
         const int16_t* l8 = reinterpret_cast<const int16_t*>(&raw);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) blk[s8[i]] = l8[i];
-    }
-    wave_lds_sync();
-    // Dequantisation of natural row `lane` and the IDCT row pass.
-    bool row_nonzero = false;
-    if (coded) {
-        const int4 rr = *reinterpret_cast<const int4*>(blk + lane * 8);
-        const int16_t* l8 = reinterpret_cast<const int16_t*>(&rr);
-        const uint8_t* w8 = reinterpret_cast<const uint8_t*>(&wrow);
-        const int qscale = m.qscale;
+        const uint8_t* s8 = reinterpret_cast<const uint8_t*>(&sraw);
+        const uint32_t qs = m.qscale, shift = intra ? 4u : 5u;
         int32_t val[8];
         uint32_t x = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int32_t level = l8[i];
-            int32_t v;
-            if (intra && lane == 0 && i == 0) {
-                v = level << job.dc_shift;  // DC: predictor << (3 - intra_dc_precision)
-            } else if (level == 0) {
-                v = 0;
-            } else {
-                const int32_t a = level < 0 ? -level : level;
-                const int32_t w = w8[i];
-                const int32_t mag = intra ? (a * qscale * w) / 16 : ((2 * a + 1) * qscale * w) / 32;
-                v = clampi(level < 0 ? -mag : mag, -2048, 2047);
-            }
+            const uint32_t a = (uint32_t)(level < 0 ? -level : level);
+            const uint32_t w = ((i < 4 ? wrow.x : wrow.y) >> (8 * (i & 3))) & 0xffu;
+            const uint32_t k = intra ? a : 2u * a + 1u;
+            const int32_t mag = (int32_t)(__umul24(k, __umul24(w, qs)) >> shift);
+            int32_t v = clampi(level < 0 ? -mag : mag, -2048, 2047);
+            v = level == 0 ? 0 : v;
+            if (i == 0 && intra && lane == 0) v = level << job.dc_shift;   // DC: predictor << (3 - intra_dc_precision)
             val[i] = v;
             x ^= (uint32_t)v;
         }
-        // Mismatch control: coefficient 63 ^= (1 ^ XOR of all values) & 1 (natural row 7, column 7).
+        // Mismatch control: coefficient 63 ^= (1 ^ XOR of all values) & 1; scan position 63 is
+        // natural 63 in both scans.
         const uint32_t all = group8_xor(x);
         if (lane == 7) val[7] ^= (int32_t)((1u ^ all) & 1u);
-        row_nonzero = (val[0] | val[1] | val[2] | val[3] | val[4] | val[5] | val[6] | val[7]) != 0;
-        idct8x8_row_pass(val, tmp + lane * 8);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) coef[s8[i]] = val[i];
     }
-    const uint32_t rows = wave_nonzero_rows(row_nonzero);
     wave_lds_sync();
+    int32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (coded) {
-        double col[8];
-        idct8x8_column_pass(tmp, lane, rows, col);
+        const int4 lo = *reinterpret_cast<const int4*>(coef + lane * 8);
+        const int4 hi = *reinterpret_cast<const int4*>(coef + lane * 8 + 4);
+        c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w; c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
+    }
+    const uint32_t rows = wave_nonzero_rows((c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) != 0);
+    wave_lds_sync();   // every row is in registers before the row pass overwrites the area
+    if (coded) idct8x8_row_pass(c, tmp + lane * 8);
+    wave_lds_sync();
+    double col[8];
+    if (coded) idct8x8_column_pass(tmp, lane, rows, col);
+    wave_lds_sync();   // the column reads are done before the residual overwrites the area
+    int16_t* blk = reinterpret_cast<int16_t*>(tmp);
+    if (coded) {
         // put_block: residual rounded and clamped to [-256, 255] (transform.rs:90-110).
 #pragma unroll
         for (int r = 0; r < 8; ++r) blk[r * 8 + lane] = (int16_t)clampi(round_half_away_i32(col[r]), -256, 255);
@@ -256,19 +334,20 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     if (!active) return;
 
     // Row `lane` of the block: prediction (0 for intra) + residual, clamped to 0..255.
-    uint64_t pred = 0;
+    uint2 pred = make_uint2(0, 0);
     if (has_f && has_b) {
-        pred = avg2_u8x8(halfpel_row(ff), halfpel_row(fb));
+        const uint2 pf = halfpel_row(ff), pb = halfpel_row(fb);
+        pred = make_uint2(avg2_u8x4(pf.x, pb.x), avg2_u8x4(pf.y, pb.y));
     } else if (has_f) {
         pred = halfpel_row(ff);
     } else if (has_b) {
         pred = halfpel_row(fb);
     }
-    uint64_t out = pred;
+    uint64_t out = (uint64_t)pred.x | ((uint64_t)pred.y << 32);
     if (coded) {   // prediction + residual in 16-bit pairs, clamped to 0..255
         const int4 rr = *reinterpret_cast<const int4*>(blk + lane * 8);
         const uint32_t r32[4] = {(uint32_t)rr.x, (uint32_t)rr.y, (uint32_t)rr.z, (uint32_t)rr.w};
-        const uint32_t p32[2] = {(uint32_t)pred, (uint32_t)(pred >> 32)};
+        const uint32_t p32[2] = {pred.x, pred.y};
         uint32_t o32[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -292,6 +371,12 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     }
     uint8_t* dst = pick3(job.planes, plane);
     store_row8(dst + (size_t)yy * stride + x0, out);
+}
+
+hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t* d_status, hipStream_t s) {
+    if (job_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(mpeg2_order_kernel, dim3(job_count), dim3(kOrderThreads), 0, s, d_jobs, d_status);
+    return hipGetLastError();
 }
 
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -116,15 +117,21 @@ struct Submission {
     char* d_block = nullptr;
     size_t jobs_bytes = 0;
     size_t counter_words = 0;
+    size_t counters_bytes = 0;
 
     Submission(mmr_recon_context* c, hipStream_t s) : ctx(c), stream(s) {}
 
-    mmr_status begin(const void* jobs, size_t bytes, size_t counters) {
+    // Device block: [jobs (uploaded)][counters + status word (zeroed)][scratch (uninitialised)].
+    // `patch` (optional) edits the host copy of the job records once the device block exists, so
+    // records can point into the block's scratch.
+    mmr_status begin(const void* jobs, size_t bytes, size_t counters, size_t scratch_bytes = 0,
+                     const std::function<void(char* host_jobs, char* scratch)>& patch = nullptr) {
         if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
         if (ctx->tickets.size() >= ctx->queue_depth)
             return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
         jobs_bytes = (bytes + 255) & ~size_t(255);
         counter_words = counters + 1;  // + status word
+        counters_bytes = (counter_words * sizeof(uint32_t) + 255) & ~size_t(255);
         rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
         hipError_t e = hipEventCreate(&rec.start);
         if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
@@ -132,8 +139,9 @@ struct Submission {
         if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
         if (e != hipSuccess) return hip_fail(e, "ticket setup");
         *rec.host_status = 0;
-        e = hipMallocAsync((void**)&d_block, jobs_bytes + counter_words * sizeof(uint32_t), stream);
+        e = hipMallocAsync((void**)&d_block, jobs_bytes + counters_bytes + scratch_bytes, stream);
         if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+        if (patch) patch(rec.host_jobs.data(), scratch_ptr());
         if (bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, stream);
         if (e == hipSuccess) e = hipEventRecord(rec.start, stream);
@@ -143,6 +151,7 @@ struct Submission {
     void* jobs_ptr() const { return d_block; }
     uint32_t* counters_ptr() const { return reinterpret_cast<uint32_t*>(d_block + jobs_bytes); }
     uint32_t* status_ptr() const { return counters_ptr() + (counter_words - 1); }
+    char* scratch_ptr() const { return d_block + jobs_bytes + counters_bytes; }
 
     mmr_status end(uint64_t* out_ticket) {
         hipError_t e = hipEventRecord(rec.kernels_done, stream);
@@ -384,6 +393,9 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         j.alt_scan = p.alternate_scan;
         j.dc_shift = 3 - p.intra_dc_precision;
         j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
+        j.refs_aligned = 1;
+        for (int k = 0; k < 3; ++k)
+            if (!aligned(j.fwd[k], 4) || !aligned(j.bwd[k], 4)) j.refs_aligned = 0;
     }
     // Dependency levels: a picture runs one launch after the latest picture it references.
     std::vector<uint32_t> level(count, 0);
@@ -417,16 +429,30 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, ordered.data(), ordered.size() * sizeof(mmr::Mpeg2Job));
     const size_t tiles_off = blob_append(blob, tile_table.data(), tile_table.size() * sizeof(uint32_t));
+    // Scratch: each picture's block order (6 entries per macroblock).
+    std::vector<size_t> order_off(ordered.size());
+    size_t scratch = 0;
+    for (size_t k = 0; k < ordered.size(); ++k) {
+        order_off[k] = scratch;
+        scratch += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * sizeof(uint32_t) + 255) & ~size_t(255);
+    }
     Submission sub(ctx, stream);
-    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
+        auto* hj = reinterpret_cast<mmr::Mpeg2Job*>(host + jobs_off);
+        for (size_t k = 0; k < ordered.size(); ++k) hj[k].order = reinterpret_cast<uint32_t*>(d_scratch + order_off[k]);
+    });
     if (s) return s;
     auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>((const char*)sub.jobs_ptr() + jobs_off);
     auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, sub.status_ptr(), stream);
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            e = mmr::launch_mpeg2_order(dj + i, 1, sub.status_ptr(), stream);
+            if (e == hipSuccess)
+                e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, sub.status_ptr(), stream);
+        }
     } else {
+        e = mmr::launch_mpeg2_order(dj, (uint32_t)ordered.size(), sub.status_ptr(), stream);
         for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
             e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv],
                                   sub.status_ptr(), stream);
@@ -484,7 +510,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::H264Job> jobs(count);
-    uint32_t mbs = 0, rows = 0;
+    uint32_t mbs = 0, bands = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
         if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
@@ -499,8 +525,6 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
         if (!aligned(p.planes[0], 4) || !aligned(p.planes[1], 4) || !aligned(p.planes[2], 4))
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 planes must be 4-byte aligned");
-        if (p.deblock && (p.width_mbs > 496 || p.height_mbs > 256))
-            return fail(MMR_STATUS_UNSUPPORTED, "H.264 deblocking supports up to 496x256 macroblocks (picture %u)", i);
         mmr::H264Job& j = jobs[i];
         memset(&j, 0, sizeof j);
         j.mbs = p.macroblocks;
@@ -530,16 +554,16 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
         j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
         j.mb_begin = per_frame ? 0 : mbs;
-        j.row_begin = per_frame ? 0 : rows;
+        j.band_begin = per_frame ? 0 : bands;
         mbs += p.width_mbs * p.height_mbs;
-        rows += j.deblock ? p.height_mbs : 0;
+        bands += j.deblock ? (p.height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand : 0;
     }
     Submission sub(ctx, stream);
     size_t counters = 0;
     if (per_frame) {
         for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
     } else {
-        counters = 1 + rows;
+        counters = 1 + bands;
     }
     mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::H264Job), counters);
     if (s) return s;
@@ -551,7 +575,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
             e = mmr::launch_h264_mc(dj + i, 1, pics[i].width_mbs * pics[i].height_mbs, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].deblock) {
-                e = mmr::launch_h264_deblock(dj + i, 1, pics[i].height_mbs, c, 1 + pics[i].height_mbs, groups, sub.status_ptr(), stream);
+                const uint32_t nb = (pics[i].height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand;
+                e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, sub.status_ptr(), stream);
                 c += 1 + pics[i].height_mbs;
             } else {
                 c += 1;
@@ -559,7 +584,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         }
     } else {
         e = mmr::launch_h264_mc(dj, count, mbs, sub.status_ptr(), stream);
-        if (e == hipSuccess && rows) e = mmr::launch_h264_deblock(dj, count, rows, sub.counters_ptr(), 1 + rows, groups, sub.status_ptr(), stream);
+        if (e == hipSuccess && bands) e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, sub.status_ptr(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "H.264 launch");
     return sub.end(out_ticket);

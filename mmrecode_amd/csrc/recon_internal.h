@@ -31,6 +31,7 @@ struct MjpegJob {
 // ---------------------------------------------------------------- MPEG-2
 struct Mpeg2Job {
     const uint8_t* matrices;      // [4][64] natural
+    uint32_t* order;              // [6 * macroblocks] coded-first block order (submission scratch)
     const mmr_recon_mpeg2_macroblock* mbs;
     const int16_t* levels;
     uint8_t* planes[3];
@@ -44,6 +45,8 @@ struct Mpeg2Job {
     uint32_t dc_shift;            // 3 - intra_dc_precision
     uint32_t tile_begin;          // first entry of this picture in the submission's tile -> job table
     uint32_t tile_count;
+    uint32_t refs_aligned;        // every reference plane pointer is 4-byte aligned
+    uint32_t pad;
 };
 
 // ---------------------------------------------------------------- DV
@@ -75,7 +78,7 @@ struct H264Job {
     uint32_t deblock;
     int32_t chroma_qp_offset[2];
     uint32_t mb_begin;            // first global macroblock index of this picture
-    uint32_t row_begin;           // first global macroblock-row ticket of this picture
+    uint32_t band_begin;          // first deblocking band ticket of this picture (bands of 16 macroblock rows)
 };
 
 // ---------------------------------------------------------------- colour / blend
@@ -100,13 +103,17 @@ struct CompositeJobDev {
 // The block kernels (MJPEG, MPEG-2, DV) run one workgroup per 32-block tile: workgroup i takes
 // tile tile_base + i, whose job index is d_tile_job[tile]. Tiles never straddle jobs.
 hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
+hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t* d_status, hipStream_t s);
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
                         uint32_t* d_status, hipStream_t s);
 hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t dv_upload_tables(hipStream_t s);
 hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s);
-hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters,
-                               uint32_t counter_words, uint32_t groups, uint32_t* d_status, hipStream_t s);
+// Deblocking in bands of kH264DeblockBand macroblock rows: d_counters[0] is the band ticket,
+// d_counters[1 + t] the progress of band t's last row (zeroed before the launch).
+constexpr uint32_t kH264DeblockBand = 16;
+hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
+                               uint32_t groups, uint32_t* d_status, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);

@@ -129,6 +129,12 @@ def _h264_refs(ctx, wmb, hmb, seed=7):
     dict(wmb=6, hmb=5, weighted=False, deblock=False),
     dict(wmb=7, hmb=6, weighted=True, deblock=True),
     dict(wmb=9, hmb=4, weighted=False, deblock=True),
+    # deblocking bands of 16 macroblock rows: 3 bands (16, 16, 8), hand-over between workgroups
+    dict(wmb=12, hmb=40, weighted=True, deblock=True),
+    # wide picture (ring slots reused), last band of a single row
+    dict(wmb=44, hmb=33, weighted=False, deblock=True),
+    # one launch per picture
+    dict(wmb=10, hmb=34, weighted=True, deblock=True, flags=abi.SUBMIT_PER_FRAME_LAUNCH),
 ])
 def test_h264_pictures_bit_exact(ctx, oracle, case):
     wmb, hmb = case["wmb"], case["hmb"]
@@ -137,7 +143,7 @@ def test_h264_pictures_bit_exact(ctx, oracle, case):
                              deblock=case["deblock"]) for i in range(3)]
     dev = [wl.DeviceUnit(ctx, u, wl.h264_ref_ptrs(u, [{n: b.ptr for n, b in r.items()} for r in dev_refs], "dev"))
            for u in units]
-    ctx.wait(ctx.submit_h264([d.struct for d in dev]))
+    ctx.wait(ctx.submit_h264([d.struct for d in dev], case.get("flags", 0)))
     for i, (u, d) in enumerate(zip(units, dev)):
         want = _oracle_run(oracle, "oracle_h264_reconstruct", u, wl.h264_ref_ptrs(u, host_refs, "host"))
         for n in ("y", "cb", "cr"):
