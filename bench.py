@@ -30,6 +30,10 @@ from mmrecode_amd import abi, metrics  # noqa: E402
 from mmrecode_amd import workloads as wl  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per step of the dominant kernel, measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
+# scaled by the counters' reading for a device-to-device copy of known size (see the summaries in
+# profiles/); absent entries report traffic as null.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -530,6 +534,32 @@ def run_served_sweep(ctx, rates, seconds, plain_limit):
 WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload}
 
 
+def calibrate_copy(nbytes=1 << 30, repeats=5):
+    """Device-to-device copies of a known size (the HIP runtime's copy kernel), for scaling the HBM
+    byte counters of rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE."""
+    ctx = mmrecode_amd.Context(device=0)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.restype = C.c_int
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipDeviceSynchronize.restype = C.c_int
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.write(src, np.ones(1 << 20, np.uint8))
+    hip_memcpy_device_to_device = 3
+    for _ in range(repeats):
+        assert hip.hipMemcpy(C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes, hip_memcpy_device_to_device) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    print(json.dumps({"calibration": "device-to-device copy", "bytes_per_copy": nbytes, "copies": repeats}), flush=True)
+    ctx.close()
+
+
+def measured_traffic(workload):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,11 +570,16 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-picture-per-launch leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--served", action="store_true", help="also run the served mixed-workload sweep")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run device-to-device copies of 1 GiB (HBM byte-counter calibration)")
     ap.add_argument("--served-rates", default="50,200,500,1000,2000")
     ap.add_argument("--served-seconds", type=float, default=2.0)
     ap.add_argument("--plain-seconds", type=float, default=4.0,
                     help="wall-time cap of each plain replay of a served trace (0: skip)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
@@ -624,6 +659,10 @@ def main():
         "cpu_baseline": cpu,
         "served": served,
     }
+    tr = measured_traffic(args.workload)
+    if tr:
+        result["roofline"]["traffic"] = int(tr["hbm_bytes_per_step"])
+        result["roofline"]["traffic_source"] = tr["source"]
     if w.f64_ops and k_ms:
         result["roofline"]["fp64_dense_tflops_equiv"] = round(w.f64_ops / (k_ms * 1e-3) / 1e12, 3)
         result["roofline"]["fp64_peak_tflops"] = FP64_PEAK_TFLOPS

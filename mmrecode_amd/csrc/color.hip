@@ -41,8 +41,9 @@ MMR_DEV void ycbcr_to_rgb(int y, int cb, int cr, uint32_t range, uint32_t matrix
 }
 
 // One thread per 4 horizontally adjacent pixels (12 RGB bytes / 16 RGBA bytes).
-__global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restrict__ jobs, uint32_t job_count, uint32_t total_quads) {
-    const uint32_t gq = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restrict__ jobs, uint32_t job_count, uint32_t total_quads,
+                                                         uint32_t per_xcd) {
+    const uint32_t gq = xcd_tile(blockIdx.x, per_xcd) * 256 + threadIdx.x;
     if (gq >= total_quads) return;
     const uint32_t ji = find_job(jobs, job_count, gq, [](const RgbJob& j) { return j.quad_begin; });
     const RgbJob& j = jobs[ji];
@@ -71,11 +72,22 @@ __global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restric
     }
     const size_t p = (size_t)y * j.width + x0;
     if (j.rgb) {
-        uint8_t* o = j.rgb + p * 3;
-        for (int i = 0; i < n; ++i) {
-            o[i * 3 + 0] = px[i * 4 + 0];
-            o[i * 3 + 1] = px[i * 4 + 1];
-            o[i * 3 + 2] = px[i * 4 + 2];
+        MMR_GLOBAL uint8_t* o = as_global(j.rgb) + p * 3;
+        if (n == 4 && (((uintptr_t)o) & 3) == 0) {   // 12 bytes as three dwords
+            uint32_t d[3];
+            d[0] = px[0] | (px[1] << 8) | (px[2] << 16) | ((uint32_t)px[4] << 24);
+            d[1] = px[5] | (px[6] << 8) | (px[8] << 16) | ((uint32_t)px[9] << 24);
+            d[2] = px[10] | (px[12] << 8) | (px[13] << 16) | ((uint32_t)px[14] << 24);
+            MMR_GLOBAL uint32_t* o32 = reinterpret_cast<MMR_GLOBAL uint32_t*>(o);
+            o32[0] = d[0];
+            o32[1] = d[1];
+            o32[2] = d[2];
+        } else {
+            for (int i = 0; i < n; ++i) {
+                o[i * 3 + 0] = px[i * 4 + 0];
+                o[i * 3 + 1] = px[i * 4 + 1];
+                o[i * 3 + 2] = px[i * 4 + 2];
+            }
         }
     }
     if (j.rgba) {
@@ -116,8 +128,9 @@ MMR_DEV void premultiplied_over(uint32_t d[4], const uint32_t s[4], uint32_t opa
 }
 
 // One thread per target pixel; the layers covering the pixel are applied in order.
-__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, uint32_t job_count, uint32_t total_px) {
-    const uint32_t gp = blockIdx.x * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, uint32_t job_count, uint32_t total_px,
+                                                        uint32_t per_xcd) {
+    const uint32_t gp = xcd_tile(blockIdx.x, per_xcd) * 256 + threadIdx.x;
     if (gp >= total_px) return;
     const uint32_t ji = find_job(jobs, job_count, gp, [](const CompositeJobDev& j) { return j.px_begin; });
     const CompositeJobDev& j = jobs[ji];
@@ -168,13 +181,15 @@ __global__ void fill_zero_kernel(uint32_t* p, uint32_t n) {
 
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s) {
     if (total_quads == 0) return hipSuccess;
-    hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3((total_quads + 255) / 256), dim3(256), 0, s, d_jobs, job_count, total_quads);
+    const uint32_t per = xcd_per((total_quads + 255) / 256);
+    hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_quads, per);
     return hipGetLastError();
 }
 
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s) {
     if (total_px == 0) return hipSuccess;
-    hipLaunchKernelGGL(composite_kernel, dim3((total_px + 255) / 256), dim3(256), 0, s, d_jobs, job_count, total_px);
+    const uint32_t per = xcd_per((total_px + 255) / 256);
+    hipLaunchKernelGGL(composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_px, per);
     return hipGetLastError();
 }
 

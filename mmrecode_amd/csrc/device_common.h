@@ -82,6 +82,16 @@ __constant__ static const uint8_t kAlternateScan[64] = {
 constexpr int kBlocksPerGroup = 32;
 constexpr int kGroupThreads = 256;
 
+// Workgroups are dispatched to the eight XCDs of an MI355X round robin (workgroup i runs on XCD
+// i % 8), and every XCD has its own L2. xcd_tile remaps the workgroup index so that each XCD walks a
+// contiguous range of `per` tiles: neighbouring tiles share reference and output cache lines, which
+// then meet in one L2 instead of being fetched and partially written back by several. A launch of
+// kXcds * per workgroups covers `tiles` tiles when per = ceil(tiles / kXcds); indices past the end
+// are idle.
+constexpr uint32_t kXcds = 8;
+MMR_DEV uint32_t xcd_tile(uint32_t block, uint32_t per) { return (block % kXcds) * per + block / kXcds; }
+inline uint32_t xcd_per(uint32_t tiles) { return (tiles + kXcds - 1) / kXcds; }
+
 // Store 8 bytes (little-endian in v) at p: one 8-byte store when aligned, else two 4-byte stores
 // (p must be 4-byte aligned), else bytes.
 MMR_DEV void store_row8(uint8_t* p, uint64_t v) {

@@ -68,17 +68,20 @@ MMR_DEV void dv_mb_coordinates(uint32_t system, int seq, int slot, int mb, int* 
 }
 
 __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restrict__ jobs,
-                                                           const uint32_t* __restrict__ tile_job, uint32_t tile_base) {
+                                                           const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                           uint32_t tiles, uint32_t per_xcd) {
     __shared__ __align__(16) int32_t coef[kBlocksPerGroup][64];
     __shared__ __align__(16) uint8_t pix[kBlocksPerGroup][64];
     __shared__ double s_cos16[64];
     __shared__ double s_cucv[64];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;   // uniform over the workgroup
     if (threadIdx.x < 64) {
         s_cos16[threadIdx.x] = kDvCos16[threadIdx.x];
         s_cucv[threadIdx.x] = kDvCuCv[threadIdx.x];
     }
-    const uint32_t tile = tile_base + blockIdx.x;
+    const uint32_t tile = tile_base + local;
     const DvJob& job = jobs[tile_job[tile]];
     const uint32_t blk = (tile - job.tile_begin) * kBlocksPerGroup + g;
     const bool active = blk < job.block_count;
@@ -230,7 +233,8 @@ hipError_t dv_upload_tables(hipStream_t s) {
 
 hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
     if (tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(dv_kernel, dim3(tiles), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base);
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(dv_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
     return hipGetLastError();
 }
 

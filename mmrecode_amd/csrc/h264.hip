@@ -224,11 +224,11 @@ struct MbShared {
 };
 
 __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict__ jobs, uint32_t job_count, uint32_t total_mbs,
-                                                       uint32_t* __restrict__ status) {
+                                                       uint32_t per_xcd, uint32_t* __restrict__ status) {
     __shared__ MbShared sh[4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     MbShared& S = sh[w];
-    const uint32_t gmb = blockIdx.x * 4 + w;
+    const uint32_t gmb = xcd_tile(blockIdx.x, per_xcd) * 4 + w;
     const bool active = gmb < total_mbs;
     const uint32_t ji = active ? find_job(jobs, job_count, gmb, [](const H264Job& j) { return j.mb_begin; }) : 0;
     const H264Job& job = jobs[ji];
@@ -399,7 +399,8 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
 
 hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s) {
     if (total_mbs == 0) return hipSuccess;
-    hipLaunchKernelGGL(h264_mb_kernel, dim3((total_mbs + 3) / 4), dim3(256), 0, s, d_jobs, job_count, total_mbs, d_status);
+    const uint32_t per = xcd_per((total_mbs + 3) / 4);
+    hipLaunchKernelGGL(h264_mb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_mbs, per, d_status);
     return hipGetLastError();
 }
 

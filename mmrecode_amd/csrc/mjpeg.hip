@@ -19,11 +19,13 @@ namespace mmr {
 
 __global__ __launch_bounds__(kGroupThreads) void mjpeg_blocks_kernel(const MjpegJob* __restrict__ jobs,
                                                                       const uint32_t* __restrict__ tile_job,
-                                                                      uint32_t tile_base) {
+                                                                      uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
     // Per block group: coefficients (int32, natural order), then the IDCT's row-pass output, then
     // the output samples, one after the other in the same LDS area.
     __shared__ __align__(16) double s_tmp[kBlocksPerGroup * kIdctTmpStride];
-    const uint32_t tile = tile_base + blockIdx.x;
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
     const MjpegJob& job = jobs[tile_job[tile]];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const uint32_t b = (tile - job.tile_begin) * kBlocksPerGroup + g;
@@ -79,7 +81,9 @@ __global__ __launch_bounds__(kGroupThreads) void mjpeg_blocks_kernel(const Mjpeg
 
 hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
     if (tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(mjpeg_blocks_kernel, dim3(tiles), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base);
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(mjpeg_blocks_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles,
+                       per);
     return hipGetLastError();
 }
 

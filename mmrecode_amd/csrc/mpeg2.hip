@@ -133,84 +133,83 @@ MMR_DEV uint2 halfpel_row(const RefFetch& f) {
     return make_uint2(avg4_u8x4(a0, b0, c0, d0), avg4_u8x4(a1, b1, c1, d1));
 }
 
-// Block order of one picture: coded blocks first (in macroblock order), then the blocks without
-// residual, so that tiles of coded blocks run the dequantisation and IDCT on every lane and tiles
-// without residual skip them. One 1024-thread workgroup per picture: each thread takes a run of
-// consecutive macroblocks, counts their coded blocks, the workgroup scans the counts, and every
-// thread writes the positions of its blocks. An entry is mb_y << 13 | mb_x << 3 | block. Field
+// Block order of a picture, row-local: the 6 * mb_w blocks of macroblock row r occupy positions
+// 6 * mb_w * r .. 6 * mb_w * (r + 1) - 1, coded blocks first (in macroblock order), then the blocks
+// without residual, so that tiles of coded blocks run the dequantisation and IDCT on every lane,
+// tiles without residual skip them, and both stay near each other in the picture (and in the tile
+// sequence). One wave per macroblock row; an entry is mb_y << 13 | mb_x << 3 | block. Field
 // prediction (not reconstructed on the device) is reported here.
-constexpr int kOrderThreads = 1024;
+constexpr int kOrderThreads = 256;
 
-__global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2Job* __restrict__ jobs, uint32_t* __restrict__ status) {
-    __shared__ uint32_t s_scan[kOrderThreads];
-    __shared__ uint32_t s_total;
-    const Mpeg2Job& job = jobs[blockIdx.x];
-    const uint32_t mbs = job.mb_w * job.mb_h;
-    const uint32_t per = (mbs + kOrderThreads - 1) / kOrderThreads;
-    const uint32_t first = min(mbs, threadIdx.x * per), last = min(mbs, first + per);
-    // First dword of every macroblock record (flags, cbp, qscale), loaded 8 at a time so the loads
-    // of a chunk are in flight together.
+MMR_DEV uint32_t wave_exclusive_sum(uint32_t v) {
+    const uint32_t lane = __lane_id();
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x - v;
+}
+MMR_DEV uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2Job* __restrict__ jobs, uint32_t job_count,
+                                                                    uint32_t total_rows, uint32_t* __restrict__ status) {
+    const uint32_t grow = blockIdx.x * (kOrderThreads / 64) + (threadIdx.x >> 6);
+    if (grow >= total_rows) return;
+    const Mpeg2Job& job = jobs[find_job(jobs, job_count, grow, [](const Mpeg2Job& j) { return j.row_begin; })];
+    const uint32_t r = grow - job.row_begin, mb_w = job.mb_w, lane = __lane_id();
     const MMR_GLOBAL uint32_t* rec = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs));
     constexpr uint32_t kRecWords = sizeof(mmr_recon_mpeg2_macroblock) / 4;
-    auto coded_mask = [](uint32_t w) -> uint32_t {   // bit (5 - b) set when block b is coded
-        return (w & MMR_MPEG2_MB_INTRA) ? 0x3fu : ((w >> 8) & 0x3fu);
-    };
-    uint32_t count = 0;
+    // first dword of a record: flags | cbp << 8 | qscale << 16; bit (5 - b) set when block b is coded
+    auto coded_mask = [](uint32_t w) -> uint32_t { return (w & MMR_MPEG2_MB_INTRA) ? 0x3fu : ((w >> 8) & 0x3fu); };
+    uint32_t row_coded = 0;
     bool field = false;
-    for (uint32_t i0 = first; i0 < last; i0 += 8) {
-        uint32_t w[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = i0 + k < last ? rec[(size_t)(i0 + k) * kRecWords] : 0u;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            count += (uint32_t)__popc(coded_mask(w[k]));
-            field |= (w[k] & MMR_MPEG2_MB_FIELD_MOTION) != 0;
-        }
+    for (uint32_t c0 = 0; c0 < mb_w; c0 += 64) {
+        const uint32_t x = c0 + lane;
+        const uint32_t w = x < mb_w ? rec[((size_t)r * mb_w + x) * kRecWords] : 0u;
+        row_coded += wave_sum((uint32_t)__popc(coded_mask(w)));
+        field |= (w & MMR_MPEG2_MB_FIELD_MOTION) != 0;
     }
     if (field) atomicOr(status, kDevStatusUnsupported);
-    // exclusive scan of the counts (Hillis-Steele over 1024 entries)
-    s_scan[threadIdx.x] = count;
-    __syncthreads();
-    for (uint32_t d = 1; d < kOrderThreads; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? s_scan[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s_scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    if (threadIdx.x == kOrderThreads - 1) s_total = s_scan[threadIdx.x];
-    __syncthreads();
-    const uint32_t total_coded = s_total;
-    uint32_t coded_pos = s_scan[threadIdx.x] - count;
-    MMR_GLOBAL uint32_t* order = as_global(job.order);
-    for (uint32_t i0 = first; i0 < last; i0 += 8) {
-        uint32_t w[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = i0 + k < last ? rec[(size_t)(i0 + k) * kRecWords] : 0u;
-        for (uint32_t k = 0; k < 8 && i0 + k < last; ++k) {
-            const uint32_t i = i0 + k, cbp = coded_mask(w[k]);
-            const uint32_t mb_y = i / job.mb_w, mb_x = i - mb_y * job.mb_w;
+    MMR_GLOBAL uint32_t* order = as_global(job.order) + (size_t)6 * mb_w * r;
+    uint32_t coded_before = 0;   // coded blocks of the row before this chunk
+    for (uint32_t c0 = 0; c0 < mb_w; c0 += 64) {
+        const uint32_t x = c0 + lane;
+        const uint32_t w = x < mb_w ? rec[((size_t)r * mb_w + x) * kRecWords] : 0u;
+        const uint32_t mask = coded_mask(w), n = (uint32_t)__popc(mask);
+        uint32_t cpos = coded_before + wave_exclusive_sum(n);   // coded blocks before this macroblock
+        if (x < mb_w) {
 #pragma unroll
             for (uint32_t b = 0; b < 6; ++b) {
-                const uint32_t entry = (mb_y << 13) | (mb_x << 3) | b;
-                if ((cbp >> (5 - b)) & 1u) {
-                    order[coded_pos++] = entry;
+                const uint32_t entry = (r << 13) | (x << 3) | b;
+                if ((mask >> (5 - b)) & 1u) {
+                    order[cpos++] = entry;
                 } else {
-                    // blocks before this one minus coded blocks before it
-                    order[total_coded + (i * 6 + b) - coded_pos] = entry;
+                    // blocks of the row before this one minus coded blocks before it
+                    order[row_coded + (x * 6 + b) - cpos] = entry;
                 }
             }
         }
+        coded_before += wave_sum(n);
     }
 }
 
 __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
                                                               const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                              uint32_t tiles, uint32_t per_xcd,
                                                               uint32_t* __restrict__ status) {
     // Per block group: natural-order coefficients, then the IDCT row-pass output, then the residual
     // rows, one after the other in the same LDS area.
     __shared__ __align__(16) double s_tmp[kBlocksPerGroup * kIdctTmpStride];
     __shared__ __align__(16) uint8_t s_wscan[kGroupThreads / 64][256];
-    const uint32_t tile = tile_base + blockIdx.x;
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
     const Mpeg2Job& job = jobs[tile_job[tile]];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const uint32_t mbs = job.mb_w * job.mb_h;
@@ -373,16 +372,21 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     store_row8(dst + (size_t)yy * stride + x0, out);
 }
 
-hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t* d_status, hipStream_t s) {
-    if (job_count == 0) return hipSuccess;
-    hipLaunchKernelGGL(mpeg2_order_kernel, dim3(job_count), dim3(kOrderThreads), 0, s, d_jobs, d_status);
+hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_status,
+                              hipStream_t s) {
+    if (job_count == 0 || total_rows == 0) return hipSuccess;
+    const uint32_t waves = kOrderThreads / 64;
+    hipLaunchKernelGGL(mpeg2_order_kernel, dim3((total_rows + waves - 1) / waves), dim3(kOrderThreads), 0, s, d_jobs, job_count,
+                       total_rows, d_status);
     return hipGetLastError();
 }
 
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
                         uint32_t* d_status, hipStream_t s) {
     if (tiles == 0) return hipSuccess;
-    hipLaunchKernelGGL(mpeg2_kernel, dim3(tiles), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, d_status);
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(mpeg2_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles, per,
+                       d_status);
     return hipGetLastError();
 }
 

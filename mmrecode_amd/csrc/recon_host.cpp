@@ -432,14 +432,20 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     // Scratch: each picture's block order (6 entries per macroblock).
     std::vector<size_t> order_off(ordered.size());
     size_t scratch = 0;
+    uint32_t order_rows = 0;
     for (size_t k = 0; k < ordered.size(); ++k) {
         order_off[k] = scratch;
         scratch += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * sizeof(uint32_t) + 255) & ~size_t(255);
+        ordered[k].row_begin = per_frame ? 0 : order_rows;
+        order_rows += ordered[k].mb_h;
     }
     Submission sub(ctx, stream);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
         auto* hj = reinterpret_cast<mmr::Mpeg2Job*>(host + jobs_off);
-        for (size_t k = 0; k < ordered.size(); ++k) hj[k].order = reinterpret_cast<uint32_t*>(d_scratch + order_off[k]);
+        for (size_t k = 0; k < ordered.size(); ++k) {
+            hj[k].order = reinterpret_cast<uint32_t*>(d_scratch + order_off[k]);
+            hj[k].row_begin = ordered[k].row_begin;
+        }
     });
     if (s) return s;
     auto* dj = reinterpret_cast<const mmr::Mpeg2Job*>((const char*)sub.jobs_ptr() + jobs_off);
@@ -447,12 +453,12 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     hipError_t e = hipSuccess;
     if (per_frame) {
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_mpeg2_order(dj + i, 1, sub.status_ptr(), stream);
+            e = mmr::launch_mpeg2_order(dj + i, 1, ordered[i].mb_h, sub.status_ptr(), stream);
             if (e == hipSuccess)
                 e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, sub.status_ptr(), stream);
         }
     } else {
-        e = mmr::launch_mpeg2_order(dj, (uint32_t)ordered.size(), sub.status_ptr(), stream);
+        e = mmr::launch_mpeg2_order(dj, (uint32_t)ordered.size(), order_rows, sub.status_ptr(), stream);
         for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
             e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv],
                                   sub.status_ptr(), stream);

@@ -46,7 +46,7 @@ struct Mpeg2Job {
     uint32_t tile_begin;          // first entry of this picture in the submission's tile -> job table
     uint32_t tile_count;
     uint32_t refs_aligned;        // every reference plane pointer is 4-byte aligned
-    uint32_t pad;
+    uint32_t row_begin;           // first macroblock row of this picture in the order pass's row space
 };
 
 // ---------------------------------------------------------------- DV
@@ -103,7 +103,8 @@ struct CompositeJobDev {
 // The block kernels (MJPEG, MPEG-2, DV) run one workgroup per 32-block tile: workgroup i takes
 // tile tile_base + i, whose job index is d_tile_job[tile]. Tiles never straddle jobs.
 hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
-hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t* d_status, hipStream_t s);
+hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_status,
+                              hipStream_t s);
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
                         uint32_t* d_status, hipStream_t s);
 hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
