@@ -40,12 +40,12 @@ MMR_DEV int chroma_qp(int luma_qp, int offset) {
     return index < 30 ? index : kChromaQp[index - 30];
 }
 
-MMR_DEV int32_t level_scale_4x4(int qp, int row, int col, const uint8_t* scaling) {
+MMR_DEV int32_t level_scale_4x4(int qp, int row, int col, const MMR_GLOBAL uint8_t* scaling) {
     const int cat = ((row & 1) == 0 && (col & 1) == 0) ? 0 : (((row & 1) && (col & 1)) ? 1 : 2);
     return kNorm4[qp % 6][cat] * (int32_t)scaling[row * 4 + col];
 }
 
-MMR_DEV int32_t level_scale_8x8(int qp, int row, int col, const uint8_t* scaling) {
+MMR_DEV int32_t level_scale_8x8(int qp, int row, int col, const MMR_GLOBAL uint8_t* scaling) {
     const int rm = row & 3, cm = col & 3;
     const bool re = (rm & 1) == 0, ce = (cm & 1) == 0;
     int cat;
@@ -57,26 +57,29 @@ MMR_DEV int32_t level_scale_8x8(int qp, int row, int col, const uint8_t* scaling
 
 MMR_DEV int clip255(int v) { return clampi(v, 0, 255); }
 
-// transform_residual_4x4 (decoder.rs:14185-14233) on one lane; out in raster order.
-MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const uint8_t* scaling, int32_t out[16]) {
-    int64_t s[16];
+// transform_residual_4x4 (decoder.rs:14185-14233) on one lane; out in raster order. T is int64_t
+// (the reference's arithmetic) or int32_t when the macroblock's levels are small enough that no
+// intermediate can leave the int32 range (then both give the same integers).
+template <typename T>
+MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const MMR_GLOBAL uint8_t* scaling, int32_t out[16]) {
+    T s[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
         const int rc = kZz4[i];
-        const int64_t v = lv[i];
+        const T v = lv[i];
         if (i == 0 && dc_scaled) {
             s[rc] = v;
         } else {
-            const int64_t scale = level_scale_4x4(qp, rc >> 2, rc & 3, scaling);
-            s[rc] = qp >= 24 ? (v * scale) << (qp / 6 - 4) : (v * scale + ((int64_t)1 << (3 - qp / 6))) >> (4 - qp / 6);
+            const T scale = level_scale_4x4(qp, rc >> 2, rc & 3, scaling);
+            s[rc] = qp >= 24 ? (v * scale) << (qp / 6 - 4) : (v * scale + ((T)1 << (3 - qp / 6))) >> (4 - qp / 6);
         }
     }
-    int64_t h[16];
+    T h[16];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int b = r * 4;
-        const int64_t e0 = s[b] + s[b + 2], e1 = s[b] - s[b + 2];
-        const int64_t e2 = (s[b + 1] >> 1) - s[b + 3], e3 = s[b + 1] + (s[b + 3] >> 1);
+        const T e0 = s[b] + s[b + 2], e1 = s[b] - s[b + 2];
+        const T e2 = (s[b + 1] >> 1) - s[b + 3], e3 = s[b + 1] + (s[b + 3] >> 1);
         h[b] = e0 + e3;
         h[b + 1] = e1 + e2;
         h[b + 2] = e1 - e2;
@@ -84,8 +87,8 @@ MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const uint8
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const int64_t g0 = h[c] + h[8 + c], g1 = h[c] - h[8 + c];
-        const int64_t g2 = (h[4 + c] >> 1) - h[12 + c], g3 = h[4 + c] + (h[12 + c] >> 1);
+        const T g0 = h[c] + h[8 + c], g1 = h[c] - h[8 + c];
+        const T g2 = (h[4 + c] >> 1) - h[12 + c], g3 = h[4 + c] + (h[12 + c] >> 1);
         out[0 * 4 + c] = (int32_t)((g0 + g3 + 32) >> 6);
         out[1 * 4 + c] = (int32_t)((g1 + g2 + 32) >> 6);
         out[2 * 4 + c] = (int32_t)((g1 - g2 + 32) >> 6);
@@ -94,15 +97,16 @@ MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const uint8
 }
 
 // inverse_transform_8 (decoder.rs:14280-14308).
-MMR_DEV void itrans8(const int64_t in[8], int64_t out[8]) {
-    const int64_t a0 = in[0] + in[4], a2 = in[0] - in[4];
-    const int64_t a4 = (in[2] >> 1) - in[6], a6 = in[2] + (in[6] >> 1);
-    const int64_t b0 = a0 + a6, b2 = a2 + a4, b4 = a2 - a4, b6 = a0 - a6;
-    const int64_t a1 = -in[3] + in[5] - in[7] - (in[7] >> 1);
-    const int64_t a3 = in[1] + in[7] - in[3] - (in[3] >> 1);
-    const int64_t a5 = -in[1] + in[7] + in[5] + (in[5] >> 1);
-    const int64_t a7 = in[3] + in[5] + in[1] + (in[1] >> 1);
-    const int64_t b1 = a1 + (a7 >> 2), b7 = a7 - (a1 >> 2), b3 = a3 + (a5 >> 2), b5 = (a3 >> 2) - a5;
+template <typename T>
+MMR_DEV void itrans8(const T in[8], T out[8]) {
+    const T a0 = in[0] + in[4], a2 = in[0] - in[4];
+    const T a4 = (in[2] >> 1) - in[6], a6 = in[2] + (in[6] >> 1);
+    const T b0 = a0 + a6, b2 = a2 + a4, b4 = a2 - a4, b6 = a0 - a6;
+    const T a1 = -in[3] + in[5] - in[7] - (in[7] >> 1);
+    const T a3 = in[1] + in[7] - in[3] - (in[3] >> 1);
+    const T a5 = -in[1] + in[7] + in[5] + (in[5] >> 1);
+    const T a7 = in[3] + in[5] + in[1] + (in[1] >> 1);
+    const T b1 = a1 + (a7 >> 2), b7 = a7 - (a1 >> 2), b3 = a3 + (a5 >> 2), b5 = (a3 >> 2) - a5;
     out[0] = b0 + b7;
     out[1] = b2 + b5;
     out[2] = b4 + b3;
@@ -118,25 +122,47 @@ MMR_DEV int div8(int v) { return v >> 3; }
 
 MMR_DEV uint32_t avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
 
+// Samples x-2 .. x+6 of reference row y (reference_sample clamps both coordinates into the
+// plane). When the 9 samples lie inside the row and the plane is 4-byte aligned they come from the
+// three aligned dwords that hold them; otherwise from per-sample clamped loads.
+MMR_DEV void load_window_row(const MMR_GLOBAL uint8_t* __restrict__ p, int W, int rowoff, int x, bool fast, int win[9]) {
+    if (fast) {
+        const int c0 = x - 2;
+        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(p + rowoff + (c0 & ~3));
+        const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], o = (uint32_t)(c0 & 3);
+        const uint32_t a0 = __builtin_amdgcn_alignbyte(w1, w0, o), a1 = __builtin_amdgcn_alignbyte(w2, w1, o);
+        const uint32_t a2 = __builtin_amdgcn_alignbyte(w2, w2, o);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            win[i] = (a0 >> (8 * i)) & 0xff;
+            win[4 + i] = (a1 >> (8 * i)) & 0xff;
+        }
+        win[8] = a2 & 0xff;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) win[i] = p[rowoff + clampi(x - 2 + i, 0, W - 1)];
+    }
+}
+
 // Four consecutive luma samples (x_q4 + 4c, y_q4), c = 0..3, of luma_qpel (decoder.rs:13770-13829)
 // with reference_sample's edge clamp. The 6x9 integer window around the row is loaded once and
 // the half-sample terms are formed from registers exactly as the reference forms them.
-MMR_DEV void luma_qpel_row4(const uint8_t* __restrict__ p, int W, int H, int x_q4, int y_q4, uint32_t out[4]) {
+MMR_DEV void luma_qpel_row4(const uint8_t* pg, int W, int H, int x_q4, int y_q4, bool aligned, uint32_t out[4]) {
+    const MMR_GLOBAL uint8_t* p = as_global(pg);
     const int x = div4(x_q4), y = div4(y_q4), fx = x_q4 & 3, fy = y_q4 & 3;
-    int cols[9], rows[6];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) cols[i] = clampi(x - 2 + i, 0, W - 1);
+    const bool fast = aligned && x >= 2 && x + 6 < W;
+    int rows[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) rows[j] = clampi(y - 2 + j, 0, H - 1) * W;
+    int win[6][9];
     if (fx == 0 && fy == 0) {
+        load_window_row(p, W, rows[2], x, fast, win[2]);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = p[rows[2] + cols[c + 2]];
+        for (int c = 0; c < 4; ++c) out[c] = win[2][c + 2];
         return;
     }
-    int win[6][9];
     if (fy == 0) {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) win[2][i] = p[rows[2] + cols[i]];
+        load_window_row(p, W, rows[2], x, fast, win[2]);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int raw = win[2][c] - 5 * win[2][c + 1] + 20 * win[2][c + 2] + 20 * win[2][c + 3] - 5 * win[2][c + 4] + win[2][c + 5];
@@ -145,11 +171,9 @@ MMR_DEV void luma_qpel_row4(const uint8_t* __restrict__ p, int W, int H, int x_q
         }
         return;
     }
+#pragma unroll
+    for (int j = 0; j < 6; ++j) load_window_row(p, W, rows[j], x, fast, win[j]);
     if (fx == 0) {
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-#pragma unroll
-            for (int i = 2; i < 6; ++i) win[j][i] = p[rows[j] + cols[i]];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int i = c + 2;
@@ -159,10 +183,6 @@ MMR_DEV void luma_qpel_row4(const uint8_t* __restrict__ p, int W, int H, int x_q
         }
         return;
     }
-#pragma unroll
-    for (int j = 0; j < 6; ++j)
-#pragma unroll
-        for (int i = 0; i < 9; ++i) win[j][i] = p[rows[j] + cols[i]];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         int rawh[6];
@@ -192,7 +212,8 @@ MMR_DEV void luma_qpel_row4(const uint8_t* __restrict__ p, int W, int H, int x_q
 }
 
 // chroma_epel (decoder.rs:13851-13886).
-MMR_DEV uint32_t chroma_epel(const uint8_t* __restrict__ p, int W, int H, int x_q8, int y_q8) {
+MMR_DEV uint32_t chroma_epel(const uint8_t* pg, int W, int H, int x_q8, int y_q8) {
+    const MMR_GLOBAL uint8_t* p = as_global(pg);
     const int ix = div8(x_q8), iy = div8(y_q8), fx = x_q8 & 7, fy = y_q8 & 7;
     const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
     const int y0 = clampi(iy, 0, H - 1) * W, y1 = clampi(iy + 1, 0, H - 1) * W;
@@ -207,7 +228,7 @@ MMR_DEV uint32_t weight1(uint32_t s, int d, int w, int o) {
     return (uint32_t)clip255(scaled + o);
 }
 // apply_bi_prediction_weight (decoder.rs:13910-13926) / average_samples (:3573-3576).
-MMR_DEV uint32_t weight2(uint32_t s0, uint32_t s1, const mmr_recon_h264_bi_weight* bw, bool weighted) {
+MMR_DEV uint32_t weight2(uint32_t s0, uint32_t s1, const MMR_GLOBAL mmr_recon_h264_bi_weight* bw, bool weighted) {
     if (!weighted) return (s0 + s1 + 1) >> 1;
     const int d = bw->log2_denom;
     const int v = (bw->w0 * (int)s0 + bw->w1 * (int)s1 + (1 << d)) >> (d + 1);
@@ -223,18 +244,107 @@ struct MbShared {
     int32_t dc[2][4];
 };
 
-__global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict__ jobs, uint32_t job_count, uint32_t total_mbs,
-                                                       uint32_t per_xcd, uint32_t* __restrict__ status) {
+// Residual of one macroblock (phases 2-4 of the kernel) into S.res_y / S.res_c, in integer type T.
+template <typename T>
+MMR_DEV void residual_phases(MbShared& S, const MMR_GLOBAL int16_t* lv, int qp, int cqp, bool t8, int l, const H264Job& job) {
+    // ---- phase 2
+    {
+        if (!t8) {
+            if (l < 16) {
+                int32_t lev[16], res[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) lev[i] = lv[l * 16 + i];
+                residual_4x4<T>(lev, qp, false, as_global(job.scaling) + 3 * 16, res);
+                const int kbx = ((l >> 2) & 1) * 2 + (l & 1), kby = (l >> 3) * 2 + ((l >> 1) & 1);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) S.res_y[(kby * 4 + (i >> 2)) * 16 + kbx * 4 + (i & 3)] = res[i];
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int idx = l * 4 + t, blk8 = idx >> 6, i = idx & 63, rc = kZz8[i];
+                const T v = (T)lv[idx] * (T)level_scale_8x8(qp, rc >> 3, rc & 7, as_global(job.scaling) + 96 + 64);
+                const int shift = qp / 6;
+                T sv = shift >= 6 ? v << (shift - 6) : (v + ((T)1 << (5 - shift))) >> (6 - shift);
+                if (rc == 0) sv += 32;
+                S.s8[blk8][rc] = sv;
+            }
+        }
+        const int dcl = t8 ? l : l - 16;
+        if (dcl == 0 || dcl == 1) {
+            // transform_chroma_dc (decoder.rs:14114-14140) with inter list 4 + component.
+            const int comp = dcl;
+            const T c0 = lv[256 + comp * 4], c1 = lv[257 + comp * 4], c2 = lv[258 + comp * 4], c3 = lv[259 + comp * 4];
+            const T f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
+            const T scale = level_scale_4x4(cqp, 0, 0, as_global(job.scaling) + (4 + comp) * 16);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) S.dc[comp][i] = (int32_t)(((f[i] * scale) << (cqp / 6)) >> 5);
+        }
+    }
+    wave_lds_sync();
+
+    // ---- phase 3
+    {
+        if (t8 && l < 32) {
+            const int blk8 = l >> 3, r = l & 7;
+            T in[8], out[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][r * 8 + i];
+            itrans8(in, out);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) S.s8[blk8][r * 8 + i] = out[i];
+        }
+        if (l >= 32 && l < 40) {
+            const int comp = (l - 32) >> 2, b = (l - 32) & 3;
+            int32_t co[16], res[16];
+            co[0] = S.dc[comp][b];
+#pragma unroll
+            for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
+            residual_4x4<T>(co, cqp, true, as_global(job.scaling) + (4 + comp) * 16, res);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) S.res_c[comp][((b >> 1) * 4 + (i >> 2)) * 8 + (b & 1) * 4 + (i & 3)] = res[i];
+        }
+    }
+    wave_lds_sync();
+
+    // ---- phase 4
+    if (t8 && l < 32) {
+        const int blk8 = l >> 3, c = l & 7;
+        T in[8], out[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][i * 8 + c];
+        itrans8(in, out);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            S.res_y[((blk8 >> 1) * 8 + i) * 16 + (blk8 & 1) * 8 + c] = (int32_t)(out[i] >> 6);
+    }
+    wave_lds_sync();
+
+}
+
+__global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                                       uint32_t tile_base, uint32_t tiles, uint32_t per_xcd,
+                                                       uint32_t* __restrict__ status) {
     __shared__ MbShared sh[4];
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     MbShared& S = sh[w];
-    const uint32_t gmb = xcd_tile(blockIdx.x, per_xcd) * 4 + w;
-    const bool active = gmb < total_mbs;
-    const uint32_t ji = active ? find_job(jobs, job_count, gmb, [](const H264Job& j) { return j.mb_begin; }) : 0;
-    const H264Job& job = jobs[ji];
-    const uint32_t mb = active ? gmb - job.mb_begin : 0;
-    const mmr_recon_h264_macroblock m = job.mbs[mb];
-    const int mbx = (int)(mb % job.width_mbs), mby = (int)(mb / job.width_mbs);
+    // Tiles of 4 macroblocks (one per wave) that never straddle pictures; tile -> picture table.
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;   // uniform over the workgroup
+    const uint32_t tile = tile_base + local;
+    const H264Job& job = jobs[tile_job[tile]];
+    const uint32_t mbs = job.width_mbs * job.height_mbs;
+    const uint32_t mb_raw = (tile - job.tile_begin) * 4 + (uint32_t)w;
+    const bool active = mb_raw < mbs;
+    const uint32_t mb = active ? mb_raw : 0;
+    mmr_recon_h264_macroblock m;
+    {
+        static_assert(sizeof(mmr_recon_h264_macroblock) == 12, "macroblock record is 3 dwords");
+        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb));
+        const uint32_t wd[3] = {q[0], q[1], q[2]};
+        __builtin_memcpy(&m, wd, sizeof m);
+    }
+    const int mby = (int)(mb / job.width_mbs), mbx = (int)(mb - (uint32_t)mby * job.width_mbs);
     const int W = (int)job.width_mbs * 16, H = (int)job.height_mbs * 16, CW = W / 2, CH = H / 2;
     const bool weighted = job.weighted == MMR_H264_WEIGHTED_TABLE;
 
@@ -242,7 +352,13 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
     const int k = l >> 2;                               // H.264 block index
     const int bx = ((k >> 2) & 1) * 2 + (k & 1), by = (k >> 3) * 2 + ((k >> 1) & 1);  // luma_block_position
     if (active) {
-        const mmr_recon_h264_block blk = job.blocks[(size_t)mb * 16 + k];
+        mmr_recon_h264_block blk;
+        {
+            static_assert(sizeof(mmr_recon_h264_block) == 12, "block record is 3 dwords");
+            const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.blocks + (size_t)mb * 16 + k));
+            const uint32_t wd[3] = {q[0], q[1], q[2]};
+            __builtin_memcpy(&blk, wd, sizeof blk);
+        }
         int r0 = blk.ref_idx[0], r1 = blk.ref_idx[1];
         // Intra macroblocks are not predicted on the device; out-of-range reference indices are
         // invalid data. Both are reported through the status word and clamped so the kernel
@@ -259,9 +375,9 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
         uint32_t cs[2];
         if (r0 >= 0 && r1 >= 0) {
             uint32_t a[4], b[4];
-            luma_qpel_row4(job.ref_planes[0][r0][0], W, H, ox * 4 + blk.mv[0][0], oy * 4 + blk.mv[0][1], a);
-            luma_qpel_row4(job.ref_planes[1][r1][0], W, H, ox * 4 + blk.mv[1][0], oy * 4 + blk.mv[1][1], b);
-            const mmr_recon_h264_bi_weight* bw = weighted ? job.bi_weights + ((size_t)r0 * MMR_H264_MAX_REFS + r1) * 3 : nullptr;
+            luma_qpel_row4(job.ref_planes[0][r0][0], W, H, ox * 4 + blk.mv[0][0], oy * 4 + blk.mv[0][1], job.refs_aligned, a);
+            luma_qpel_row4(job.ref_planes[1][r1][0], W, H, ox * 4 + blk.mv[1][0], oy * 4 + blk.mv[1][1], job.refs_aligned, b);
+            const MMR_GLOBAL mmr_recon_h264_bi_weight* bw = weighted ? as_global(job.bi_weights) + ((size_t)r0 * MMR_H264_MAX_REFS + r1) * 3 : nullptr;
 #pragma unroll
             for (int c = 0; c < 4; ++c) ys[c] = weight2(a[c], b[c], bw, weighted);
 #pragma unroll
@@ -274,8 +390,8 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
             const int list = r0 >= 0 ? 0 : 1, r = r0 >= 0 ? r0 : r1;
             const int mvx = blk.mv[list][0], mvy = blk.mv[list][1];
             uint32_t a[4];
-            luma_qpel_row4(job.ref_planes[list][r][0], W, H, ox * 4 + mvx, oy * 4 + mvy, a);
-            const mmr_recon_h264_weight* wt = weighted ? job.weights + ((size_t)list * MMR_H264_MAX_REFS + r) * 3 : nullptr;
+            luma_qpel_row4(job.ref_planes[list][r][0], W, H, ox * 4 + mvx, oy * 4 + mvy, job.refs_aligned, a);
+            const MMR_GLOBAL mmr_recon_h264_weight* wt = weighted ? as_global(job.weights) + ((size_t)list * MMR_H264_MAX_REFS + r) * 3 : nullptr;
 #pragma unroll
             for (int c = 0; c < 4; ++c) ys[c] = wt ? weight1(a[c], wt[0].log2_denom, wt[0].weight, wt[0].offset) : a[c];
 #pragma unroll
@@ -294,82 +410,27 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
 
     const bool has_res = active && m.cbp != 0;
     const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
-    const int16_t* lv = job.levels + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+    const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
     const int qp = m.qp;
     const int cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both (decoder.rs:1921)
 
-    // ---- phase 2
+    // ---- phases 2-4: residual (64-bit like the reference, or 32-bit when the levels are small)
     if (has_res) {
-        if (!t8) {
-            if (l < 16) {
-                int32_t lev[16], res[16];
+        // Largest |level| of the macroblock: below 250 every intermediate of the 4x4 / 8x8 /
+        // chroma paths stays below 2^31 (8x8: |v| * 58 * 255 * 4 growing at most 144-fold through
+        // the two 8-point passes; 4x4 and chroma less).
+        uint32_t mx = 0;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) lev[i] = lv[l * 16 + i];
-                residual_4x4(lev, qp, false, job.scaling + 3 * 16, res);
-                const int kbx = ((l >> 2) & 1) * 2 + (l & 1), kby = (l >> 3) * 2 + ((l >> 1) & 1);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) S.res_y[(kby * 4 + (i >> 2)) * 16 + kbx * 4 + (i & 3)] = res[i];
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int idx = l * 4 + t, blk8 = idx >> 6, i = idx & 63, rc = kZz8[i];
-                const int64_t v = (int64_t)lv[idx] * level_scale_8x8(qp, rc >> 3, rc & 7, job.scaling + 96 + 64);
-                const int shift = qp / 6;
-                int64_t sv = shift >= 6 ? v << (shift - 6) : (v + ((int64_t)1 << (5 - shift))) >> (6 - shift);
-                if (rc == 0) sv += 32;
-                S.s8[blk8][rc] = sv;
-            }
+        for (int i = 0; i < MMR_H264_LEVELS_PER_MB / 64; ++i) {
+            const int v = lv[l + 64 * i];
+            mx = max(mx, (uint32_t)(v < 0 ? -v : v));
         }
-        const int dcl = t8 ? l : l - 16;
-        if (dcl == 0 || dcl == 1) {
-            // transform_chroma_dc (decoder.rs:14114-14140) with inter list 4 + component.
-            const int comp = dcl;
-            const int64_t c0 = lv[256 + comp * 4], c1 = lv[257 + comp * 4], c2 = lv[258 + comp * 4], c3 = lv[259 + comp * 4];
-            const int64_t f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
-            const int64_t scale = level_scale_4x4(cqp, 0, 0, job.scaling + (4 + comp) * 16);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) S.dc[comp][i] = (int32_t)(((f[i] * scale) << (cqp / 6)) >> 5);
-        }
+        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, t8, l, job);
+        else residual_phases<int64_t>(S, lv, qp, cqp, t8, l, job);
     }
-    __syncthreads();
-
-    // ---- phase 3
-    if (has_res) {
-        if (t8 && l < 32) {
-            const int blk8 = l >> 3, r = l & 7;
-            int64_t in[8], out[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][r * 8 + i];
-            itrans8(in, out);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) S.s8[blk8][r * 8 + i] = out[i];
-        }
-        if (l >= 32 && l < 40) {
-            const int comp = (l - 32) >> 2, b = (l - 32) & 3;
-            int32_t co[16], res[16];
-            co[0] = S.dc[comp][b];
-#pragma unroll
-            for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
-            residual_4x4(co, cqp, true, job.scaling + (4 + comp) * 16, res);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) S.res_c[comp][((b >> 1) * 4 + (i >> 2)) * 8 + (b & 1) * 4 + (i & 3)] = res[i];
-        }
-    }
-    __syncthreads();
-
-    // ---- phase 4
-    if (has_res && t8 && l < 32) {
-        const int blk8 = l >> 3, c = l & 7;
-        int64_t in[8], out[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][i * 8 + c];
-        itrans8(in, out);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            S.res_y[((blk8 >> 1) * 8 + i) * 16 + (blk8 & 1) * 8 + c] = (int32_t)(out[i] >> 6);
-    }
-    __syncthreads();
+    wave_lds_sync();
 
     // ---- phase 5: residual add + store
     if (!active) return;
@@ -382,7 +443,7 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
             if (has_res) v = clip255(v + S.res_y[row * 16 + c0 + c]);
             word |= (uint32_t)v << (8 * c);
         }
-        *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)(mby * 16 + row) * W + mbx * 16 + c0) = word;
+        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(mby * 16 + row) * W + mbx * 16 + c0) = word;
     }
     {
         const int comp = l >> 5, row = (l >> 2) & 7, c0 = (l & 3) * 2;
@@ -393,14 +454,15 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
             if (has_res) v = clip255(v + S.res_c[comp][row * 8 + c0 + c]);
             half |= (uint32_t)v << (8 * c);
         }
-        *reinterpret_cast<uint16_t*>(job.planes[1 + comp] + (size_t)(mby * 8 + row) * CW + mbx * 8 + c0) = (uint16_t)half;
+        *reinterpret_cast<MMR_GLOBAL uint16_t*>(as_global(job.planes[1 + comp]) + (size_t)(mby * 8 + row) * CW + mbx * 8 + c0) = (uint16_t)half;
     }
 }
 
-hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s) {
-    if (total_mbs == 0) return hipSuccess;
-    const uint32_t per = xcd_per((total_mbs + 3) / 4);
-    hipLaunchKernelGGL(h264_mb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_mbs, per, d_status);
+hipError_t launch_h264_mc(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t* d_status,
+                          hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(h264_mb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, per, d_status);
     return hipGetLastError();
 }
 

@@ -516,6 +516,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::H264Job> jobs(count);
+    std::vector<uint32_t> tiles;
     uint32_t mbs = 0, bands = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
@@ -559,7 +560,12 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         j.deblock = p.deblock ? 1u : 0u;
         j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
         j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
-        j.mb_begin = per_frame ? 0 : mbs;
+        j.tile_begin = add_tiles(tiles, i, (p.width_mbs * p.height_mbs) * 8);   // 4 macroblocks per 32-entry tile unit
+        j.refs_aligned = 1;
+        for (int list = 0; list < 2; ++list)
+            for (uint32_t r = 0; r < MMR_H264_MAX_REFS; ++r)
+                for (int k = 0; k < 3; ++k)
+                    if (!aligned(j.ref_planes[list][r][k], 4)) j.refs_aligned = 0;
         j.band_begin = per_frame ? 0 : bands;
         mbs += p.width_mbs * p.height_mbs;
         bands += j.deblock ? (p.height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand : 0;
@@ -571,15 +577,19 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     } else {
         counters = 1 + bands;
     }
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::H264Job), counters);
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::H264Job));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
+    mmr_status s = sub.begin(blob.data(), blob.size(), counters);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::H264Job*>(sub.jobs_ptr());
+    auto* dj = reinterpret_cast<const mmr::H264Job*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count;
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_h264_mc(dj + i, 1, pics[i].width_mbs * pics[i].height_mbs, sub.status_ptr(), stream);
+            e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, (pics[i].width_mbs * pics[i].height_mbs + 3) / 4, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].deblock) {
                 const uint32_t nb = (pics[i].height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand;
                 e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, sub.status_ptr(), stream);
@@ -589,7 +599,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             }
         }
     } else {
-        e = mmr::launch_h264_mc(dj, count, mbs, sub.status_ptr(), stream);
+        e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
         if (e == hipSuccess && bands) e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, sub.status_ptr(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "H.264 launch");

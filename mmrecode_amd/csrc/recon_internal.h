@@ -77,7 +77,8 @@ struct H264Job {
     uint32_t weighted;
     uint32_t deblock;
     int32_t chroma_qp_offset[2];
-    uint32_t mb_begin;            // first global macroblock index of this picture
+    uint32_t tile_begin;          // first entry of this picture in the tile -> job table (4 macroblocks per tile)
+    uint32_t refs_aligned;        // every reference plane is 4-byte aligned
     uint32_t band_begin;          // first deblocking band ticket of this picture (bands of 16 macroblock rows)
 };
 
@@ -109,7 +110,8 @@ hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint
                         uint32_t* d_status, hipStream_t s);
 hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t dv_upload_tables(hipStream_t s);
-hipError_t launch_h264_mc(const H264Job* d_jobs, uint32_t job_count, uint32_t total_mbs, uint32_t* d_status, hipStream_t s);
+hipError_t launch_h264_mc(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t* d_status,
+                          hipStream_t s);
 // Deblocking in bands of kH264DeblockBand macroblock rows: d_counters[0] is the band ticket,
 // d_counters[1 + t] the progress of band t's last row (zeroed before the launch).
 constexpr uint32_t kH264DeblockBand = 16;
