@@ -422,6 +422,37 @@ MMR_RECON_API mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context *context,
                                                     uint32_t job_count, uint32_t flags,
                                                     uint64_t *out_ticket);
 
+/* Codec selectors of the prepared-batch and serving calls. */
+#define MMR_CODEC_MJPEG 1u
+#define MMR_CODEC_MPEG2 2u
+#define MMR_CODEC_DV 3u
+#define MMR_CODEC_H264 4u
+
+/* ------------------------------------------------------ prepared batches */
+
+/*
+ * New (no counterpart in the reference, whose reconstruction loops run per frame on the CPU): a
+ * batch prepared once and replayed. mmr_recon_prepare uploads the descriptors' job records once
+ * into a persistent device block and captures the batch's kernel launches in a HIP graph;
+ * mmr_recon_prepared_launch replays that graph on the context's stream and returns a ticket like a
+ * submit call. The device buffers the descriptors name must stay allocated at the same addresses
+ * while the prepared batch lives (their contents may change between launches: a replay reads the
+ * inputs anew). Launches of one prepared batch run in order on the context's stream.
+ * codec: MMR_CODEC_MJPEG / MPEG2 / DV / H264; descriptors: an array of that codec's frame /
+ * picture descriptors; flags as for the codec's submit call.
+ */
+typedef struct mmr_recon_prepared mmr_recon_prepared;
+
+MMR_RECON_API mmr_status mmr_recon_prepare(mmr_recon_context *context, uint32_t codec,
+                                           const void *descriptors, uint32_t count, uint32_t flags,
+                                           mmr_recon_prepared **out_prepared);
+MMR_RECON_API mmr_status mmr_recon_prepared_launch(mmr_recon_context *context,
+                                                   mmr_recon_prepared *prepared,
+                                                   uint64_t *out_ticket);
+/* Waits for the context's stream, then releases the graph and the device block. */
+MMR_RECON_API void mmr_recon_prepared_destroy(mmr_recon_context *context,
+                                              mmr_recon_prepared *prepared);
+
 /* --------------------------------------------------------------- serving */
 
 /*
@@ -440,10 +471,6 @@ MMR_RECON_API mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context *context,
  */
 typedef struct mmr_recon_server mmr_recon_server;
 
-#define MMR_CODEC_MJPEG 1u
-#define MMR_CODEC_MPEG2 2u
-#define MMR_CODEC_DV 3u
-#define MMR_CODEC_H264 4u
 
 typedef struct mmr_recon_server_config {
     size_t struct_size;

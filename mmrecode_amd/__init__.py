@@ -154,6 +154,10 @@ class Context:
     def submit_yuv_blend(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_yuv_blend_submit, abi.YuvBlendJob, jobs, flags)
 
+    def prepare(self, codec, items, flags=0):
+        """A Prepared batch of one codec's descriptors (uploaded once, launches captured in a graph)."""
+        return Prepared(self, codec, items, flags)
+
     def wait(self, ticket):
         check(self.lib.mmr_recon_wait(self.handle, ticket))
 
@@ -169,6 +173,29 @@ class Context:
         ms = C.c_double()
         check(self.lib.mmr_recon_last_kernel_ms(self.handle, ticket, C.byref(ms)))
         return ms.value
+
+
+class Prepared:
+    """mmr_recon_prepare / mmr_recon_prepared_launch: a batch whose job records are uploaded once and
+    whose kernel launches are replayed from a HIP graph."""
+
+    def __init__(self, ctx, codec, items, flags=0):
+        self.ctx = ctx
+        struct = Server.STRUCTS[codec]
+        arr = (struct * len(items))(*items)
+        h = C.c_void_p()
+        check(ctx.lib.mmr_recon_prepare(ctx.handle, codec, C.addressof(arr), len(items), flags, C.byref(h)))
+        self.handle = h
+
+    def launch(self):
+        ticket = C.c_uint64()
+        check(self.ctx.lib.mmr_recon_prepared_launch(self.ctx.handle, self.handle, C.byref(ticket)))
+        return ticket.value
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.mmr_recon_prepared_destroy(self.ctx.handle, self.handle)
+            self.handle = None
 
 
 class Server:

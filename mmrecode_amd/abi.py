@@ -352,4 +352,7 @@ ENTRY_POINTS = [
     ("mmr_recon_server_drain", C.c_int32, [C.c_void_p]),
     ("mmr_recon_server_stats_get", C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
     ("mmr_recon_server_destroy", None, [C.c_void_p]),
+    ("mmr_recon_prepare", C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("mmr_recon_prepared_launch", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_prepared_destroy", None, [C.c_void_p, C.c_void_p]),
 ]

@@ -65,58 +65,53 @@ MMR_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on LDS samples:
-// p_i at s[-(i+1)*step], q_i at s[i*step].
-MMR_DEV void filter_samples(uint8_t* s, int step, int qp, int bs, bool chroma, int offset_a, int offset_b) {
+// filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on one line of samples held in
+// registers as packed bytes: sample i is byte i of the line, p_i is sample k - 1 - i and q_i sample
+// k + i (k is a compile-time index, so every extract / insert is one bitfield instruction).
+// Samples are only rewritten when the filter applies, which leaves the same values as the
+// reference's unconditional write-back of unchanged samples.
+template <int NW>
+MMR_DEV int get_byte(const uint32_t (&w)[NW], int i) { return (w[i >> 2] >> (8 * (i & 3))) & 0xff; }
+template <int NW>
+MMR_DEV void set_byte(uint32_t (&w)[NW], int i, int v) {
+    const uint32_t sh = 8 * (i & 3);
+    w[i >> 2] = (w[i >> 2] & ~(0xffu << sh)) | ((uint32_t)v << sh);
+}
+
+template <int k, int NW>
+MMR_DEV void filter_line(uint32_t (&w)[NW], int qp, int bs, bool chroma, int offset_a, int offset_b) {
     const int ia = clampi(qp + offset_a, 0, 51), ib = clampi(qp + offset_b, 0, 51);
     const int alpha = kAlpha[ia], beta = kBeta[ib];
-    int p[4], q[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        p[i] = s[-(i + 1) * step];
-        q[i] = s[i * step];
-    }
-    if (iabs(p[0] - q[0]) >= alpha || iabs(p[1] - p[0]) >= beta || iabs(q[1] - q[0]) >= beta) return;
-    int fp[3], fq[3];
+    const int p0 = get_byte(w, k - 1), p1 = get_byte(w, k - 2), p2 = get_byte(w, k - 3), p3 = get_byte(w, k - 4);
+    const int q0 = get_byte(w, k), q1 = get_byte(w, k + 1), q2 = get_byte(w, k + 2), q3 = get_byte(w, k + 3);
+    if (iabs(p0 - q0) >= alpha || iabs(p1 - p0) >= beta || iabs(q1 - q0) >= beta) return;
     if (bs == 4) {
-        const bool sp = !chroma && iabs(p[2] - p[0]) < beta && iabs(p[0] - q[0]) < (alpha >> 2) + 2;
-        const bool sq = !chroma && iabs(q[2] - q[0]) < beta && iabs(p[0] - q[0]) < (alpha >> 2) + 2;
+        const bool near = iabs(p0 - q0) < (alpha >> 2) + 2;
+        const bool sp = !chroma && iabs(p2 - p0) < beta && near;
+        const bool sq = !chroma && iabs(q2 - q0) < beta && near;
         if (sp) {
-            fp[0] = (p[2] + 2 * p[1] + 2 * p[0] + 2 * q[0] + q[1] + 4) >> 3;
-            fp[1] = (p[2] + p[1] + p[0] + q[0] + 2) >> 2;
-            fp[2] = (2 * p[3] + 3 * p[2] + p[1] + p[0] + q[0] + 4) >> 3;
+            set_byte(w, k - 1, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
+            set_byte(w, k - 2, (p2 + p1 + p0 + q0 + 2) >> 2);
+            set_byte(w, k - 3, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
         } else {
-            fp[0] = (2 * p[1] + p[0] + q[1] + 2) >> 2;
-            fp[1] = p[1];
-            fp[2] = p[2];
+            set_byte(w, k - 1, (2 * p1 + p0 + q1 + 2) >> 2);
         }
         if (sq) {
-            fq[0] = (p[1] + 2 * p[0] + 2 * q[0] + 2 * q[1] + q[2] + 4) >> 3;
-            fq[1] = (p[0] + q[0] + q[1] + q[2] + 2) >> 2;
-            fq[2] = (2 * q[3] + 3 * q[2] + q[1] + q[0] + p[0] + 4) >> 3;
+            set_byte(w, k, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
+            set_byte(w, k + 1, (p0 + q0 + q1 + q2 + 2) >> 2);
+            set_byte(w, k + 2, (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3);
         } else {
-            fq[0] = (2 * q[1] + q[0] + p[1] + 2) >> 2;
-            fq[1] = q[1];
-            fq[2] = q[2];
+            set_byte(w, k, (2 * q1 + q0 + p1 + 2) >> 2);
         }
     } else {
         const int tc0 = kTc0[bs - 1][ia];
-        const int ap = iabs(p[2] - p[0]), aq = iabs(q[2] - q[0]);
+        const int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
         const int tc = tc0 + (chroma ? 1 : ((ap < beta) + (aq < beta)));
-        const int delta = clampi((((q[0] - p[0]) << 2) + (p[1] - q[1]) + 4) >> 3, -tc, tc);
-        fp[0] = p[0] + delta;
-        fp[1] = p[1];
-        fp[2] = p[2];
-        fq[0] = q[0] - delta;
-        fq[1] = q[1];
-        fq[2] = q[2];
-        if (!chroma && ap < beta) fp[1] = p[1] + clampi((p[2] + ((p[0] + q[0] + 1) >> 1) - (p[1] << 1)) >> 1, -tc0, tc0);
-        if (!chroma && aq < beta) fq[1] = q[1] + clampi((q[2] + ((p[0] + q[0] + 1) >> 1) - (q[1] << 1)) >> 1, -tc0, tc0);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        s[-(i + 1) * step] = (uint8_t)clampi(fp[i], 0, 255);
-        s[i * step] = (uint8_t)clampi(fq[i], 0, 255);
+        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        set_byte(w, k - 1, clampi(p0 + delta, 0, 255));
+        set_byte(w, k, clampi(q0 - delta, 0, 255));
+        if (!chroma && ap < beta) set_byte(w, k - 2, clampi(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0), 0, 255));
+        if (!chroma && aq < beta) set_byte(w, k + 1, clampi(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0), 0, 255));
     }
 }
 
@@ -175,12 +170,28 @@ struct DbPrefetch {
     bool edge_on;
 };
 
+// Records read through the global address space (the job holds generic pointers).
+MMR_DEV mmr_recon_h264_macroblock db_mb(const H264Job& job, uint32_t a) {
+    const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + a));
+    const uint32_t w[3] = {q[0], q[1], q[2]};
+    mmr_recon_h264_macroblock m;
+    __builtin_memcpy(&m, w, sizeof m);
+    return m;
+}
+MMR_DEV mmr_recon_h264_block db_block(const H264Job& job, size_t i) {
+    const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.blocks + i));
+    const uint32_t w[3] = {q[0], q[1], q[2]};
+    mmr_recon_h264_block b;
+    __builtin_memcpy(&b, w, sizeof b);
+    return b;
+}
+
 MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPrefetch& f) {
     const int W = MW * 16, CW = MW * 8;
-    f.own_y = *reinterpret_cast<const uint32_t*>(job.planes[0] + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+    f.own_y = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
     if (l < 32) {
         const int comp = l >> 4, row = (l >> 1) & 7;
-        f.own_c = *reinterpret_cast<const uint32_t*>(job.planes[1 + comp] + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
+        f.own_c = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
     }
     const uint32_t a = (uint32_t)(y * MW + x);
     f.edge_on = false;
@@ -200,17 +211,17 @@ MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPref
                 pm = on ? a - MW : a;
             }
             if (on) {
-                const mmr_recon_h264_macroblock q = job.mbs[a];
-                on = !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || job.mbs[pm].slice_id == q.slice_id;   // external_edge_enabled
+                const mmr_recon_h264_macroblock q = db_mb(job, a);
+                on = !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || db_mb(job, pm).slice_id == q.slice_id;   // external_edge_enabled
             }
         }
         f.edge_on = on;
         const int pb = vertical ? luma_block_index(external ? 3 : e - 1, sgm) : luma_block_index(sgm, external ? 3 : e - 1);
         const int qb = vertical ? luma_block_index(e, sgm) : luma_block_index(sgm, e);
-        f.P = job.blocks[(size_t)pm * 16 + pb];
-        f.Q = job.blocks[(size_t)a * 16 + qb];
-        f.p_flags = job.mbs[pm].flags;
-        f.q_flags = job.mbs[a].flags;
+        f.P = db_block(job, (size_t)pm * 16 + pb);
+        f.Q = db_block(job, (size_t)a * 16 + qb);
+        f.p_flags = as_global(job.mbs)[pm].flags;
+        f.q_flags = as_global(job.mbs)[a].flags;
     }
 }
 
@@ -263,7 +274,7 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
     db_prefetch(job, MW, 0, y, l, cur);
     for (int x = 0; x < MW; ++x) {
         const uint32_t a = (uint32_t)(y * MW + x);
-        const mmr_recon_h264_macroblock m = job.mbs[a];
+        const mmr_recon_h264_macroblock m = db_mb(job, a);
         const bool filter = !(m.flags & MMR_H264_MB_NO_DEBLOCK);
         // own samples into the window (rows 4.., cols 4..)
         *reinterpret_cast<uint32_t*>(&S.ly[(4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]) = cur.own_y;
@@ -279,7 +290,7 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
             else S.bs_h[e][sgm] = bs;
         }
         if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
-        const int qp_left = x > 0 ? job.mbs[a - 1].qp : 0, qp_top = y > 0 ? job.mbs[a - MW].qp : 0;
+        const int qp_left = x > 0 ? as_global(job.mbs)[a - 1].qp : 0, qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
         // top context from the row above (slot x), after it finished macroblock x + 1
         if (y > 0 && band_first) {
             if (!db_wait_band(gprev, (uint32_t)min(x + 2, MW), seen, status)) return;
@@ -307,44 +318,66 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
         if (filter) {
             const int oa = m.offset_a, ob = m.offset_b;
             const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
-            // vertical edges: lanes 0-15 luma rows, 16-23 Cb rows, 24-31 Cr rows
+            // vertical edges, one line of the window per lane, held in registers: lanes 0-15 luma
+            // rows (20 samples: 4 of left context + 16), 16-23 Cb rows, 24-31 Cr rows (12 samples)
             if (l < 16) {
-                for (int e = 0; e < 4; ++e) {
-                    if (t8 && (e == 1 || e == 3)) continue;
-                    const int bs = S.bs_v[e][l >> 2];
-                    if (bs == 0) continue;
-                    const int qp = ((e == 0 ? qp_left : m.qp) + m.qp + 1) >> 1;
-                    filter_samples(&S.ly[(4 + l) * 20 + 4 + e * 4], 1, qp, bs, false, oa, ob);
-                }
+                uint32_t* row = reinterpret_cast<uint32_t*>(&S.ly[(4 + l) * 20]);
+                uint32_t v[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) v[i] = row[i];
+                const int sg = l >> 2, qpe = (qp_left + m.qp + 1) >> 1, qpi = m.qp;
+                if (S.bs_v[0][sg]) filter_line<4>(v, qpe, S.bs_v[0][sg], false, oa, ob);
+                if (!t8 && S.bs_v[1][sg]) filter_line<8>(v, qpi, S.bs_v[1][sg], false, oa, ob);
+                if (S.bs_v[2][sg]) filter_line<12>(v, qpi, S.bs_v[2][sg], false, oa, ob);
+                if (!t8 && S.bs_v[3][sg]) filter_line<16>(v, qpi, S.bs_v[3][sg], false, oa, ob);
+#pragma unroll
+                for (int i = 0; i < 5; ++i) row[i] = v[i];
             } else if (l < 32) {
-                const int comp = (l - 16) >> 3, row = (l - 16) & 7;
+                const int comp = (l - 16) >> 3, r = (l - 16) & 7;
                 const int off = job.chroma_qp_offset[comp];
-                for (int e = 0; e < 2; ++e) {
-                    const int bs = S.bs_v[e * 2][row >> 1];
-                    if (bs == 0) continue;
-                    const int qp = (db_chroma_qp(e == 0 ? qp_left : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                    filter_samples(&S.lc[comp][(4 + row) * 12 + 4 + e * 4], 1, qp, bs, true, oa, ob);
-                }
+                uint32_t* row = reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + r) * 12]);
+                uint32_t v[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) v[i] = row[i];
+                const int sg = r >> 1;
+                const int qpe = (db_chroma_qp(qp_left, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
+                const int qpi = db_chroma_qp(m.qp, off);
+                if (S.bs_v[0][sg]) filter_line<4>(v, qpe, S.bs_v[0][sg], true, oa, ob);
+                if (S.bs_v[2][sg]) filter_line<8>(v, qpi, S.bs_v[2][sg], true, oa, ob);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) row[i] = v[i];
             }
             wave_sync();
-            // horizontal edges: lanes 0-15 luma columns, 16-23 Cb columns, 24-31 Cr columns
+            // horizontal edges, one column of the window per lane: lanes 0-15 luma columns (20
+            // samples: 4 of top context + 16), 16-23 Cb columns, 24-31 Cr columns (12 samples)
             if (l < 16) {
-                for (int e = 0; e < 4; ++e) {
-                    if (t8 && (e == 1 || e == 3)) continue;
-                    const int bs = S.bs_h[e][l >> 2];
-                    if (bs == 0) continue;
-                    const int qp = ((e == 0 ? qp_top : m.qp) + m.qp + 1) >> 1;
-                    filter_samples(&S.ly[(4 + e * 4) * 20 + 4 + l], 20, qp, bs, false, oa, ob);
-                }
+                uint8_t* col = &S.ly[4 + l];
+                uint32_t v[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i)
+                    v[i] = col[(4 * i) * 20] | (col[(4 * i + 1) * 20] << 8) | (col[(4 * i + 2) * 20] << 16) | ((uint32_t)col[(4 * i + 3) * 20] << 24);
+                const int sg = l >> 2, qpe = (qp_top + m.qp + 1) >> 1, qpi = m.qp;
+                if (S.bs_h[0][sg]) filter_line<4>(v, qpe, S.bs_h[0][sg], false, oa, ob);
+                if (!t8 && S.bs_h[1][sg]) filter_line<8>(v, qpi, S.bs_h[1][sg], false, oa, ob);
+                if (S.bs_h[2][sg]) filter_line<12>(v, qpi, S.bs_h[2][sg], false, oa, ob);
+                if (!t8 && S.bs_h[3][sg]) filter_line<16>(v, qpi, S.bs_h[3][sg], false, oa, ob);
+#pragma unroll
+                for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)get_byte(v, i);
             } else if (l < 32) {
-                const int comp = (l - 16) >> 3, col = (l - 16) & 7;
+                const int comp = (l - 16) >> 3, c = (l - 16) & 7;
                 const int off = job.chroma_qp_offset[comp];
-                for (int e = 0; e < 2; ++e) {
-                    const int bs = S.bs_h[e * 2][col >> 1];
-                    if (bs == 0) continue;
-                    const int qp = (db_chroma_qp(e == 0 ? qp_top : m.qp, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                    filter_samples(&S.lc[comp][(4 + e * 4) * 12 + 4 + col], 12, qp, bs, true, oa, ob);
-                }
+                uint8_t* col = &S.lc[comp][4 + c];
+                uint32_t v[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    v[i] = col[(4 * i) * 12] | (col[(4 * i + 1) * 12] << 8) | (col[(4 * i + 2) * 12] << 16) | ((uint32_t)col[(4 * i + 3) * 12] << 24);
+                const int sg = c >> 1;
+                const int qpe = (db_chroma_qp(qp_top, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
+                const int qpi = db_chroma_qp(m.qp, off);
+                if (S.bs_h[0][sg]) filter_line<4>(v, qpe, S.bs_h[0][sg], true, oa, ob);
+                if (S.bs_h[2][sg]) filter_line<8>(v, qpi, S.bs_h[2][sg], true, oa, ob);
+#pragma unroll
+                for (int i = 1; i < 11; ++i) col[i * 12] = (uint8_t)get_byte(v, i);
             }
             wave_sync();
         }
@@ -375,18 +408,18 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
             const int r = 1 + (l >> 2), c4 = 4 * (l & 3);
             const int gy = y * 16 - 4 + r, gx = x * 16 - 4 + c4;
             if (gy >= 0 && gx >= 0)
-                *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&S.ly[r * 20 + c4]);
+                *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&S.ly[r * 20 + c4]);
             if (last_row && l < 12) {   // rows y*16+13 .. +15 (window rows 17..19)
                 const int r2 = 17 + (l >> 2), gx2 = x * 16 - 4 + c4;
                 if (gx2 >= 0)
-                    *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)(y * 16 - 4 + r2) * W + gx2) =
+                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y * 16 - 4 + r2) * W + gx2) =
                         *reinterpret_cast<const uint32_t*>(&S.ly[r2 * 20 + c4]);
             }
             if (x == MW - 1 && l < 19) {   // last column: columns x*16+12 .. +15 (window col 16)
                 const int r3 = 1 + l;
                 const int gy3 = y * 16 - 4 + r3;
                 if (gy3 >= 0 && (r3 <= 16 || last_row))
-                    *reinterpret_cast<uint32_t*>(job.planes[0] + (size_t)gy3 * W + x * 16 + 12) =
+                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)gy3 * W + x * 16 + 12) =
                         *reinterpret_cast<const uint32_t*>(&S.ly[r3 * 20 + 16]);
             }
             // chroma: rows y*8-3 .. y*8+4 (window rows 1..8), columns x*8-4 .. x*8+3 (window cols 0..7)
@@ -394,7 +427,7 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
                 const int comp = l >> 4, rr = 1 + ((l >> 1) & 7), cc = 4 * (l & 1);
                 const int cy = y * 8 - 4 + rr, cx = x * 8 - 4 + cc;
                 if (cy >= 0 && cx >= 0)
-                    *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)cy * CW + cx) =
+                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + cx) =
                         *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
             }
             // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
@@ -405,13 +438,13 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
                     const int comp = i / 6, rr = 9 + (i % 6) / 2, cc = 4 * (i & 1);
                     const int cx = x * 8 - 4 + cc;
                     if (cx >= 0)
-                        *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)(y * 8 - 4 + rr) * CW + cx) =
+                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + rr) * CW + cx) =
                             *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
                 } else {
                     const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
                     const int cy = y * 8 - 4 + rr;
                     if (cy >= 0 && (rr <= 8 || last_row))
-                        *reinterpret_cast<uint32_t*>(job.planes[1 + comp] + (size_t)cy * CW + x * 8 + 4) =
+                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + x * 8 + 4) =
                             *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + 8]);
                 }
             }

@@ -110,22 +110,68 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
 
 // A submission in flight: records the start event, owns the device allocation of job records +
 // counters + status word, and finishes by recording the completion event.
+// A submission prepared for replay: the device block is a persistent allocation and the kernel
+// launches (counter fill included) are captured into a graph on `stream`.
+struct PreparedCapture {
+    hipStream_t stream = nullptr;
+    char* d_block = nullptr;
+    uint32_t* status = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct Submission {
     mmr_recon_context* ctx;
     hipStream_t stream;
+    PreparedCapture* cap;
+    bool capturing = false;
     TicketRecord rec;
     char* d_block = nullptr;
     size_t jobs_bytes = 0;
     size_t counter_words = 0;
     size_t counters_bytes = 0;
 
-    Submission(mmr_recon_context* c, hipStream_t s) : ctx(c), stream(s) {}
+    Submission(mmr_recon_context* c, hipStream_t s, PreparedCapture* capture = nullptr) : ctx(c), stream(s), cap(capture) {}
+
+    mmr_status begin_capture(const void* jobs, size_t bytes, size_t scratch_bytes,
+                             const std::function<void(char* host_jobs, char* scratch)>& patch) {
+        rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
+        hipError_t e = hipMalloc((void**)&d_block, jobs_bytes + counters_bytes + scratch_bytes);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc (prepared batch)");
+        if (patch) patch(rec.host_jobs.data(), scratch_ptr());
+        if (bytes) e = hipMemcpy(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(e, "prepared batch upload / capture");
+        capturing = true;
+        e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, stream);
+        if (e != hipSuccess) return hip_fail(e, "counter fill");
+        return MMR_STATUS_OK;
+    }
+    mmr_status end_capture() {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(stream, &g);
+        capturing = false;
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+        cap->graph = g;
+        e = hipGraphInstantiate(&cap->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+        cap->d_block = d_block;
+        cap->status = status_ptr();
+        d_block = nullptr;
+        return MMR_STATUS_OK;
+    }
 
     // Device block: [jobs (uploaded)][counters + status word (zeroed)][scratch (uninitialised)].
     // `patch` (optional) edits the host copy of the job records once the device block exists, so
     // records can point into the block's scratch.
     mmr_status begin(const void* jobs, size_t bytes, size_t counters, size_t scratch_bytes = 0,
                      const std::function<void(char* host_jobs, char* scratch)>& patch = nullptr) {
+        if (cap) {
+            jobs_bytes = (bytes + 255) & ~size_t(255);
+            counter_words = counters + 1;
+            counters_bytes = (counter_words * sizeof(uint32_t) + 255) & ~size_t(255);
+            return begin_capture(jobs, bytes, scratch_bytes, patch);
+        }
         if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
         if (ctx->tickets.size() >= ctx->queue_depth)
             return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
@@ -154,6 +200,7 @@ struct Submission {
     char* scratch_ptr() const { return d_block + jobs_bytes + counters_bytes; }
 
     mmr_status end(uint64_t* out_ticket) {
+        if (cap) return end_capture();
         hipError_t e = hipEventRecord(rec.kernels_done, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipEventRecord(rec.done, stream);
@@ -167,7 +214,14 @@ struct Submission {
         return MMR_STATUS_OK;
     }
     ~Submission() {
-        if (d_block) (void)hipFreeAsync(d_block, stream);
+        if (capturing) {   // a launch failed during capture: end it and drop the partial graph
+            hipGraph_t g = nullptr;
+            if (hipStreamEndCapture(stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        }
+        if (d_block) {
+            if (cap) (void)hipFree(d_block);
+            else (void)hipFreeAsync(d_block, stream);
+        }
         if (rec.start) (void)hipEventDestroy(rec.start);
         if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
         if (rec.done) (void)hipEventDestroy(rec.done);
@@ -304,7 +358,7 @@ mmr_status check_rgb_job(const mmr_recon_yuv_to_rgb_job& s) {
 namespace mmr_host {
 
 mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
-                                  uint64_t* out_ticket) {
+                                  uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
     CHECK_CTX(ctx);
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     std::vector<mmr::MjpegJob> jobs(count);
@@ -327,7 +381,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
     const size_t rgb_off = blob_append(blob, rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
     const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
-    Submission sub(ctx, stream);
+    Submission sub(ctx, stream, cap);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
     const char* base = (const char*)sub.jobs_ptr();
@@ -353,7 +407,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
 }
 
 mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
-                                  uint64_t* out_ticket) {
+                                  uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
     CHECK_CTX(ctx);
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
@@ -439,7 +493,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         ordered[k].row_begin = per_frame ? 0 : order_rows;
         order_rows += ordered[k].mb_h;
     }
-    Submission sub(ctx, stream);
+    Submission sub(ctx, stream, cap);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
         auto* hj = reinterpret_cast<mmr::Mpeg2Job*>(host + jobs_off);
         for (size_t k = 0; k < ordered.size(); ++k) {
@@ -468,7 +522,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
 }
 
 mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_dv_frame* frames, uint32_t count, uint32_t flags,
-                               uint64_t* out_ticket) {
+                               uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
     CHECK_CTX(ctx);
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
@@ -494,7 +548,7 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::DvJob));
     const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
-    Submission sub(ctx, stream);
+    Submission sub(ctx, stream, cap);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
     auto* dj = reinterpret_cast<const mmr::DvJob*>((const char*)sub.jobs_ptr() + jobs_off);
@@ -511,7 +565,7 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
 }
 
 mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_h264_picture* pics, uint32_t count, uint32_t flags,
-                                 uint64_t* out_ticket) {
+                                 uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
     CHECK_CTX(ctx);
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
@@ -570,7 +624,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         mbs += p.width_mbs * p.height_mbs;
         bands += j.deblock ? (p.height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand : 0;
     }
-    Submission sub(ctx, stream);
+    Submission sub(ctx, stream, cap);
     size_t counters = 0;
     if (per_frame) {
         for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
@@ -957,6 +1011,114 @@ mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yu
     CHECK_CTX(ctx);
     LOCK_AND_SET_DEVICE(ctx);
     return mmr_host::yuv_blend_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+}
+
+}  // extern "C"
+
+// ====================================================================== prepared batches
+//
+// A batch prepared once and replayed: the submission's job records go once into a persistent
+// device block and its launches (counter fill, kernels) are captured into a HIP graph on a private
+// stream; a launch replays the graph on the context's stream between the ticket's events. The
+// graph holds no memset node (the counters are cleared by a fill kernel), so every replay starts
+// from zeroed counters.
+
+struct mmr_recon_prepared {
+    PreparedCapture cap;
+    uint32_t codec = 0;
+};
+
+extern "C" {
+
+mmr_status mmr_recon_prepare(mmr_recon_context* ctx, uint32_t codec, const void* descriptors, uint32_t count, uint32_t flags,
+                             mmr_recon_prepared** out) {
+    CHECK_CTX(ctx);
+    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_prepared is NULL");
+    *out = nullptr;
+    if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
+    LOCK_AND_SET_DEVICE(ctx);
+    auto* p = new mmr_recon_prepared();
+    p->codec = codec;
+    hipError_t e = hipStreamCreateWithFlags(&p->cap.stream, hipStreamNonBlocking);   // default priority
+    if (e != hipSuccess) {
+        delete p;
+        return hip_fail(e, "capture stream");
+    }
+    mmr_status s;
+    switch (codec) {
+    case MMR_CODEC_MJPEG:
+        s = mmr_host::mjpeg_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, nullptr, &p->cap);
+        break;
+    case MMR_CODEC_MPEG2:
+        s = mmr_host::mpeg2_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, nullptr, &p->cap);
+        break;
+    case MMR_CODEC_DV:
+        s = mmr_host::dv_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, nullptr, &p->cap);
+        break;
+    case MMR_CODEC_H264:
+        s = mmr_host::h264_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, nullptr, &p->cap);
+        break;
+    default:
+        s = fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
+    }
+    if (s != MMR_STATUS_OK || !p->cap.exec) {
+        if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
+        if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
+        if (p->cap.d_block) (void)hipFree(p->cap.d_block);
+        (void)hipStreamDestroy(p->cap.stream);
+        delete p;
+        return s != MMR_STATUS_OK ? s : fail(MMR_STATUS_INTERNAL_ERROR, "graph capture produced no graph");
+    }
+    *out = p;
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_prepared_launch(mmr_recon_context* ctx, mmr_recon_prepared* p, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!p) return fail(MMR_STATUS_INVALID_ARGUMENT, "prepared batch is NULL");
+    LOCK_AND_SET_DEVICE(ctx);
+    if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
+    if (ctx->tickets.size() >= ctx->queue_depth) return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
+    TicketRecord rec;
+    hipError_t e = hipEventCreate(&rec.start);
+    if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
+    if (e == hipSuccess) e = hipEventCreate(&rec.done);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
+    if (e == hipSuccess) {
+        *rec.host_status = 0;
+        e = hipEventRecord(rec.start, ctx->stream);
+    }
+    if (e == hipSuccess) e = hipGraphLaunch(p->cap.exec, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(rec.kernels_done, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, p->cap.status, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(rec.done, ctx->stream);
+    if (e != hipSuccess) {
+        // the events may be in flight: wait for the stream before releasing them
+        (void)hipStreamSynchronize(ctx->stream);
+        if (rec.start) (void)hipEventDestroy(rec.start);
+        if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
+        if (rec.done) (void)hipEventDestroy(rec.done);
+        if (rec.host_status) (void)hipHostFree(rec.host_status);
+        return hip_fail(e, "prepared batch launch");
+    }
+    const uint64_t id = ctx->next_ticket++;
+    ctx->tickets.emplace(id, std::move(rec));
+    if (out_ticket) *out_ticket = id;
+    return MMR_STATUS_OK;
+}
+
+void mmr_recon_prepared_destroy(mmr_recon_context* ctx, mmr_recon_prepared* p) {
+    if (!p) return;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);   // no launch of this batch may still be running
+    }
+    if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
+    if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
+    if (p->cap.d_block) (void)hipFree(p->cap.d_block);
+    if (p->cap.stream) (void)hipStreamDestroy(p->cap.stream);
+    delete p;
 }
 
 }  // extern "C"
