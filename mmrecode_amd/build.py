@@ -27,8 +27,7 @@ FLAGS = [
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *[("-x" if False else s) for s in srcs]]
-    # .cpp host file holds HIP runtime calls only; compile every file as HIP.
+    # The .cpp host file holds HIP runtime calls only; every file is compiled as HIP.
     cmd = [HIPCC, *FLAGS, "-o", tmp, "-x", "hip", *srcs]
     if verbose:
         print(" ".join(cmd), flush=True)
