@@ -98,7 +98,9 @@ MMR_RECON_API mmr_status mmr_recon_copy_to_device(mmr_recon_context *context, vo
 MMR_RECON_API mmr_status mmr_recon_copy_to_host(mmr_recon_context *context, void *host_dst,
                                                 const void *device_src, size_t bytes);
 
-/* Completion. Tickets are issued in increasing order per context. */
+/* Completion. Tickets are issued in increasing order per context. wait / poll return the
+ * status of one submission; synchronize waits for every submission of the context and returns
+ * the first error of a submission whose status was not yet returned by wait or poll. */
 MMR_RECON_API mmr_status mmr_recon_wait(mmr_recon_context *context, uint64_t ticket);
 MMR_RECON_API mmr_status mmr_recon_poll(mmr_recon_context *context, uint64_t ticket,
                                         int32_t *out_done);

@@ -191,18 +191,22 @@ MMR_DEV uint8_t clamp_u8_d(double r) {
 
 MMR_DEV int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// XOR / OR reductions across the 8 lanes of a block group (lanes 8g..8g+7 of the wave).
+// XOR / OR reductions across the 8 lanes of a block group (lanes 8g..8g+7 of the wave), with DPP
+// lane moves: swap neighbours, swap pairs (now every lane of a quad holds the quad's value), then
+// the half-row mirror pairs each lane with one of the other quad. All 8 lanes of a group must be
+// active (the callers' conditions are uniform per block).
+MMR_DEV uint32_t dpp_quad_swap1(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xf, 0xf, false); }
+MMR_DEV uint32_t dpp_quad_swap2(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xf, 0xf, false); }
+MMR_DEV uint32_t dpp_half_mirror(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xf, 0xf, false); }
 MMR_DEV uint32_t group8_xor(uint32_t v) {
-    v ^= __shfl_xor(v, 1, 8);
-    v ^= __shfl_xor(v, 2, 8);
-    v ^= __shfl_xor(v, 4, 8);
-    return v;
+    v ^= dpp_quad_swap1(v);
+    v ^= dpp_quad_swap2(v);
+    return v ^ dpp_half_mirror(v);
 }
 MMR_DEV uint32_t group8_or(uint32_t v) {
-    v |= __shfl_xor(v, 1, 8);
-    v |= __shfl_xor(v, 2, 8);
-    v |= __shfl_xor(v, 4, 8);
-    return v;
+    v |= dpp_quad_swap1(v);
+    v |= dpp_quad_swap2(v);
+    return v | dpp_half_mirror(v);
 }
 
 // Agent-scope acquire / release on work-queue counters (coherent across the XCDs' L2 caches).

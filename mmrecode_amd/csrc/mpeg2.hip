@@ -82,6 +82,36 @@ MMR_DEV void ref_row_bytes(const RefRow& r, uint32_t o, uint32_t& a0, uint32_t& 
     b1 = __builtin_amdgcn_alignbyte(s8, a1, 1);
 }
 
+// The plane record of a lane's block (two 16-byte loads from the job).
+MMR_DEV Mpeg2Plane load_plane(const Mpeg2Plane* p) {
+    const int4 a = load_global_int4(p);
+    const int4 c = load_global_int4(reinterpret_cast<const char*>(p) + 16);
+    Mpeg2Plane r;
+    r.dst = reinterpret_cast<uint8_t*>(((uint64_t)(uint32_t)a.y << 32) | (uint32_t)a.x);
+    r.fwd = reinterpret_cast<const uint8_t*>(((uint64_t)(uint32_t)a.w << 32) | (uint32_t)a.z);
+    r.bwd = reinterpret_cast<const uint8_t*>(((uint64_t)(uint32_t)c.y << 32) | (uint32_t)c.x);
+    r.stride = (uint32_t)c.z;
+    r.height = (uint32_t)c.w;
+    return r;
+}
+
+// chroma_motion (decoder.rs:1220-1222) on a packed (x, y) vector: both halves v / 2, truncated
+// toward zero.
+typedef uint16_t mmr_v2u16 __attribute__((ext_vector_type(2)));
+MMR_DEV uint32_t halve_mv(uint32_t packed) {
+    mmr_v2i16 v;
+    __builtin_memcpy(&v, &packed, 4);
+    mmr_v2u16 u;
+    __builtin_memcpy(&u, &packed, 4);
+    const mmr_v2u16 neg = u >> (mmr_v2u16){15, 15};
+    mmr_v2i16 n;
+    __builtin_memcpy(&n, &neg, 4);
+    v = (v + n) >> (mmr_v2i16){1, 1};
+    uint32_t out;
+    __builtin_memcpy(&out, &v, 4);
+    return out;
+}
+
 struct RefFetch {
     RefRow r0, r1;
     const MMR_GLOBAL uint8_t* row0;
@@ -115,7 +145,10 @@ MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int height, int x0, i
     return f;
 }
 
-// halfpel_sample (decoder.rs:1181-1206) for the 8 samples of the fetched row, as two dwords.
+// halfpel_sample (decoder.rs:1181-1206) for the 8 samples of the fetched row, as two dwords,
+// without branches on the half-sample flags: with p = fx ? avg2(a, b) : a and q the same on the
+// next row, the result is fy ? (fx ? avg4 : avg2(a, c)) : p, and avg4 = lerp(p, q, r) with r the
+// avg4_u8x4 correction; lerp(x, x, 0) = x covers the cases without averaging.
 MMR_DEV uint2 halfpel_row(const RefFetch& f) {
     RefRow r0 = f.r0, r1 = f.r1;
     uint32_t o = (uint32_t)(f.x & 3);
@@ -124,13 +157,20 @@ MMR_DEV uint2 halfpel_row(const RefFetch& f) {
         r1 = gather_ref_row(f.row1, f.stride, f.x);
         o = 0;
     }
-    uint32_t a0, a1, b0, b1;
-    ref_row_bytes(r0, o, a0, a1, b0, b1);
-    if (!f.fy) return f.fx ? make_uint2(avg2_u8x4(a0, b0), avg2_u8x4(a1, b1)) : make_uint2(a0, a1);
-    uint32_t c0, c1, d0, d1;
-    ref_row_bytes(r1, o, c0, c1, d0, d1);
-    if (!f.fx) return make_uint2(avg2_u8x4(a0, c0), avg2_u8x4(a1, c1));
-    return make_uint2(avg4_u8x4(a0, b0, c0, d0), avg4_u8x4(a1, b1, c1, d1));
+    uint32_t a[2], b[2], c[2], d[2];
+    ref_row_bytes(r0, o, a[0], a[1], b[0], b[1]);
+    ref_row_bytes(r1, o, c[0], c[1], d[0], d[1]);
+    const uint32_t rx = f.fx ? kOnes : 0u;
+    uint32_t out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t p = __builtin_amdgcn_lerp(a[h], f.fx ? b[h] : a[h], rx);
+        const uint32_t q = __builtin_amdgcn_lerp(c[h], f.fx ? d[h] : c[h], rx);
+        const uint32_t r4 = ~((a[h] ^ b[h]) | (c[h] ^ d[h])) & kOnes;
+        const uint32_t ry = f.fy ? (f.fx ? r4 : kOnes) : 0u;
+        out[h] = __builtin_amdgcn_lerp(p, f.fy ? q : p, ry);
+    }
+    return make_uint2(out[0], out[1]);
 }
 
 // Block order of a picture, row-local: the 6 * mb_w blocks of macroblock row r occupy positions
@@ -199,6 +239,16 @@ __global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2J
     }
 }
 
+// Scan position -> byte offset of the natural-order int32 coefficient (4 * natural index), for the
+// zig-zag and the alternate scan.
+__constant__ static const uint8_t kScanCoefOffset[2][64] = {
+    {0, 4, 32, 64, 36, 8, 12, 40, 68, 96, 128, 100, 72, 44, 16, 20, 48, 76, 104, 132, 160, 192, 164, 136, 108, 80, 52, 24,
+     28, 56, 84, 112, 140, 168, 196, 224, 228, 200, 172, 144, 116, 88, 60, 92, 120, 148, 176, 204, 232, 236, 208, 180,
+     152, 124, 156, 184, 212, 240, 244, 216, 188, 220, 248, 252},
+    {0, 32, 64, 96, 4, 36, 8, 40, 68, 100, 128, 160, 192, 224, 228, 196, 164, 132, 104, 72, 12, 44, 16, 48, 76, 108, 136,
+     168, 200, 232, 140, 172, 204, 236, 80, 112, 20, 52, 24, 56, 84, 116, 144, 176, 208, 240, 148, 180, 212, 244, 88, 120,
+     28, 60, 92, 124, 152, 184, 216, 248, 156, 188, 220, 252}};
+
 __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
                                                               const uint32_t* __restrict__ tile_job, uint32_t tile_base,
                                                               uint32_t tiles, uint32_t per_xcd,
@@ -221,12 +271,18 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     const int b = (int)(entry & 7u);
     const uint32_t mb = __umul24(mb_y, job.mb_w) + mb_x;
     mmr_recon_mpeg2_macroblock m;
+    uint32_t mv_words[2];   // forward, backward (x, y) as packed int16
     {
         static_assert(sizeof(mmr_recon_mpeg2_macroblock) == 16, "macroblock record is 4 dwords");
         const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb));
         const uint32_t w[4] = {q[0], q[1], q[2], q[3]};
         __builtin_memcpy(&m, w, sizeof m);
+        mv_words[0] = w[1];
+        mv_words[1] = w[2];
     }
+    // Plane of block b (Y for 0..3, Cb for 4, Cr for 5) with its reference planes.
+    const int plane = b < 4 ? 0 : b - 3;
+    const Mpeg2Plane pr = load_plane(job.plane + plane);
     const bool intra = (m.flags & MMR_MPEG2_MB_INTRA) != 0;
     const bool coded = active && (intra || ((m.cbp >> (5 - b)) & 1));
     double* tmp = s_tmp + g * kIdctTmpStride;
@@ -250,43 +306,47 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     const uint32_t index = m.block_offset + (intra ? (uint32_t)b : (uint32_t)__popc((m.cbp & 0x3fu) >> (6 - b)));
     const int16_t* lv = coded ? job.levels + (size_t)index * 64 : reinterpret_cast<const int16_t*>(job.matrices);
     const int4 raw = load_global_int4(lv + lane * 8);
-    const uint2 sraw = *reinterpret_cast<const uint2*>((job.alt_scan ? kAlternateScan : kZigzag) + lane * 8);
+    const uint2 sraw = *reinterpret_cast<const uint2*>(kScanCoefOffset[job.alt_scan ? 1 : 0] + lane * 8);
 
     // Geometry of block b and of the row this lane predicts / stores.
     const bool interlaced = (m.flags & MMR_MPEG2_MB_INTERLACED_DCT) != 0;
-    const int stride_y = (int)job.mb_w * 16, height_y = (int)job.mb_h * 16;
-    int plane, stride, height, x0, y0, step;
+    const int stride = (int)pr.stride, height = (int)pr.height;
+    int x0, y0, step;
     if (b < 4) {
-        plane = 0; stride = stride_y; height = height_y;
         x0 = (int)mb_x * 16 + (b & 1) * 8;
         y0 = (int)mb_y * 16 + (interlaced ? (b >> 1) : (b >> 1) * 8);
         step = interlaced ? 2 : 1;
     } else {
-        plane = b - 3; stride = stride_y / 2; height = height_y / 2;
-        x0 = (int)mb_x * 8; y0 = (int)mb_y * 8; step = 1;
+        x0 = (int)mb_x * 8;
+        y0 = (int)mb_y * 8;
+        step = 1;
     }
     const int yy = y0 + lane * step;
     const bool has_f = !intra && (m.flags & MMR_MPEG2_MB_FORWARD) != 0;
     const bool has_b = !intra && (m.flags & MMR_MPEG2_MB_BACKWARD) != 0;
-    int fmx = m.mv[0][0], fmy = m.mv[0][1], bmx = m.mv[1][0], bmy = m.mv[1][1];
-    if (b >= 4) {  // chroma_motion: v / 2 truncating toward zero
-        fmx /= 2; fmy /= 2; bmx /= 2; bmy /= 2;
+    uint32_t fmv = mv_words[0], bmv = mv_words[1];
+    if (b >= 4) {
+        fmv = halve_mv(fmv);
+        bmv = halve_mv(bmv);
     }
-    const uint8_t* fwd = pick3(job.fwd, plane);
-    const uint8_t* bwd = pick3(job.bwd, plane);
-    const RefFetch ff = fetch_ref(fwd, stride, height, x0, yy, fmx, fmy, has_f, job.refs_aligned != 0);
-    const RefFetch fb = fetch_ref(bwd, stride, height, x0, yy, bmx, bmy, has_b, job.refs_aligned != 0);
+    const int fmx = (int16_t)(fmv & 0xffffu), fmy = (int16_t)(fmv >> 16);
+    const int bmx = (int16_t)(bmv & 0xffffu), bmy = (int16_t)(bmv >> 16);
+    const RefFetch ff = fetch_ref(pr.fwd, stride, height, x0, yy, fmx, fmy, has_f, job.refs_aligned != 0);
+    const RefFetch fb = fetch_ref(pr.bwd, stride, height, x0, yy, bmx, bmy, has_b, job.refs_aligned != 0);
 
-    // Dequantisation in scan order (decode_intra_block / decode_non_intra_block): branch-free, with
-    // 24-bit multiplies. |level| < 2^16, W * qscale < 2^16, so every product is below 2^32 and
-    // equals the reference's i32 product for all inputs where that one does not overflow.
+    // Dequantisation in scan order (decode_intra_block / decode_non_intra_block), branch-free with
+    // 24-bit multiplies. Intra: |level| * W * qscale / 16; non-intra: (2 |level| + 1) * W * qscale
+    // / 32 for level != 0. Both are (k * W * qscale) >> 5 with k = 2 |level| + (non-intra and
+    // level != 0) = (|level| << 1) | min(|level|, non_intra), which is 0 for a zero level.
+    // |level| <= 2^15 and W * qscale < 2^15, so k * W * qscale < 2^32 is exact in 32 bits and equals
+    // the reference's i32 product wherever that one does not overflow.
     wave_lds_sync();   // wscan
     int32_t* coef = reinterpret_cast<int32_t*>(tmp);   // natural-order coefficients, then tmp, then residual
     if (coded) {
         const uint2 wrow = *reinterpret_cast<const uint2*>(wscan + 64 * (intra ? (component ? 2 : 0) : (component ? 3 : 1)) + lane * 8);
         const int16_t* l8 = reinterpret_cast<const int16_t*>(&raw);
         const uint8_t* s8 = reinterpret_cast<const uint8_t*>(&sraw);
-        const uint32_t qs = m.qscale, shift = intra ? 4u : 5u;
+        const uint32_t qs = m.qscale, non_intra = intra ? 0u : 1u;
         int32_t val[8];
         uint32_t x = 0;
 #pragma unroll
@@ -294,10 +354,9 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
             const int32_t level = l8[i];
             const uint32_t a = (uint32_t)(level < 0 ? -level : level);
             const uint32_t w = ((i < 4 ? wrow.x : wrow.y) >> (8 * (i & 3))) & 0xffu;
-            const uint32_t k = intra ? a : 2u * a + 1u;
-            const int32_t mag = (int32_t)(__umul24(k, __umul24(w, qs)) >> shift);
+            const uint32_t k = (a << 1) | min(a, non_intra);
+            const int32_t mag = (int32_t)(__umul24(k, __umul24(w, qs)) >> 5);
             int32_t v = clampi(level < 0 ? -mag : mag, -2048, 2047);
-            v = level == 0 ? 0 : v;
             if (i == 0 && intra && lane == 0) v = level << job.dc_shift;   // DC: predictor << (3 - intra_dc_precision)
             val[i] = v;
             x ^= (uint32_t)v;
@@ -306,8 +365,9 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
         // natural 63 in both scans.
         const uint32_t all = group8_xor(x);
         if (lane == 7) val[7] ^= (int32_t)((1u ^ all) & 1u);
+        char* coef_bytes = reinterpret_cast<char*>(coef);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) coef[s8[i]] = val[i];
+        for (int i = 0; i < 8; ++i) *reinterpret_cast<int32_t*>(coef_bytes + s8[i]) = val[i];
     }
     wave_lds_sync();
     int32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -332,44 +392,46 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     wave_lds_sync();
     if (!active) return;
 
-    // Row `lane` of the block: prediction (0 for intra) + residual, clamped to 0..255.
-    uint2 pred = make_uint2(0, 0);
-    if (has_f && has_b) {
-        const uint2 pf = halfpel_row(ff), pb = halfpel_row(fb);
-        pred = make_uint2(avg2_u8x4(pf.x, pb.x), avg2_u8x4(pf.y, pb.y));
-    } else if (has_f) {
-        pred = halfpel_row(ff);
-    } else if (has_b) {
-        pred = halfpel_row(fb);
+    // Row `lane` of the block: prediction (0 for intra) + residual, clamped to 0..255. Each
+    // direction is formed only where a lane of the wave predicts from it; the average of both is
+    // lerp(f, b, 1), a single direction is lerp(x, x, 0).
+    uint32_t pf[2] = {0u, 0u}, pb[2] = {0u, 0u};
+    if (has_f) {
+        const uint2 t = halfpel_row(ff);
+        pf[0] = t.x;
+        pf[1] = t.y;
     }
-    uint64_t out = (uint64_t)pred.x | ((uint64_t)pred.y << 32);
-    if (coded) {   // prediction + residual in 16-bit pairs, clamped to 0..255
+    if (has_b) {
+        const uint2 t = halfpel_row(fb);
+        pb[0] = t.x;
+        pb[1] = t.y;
+    }
+    const uint32_t rb = (has_f && has_b) ? kOnes : 0u;
+    uint32_t o32[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) o32[h] = __builtin_amdgcn_lerp(has_f ? pf[h] : pb[h], has_b ? pb[h] : pf[h], rb);
+    if (coded) {   // prediction + residual as 16-bit pairs, clamped to 0..255
         const int4 rr = *reinterpret_cast<const int4*>(blk + lane * 8);
         const uint32_t r32[4] = {(uint32_t)rr.x, (uint32_t)rr.y, (uint32_t)rr.z, (uint32_t)rr.w};
-        const uint32_t p32[2] = {pred.x, pred.y};
-        uint32_t o32[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            uint32_t packed[2];
+            uint32_t sums[2];
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                const uint32_t pw = (p32[h] >> (16 * k)) & 0xffffu;      // two prediction bytes
-                const mmr_v2i16 pv = {(int16_t)(pw & 0xffu), (int16_t)(pw >> 8)};
-                mmr_v2i16 rv;
+                // prediction bytes 2k, 2k + 1 zero-extended to 16 bits
+                const uint32_t pw = __builtin_amdgcn_perm(0u, o32[h], k ? 0x0c030c02u : 0x0c010c00u);
+                mmr_v2i16 pv, rv;
+                __builtin_memcpy(&pv, &pw, 4);
                 __builtin_memcpy(&rv, &r32[h * 2 + k], 4);
                 mmr_v2i16 v = pv + rv;                                    // |v| <= 511: no overflow
                 v = __builtin_elementwise_max(v, (mmr_v2i16){0, 0});
                 v = __builtin_elementwise_min(v, (mmr_v2i16){255, 255});
-                uint32_t vw;
-                __builtin_memcpy(&vw, &v, 4);
-                packed[k] = (vw & 0xffu) | ((vw >> 8) & 0xff00u);
+                __builtin_memcpy(&sums[k], &v, 4);
             }
-            o32[h] = packed[0] | (packed[1] << 16);
+            o32[h] = __builtin_amdgcn_perm(sums[1], sums[0], 0x06040200u);   // low bytes of the four halves
         }
-        out = (uint64_t)o32[0] | ((uint64_t)o32[1] << 32);
     }
-    uint8_t* dst = pick3(job.planes, plane);
-    store_row8(dst + (size_t)yy * stride + x0, out);
+    store_row8(pr.dst + (uint32_t)(yy * stride + x0), (uint64_t)o32[0] | ((uint64_t)o32[1] << 32));
 }
 
 hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_status,

@@ -48,6 +48,7 @@ struct TicketRecord {
     std::vector<char> host_jobs;      // kept alive until completion (async upload source)
     uint32_t* host_status = nullptr;  // pinned; device status word copied here
     bool finished = false;
+    bool reported = false;            // status returned to the caller (wait / poll / server completion)
     mmr_status status = MMR_STATUS_OK;
     double kernel_ms = -1.0;
 };
@@ -436,20 +437,25 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         j.mbs = p.macroblocks;
         j.levels = p.levels;
         for (int k = 0; k < 3; ++k) {
-            j.planes[k] = p.planes[k];
-            j.fwd[k] = p.forward[k] ? p.forward[k] : p.planes[k];
-            j.bwd[k] = p.backward[k] ? p.backward[k] : (p.forward[k] ? p.forward[k] : p.planes[k]);
+            mmr::Mpeg2Plane& pl = j.plane[k];
+            pl.dst = p.planes[k];
+            pl.fwd = p.forward[k] ? p.forward[k] : p.planes[k];
+            pl.bwd = p.backward[k] ? p.backward[k] : (p.forward[k] ? p.forward[k] : p.planes[k]);
         }
         j.fwd_src = per_frame ? -1 : p.forward_source;
         j.bwd_src = per_frame ? -1 : p.backward_source;
         j.mb_w = (p.width + 15) / 16;
         j.mb_h = (p.height + 15) / 16;
+        for (int k = 0; k < 3; ++k) {
+            j.plane[k].stride = k ? j.mb_w * 8 : j.mb_w * 16;
+            j.plane[k].height = k ? j.mb_h * 8 : j.mb_h * 16;
+        }
         j.alt_scan = p.alternate_scan;
         j.dc_shift = 3 - p.intra_dc_precision;
         j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
         j.refs_aligned = 1;
         for (int k = 0; k < 3; ++k)
-            if (!aligned(j.fwd[k], 4) || !aligned(j.bwd[k], 4)) j.refs_aligned = 0;
+            if (!aligned(j.plane[k].fwd, 4) || !aligned(j.plane[k].bwd, 4)) j.refs_aligned = 0;
     }
     // Dependency levels: a picture runs one launch after the latest picture it references.
     std::vector<uint32_t> level(count, 0);
@@ -571,7 +577,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::H264Job> jobs(count);
     std::vector<uint32_t> tiles;
-    uint32_t mbs = 0, bands = 0;
+    uint32_t bands = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
         if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
@@ -621,7 +627,6 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
                 for (int k = 0; k < 3; ++k)
                     if (!aligned(j.ref_planes[list][r][k], 4)) j.refs_aligned = 0;
         j.band_begin = per_frame ? 0 : bands;
-        mbs += p.width_mbs * p.height_mbs;
         bands += j.deblock ? (p.height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand : 0;
     }
     Submission sub(ctx, stream, cap);
@@ -894,6 +899,7 @@ mmr_status mmr_recon_wait(mmr_recon_context* ctx, uint64_t ticket) {
         return MMR_STATUS_OK;  // already retired
     }
     mmr_status s = finish(ctx, it->second);
+    it->second.reported = true;
     return s;
 }
 
@@ -913,6 +919,7 @@ mmr_status mmr_recon_poll(mmr_recon_context* ctx, uint64_t ticket, int32_t* out_
         return MMR_STATUS_OK;
     }
     *out_done = 1;
+    it->second.reported = true;
     return finish(ctx, it->second);
 }
 
@@ -921,10 +928,12 @@ mmr_status mmr_recon_synchronize(mmr_recon_context* ctx) {
     LOCK_AND_SET_DEVICE(ctx);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    // Errors already returned by wait / poll (or carried by server completions) are not repeated.
     mmr_status worst = MMR_STATUS_OK;
     for (auto& kv : ctx->tickets) {
         mmr_status s = finish(ctx, kv.second);
-        if (s != MMR_STATUS_OK && worst == MMR_STATUS_OK) worst = s;
+        if (s != MMR_STATUS_OK && worst == MMR_STATUS_OK && !kv.second.reported) worst = s;
+        kv.second.reported = true;
     }
     reap(ctx);
     return worst;
@@ -1302,6 +1311,7 @@ void server_loop(mmr_recon_server* s) {
                 for (uint64_t t : it->tickets) {
                     auto tr = s->ctx->tickets.find(t);
                     if (tr != s->ctx->tickets.end()) {
+                        tr->second.reported = true;   // carried by the frame completions
                         mmr_status x = finish(s->ctx, tr->second);
                         if (x && !st) st = x;
                     }

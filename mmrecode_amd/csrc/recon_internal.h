@@ -29,14 +29,23 @@ struct MjpegJob {
 };
 
 // ---------------------------------------------------------------- MPEG-2
+// One plane of an MPEG-2 picture with its reference planes; a lane loads the record of its
+// block's plane with two 16-byte loads.
+struct alignas(16) Mpeg2Plane {
+    uint8_t* dst;
+    const uint8_t* fwd;           // the plane itself when the picture has no forward reference
+    const uint8_t* bwd;
+    uint32_t stride;
+    uint32_t height;
+};
+static_assert(sizeof(Mpeg2Plane) == 32, "plane record is two 16-byte loads");
+
 struct Mpeg2Job {
+    Mpeg2Plane plane[3];          // Y, Cb, Cr
     const uint8_t* matrices;      // [4][64] natural
     uint32_t* order;              // [6 * macroblocks] coded-first block order (submission scratch)
     const mmr_recon_mpeg2_macroblock* mbs;
     const int16_t* levels;
-    uint8_t* planes[3];
-    const uint8_t* fwd[3];
-    const uint8_t* bwd[3];
     int32_t fwd_src;
     int32_t bwd_src;
     uint32_t mb_w;

@@ -101,6 +101,7 @@ def test_mpeg2_field_motion_reports_unsupported(ctx):
     with pytest.raises(mmrecode_amd.ReconError) as err:
         _mpeg2_gpu(ctx, units)
     assert err.value.status == abi.STATUS_UNSUPPORTED
+    ctx.synchronize()   # the error was returned by wait and is not repeated
 
 
 # ------------------------------------------------------------------ DV
