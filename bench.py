@@ -573,6 +573,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain one-picture-per-launch leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes instead of autotuning them")
     ap.add_argument("--served", action="store_true", help="also run the served mixed-workload sweep")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run device-to-device copies of 1 GiB (HBM byte-counter calibration)")
@@ -593,6 +594,13 @@ def main():
     w = WORKLOADS[args.workload](ctx)
     log(f"{w.pictures} pictures, {w.mpix:.1f} Mpix, {w.bytes / 1e9:.3f} GB algorithmic per step")
 
+    tuning = {"chosen": ctx.get_tuning(), "trials": None}
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        best, trials = ctx.autotune(w.codec, w.structs, repeats=3)
+        tuning = {"chosen": best, "trials": [{"tuning": t, "kernel_ms": round(ms, 4)} for t, ms in trials],
+                  "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"autotuned in {tuning['seconds']} s: {best}")
     for _ in range(args.warmup):
         ctx.wait(w.submit())
     ctx.synchronize()
@@ -674,6 +682,7 @@ def main():
                                      "note": "every picture of a batch completes with its launch"},
         "speedup_vs_plain": round(value / plain, 2) if plain else None,
         "prepared_graph": prepared,
+        "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None, "kernel": w.kernel, "kernel_ms": k_ms,

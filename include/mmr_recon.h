@@ -455,6 +455,51 @@ MMR_RECON_API mmr_status mmr_recon_prepared_launch(mmr_recon_context *context,
 MMR_RECON_API void mmr_recon_prepared_destroy(mmr_recon_context *context,
                                               mmr_recon_prepared *prepared);
 
+/* ---------------------------------------------------------------- tuning */
+
+/*
+ * New (no counterpart in the reference, whose reconstruction runs on the CPU): the launch shapes
+ * of the device kernels. Every setting reconstructs the same bytes; settings differ in speed
+ * only. A context starts from the defaults below; mmr_recon_set_tuning replaces them (for the
+ * submissions, prepared batches and server batches made afterwards), and mmr_recon_autotune
+ * times the candidates of one codec on a caller's batch and keeps the fastest.
+ */
+typedef struct mmr_recon_tuning {
+    size_t struct_size;
+    uint32_t mjpeg_group_blocks;         /* 8x8 blocks per workgroup: 16, 32 (default) or 64 */
+    uint32_t mpeg2_group_blocks;         /* same for MPEG-2 (default 32) */
+    uint32_t dv_group_blocks;            /* same for DV (default 32) */
+    uint32_t h264_deblock_band_rows;     /* macroblock rows per deblocking band, one wave each:
+                                            4, 8 or 16 */
+    uint32_t h264_deblock_groups_per_cu; /* persistent deblocking workgroups per compute unit: 1..4 */
+    uint32_t h264_deblock_publish;       /* macroblocks between two progress publications of a
+                                            band's last row to the next band: 1..32 */
+} mmr_recon_tuning;
+
+MMR_RECON_API mmr_status mmr_recon_get_tuning(mmr_recon_context *context, mmr_recon_tuning *out_tuning);
+/* MMR_STATUS_INVALID_ARGUMENT for a value outside the ranges above (the tuning is unchanged). */
+MMR_RECON_API mmr_status mmr_recon_set_tuning(mmr_recon_context *context, const mmr_recon_tuning *tuning);
+
+typedef struct mmr_recon_tuning_trial {
+    mmr_recon_tuning tuning;
+    double kernel_ms;                    /* median device time of the batch's kernels */
+} mmr_recon_tuning_trial;
+
+/*
+ * Reconstructs the batch (descriptors and flags as for the codec's submit call) with every
+ * candidate setting of that codec's fields (the other fields keep their current values): once
+ * untimed, then `repeats` timed times (0: 3). Keeps the candidate with the smallest median kernel
+ * time in the context and copies it to out_best (optional). trials (optional, trial_capacity
+ * entries) receives every candidate with its time, out_trials (optional) their number. Blocks
+ * until done; the outputs then hold the batch's reconstruction. A failing run returns its status
+ * and leaves the tuning as it was.
+ */
+MMR_RECON_API mmr_status mmr_recon_autotune(mmr_recon_context *context, uint32_t codec,
+                                            const void *descriptors, uint32_t count, uint32_t flags,
+                                            uint32_t repeats, mmr_recon_tuning *out_best,
+                                            mmr_recon_tuning_trial *trials, uint32_t trial_capacity,
+                                            uint32_t *out_trials);
+
 /* --------------------------------------------------------------- serving */
 
 /*

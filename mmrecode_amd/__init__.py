@@ -154,6 +154,38 @@ class Context:
     def submit_yuv_blend(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_yuv_blend_submit, abi.YuvBlendJob, jobs, flags)
 
+    # ---- tuning
+    TUNING_FIELDS = [f for f, _ in abi.Tuning._fields_ if f != "struct_size"]
+
+    def get_tuning(self):
+        """The context's launch shapes as a dict (mmr_recon_get_tuning)."""
+        t = abi.new(abi.Tuning)
+        check(self.lib.mmr_recon_get_tuning(self.handle, C.byref(t)))
+        return {f: getattr(t, f) for f in self.TUNING_FIELDS}
+
+    def set_tuning(self, **fields):
+        """Replaces the given launch-shape fields, keeping the others (mmr_recon_set_tuning)."""
+        cur = self.get_tuning()
+        unknown = set(fields) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(fields)
+        t = abi.new(abi.Tuning, **cur)
+        check(self.lib.mmr_recon_set_tuning(self.handle, C.byref(t)))
+
+    def autotune(self, codec, items, flags=0, repeats=3):
+        """Times every candidate launch shape of `codec` on a batch and keeps the fastest
+        (mmr_recon_autotune). Returns (best tuning dict, [(tuning dict, kernel_ms), ...])."""
+        struct = Server.STRUCTS[codec]
+        arr = (struct * len(items))(*items)
+        best = abi.new(abi.Tuning)
+        trials = (abi.TuningTrial * 64)()
+        n = C.c_uint32()
+        check(self.lib.mmr_recon_autotune(self.handle, codec, C.addressof(arr), len(items), flags, repeats, C.byref(best),
+                                          trials, 64, C.byref(n)))
+        as_dict = lambda t: {f: getattr(t, f) for f in self.TUNING_FIELDS}  # noqa: E731
+        return as_dict(best), [(as_dict(tr.tuning), tr.kernel_ms) for tr in trials[: n.value]]
+
     def prepare(self, codec, items, flags=0):
         """A Prepared batch of one codec's descriptors (uploaded once, launches captured in a graph)."""
         return Prepared(self, codec, items, flags)

@@ -305,6 +305,22 @@ class ServerStats(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("mjpeg_group_blocks", C.c_uint32),
+        ("mpeg2_group_blocks", C.c_uint32),
+        ("dv_group_blocks", C.c_uint32),
+        ("h264_deblock_band_rows", C.c_uint32),
+        ("h264_deblock_groups_per_cu", C.c_uint32),
+        ("h264_deblock_publish", C.c_uint32),
+    ]
+
+
+class TuningTrial(C.Structure):
+    _fields_ = [("tuning", Tuning), ("kernel_ms", C.c_double)]
+
+
 def new(struct_type, **fields):
     """Zero-initialised struct with struct_size set (if it has one) and fields assigned."""
     s = struct_type()
@@ -355,4 +371,8 @@ ENTRY_POINTS = [
     ("mmr_recon_prepare", C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     ("mmr_recon_prepared_launch", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     ("mmr_recon_prepared_destroy", None, [C.c_void_p, C.c_void_p]),
+    ("mmr_recon_get_tuning", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mmr_recon_set_tuning", C.c_int32, [C.c_void_p, C.c_void_p]),
+    ("mmr_recon_autotune", C.c_int32, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
 ]

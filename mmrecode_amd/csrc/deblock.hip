@@ -44,16 +44,35 @@ __constant__ static const int kTc0[3][52] = {
     {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 23, 25}};
 __constant__ static const int kDbChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
 
-constexpr int kDbWaves = 16;
 // Ring slots per wave: a row runs at most kDbRing-1 macroblocks ahead of the row below.
 constexpr int kDbRing = 32;
-// Macroblocks between two progress publications of a band's last row.
-constexpr int kDbPublish = 8;
-static_assert(kDbWaves == (int)kH264DeblockBand, "one band row per wave");
 
-MMR_DEV int db_chroma_qp(int luma_qp, int offset) {
-    const int index = clampi(luma_qp + offset, 0, 51);
-    return index < 30 ? index : kDbChromaQp[index - 30];
+// Per-workgroup LDS copies of the tables: tab[i] = ALPHA[i] | BETA[i] << 8 | TC0[0][i] << 16 |
+// TC0[1][i] << 21 | TC0[2][i] << 26, cqp[i] = the chroma QP of luma QP index i. Per-lane lookups in
+// __constant__ memory would be vector memory loads on every filtered edge.
+struct DbTables {
+    uint32_t tab[52];
+    int32_t cqp[52];
+};
+MMR_DEV void db_load_tables(DbTables& t) {
+    for (int i = threadIdx.x; i < 52; i += blockDim.x) {
+        t.tab[i] = (uint32_t)kAlpha[i] | ((uint32_t)kBeta[i] << 8) | ((uint32_t)kTc0[0][i] << 16) | ((uint32_t)kTc0[1][i] << 21) |
+                   ((uint32_t)kTc0[2][i] << 26);
+        t.cqp[i] = i < 30 ? i : kDbChromaQp[i - 30];
+    }
+}
+MMR_DEV int db_chroma_qp(const DbTables& t, int luma_qp, int offset) { return t.cqp[clampi(luma_qp + offset, 0, 51)]; }
+
+// Filter parameters of one edge class: alpha and TC0 from index A = clamp(qp + offset_a), beta
+// from index B = clamp(qp + offset_b).
+struct EdgeParams {
+    uint32_t a, b;   // tab[A], tab[B]
+};
+MMR_DEV EdgeParams edge_params(const DbTables& t, int qp, int offset_a, int offset_b) {
+    EdgeParams e;
+    e.a = t.tab[clampi(qp + offset_a, 0, 51)];
+    e.b = t.tab[clampi(qp + offset_b, 0, 51)];
+    return e;
 }
 MMR_DEV int iabs(int v) { return v < 0 ? -v : v; }
 
@@ -79,9 +98,8 @@ MMR_DEV void set_byte(uint32_t (&w)[NW], int i, int v) {
 }
 
 template <int k, int NW>
-MMR_DEV void filter_line(uint32_t (&w)[NW], int qp, int bs, bool chroma, int offset_a, int offset_b) {
-    const int ia = clampi(qp + offset_a, 0, 51), ib = clampi(qp + offset_b, 0, 51);
-    const int alpha = kAlpha[ia], beta = kBeta[ib];
+MMR_DEV void filter_line(uint32_t (&w)[NW], EdgeParams ep, int bs, bool chroma) {
+    const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
     const int p0 = get_byte(w, k - 1), p1 = get_byte(w, k - 2), p2 = get_byte(w, k - 3), p3 = get_byte(w, k - 4);
     const int q0 = get_byte(w, k), q1 = get_byte(w, k + 1), q2 = get_byte(w, k + 2), q3 = get_byte(w, k + 3);
     if (iabs(p0 - q0) >= alpha || iabs(p1 - p0) >= beta || iabs(q1 - q0) >= beta) return;
@@ -104,7 +122,7 @@ MMR_DEV void filter_line(uint32_t (&w)[NW], int qp, int bs, bool chroma, int off
             set_byte(w, k, (2 * q1 + q0 + p1 + 2) >> 2);
         }
     } else {
-        const int tc0 = kTc0[bs - 1][ia];
+        const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);
         const int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
         const int tc = tc0 + (chroma ? 1 : ((ap < beta) + (aq < beta)));
         const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
@@ -163,6 +181,8 @@ struct DbWave {
 
 // Per-lane prefetch of one macroblock step: own samples and the records its strengths need.
 struct DbPrefetch {
+    mmr_recon_h264_macroblock m;   // record of the macroblock
+    int qp_top;            // QP of the macroblock above (0 in the first row)
     uint32_t own_y;        // luma dword (row l>>2, cols 4*(l&3)) of the macroblock
     uint32_t own_c;        // chroma dword, lanes < 32 (comp l>>4, row (l>>1)&7, cols 4*(l&1))
     mmr_recon_h264_block P, Q;
@@ -194,6 +214,8 @@ MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPref
         f.own_c = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
     }
     const uint32_t a = (uint32_t)(y * MW + x);
+    f.m = db_mb(job, a);
+    f.qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
     f.edge_on = false;
     if (l < 32) {
         // lanes 0-15: vertical edge e = (l >> 2), segment l & 3; lanes 16-31: horizontal
@@ -210,10 +232,8 @@ MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPref
                 on = y > 0;
                 pm = on ? a - MW : a;
             }
-            if (on) {
-                const mmr_recon_h264_macroblock q = db_mb(job, a);
-                on = !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || db_mb(job, pm).slice_id == q.slice_id;   // external_edge_enabled
-            }
+            if (on)   // external_edge_enabled
+                on = !(f.m.flags & MMR_H264_MB_SLICE_EDGE_OFF) || db_mb(job, pm).slice_id == f.m.slice_id;
         }
         f.edge_on = on;
         const int pb = vertical ? luma_block_index(external ? 3 : e - 1, sgm) : luma_block_index(sgm, external ? 3 : e - 1);
@@ -262,20 +282,31 @@ MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint
 // One macroblock row. band_first: the top context comes from the picture (gprev: the previous
 // band's progress); band_last: no row below in this band (gself: progress for the next band, or
 // null for the picture's last row) -- the row writes its bottom sample rows itself.
-MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status,
-                    bool band_first, bool band_last, uint32_t* gprev, uint32_t* gself) {
+template <int ROWS>
+MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status,
+                    bool band_first, bool band_last, uint32_t* gprev, uint32_t* gself, uint32_t publish) {
     DbWave& S = waves[w];
-    const DbWave& Above = waves[(w + kDbWaves - 1) % kDbWaves];
+    const DbWave& Above = waves[(w + ROWS - 1) % ROWS];
     const int MW = (int)job.width_mbs, MH = (int)job.height_mbs;
     const int W = MW * 16, CW = MW * 8;
     const bool last_row = band_last;   // writes its own bottom rows
     uint32_t seen = 0;
     DbPrefetch cur, nxt;
     db_prefetch(job, MW, 0, y, l, cur);
+    // Lanes 0-15 filter luma lines, 16-23 Cb and 24-31 Cr lines (rows in the vertical pass, columns
+    // in the horizontal one); their QPs go through the chroma QP table.
+    const bool chroma_lane = l >= 16;
+    const int chroma_off = job.chroma_qp_offset[(l >> 3) & 1];
+    int qp_left = 0;
     for (int x = 0; x < MW; ++x) {
-        const uint32_t a = (uint32_t)(y * MW + x);
-        const mmr_recon_h264_macroblock m = db_mb(job, a);
+        const mmr_recon_h264_macroblock m = cur.m;
         const bool filter = !(m.flags & MMR_H264_MB_NO_DEBLOCK);
+        // filter parameters of the left edge, the internal edges and the top edge
+        auto lane_qp = [&](int qp) { return chroma_lane ? db_chroma_qp(T, qp, chroma_off) : qp; };
+        const int qpc = lane_qp(m.qp);
+        const EdgeParams ev = edge_params(T, (lane_qp(qp_left) + qpc + 1) >> 1, m.offset_a, m.offset_b);
+        const EdgeParams ei = edge_params(T, qpc, m.offset_a, m.offset_b);
+        const EdgeParams eh = edge_params(T, (lane_qp(cur.qp_top) + qpc + 1) >> 1, m.offset_a, m.offset_b);
         // own samples into the window (rows 4.., cols 4..)
         *reinterpret_cast<uint32_t*>(&S.ly[(4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]) = cur.own_y;
         if (l < 32) {
@@ -290,7 +321,6 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
             else S.bs_h[e][sgm] = bs;
         }
         if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
-        const int qp_left = x > 0 ? as_global(job.mbs)[a - 1].qp : 0, qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
         // top context from the row above (slot x), after it finished macroblock x + 1
         if (y > 0 && band_first) {
             if (!db_wait_band(gprev, (uint32_t)min(x + 2, MW), seen, status)) return;
@@ -316,7 +346,6 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
         }
         wave_sync();
         if (filter) {
-            const int oa = m.offset_a, ob = m.offset_b;
             const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
             // vertical edges, one line of the window per lane, held in registers: lanes 0-15 luma
             // rows (20 samples: 4 of left context + 16), 16-23 Cb rows, 24-31 Cr rows (12 samples)
@@ -325,25 +354,22 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
                 uint32_t v[5];
 #pragma unroll
                 for (int i = 0; i < 5; ++i) v[i] = row[i];
-                const int sg = l >> 2, qpe = (qp_left + m.qp + 1) >> 1, qpi = m.qp;
-                if (S.bs_v[0][sg]) filter_line<4>(v, qpe, S.bs_v[0][sg], false, oa, ob);
-                if (!t8 && S.bs_v[1][sg]) filter_line<8>(v, qpi, S.bs_v[1][sg], false, oa, ob);
-                if (S.bs_v[2][sg]) filter_line<12>(v, qpi, S.bs_v[2][sg], false, oa, ob);
-                if (!t8 && S.bs_v[3][sg]) filter_line<16>(v, qpi, S.bs_v[3][sg], false, oa, ob);
+                const int sg = l >> 2;
+                if (S.bs_v[0][sg]) filter_line<4>(v, ev, S.bs_v[0][sg], false);
+                if (!t8 && S.bs_v[1][sg]) filter_line<8>(v, ei, S.bs_v[1][sg], false);
+                if (S.bs_v[2][sg]) filter_line<12>(v, ei, S.bs_v[2][sg], false);
+                if (!t8 && S.bs_v[3][sg]) filter_line<16>(v, ei, S.bs_v[3][sg], false);
 #pragma unroll
                 for (int i = 0; i < 5; ++i) row[i] = v[i];
             } else if (l < 32) {
                 const int comp = (l - 16) >> 3, r = (l - 16) & 7;
-                const int off = job.chroma_qp_offset[comp];
                 uint32_t* row = reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + r) * 12]);
                 uint32_t v[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i) v[i] = row[i];
                 const int sg = r >> 1;
-                const int qpe = (db_chroma_qp(qp_left, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                const int qpi = db_chroma_qp(m.qp, off);
-                if (S.bs_v[0][sg]) filter_line<4>(v, qpe, S.bs_v[0][sg], true, oa, ob);
-                if (S.bs_v[2][sg]) filter_line<8>(v, qpi, S.bs_v[2][sg], true, oa, ob);
+                if (S.bs_v[0][sg]) filter_line<4>(v, ev, S.bs_v[0][sg], true);
+                if (S.bs_v[2][sg]) filter_line<8>(v, ei, S.bs_v[2][sg], true);
 #pragma unroll
                 for (int i = 0; i < 3; ++i) row[i] = v[i];
             }
@@ -356,26 +382,23 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
 #pragma unroll
                 for (int i = 0; i < 5; ++i)
                     v[i] = col[(4 * i) * 20] | (col[(4 * i + 1) * 20] << 8) | (col[(4 * i + 2) * 20] << 16) | ((uint32_t)col[(4 * i + 3) * 20] << 24);
-                const int sg = l >> 2, qpe = (qp_top + m.qp + 1) >> 1, qpi = m.qp;
-                if (S.bs_h[0][sg]) filter_line<4>(v, qpe, S.bs_h[0][sg], false, oa, ob);
-                if (!t8 && S.bs_h[1][sg]) filter_line<8>(v, qpi, S.bs_h[1][sg], false, oa, ob);
-                if (S.bs_h[2][sg]) filter_line<12>(v, qpi, S.bs_h[2][sg], false, oa, ob);
-                if (!t8 && S.bs_h[3][sg]) filter_line<16>(v, qpi, S.bs_h[3][sg], false, oa, ob);
+                const int sg = l >> 2;
+                if (S.bs_h[0][sg]) filter_line<4>(v, eh, S.bs_h[0][sg], false);
+                if (!t8 && S.bs_h[1][sg]) filter_line<8>(v, ei, S.bs_h[1][sg], false);
+                if (S.bs_h[2][sg]) filter_line<12>(v, ei, S.bs_h[2][sg], false);
+                if (!t8 && S.bs_h[3][sg]) filter_line<16>(v, ei, S.bs_h[3][sg], false);
 #pragma unroll
                 for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)get_byte(v, i);
             } else if (l < 32) {
                 const int comp = (l - 16) >> 3, c = (l - 16) & 7;
-                const int off = job.chroma_qp_offset[comp];
                 uint8_t* col = &S.lc[comp][4 + c];
                 uint32_t v[3];
 #pragma unroll
                 for (int i = 0; i < 3; ++i)
                     v[i] = col[(4 * i) * 12] | (col[(4 * i + 1) * 12] << 8) | (col[(4 * i + 2) * 12] << 16) | ((uint32_t)col[(4 * i + 3) * 12] << 24);
                 const int sg = c >> 1;
-                const int qpe = (db_chroma_qp(qp_top, off) + db_chroma_qp(m.qp, off) + 1) >> 1;
-                const int qpi = db_chroma_qp(m.qp, off);
-                if (S.bs_h[0][sg]) filter_line<4>(v, qpe, S.bs_h[0][sg], true, oa, ob);
-                if (S.bs_h[2][sg]) filter_line<8>(v, qpi, S.bs_h[2][sg], true, oa, ob);
+                if (S.bs_h[0][sg]) filter_line<4>(v, eh, S.bs_h[0][sg], true);
+                if (S.bs_h[2][sg]) filter_line<8>(v, ei, S.bs_h[2][sg], true);
 #pragma unroll
                 for (int i = 1; i < 11; ++i) col[i * 12] = (uint8_t)get_byte(v, i);
             }
@@ -451,7 +474,7 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
         }
         wave_sync();
         if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (gself && ((x + 1) % kDbPublish == 0 || x == MW - 1))
+        if (gself && ((x + 1) % publish == 0 || x == MW - 1))
             __hip_atomic_store(gself, (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         // shift the window left by one macroblock (its right 4 columns become the left context)
         if (l < 20) {
@@ -461,42 +484,64 @@ MMR_DEV void db_row(const H264Job& job, int y, int w, int l, DbWave* waves, uint
             *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][row * 12 + 8]);
         }
         wave_sync();
+        qp_left = m.qp;
         cur = nxt;
     }
 }
 
-__global__ __launch_bounds__(1024) void h264_deblock_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
-                                                            uint32_t total_bands, uint32_t* __restrict__ counters,
-                                                            uint32_t* __restrict__ status) {
-    __shared__ __align__(16) DbWave waves[kDbWaves];
-    __shared__ uint32_t prog[kDbWaves];
+// ROWS macroblock rows per band, one wave per row.
+template <int ROWS>
+__global__ __launch_bounds__(ROWS * 64) void h264_deblock_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
+                                                                 uint32_t total_bands, uint32_t* __restrict__ counters,
+                                                                 uint32_t publish, uint32_t* __restrict__ status) {
+    __shared__ __align__(16) DbWave waves[ROWS];
+    __shared__ uint32_t prog[ROWS];
     __shared__ uint32_t s_ticket;
+    __shared__ DbTables s_tables;
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    db_load_tables(s_tables);   // visible after the loop's first barrier
     for (;;) {
         if (threadIdx.x == 0) s_ticket = fetch_add_relaxed(&counters[0], 1);
-        if (threadIdx.x < kDbWaves) prog[threadIdx.x] = 0;
+        if (threadIdx.x < ROWS) prog[threadIdx.x] = 0;
         __syncthreads();
         const uint32_t t = s_ticket;
         if (t >= total_bands) break;
         const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.band_begin; })];
         const uint32_t band = t - job.band_begin;
-        const int y = (int)(band * kDbWaves) + w;
+        const int y = (int)(band * ROWS) + w;
         if (job.deblock && y < (int)job.height_mbs) {
-            const bool first = w == 0, last = w == kDbWaves - 1 || y == (int)job.height_mbs - 1;
+            const bool first = w == 0, last = w == ROWS - 1 || y == (int)job.height_mbs - 1;
             uint32_t* gprev = first && band > 0 ? &counters[1 + t - 1] : nullptr;
             uint32_t* gself = last && y < (int)job.height_mbs - 1 ? &counters[1 + t] : nullptr;
-            db_row(job, y, w, l, waves, prog, status, first, last, gprev, gself);
+            db_row<ROWS>(job, s_tables, y, w, l, waves, prog, status, first, last, gprev, gself, publish);
         }
         __syncthreads();
     }
 }
 
+bool h264_deblock_rows_supported(uint32_t rows) { return rows == 4 || rows == 8 || rows == 16; }
+
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
-                               uint32_t groups, uint32_t* d_status, hipStream_t s) {
+                               uint32_t groups, uint32_t band_rows, uint32_t publish, uint32_t* d_status, hipStream_t s) {
     if (job_count == 0 || total_bands == 0) return hipSuccess;
     if (groups > total_bands) groups = total_bands;
-    hipLaunchKernelGGL(h264_deblock_kernel, dim3(groups), dim3(kDbWaves * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
-                       d_status);
+    if (publish == 0) publish = 1;
+    switch (band_rows) {
+    case 4:
+        hipLaunchKernelGGL(h264_deblock_kernel<4>, dim3(groups), dim3(4 * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
+                           publish, d_status);
+        break;
+    case 8:
+        hipLaunchKernelGGL(h264_deblock_kernel<8>, dim3(groups), dim3(8 * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
+                           publish, d_status);
+        break;
+    case 16:
+        hipLaunchKernelGGL(h264_deblock_kernel<16>, dim3(groups), dim3(16 * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
+                           publish, d_status);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -76,11 +76,10 @@ __constant__ static const uint8_t kAlternateScan[64] = {
     27, 34, 42, 50, 58, 35, 43, 51, 59, 20, 28, 5, 13, 6, 14, 21, 29, 36, 44, 52, 60, 37, 45, 53,
     61, 22, 30, 7, 15, 23, 31, 38, 46, 54, 62, 39, 47, 55, 63};
 
-// Blocks handled by one 256-thread workgroup in the 8-lanes-per-block kernels. Work is split into
-// tiles of 32 blocks that never straddle two frames; the host uploads a tile -> job table with the
-// jobs, so a workgroup finds its (uniform) job with one scalar load.
-constexpr int kBlocksPerGroup = 32;
-constexpr int kGroupThreads = 256;
+// The 8-lanes-per-block kernels (MJPEG, MPEG-2, DV) take BPG = 16, 32 or 64 blocks per workgroup
+// (a template parameter; the launch shape is a tuning choice). Work is split into tiles of BPG
+// blocks that never straddle two frames; the host uploads a tile -> job table with the jobs, so a
+// workgroup finds its (uniform) job with one scalar load.
 
 // Workgroups are dispatched to the eight XCDs of an MI355X round robin (workgroup i runs on XCD
 // i % 8), and every XCD has its own L2. xcd_tile remaps the workgroup index so that each XCD walks a

@@ -67,11 +67,13 @@ MMR_DEV void dv_mb_coordinates(uint32_t system, int seq, int slot, int mb, int* 
     }
 }
 
-__global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restrict__ jobs,
-                                                           const uint32_t* __restrict__ tile_job, uint32_t tile_base,
-                                                           uint32_t tiles, uint32_t per_xcd) {
-    __shared__ __align__(16) int32_t coef[kBlocksPerGroup][64];
-    __shared__ __align__(16) uint8_t pix[kBlocksPerGroup][64];
+// BPG 8x8 blocks (8 lanes each) per workgroup.
+template <int BPG>
+__global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ jobs,
+                                                     const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                     uint32_t tiles, uint32_t per_xcd) {
+    __shared__ __align__(16) int32_t coef[BPG][64];
+    __shared__ __align__(16) uint8_t pix[BPG][64];
     __shared__ double s_cos16[64];
     __shared__ double s_cucv[64];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(kGroupThreads) void dv_kernel(const DvJob* __restri
     }
     const uint32_t tile = tile_base + local;
     const DvJob& job = jobs[tile_job[tile]];
-    const uint32_t blk = (tile - job.tile_begin) * kBlocksPerGroup + g;
+    const uint32_t blk = (tile - job.tile_begin) * BPG + g;
     const bool active = blk < job.block_count;
     const uint8_t info = active ? as_global(job.info)[blk] : 0;
     const int mode248 = info & 1, cls = (info >> 1) & 3, qno = (info >> 3) & 15;
@@ -231,10 +233,16 @@ hipError_t dv_upload_tables(hipStream_t s) {
     return e;
 }
 
-hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
+                     hipStream_t s) {
     if (tiles == 0) return hipSuccess;
     const uint32_t per = xcd_per(tiles);
-    hipLaunchKernelGGL(dv_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
+    switch (group_blocks) {
+    case 16: hipLaunchKernelGGL(dv_kernel<16>, dim3(per * kXcds), dim3(16 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    case 32: hipLaunchKernelGGL(dv_kernel<32>, dim3(per * kXcds), dim3(32 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    case 64: hipLaunchKernelGGL(dv_kernel<64>, dim3(per * kXcds), dim3(64 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -17,18 +17,20 @@
 
 namespace mmr {
 
-__global__ __launch_bounds__(kGroupThreads) void mjpeg_blocks_kernel(const MjpegJob* __restrict__ jobs,
-                                                                      const uint32_t* __restrict__ tile_job,
-                                                                      uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
+// BPG 8x8 blocks (8 lanes each) per workgroup.
+template <int BPG>
+__global__ __launch_bounds__(BPG * 8) void mjpeg_blocks_kernel(const MjpegJob* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ tile_job,
+                                                                uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
     // Per block group: coefficients (int32, natural order), then the IDCT's row-pass output, then
     // the output samples, one after the other in the same LDS area.
-    __shared__ __align__(16) double s_tmp[kBlocksPerGroup * kIdctTmpStride];
+    __shared__ __align__(16) double s_tmp[BPG * kIdctTmpStride];
     const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
     if (local >= tiles) return;
     const uint32_t tile = tile_base + local;
     const MjpegJob& job = jobs[tile_job[tile]];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
-    const uint32_t b = (tile - job.tile_begin) * kBlocksPerGroup + g;
+    const uint32_t b = (tile - job.tile_begin) * BPG + g;
     const bool active = b < job.block_count;
     const uint32_t mcu = active ? b / job.blocks_per_mcu : 0, k = active ? b % job.blocks_per_mcu : 0;
     const int comp = job.blk_comp[k];
@@ -79,11 +81,16 @@ __global__ __launch_bounds__(kGroupThreads) void mjpeg_blocks_kernel(const Mjpeg
     }
 }
 
-hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
+                        hipStream_t s) {
     if (tiles == 0) return hipSuccess;
     const uint32_t per = xcd_per(tiles);
-    hipLaunchKernelGGL(mjpeg_blocks_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles,
-                       per);
+    switch (group_blocks) {
+    case 16: hipLaunchKernelGGL(mjpeg_blocks_kernel<16>, dim3(per * kXcds), dim3(16 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    case 32: hipLaunchKernelGGL(mjpeg_blocks_kernel<32>, dim3(per * kXcds), dim3(32 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    case 64: hipLaunchKernelGGL(mjpeg_blocks_kernel<64>, dim3(per * kXcds), dim3(64 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -249,14 +249,15 @@ __constant__ static const uint8_t kScanCoefOffset[2][64] = {
      168, 200, 232, 140, 172, 204, 236, 80, 112, 20, 52, 24, 56, 84, 116, 144, 176, 208, 240, 148, 180, 212, 244, 88, 120,
      28, 60, 92, 124, 152, 184, 216, 248, 156, 188, 220, 252}};
 
-__global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
-                                                              const uint32_t* __restrict__ tile_job, uint32_t tile_base,
-                                                              uint32_t tiles, uint32_t per_xcd,
-                                                              uint32_t* __restrict__ status) {
+// BPG 8x8 blocks (8 lanes each) per workgroup.
+template <int BPG>
+__global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                        uint32_t tiles, uint32_t per_xcd, uint32_t* __restrict__ status) {
     // Per block group: natural-order coefficients, then the IDCT row-pass output, then the residual
     // rows, one after the other in the same LDS area.
-    __shared__ __align__(16) double s_tmp[kBlocksPerGroup * kIdctTmpStride];
-    __shared__ __align__(16) uint8_t s_wscan[kGroupThreads / 64][256];
+    __shared__ __align__(16) double s_tmp[BPG * kIdctTmpStride];
+    __shared__ __align__(16) uint8_t s_wscan[BPG / 8][256];
     const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
     if (local >= tiles) return;
     const uint32_t tile = tile_base + local;
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(kGroupThreads) void mpeg2_kernel(const Mpeg2Job* __
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
     const uint32_t mbs = job.mb_w * job.mb_h;
     // Block of this group, from the picture's coded-first block order.
-    const uint32_t pos = (tile - job.tile_begin) * kBlocksPerGroup + (uint32_t)g;
+    const uint32_t pos = (tile - job.tile_begin) * BPG + (uint32_t)g;
     const bool active = pos < mbs * 6;
     const uint32_t entry = as_global(job.order)[active ? pos : 0];
     const uint32_t mb_x = (entry >> 3) & 0x3ffu, mb_y = entry >> 13;
@@ -444,11 +445,15 @@ hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32
 }
 
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
-                        uint32_t* d_status, hipStream_t s) {
+                        uint32_t group_blocks, uint32_t* d_status, hipStream_t s) {
     if (tiles == 0) return hipSuccess;
     const uint32_t per = xcd_per(tiles);
-    hipLaunchKernelGGL(mpeg2_kernel, dim3(per * kXcds), dim3(kGroupThreads), 0, s, d_jobs, d_tile_job, tile_base, tiles, per,
-                       d_status);
+    switch (group_blocks) {
+    case 16: hipLaunchKernelGGL(mpeg2_kernel<16>, dim3(per * kXcds), dim3(16 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per, d_status); break;
+    case 32: hipLaunchKernelGGL(mpeg2_kernel<32>, dim3(per * kXcds), dim3(32 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per, d_status); break;
+    case 64: hipLaunchKernelGGL(mpeg2_kernel<64>, dim3(per * kXcds), dim3(64 * 8), 0, s, d_jobs, d_tile_job, tile_base, tiles, per, d_status); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -68,6 +68,7 @@ struct mmr_recon_context {
     uint64_t next_ticket = 1;
     std::map<uint64_t, TicketRecord> tickets;
     uint8_t* flat_scaling = nullptr;  // device Flat_16 scaling lists ([6][16] + [2][64])
+    mmr_recon_tuning tuning{};        // launch shapes used by the next submissions
 };
 
 namespace {
@@ -239,9 +240,11 @@ uint32_t groups_for(mmr_recon_context* ctx, uint32_t per_cu) {
 
 // Appends the tiles of one job (ceil(blocks / 32) entries holding `job`) to a tile -> job table
 // and returns the index of its first tile.
-uint32_t add_tiles(std::vector<uint32_t>& table, uint32_t job, uint32_t blocks) {
+// Appends the tiles of `blocks` work items (tiles of `per_tile`) of job `job` to a tile -> job
+// table; returns the job's first tile.
+uint32_t add_tiles(std::vector<uint32_t>& table, uint32_t job, uint32_t blocks, uint32_t per_tile = 32) {
     const uint32_t first = (uint32_t)table.size();
-    table.insert(table.end(), (blocks + 31) / 32, job);
+    table.insert(table.end(), (blocks + per_tile - 1) / per_tile, job);
     return first;
 }
 
@@ -358,6 +361,31 @@ mmr_status check_rgb_job(const mmr_recon_yuv_to_rgb_job& s) {
 
 namespace mmr_host {
 
+mmr_recon_tuning default_tuning() {
+    mmr_recon_tuning t;
+    memset(&t, 0, sizeof t);
+    t.struct_size = sizeof t;
+    t.mjpeg_group_blocks = 32;
+    t.mpeg2_group_blocks = 32;
+    t.dv_group_blocks = 32;
+    t.h264_deblock_band_rows = 16;
+    t.h264_deblock_groups_per_cu = 1;
+    t.h264_deblock_publish = 8;
+    return t;
+}
+
+mmr_status check_tuning(const mmr_recon_tuning& t) {
+    for (uint32_t b : {t.mjpeg_group_blocks, t.mpeg2_group_blocks, t.dv_group_blocks})
+        if (b != 16 && b != 32 && b != 64) return fail(MMR_STATUS_INVALID_ARGUMENT, "blocks per workgroup must be 16, 32 or 64 (got %u)", b);
+    if (!mmr::h264_deblock_rows_supported(t.h264_deblock_band_rows))
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "deblocking band rows must be 4, 8 or 16 (got %u)", t.h264_deblock_band_rows);
+    if (t.h264_deblock_groups_per_cu < 1 || t.h264_deblock_groups_per_cu > 4)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "deblocking groups per compute unit must be 1..4");
+    if (t.h264_deblock_publish < 1 || t.h264_deblock_publish > 32)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "deblocking publish interval must be 1..32");
+    return MMR_STATUS_OK;
+}
+
 mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
                                   uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
     CHECK_CTX(ctx);
@@ -366,11 +394,12 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<mmr::RgbJob> rgb;
     std::vector<uint32_t> tiles;
     uint32_t total_quads = 0;
+    const uint32_t bpg = ctx->tuning.mjpeg_group_blocks;
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t quads = 0;
         mmr_status s = build_mjpeg(frames[i], jobs[i], quads);
         if (s) return s;
-        jobs[i].tile_begin = add_tiles(tiles, i, jobs[i].block_count);
+        jobs[i].tile_begin = add_tiles(tiles, i, jobs[i].block_count, bpg);
         if (quads) {
             mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
             r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
@@ -393,14 +422,14 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
         size_t r = 0;
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_mjpeg(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + 31) / 32, stream);
+            e = mmr::launch_mjpeg(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + bpg - 1) / bpg, bpg, stream);
             if (e == hipSuccess && frames[i].rgb) {
                 e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, stream);
                 ++r;
             }
         }
     } else {
-        e = mmr::launch_mjpeg(dj, dt, 0, (uint32_t)tiles.size(), stream);
+        e = mmr::launch_mjpeg(dj, dt, 0, (uint32_t)tiles.size(), bpg, stream);
         if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, stream);
     }
     if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
@@ -412,6 +441,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     CHECK_CTX(ctx);
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    const uint32_t bpg = ctx->tuning.mpeg2_group_blocks;
     std::vector<mmr::Mpeg2Job> jobs(count);
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_mpeg2_picture& p = pics[i];
@@ -452,7 +482,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         }
         j.alt_scan = p.alternate_scan;
         j.dc_shift = 3 - p.intra_dc_precision;
-        j.tile_count = (j.mb_w * j.mb_h * 6 + 31) / 32;
+        j.tile_count = (j.mb_w * j.mb_h * 6 + bpg - 1) / bpg;
         j.refs_aligned = 1;
         for (int k = 0; k < 3; ++k)
             if (!aligned(j.plane[k].fwd, 4) || !aligned(j.plane[k].bwd, 4)) j.refs_aligned = 0;
@@ -473,14 +503,14 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<uint32_t> level_tile_first(levels + 1, 0);
     if (per_frame) {
         ordered = jobs;
-        for (uint32_t i = 0; i < count; ++i) ordered[i].tile_begin = add_tiles(tile_table, i, ordered[i].tile_count * 32);
+        for (uint32_t i = 0; i < count; ++i) ordered[i].tile_begin = add_tiles(tile_table, i, ordered[i].tile_count * bpg, bpg);
     } else {
         for (uint32_t lv = 0; lv < levels; ++lv) {
             level_tile_first[lv] = (uint32_t)tile_table.size();
             for (uint32_t i = 0; i < count; ++i)
                 if (level[i] == lv) {
                     mmr::Mpeg2Job j = jobs[i];
-                    j.tile_begin = add_tiles(tile_table, (uint32_t)ordered.size(), j.tile_count * 32);
+                    j.tile_begin = add_tiles(tile_table, (uint32_t)ordered.size(), j.tile_count * bpg, bpg);
                     ordered.push_back(j);
                 }
         }
@@ -515,12 +545,12 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
             e = mmr::launch_mpeg2_order(dj + i, 1, ordered[i].mb_h, sub.status_ptr(), stream);
             if (e == hipSuccess)
-                e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, sub.status_ptr(), stream);
+                e = mmr::launch_mpeg2(dj, dt, ordered[i].tile_begin, ordered[i].tile_count, bpg, sub.status_ptr(), stream);
         }
     } else {
         e = mmr::launch_mpeg2_order(dj, (uint32_t)ordered.size(), order_rows, sub.status_ptr(), stream);
         for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
-            e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv],
+            e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv], bpg,
                                   sub.status_ptr(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "MPEG-2 launch");
@@ -532,6 +562,7 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
     CHECK_CTX(ctx);
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    const uint32_t bpg = ctx->tuning.dv_group_blocks;
     std::vector<mmr::DvJob> jobs(count);
     std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
@@ -549,7 +580,7 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
         for (int k = 0; k < 3; ++k) j.planes[k] = f.planes[k];
         j.system = f.system;
         j.block_count = (f.system == MMR_DV_SYSTEM_625_50 ? 12u : 10u) * 27u * 5u * 6u;
-        j.tile_begin = add_tiles(tiles, i, j.block_count);
+        j.tile_begin = add_tiles(tiles, i, j.block_count, bpg);
     }
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::DvJob));
@@ -562,9 +593,9 @@ mmr_status dv_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_re
     hipError_t e = hipSuccess;
     if (per_frame) {
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_dv(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + 31) / 32, stream);
+            e = mmr::launch_dv(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + bpg - 1) / bpg, bpg, stream);
     } else {
-        e = mmr::launch_dv(dj, dt, 0, (uint32_t)tiles.size(), stream);
+        e = mmr::launch_dv(dj, dt, 0, (uint32_t)tiles.size(), bpg, stream);
     }
     if (e != hipSuccess) return hip_fail(e, "DV launch");
     return sub.end(out_ticket);
@@ -578,6 +609,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     std::vector<mmr::H264Job> jobs(count);
     std::vector<uint32_t> tiles;
     uint32_t bands = 0;
+    const uint32_t band_rows = ctx->tuning.h264_deblock_band_rows, publish = ctx->tuning.h264_deblock_publish;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
         if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
@@ -627,7 +659,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
                 for (int k = 0; k < 3; ++k)
                     if (!aligned(j.ref_planes[list][r][k], 4)) j.refs_aligned = 0;
         j.band_begin = per_frame ? 0 : bands;
-        bands += j.deblock ? (p.height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand : 0;
+        bands += j.deblock ? (p.height_mbs + band_rows - 1) / band_rows : 0;
     }
     Submission sub(ctx, stream, cap);
     size_t counters = 0;
@@ -644,14 +676,14 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     auto* dj = reinterpret_cast<const mmr::H264Job*>((const char*)sub.jobs_ptr() + jobs_off);
     auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
-    const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count;
+    const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count * ctx->tuning.h264_deblock_groups_per_cu;
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
             e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, (pics[i].width_mbs * pics[i].height_mbs + 3) / 4, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].deblock) {
-                const uint32_t nb = (pics[i].height_mbs + mmr::kH264DeblockBand - 1) / mmr::kH264DeblockBand;
-                e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, sub.status_ptr(), stream);
+                const uint32_t nb = (pics[i].height_mbs + band_rows - 1) / band_rows;
+                e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, band_rows, publish, sub.status_ptr(), stream);
                 c += 1 + pics[i].height_mbs;
             } else {
                 c += 1;
@@ -659,7 +691,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         }
     } else {
         e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
-        if (e == hipSuccess && bands) e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, sub.status_ptr(), stream);
+        if (e == hipSuccess && bands)
+            e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, band_rows, publish, sub.status_ptr(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "H.264 launch");
     return sub.end(out_ticket);
@@ -818,6 +851,7 @@ mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_co
     ctx->device = cfg.device_ordinal;
     ctx->queue_depth = cfg.queue_depth ? cfg.queue_depth : 256;
     ctx->persistent_groups = cfg.persistent_groups;
+    ctx->tuning = mmr_host::default_tuning();
     e = hipSetDevice(ctx->device);
     hipDeviceProp_t prop;
     if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
@@ -1128,6 +1162,118 @@ void mmr_recon_prepared_destroy(mmr_recon_context* ctx, mmr_recon_prepared* p) {
     if (p->cap.d_block) (void)hipFree(p->cap.d_block);
     if (p->cap.stream) (void)hipStreamDestroy(p->cap.stream);
     delete p;
+}
+
+}  // extern "C"
+
+// ====================================================================== tuning
+
+extern "C" {
+
+mmr_status mmr_recon_get_tuning(mmr_recon_context* ctx, mmr_recon_tuning* out) {
+    CHECK_CTX(ctx);
+    if (!out || out->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = ctx->tuning;
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_set_tuning(mmr_recon_context* ctx, const mmr_recon_tuning* in) {
+    CHECK_CTX(ctx);
+    if (!in || in->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
+    mmr_status s = mmr_host::check_tuning(*in);
+    if (s) return s;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuning = *in;
+    ctx->tuning.struct_size = sizeof(mmr_recon_tuning);
+    return MMR_STATUS_OK;
+}
+
+mmr_status mmr_recon_autotune(mmr_recon_context* ctx, uint32_t codec, const void* descriptors, uint32_t count, uint32_t flags,
+                              uint32_t repeats, mmr_recon_tuning* out_best, mmr_recon_tuning_trial* trials, uint32_t trial_capacity,
+                              uint32_t* out_trials) {
+    CHECK_CTX(ctx);
+    if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
+    if (repeats == 0) repeats = 3;
+    mmr_recon_tuning base;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        base = ctx->tuning;
+    }
+    // Candidates: the codec's own fields varied, the others as they are.
+    std::vector<mmr_recon_tuning> cands;
+    const uint32_t groups[3] = {16, 32, 64};
+    switch (codec) {
+    case MMR_CODEC_MJPEG:
+    case MMR_CODEC_MPEG2:
+    case MMR_CODEC_DV:
+        for (uint32_t g : groups) {
+            mmr_recon_tuning t = base;
+            (codec == MMR_CODEC_MJPEG ? t.mjpeg_group_blocks : codec == MMR_CODEC_MPEG2 ? t.mpeg2_group_blocks : t.dv_group_blocks) = g;
+            cands.push_back(t);
+        }
+        break;
+    case MMR_CODEC_H264:
+        for (uint32_t rows : {4u, 8u, 16u})
+            for (uint32_t per_cu : {1u, 2u})
+                for (uint32_t publish : {2u, 8u}) {
+                    mmr_recon_tuning t = base;
+                    t.h264_deblock_band_rows = rows;
+                    t.h264_deblock_groups_per_cu = per_cu;
+                    t.h264_deblock_publish = publish;
+                    cands.push_back(t);
+                }
+        break;
+    default:
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
+    }
+    auto submit = [&](uint64_t* ticket) -> mmr_status {
+        switch (codec) {
+        case MMR_CODEC_MJPEG: return mmr_recon_mjpeg_submit(ctx, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, ticket);
+        case MMR_CODEC_MPEG2: return mmr_recon_mpeg2_submit(ctx, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, ticket);
+        case MMR_CODEC_DV: return mmr_recon_dv_submit(ctx, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, ticket);
+        default: return mmr_recon_h264_submit(ctx, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, ticket);
+        }
+    };
+    auto restore = [&]() {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->tuning = base;
+    };
+    size_t best = 0;
+    std::vector<double> medians(cands.size());
+    for (size_t c = 0; c < cands.size(); ++c) {
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = cands[c];
+        }
+        std::vector<double> ms;
+        for (uint32_t r = 0; r <= repeats; ++r) {
+            uint64_t t = 0;
+            mmr_status s = submit(&t);
+            if (s == MMR_STATUS_OK) s = mmr_recon_wait(ctx, t);
+            double k = 0.0;
+            if (s == MMR_STATUS_OK) s = mmr_recon_last_kernel_ms(ctx, t, &k);
+            if (s != MMR_STATUS_OK) {
+                restore();
+                return s;
+            }
+            if (r > 0) ms.push_back(k);   // the first run is untimed
+        }
+        std::sort(ms.begin(), ms.end());
+        medians[c] = ms[ms.size() / 2];
+        if (medians[c] < medians[best]) best = c;
+        if (trials && c < trial_capacity) {
+            trials[c].tuning = cands[c];
+            trials[c].kernel_ms = medians[c];
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->tuning = cands[best];
+    }
+    if (out_best) *out_best = cands[best];
+    if (out_trials) *out_trials = (uint32_t)cands.size();
+    return MMR_STATUS_OK;
 }
 
 }  // extern "C"

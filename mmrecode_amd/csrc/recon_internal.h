@@ -88,7 +88,7 @@ struct H264Job {
     int32_t chroma_qp_offset[2];
     uint32_t tile_begin;          // first entry of this picture in the tile -> job table (4 macroblocks per tile)
     uint32_t refs_aligned;        // every reference plane is 4-byte aligned
-    uint32_t band_begin;          // first deblocking band ticket of this picture (bands of 16 macroblock rows)
+    uint32_t band_begin;          // first deblocking band ticket of this picture
 };
 
 // ---------------------------------------------------------------- colour / blend
@@ -112,20 +112,23 @@ struct CompositeJobDev {
 // Kernel launchers (stream-ordered). Each returns hipSuccess or the launch error.
 // The block kernels (MJPEG, MPEG-2, DV) run one workgroup per 32-block tile: workgroup i takes
 // tile tile_base + i, whose job index is d_tile_job[tile]. Tiles never straddle jobs.
-hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
+hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
+                        hipStream_t s);
 hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_status,
                               hipStream_t s);
 hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
-                        uint32_t* d_status, hipStream_t s);
-hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
+                        uint32_t group_blocks, uint32_t* d_status, hipStream_t s);
+hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
+                     hipStream_t s);
 hipError_t dv_upload_tables(hipStream_t s);
 hipError_t launch_h264_mc(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t* d_status,
                           hipStream_t s);
-// Deblocking in bands of kH264DeblockBand macroblock rows: d_counters[0] is the band ticket,
-// d_counters[1 + t] the progress of band t's last row (zeroed before the launch).
-constexpr uint32_t kH264DeblockBand = 16;
+// Deblocking in bands of `band_rows` (4, 8 or 16) macroblock rows: d_counters[0] is the band
+// ticket, d_counters[1 + t] the progress of band t's last row (zeroed before the launch), published
+// every `publish` macroblocks.
+bool h264_deblock_rows_supported(uint32_t rows);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
-                               uint32_t groups, uint32_t* d_status, hipStream_t s);
+                               uint32_t groups, uint32_t band_rows, uint32_t publish, uint32_t* d_status, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);

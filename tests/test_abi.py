@@ -37,6 +37,8 @@ STRUCTS = {
     "mmr_recon_request": abi.Request,
     "mmr_recon_frame_completion": abi.FrameCompletion,
     "mmr_recon_server_stats": abi.ServerStats,
+    "mmr_recon_tuning": abi.Tuning,
+    "mmr_recon_tuning_trial": abi.TuningTrial,
 }
 
 needs_cc = pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc is not available")
