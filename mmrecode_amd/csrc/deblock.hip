@@ -45,7 +45,10 @@ __constant__ static const int kTc0[3][52] = {
 __constant__ static const int kDbChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
 
 // Ring slots per wave: a row runs at most kDbRing-1 macroblocks ahead of the row below.
-constexpr int kDbRing = 32;
+constexpr int kDbRing = 16;
+// Row stride of the chroma windows: the luma one's, so one code path filters luma and chroma
+// lines (rows 12..19 and columns 12..19 of a chroma window are padding).
+constexpr int kLc = 20;
 
 // Per-workgroup LDS copies of the tables: tab[i] = ALPHA[i] | BETA[i] << 8 | TC0[0][i] << 16 |
 // TC0[1][i] << 21 | TC0[2][i] << 26, cqp[i] = the chroma QP of luma QP index i. Per-lane lookups in
@@ -97,40 +100,39 @@ MMR_DEV void set_byte(uint32_t (&w)[NW], int i, int v) {
     w[i >> 2] = (w[i >> 2] & ~(0xffu << sh)) | ((uint32_t)v << sh);
 }
 
-template <int k, int NW>
-MMR_DEV void filter_line(uint32_t (&w)[NW], EdgeParams ep, int bs, bool chroma) {
+template <int k>
+MMR_DEV void filter_line(uint32_t (&w)[5], EdgeParams ep, int bs, bool chroma) {
     const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
     const int p0 = get_byte(w, k - 1), p1 = get_byte(w, k - 2), p2 = get_byte(w, k - 3), p3 = get_byte(w, k - 4);
     const int q0 = get_byte(w, k), q1 = get_byte(w, k + 1), q2 = get_byte(w, k + 2), q3 = get_byte(w, k + 3);
-    if (iabs(p0 - q0) >= alpha || iabs(p1 - p0) >= beta || iabs(q1 - q0) >= beta) return;
-    if (bs == 4) {
-        const bool near = iabs(p0 - q0) < (alpha >> 2) + 2;
-        const bool sp = !chroma && iabs(p2 - p0) < beta && near;
-        const bool sq = !chroma && iabs(q2 - q0) < beta && near;
-        if (sp) {
-            set_byte(w, k - 1, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3);
-            set_byte(w, k - 2, (p2 + p1 + p0 + q0 + 2) >> 2);
-            set_byte(w, k - 3, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3);
-        } else {
-            set_byte(w, k - 1, (2 * p1 + p0 + q1 + 2) >> 2);
-        }
-        if (sq) {
-            set_byte(w, k, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3);
-            set_byte(w, k + 1, (p0 + q0 + q1 + q2 + 2) >> 2);
-            set_byte(w, k + 2, (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3);
-        } else {
-            set_byte(w, k, (2 * q1 + q0 + p1 + 2) >> 2);
-        }
-    } else {
-        const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);
-        const int ap = iabs(p2 - p0), aq = iabs(q2 - q0);
-        const int tc = tc0 + (chroma ? 1 : ((ap < beta) + (aq < beta)));
-        const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        set_byte(w, k - 1, clampi(p0 + delta, 0, 255));
-        set_byte(w, k, clampi(q0 - delta, 0, 255));
-        if (!chroma && ap < beta) set_byte(w, k - 2, clampi(p1 + clampi((p2 + ((p0 + q0 + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0), 0, 255));
-        if (!chroma && aq < beta) set_byte(w, k + 1, clampi(q1 + clampi((q2 + ((p0 + q0 + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0), 0, 255));
-    }
+    if (bs == 0 || iabs(p0 - q0) >= alpha || iabs(p1 - p0) >= beta || iabs(q1 - q0) >= beta) return;
+    // Both filters are evaluated and the results selected (no branches inside a wave).
+    const bool luma = !chroma;
+    const bool ap = iabs(p2 - p0) < beta, aq = iabs(q2 - q0) < beta;
+    // bS < 4
+    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);   // bs 1..3 (unused for 4)
+    const int tc = tc0 + (chroma ? 1 : ((int)ap + (int)aq));
+    const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+    const int avg = (p0 + q0 + 1) >> 1;
+    const int np0 = clampi(p0 + delta, 0, 255), nq0 = clampi(q0 - delta, 0, 255);
+    const int np1 = (luma && ap) ? clampi(p1 + clampi((p2 + avg - (p1 << 1)) >> 1, -tc0, tc0), 0, 255) : p1;
+    const int nq1 = (luma && aq) ? clampi(q1 + clampi((q2 + avg - (q1 << 1)) >> 1, -tc0, tc0), 0, 255) : q1;
+    // bS == 4
+    const bool near = iabs(p0 - q0) < (alpha >> 2) + 2;
+    const bool sp = luma && ap && near, sq = luma && aq && near;
+    const int sp0 = sp ? (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3 : (2 * p1 + p0 + q1 + 2) >> 2;
+    const int sp1 = sp ? (p2 + p1 + p0 + q0 + 2) >> 2 : p1;
+    const int sp2 = sp ? (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3 : p2;
+    const int sq0 = sq ? (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3 : (2 * q1 + q0 + p1 + 2) >> 2;
+    const int sq1 = sq ? (p0 + q0 + q1 + q2 + 2) >> 2 : q1;
+    const int sq2 = sq ? (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3 : q2;
+    const bool strong = bs == 4;
+    set_byte(w, k - 3, strong ? sp2 : p2);
+    set_byte(w, k - 2, strong ? sp1 : np1);
+    set_byte(w, k - 1, strong ? sp0 : np0);
+    set_byte(w, k, strong ? sq0 : nq0);
+    set_byte(w, k + 1, strong ? sq1 : nq1);
+    set_byte(w, k + 2, strong ? sq2 : q2);
 }
 
 struct RefMotion {
@@ -172,22 +174,18 @@ MMR_DEV int luma_block_index(int x, int y) { return (y / 2) * 8 + (x / 2) * 4 + 
 
 struct DbWave {
     uint8_t ly[20 * 20];               // rows y*16-4 .., cols x*16-4 ..
-    uint8_t lc[2][12 * 12];            // rows y*8-4 .., cols x*8-4 ..
+    uint8_t lc[2][20 * kLc];           // rows y*8-4 .., cols x*8-4 .. (12 x 12 used)
     uint8_t ring_y[kDbRing][4 * 16];   // bottom 4 luma rows of a macroblock, for the row below
     uint8_t ring_c[kDbRing][2][4 * 8];
-    int bs_v[4][4];
-    int bs_h[4][4];
 };
 
-// Per-lane prefetch of one macroblock step: own samples and the records its strengths need.
+// Per-lane prefetch of one macroblock step: its record, own samples and boundary strengths.
 struct DbPrefetch {
     mmr_recon_h264_macroblock m;   // record of the macroblock
     int qp_top;            // QP of the macroblock above (0 in the first row)
     uint32_t own_y;        // luma dword (row l>>2, cols 4*(l&3)) of the macroblock
     uint32_t own_c;        // chroma dword, lanes < 32 (comp l>>4, row (l>>1)&7, cols 4*(l&1))
-    mmr_recon_h264_block P, Q;
-    uint8_t p_flags, q_flags;
-    bool edge_on;
+    int4 bs;               // the macroblock's 32 strengths (h264_bs_kernel)
 };
 
 // Records read through the global address space (the job holds generic pointers).
@@ -208,41 +206,69 @@ MMR_DEV mmr_recon_h264_block db_block(const H264Job& job, size_t i) {
 
 MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPrefetch& f) {
     const int W = MW * 16, CW = MW * 8;
+    const uint32_t a = (uint32_t)(y * MW + x);
+    f.m = db_mb(job, a);
+    f.qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
+    f.bs = load_global_int4(job.bs + (size_t)a * 4);
     f.own_y = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
     if (l < 32) {
         const int comp = l >> 4, row = (l >> 1) & 7;
         f.own_c = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
     }
-    const uint32_t a = (uint32_t)(y * MW + x);
-    f.m = db_mb(job, a);
-    f.qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
-    f.edge_on = false;
-    if (l < 32) {
-        // lanes 0-15: vertical edge e = (l >> 2), segment l & 3; lanes 16-31: horizontal
-        const bool vertical = l < 16;
-        const int e = (l & 15) >> 2, sgm = l & 3;
-        const bool external = e == 0;
-        bool on = true;
-        uint32_t pm = a;
-        if (external) {
-            if (vertical) {
-                on = x > 0;
-                pm = on ? a - 1 : a;
-            } else {
-                on = y > 0;
-                pm = on ? a - MW : a;
-            }
-            if (on)   // external_edge_enabled
-                on = !(f.m.flags & MMR_H264_MB_SLICE_EDGE_OFF) || db_mb(job, pm).slice_id == f.m.slice_id;
-        }
-        f.edge_on = on;
+}
+
+// Strength n of a macroblock's record: n = e * 4 + segment for vertical edge e, 16 + e * 4 +
+// segment for horizontal edge e; four bits each, eight per dword.
+MMR_DEV int bs_nibble(const int4& r, int n) {
+    const uint32_t w = (uint32_t)(n < 8 ? r.x : n < 16 ? r.y : n < 24 ? r.z : r.w);
+    return (int)((w >> (4 * (n & 7))) & 0xfu);
+}
+
+// derive_boundary_strengths (deblock.rs:144-200) for every macroblock of the deblocked pictures,
+// ahead of the wavefront: one lane per (macroblock, edge segment), lanes 0-15 of a 32-lane group
+// the vertical edges (e = lane >> 2, segment lane & 3), 16-31 the horizontal ones. Edges that are
+// not filtered get strength 0: external edges on the picture border or across a slice edge with
+// filtering off there, the internal edges 1 and 3 of 8x8-transform macroblocks, and every edge of
+// a macroblock with filtering disabled. Writes 16 bytes per macroblock (job.bs).
+__global__ __launch_bounds__(128) void h264_bs_kernel(const H264Job* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                                      uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const H264Job& job = jobs[tile_job[tile]];
+    if (!job.deblock) return;
+    const uint32_t a = (tile - job.tile_begin) * 4 + (threadIdx.x >> 5);   // uniform over 32 lanes
+    const int MW = (int)job.width_mbs;
+    if (a >= (uint32_t)MW * job.height_mbs) return;
+    const int x = (int)(a % (uint32_t)MW), y = (int)(a / (uint32_t)MW), l = threadIdx.x & 31;
+    const mmr_recon_h264_macroblock q = db_mb(job, a);
+    const bool vertical = l < 16;
+    const int e = (l & 15) >> 2, sgm = l & 3;
+    const bool external = e == 0;
+    bool on = !(q.flags & MMR_H264_MB_NO_DEBLOCK) && !(e & 1 && (q.flags & MMR_H264_MB_TRANSFORM_8X8));
+    uint32_t pm = a;
+    if (external) {
+        on = on && (vertical ? x > 0 : y > 0);
+        pm = on ? (vertical ? a - 1 : a - (uint32_t)MW) : a;
+        if (on)   // external_edge_enabled
+            on = !(q.flags & MMR_H264_MB_SLICE_EDGE_OFF) || db_mb(job, pm).slice_id == q.slice_id;
+    }
+    int bs = 0;
+    if (on) {
         const int pb = vertical ? luma_block_index(external ? 3 : e - 1, sgm) : luma_block_index(sgm, external ? 3 : e - 1);
         const int qb = vertical ? luma_block_index(e, sgm) : luma_block_index(sgm, e);
-        f.P = db_block(job, (size_t)pm * 16 + pb);
-        f.Q = db_block(job, (size_t)a * 16 + qb);
-        f.p_flags = as_global(job.mbs)[pm].flags;
-        f.q_flags = as_global(job.mbs)[a].flags;
+        const mmr_recon_h264_block P = db_block(job, (size_t)pm * 16 + pb), Q = db_block(job, (size_t)a * 16 + qb);
+        bs = boundary_strength(job, as_global(job.mbs)[pm].flags, q.flags, P, Q, external);
     }
+    const uint32_t packed = group8_or((uint32_t)bs << (4 * (l & 7)));
+    if ((l & 7) == 0) as_global(job.bs)[(size_t)a * 4 + (l >> 3)] = packed;
+}
+
+hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(h264_bs_kernel, dim3(per * kXcds), dim3(128), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
+    return hipGetLastError();
 }
 
 // Bounded wait on a workgroup progress counter: gives up (and reports) after about a second, so a
@@ -311,14 +337,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         *reinterpret_cast<uint32_t*>(&S.ly[(4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]) = cur.own_y;
         if (l < 32) {
             const int comp = l >> 4, row = (l >> 1) & 7;
-            *reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + row) * 12 + 4 + 4 * (l & 1)]) = cur.own_c;
-        }
-        // derive_boundary_strengths (deblock.rs:144-200)
-        if (l < 32) {
-            const int e = (l & 15) >> 2, sgm = l & 3;
-            const int bs = cur.edge_on ? boundary_strength(job, cur.p_flags, cur.q_flags, cur.P, cur.Q, e == 0) : 0;
-            if (l < 16) S.bs_v[e][sgm] = bs;
-            else S.bs_h[e][sgm] = bs;
+            *reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + row) * kLc + 4 + 4 * (l & 1)]) = cur.own_c;
         }
         if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
         // top context from the row above (slot x), after it finished macroblock x + 1
@@ -329,7 +348,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
                     as_global(job.planes[0]) + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
             } else if (l < 32) {
                 const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12 + 4 + 4 * (k & 1)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
+                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc + 4 + 4 * (k & 1)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
                     as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
             }
         } else if (y > 0) {
@@ -340,70 +359,55 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
                     *reinterpret_cast<const uint32_t*>(&Above.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]);
             } else if (l < 32) {
                 const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12 + 4 + 4 * (k & 1)]) =
+                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc + 4 + 4 * (k & 1)]) =
                     *reinterpret_cast<const uint32_t*>(&Above.ring_c[slot][comp][row * 8 + 4 * (k & 1)]);
             }
         }
         wave_sync();
-        if (filter) {
-            const bool t8 = (m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0;
-            // vertical edges, one line of the window per lane, held in registers: lanes 0-15 luma
-            // rows (20 samples: 4 of left context + 16), 16-23 Cb rows, 24-31 Cr rows (12 samples)
-            if (l < 16) {
-                uint32_t* row = reinterpret_cast<uint32_t*>(&S.ly[(4 + l) * 20]);
+        if (filter && l < 32) {
+            // One line per lane, in registers: lanes 0-15 luma (rows in the vertical pass,
+            // columns in the horizontal one: 4 samples of context + 16), 16-23 Cb and 24-31 Cr (4 +
+            // 8). Edge slots k = 4, 8, 12, 16 are the luma edges 0..3 and the chroma edges 0 and 2
+            // (slots 12, 16 have strength 0 on chroma lines). Strengths come from the macroblock's
+            // record: vertical edge e, segment g at nibble e*4+g of dwords x / y, horizontal at
+            // nibble e*4+g of dwords z / w.
+            const int li = chroma_lane ? ((l - 16) & 7) : l;
+            uint8_t* win = chroma_lane ? S.lc[(l - 16) >> 3] : S.ly;
+            const uint32_t sh = 4u * (uint32_t)(chroma_lane ? (li >> 1) : (li >> 2));
+            const int4 r = cur.bs;
+            auto nib = [&](int v, uint32_t shift) { return (int)(((uint32_t)v >> shift) & 0xfu); };
+            const int bv[4] = {nib(r.x, sh), chroma_lane ? nib(r.y, sh) : nib(r.x, sh + 16), chroma_lane ? 0 : nib(r.y, sh),
+                               chroma_lane ? 0 : nib(r.y, sh + 16)};
+            const int bh[4] = {nib(r.z, sh), chroma_lane ? nib(r.w, sh) : nib(r.z, sh + 16), chroma_lane ? 0 : nib(r.w, sh),
+                               chroma_lane ? 0 : nib(r.w, sh + 16)};
+            {   // vertical edges
+                uint32_t* row = reinterpret_cast<uint32_t*>(win + (4 + li) * 20);
                 uint32_t v[5];
 #pragma unroll
                 for (int i = 0; i < 5; ++i) v[i] = row[i];
-                const int sg = l >> 2;
-                if (S.bs_v[0][sg]) filter_line<4>(v, ev, S.bs_v[0][sg], false);
-                if (!t8 && S.bs_v[1][sg]) filter_line<8>(v, ei, S.bs_v[1][sg], false);
-                if (S.bs_v[2][sg]) filter_line<12>(v, ei, S.bs_v[2][sg], false);
-                if (!t8 && S.bs_v[3][sg]) filter_line<16>(v, ei, S.bs_v[3][sg], false);
+                filter_line<4>(v, ev, bv[0], chroma_lane);
+                filter_line<8>(v, ei, bv[1], chroma_lane);
+                filter_line<12>(v, ei, bv[2], chroma_lane);
+                filter_line<16>(v, ei, bv[3], chroma_lane);
 #pragma unroll
                 for (int i = 0; i < 5; ++i) row[i] = v[i];
-            } else if (l < 32) {
-                const int comp = (l - 16) >> 3, r = (l - 16) & 7;
-                uint32_t* row = reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + r) * 12]);
-                uint32_t v[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i) v[i] = row[i];
-                const int sg = r >> 1;
-                if (S.bs_v[0][sg]) filter_line<4>(v, ev, S.bs_v[0][sg], true);
-                if (S.bs_v[2][sg]) filter_line<8>(v, ei, S.bs_v[2][sg], true);
-#pragma unroll
-                for (int i = 0; i < 3; ++i) row[i] = v[i];
             }
             wave_sync();
-            // horizontal edges, one column of the window per lane: lanes 0-15 luma columns (20
-            // samples: 4 of top context + 16), 16-23 Cb columns, 24-31 Cr columns (12 samples)
-            if (l < 16) {
-                uint8_t* col = &S.ly[4 + l];
+            {   // horizontal edges
+                uint8_t* col = win + 4 + li;
                 uint32_t v[5];
 #pragma unroll
                 for (int i = 0; i < 5; ++i)
                     v[i] = col[(4 * i) * 20] | (col[(4 * i + 1) * 20] << 8) | (col[(4 * i + 2) * 20] << 16) | ((uint32_t)col[(4 * i + 3) * 20] << 24);
-                const int sg = l >> 2;
-                if (S.bs_h[0][sg]) filter_line<4>(v, eh, S.bs_h[0][sg], false);
-                if (!t8 && S.bs_h[1][sg]) filter_line<8>(v, ei, S.bs_h[1][sg], false);
-                if (S.bs_h[2][sg]) filter_line<12>(v, ei, S.bs_h[2][sg], false);
-                if (!t8 && S.bs_h[3][sg]) filter_line<16>(v, ei, S.bs_h[3][sg], false);
+                filter_line<4>(v, eh, bh[0], chroma_lane);
+                filter_line<8>(v, ei, bh[1], chroma_lane);
+                filter_line<12>(v, ei, bh[2], chroma_lane);
+                filter_line<16>(v, ei, bh[3], chroma_lane);
 #pragma unroll
                 for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)get_byte(v, i);
-            } else if (l < 32) {
-                const int comp = (l - 16) >> 3, c = (l - 16) & 7;
-                uint8_t* col = &S.lc[comp][4 + c];
-                uint32_t v[3];
-#pragma unroll
-                for (int i = 0; i < 3; ++i)
-                    v[i] = col[(4 * i) * 12] | (col[(4 * i + 1) * 12] << 8) | (col[(4 * i + 2) * 12] << 16) | ((uint32_t)col[(4 * i + 3) * 12] << 24);
-                const int sg = c >> 1;
-                if (S.bs_h[0][sg]) filter_line<4>(v, eh, S.bs_h[0][sg], true);
-                if (S.bs_h[2][sg]) filter_line<8>(v, ei, S.bs_h[2][sg], true);
-#pragma unroll
-                for (int i = 1; i < 11; ++i) col[i * 12] = (uint8_t)get_byte(v, i);
             }
-            wave_sync();
         }
+        wave_sync();
         // hand the bottom rows to the row below: slot x (all columns) and the right columns of
         // slot x - 1, which this step's left edge modified
         if (!band_last) {
@@ -416,13 +420,13 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             } else if (l < 32) {
                 const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
                 *reinterpret_cast<uint32_t*>(&S.ring_c[slot][comp][row * 8 + 4 * (k & 1)]) =
-                    *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * 12 + 4 + 4 * (k & 1)]);
+                    *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * kLc + 4 + 4 * (k & 1)]);
             } else if (x > 0 && l < 36) {
                 const int row = l - 32;
                 *reinterpret_cast<uint32_t*>(&S.ring_y[prev][row * 16 + 12]) = *reinterpret_cast<const uint32_t*>(&S.ly[(16 + row) * 20]);
             } else if (x > 0 && l >= 40 && l < 48) {
                 const int k = l - 40, comp = k >> 2, row = k & 3;
-                *reinterpret_cast<uint32_t*>(&S.ring_c[prev][comp][row * 8 + 4]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * 12]);
+                *reinterpret_cast<uint32_t*>(&S.ring_c[prev][comp][row * 8 + 4]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * kLc]);
             }
         }
         // write back the samples that are now final
@@ -451,7 +455,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
                 const int cy = y * 8 - 4 + rr, cx = x * 8 - 4 + cc;
                 if (cy >= 0 && cx >= 0)
                     *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + cx) =
-                        *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
+                        *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + cc]);
             }
             // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
             // columns x*8+4 .. +7 (window col 8, 2 x 11 rows)
@@ -462,13 +466,13 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
                     const int cx = x * 8 - 4 + cc;
                     if (cx >= 0)
                         *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + rr) * CW + cx) =
-                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + cc]);
+                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + cc]);
                 } else {
                     const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
                     const int cy = y * 8 - 4 + rr;
                     if (cy >= 0 && (rr <= 8 || last_row))
                         *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + x * 8 + 4) =
-                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * 12 + 8]);
+                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + 8]);
                 }
             }
         }
@@ -481,7 +485,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             *reinterpret_cast<uint32_t*>(&S.ly[l * 20]) = *reinterpret_cast<const uint32_t*>(&S.ly[l * 20 + 16]);
         } else if (l < 44) {
             const int k = l - 20, comp = k / 12, row = k % 12;
-            *reinterpret_cast<uint32_t*>(&S.lc[comp][row * 12]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][row * 12 + 8]);
+            *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][row * kLc + 8]);
         }
         wave_sync();
         qp_left = m.qp;

@@ -668,10 +668,20 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     } else {
         counters = 1 + bands;
     }
+    // Scratch: 16 bytes of boundary strengths per macroblock of every deblocked picture.
+    std::vector<size_t> bs_off(count, 0);
+    size_t scratch = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        bs_off[i] = scratch;
+        if (jobs[i].deblock) scratch += ((size_t)pics[i].width_mbs * pics[i].height_mbs * 16 + 255) & ~size_t(255);
+    }
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::H264Job));
     const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
-    mmr_status s = sub.begin(blob.data(), blob.size(), counters);
+    mmr_status s = sub.begin(blob.data(), blob.size(), counters, scratch, [&](char* host, char* d_scratch) {
+        auto* hj = reinterpret_cast<mmr::H264Job*>(host + jobs_off);
+        for (uint32_t i = 0; i < count; ++i) hj[i].bs = jobs[i].deblock ? reinterpret_cast<uint32_t*>(d_scratch + bs_off[i]) : nullptr;
+    });
     if (s) return s;
     auto* dj = reinterpret_cast<const mmr::H264Job*>((const char*)sub.jobs_ptr() + jobs_off);
     auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
@@ -680,7 +690,9 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, (pics[i].width_mbs * pics[i].height_mbs + 3) / 4, sub.status_ptr(), stream);
+            const uint32_t mc_tiles = (pics[i].width_mbs * pics[i].height_mbs + 3) / 4;
+            if (jobs[i].deblock) e = mmr::launch_h264_bs(dj, dt, jobs[i].tile_begin, mc_tiles, stream);
+            if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, mc_tiles, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].deblock) {
                 const uint32_t nb = (pics[i].height_mbs + band_rows - 1) / band_rows;
                 e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, band_rows, publish, sub.status_ptr(), stream);
@@ -690,7 +702,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             }
         }
     } else {
-        e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
+        if (bands) e = mmr::launch_h264_bs(dj, dt, 0, (uint32_t)tiles.size(), stream);
+        if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
         if (e == hipSuccess && bands)
             e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, band_rows, publish, sub.status_ptr(), stream);
     }

@@ -89,6 +89,8 @@ struct H264Job {
     uint32_t tile_begin;          // first entry of this picture in the tile -> job table (4 macroblocks per tile)
     uint32_t refs_aligned;        // every reference plane is 4-byte aligned
     uint32_t band_begin;          // first deblocking band ticket of this picture
+    uint32_t pad;
+    uint32_t* bs;                 // [macroblocks][4] packed boundary strengths (submission scratch)
 };
 
 // ---------------------------------------------------------------- colour / blend
@@ -110,8 +112,9 @@ struct CompositeJobDev {
 };
 
 // Kernel launchers (stream-ordered). Each returns hipSuccess or the launch error.
-// The block kernels (MJPEG, MPEG-2, DV) run one workgroup per 32-block tile: workgroup i takes
-// tile tile_base + i, whose job index is d_tile_job[tile]. Tiles never straddle jobs.
+// The block kernels (MJPEG, MPEG-2, DV) run one workgroup per tile of group_blocks blocks (16, 32
+// or 64): workgroup i takes tile tile_base + i (XCD-remapped), whose job index is
+// d_tile_job[tile]. Tiles never straddle jobs.
 hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
                         hipStream_t s);
 hipError_t launch_mpeg2_order(const Mpeg2Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_status,
@@ -127,6 +130,8 @@ hipError_t launch_h264_mc(const H264Job* d_jobs, const uint32_t* d_tile_job, uin
 // ticket, d_counters[1 + t] the progress of band t's last row (zeroed before the launch), published
 // every `publish` macroblocks.
 bool h264_deblock_rows_supported(uint32_t rows);
+// Boundary strengths of every macroblock of the deblocked pictures into job.bs (MC tiles).
+hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
                                uint32_t groups, uint32_t band_rows, uint32_t publish, uint32_t* d_status, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
