@@ -165,7 +165,8 @@ typedef struct mmr_recon_mpeg2_macroblock {
     uint8_t qscale;         /* quantiser_scale value (decode_quantiser_scale, decoder.rs:721-732) */
     uint8_t reserved;
     int16_t mv[2][2];       /* [forward, backward][x, y], half-sample units */
-    uint32_t block_offset;  /* index of the first coded block of this macroblock in levels */
+    uint32_t block_offset;  /* index of the first coded block of this macroblock in levels
+                               (below 2^24 - 6; larger values are reported INVALID_DATA) */
 } mmr_recon_mpeg2_macroblock;
 
 #define MMR_MPEG2_PICTURE_I 1u

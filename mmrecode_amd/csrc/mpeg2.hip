@@ -177,8 +177,11 @@ MMR_DEV uint2 halfpel_row(const RefFetch& f) {
 // 6 * mb_w * r .. 6 * mb_w * (r + 1) - 1, coded blocks first (in macroblock order), then the blocks
 // without residual, so that tiles of coded blocks run the dequantisation and IDCT on every lane,
 // tiles without residual skip them, and both stay near each other in the picture (and in the tile
-// sequence). One wave per macroblock row; an entry is mb_y << 13 | mb_x << 3 | block. Field
-// prediction (not reconstructed on the device) is reported here.
+// sequence). One wave per macroblock row. An entry is 16 bytes and holds everything the block's
+// reconstruction needs from its macroblock record, so the main kernel's first load is the entry
+// itself: x = block | mb_x << 3 | mb_y << 13 | coded << 23 | macroblock flags << 24,
+// y = level record index | qscale << 24, z / w = forward / backward vector (packed int16 x, y).
+// Field prediction (not reconstructed on the device) is reported here.
 constexpr int kOrderThreads = 256;
 
 MMR_DEV uint32_t wave_exclusive_sum(uint32_t v) {
@@ -216,22 +219,33 @@ __global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2J
         field |= (w & MMR_MPEG2_MB_FIELD_MOTION) != 0;
     }
     if (field) atomicOr(status, kDevStatusUnsupported);
-    MMR_GLOBAL uint32_t* order = as_global(job.order) + (size_t)6 * mb_w * r;
+    MMR_GLOBAL mmr_v4i32* order = reinterpret_cast<MMR_GLOBAL mmr_v4i32*>(as_global(job.order)) + (size_t)6 * mb_w * r;
     uint32_t coded_before = 0;   // coded blocks of the row before this chunk
     for (uint32_t c0 = 0; c0 < mb_w; c0 += 64) {
         const uint32_t x = c0 + lane;
-        const uint32_t w = x < mb_w ? rec[((size_t)r * mb_w + x) * kRecWords] : 0u;
+        mmr_v4i32 mbr = {0, 0, 0, 0};
+        if (x < mb_w) mbr = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(rec + ((size_t)r * mb_w + x) * kRecWords);
+        const uint32_t w = (uint32_t)mbr.x;
         const uint32_t mask = coded_mask(w), n = (uint32_t)__popc(mask);
         uint32_t cpos = coded_before + wave_exclusive_sum(n);   // coded blocks before this macroblock
         if (x < mb_w) {
+            const bool intra = (w & MMR_MPEG2_MB_INTRA) != 0;
+            const uint32_t base = (uint32_t)mbr.w;   // block_offset: first level record of the macroblock
+            const uint32_t head = (r << 13) | (x << 3) | ((w & 0xffu) << 24);
+            const uint32_t qs = ((w >> 16) & 0xffu) << 24;
 #pragma unroll
             for (uint32_t b = 0; b < 6; ++b) {
-                const uint32_t entry = (r << 13) | (x << 3) | b;
-                if ((mask >> (5 - b)) & 1u) {
-                    order[cpos++] = entry;
+                const bool coded = (mask >> (5 - b)) & 1u;
+                // level record of block b: intra blocks are all coded; non-intra ones count the
+                // coded blocks before b in the coded-block pattern
+                const uint32_t index = base + (intra ? b : (uint32_t)__popc(((w >> 8) & 0x3fu) >> (6 - b)));
+                if (coded && index >= (1u << 24)) atomicOr(status, kDevStatusInvalidData);   // 24-bit entry field
+                const mmr_v4i32 e = {(int32_t)(head | b | (coded ? 1u << 23 : 0u)), (int32_t)((coded ? index : 0u) | qs), mbr.y, mbr.z};
+                if (coded) {
+                    order[cpos++] = e;
                 } else {
                     // blocks of the row before this one minus coded blocks before it
-                    order[row_coded + (x * 6 + b) - cpos] = entry;
+                    order[row_coded + (x * 6 + b) - cpos] = e;
                 }
             }
         }
@@ -267,25 +281,17 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     // Block of this group, from the picture's coded-first block order.
     const uint32_t pos = (tile - job.tile_begin) * BPG + (uint32_t)g;
     const bool active = pos < mbs * 6;
-    const uint32_t entry = as_global(job.order)[active ? pos : 0];
-    const uint32_t mb_x = (entry >> 3) & 0x3ffu, mb_y = entry >> 13;
+    const int4 ent = load_global_int4(job.order + (size_t)(active ? pos : 0) * 4);
+    const uint32_t entry = (uint32_t)ent.x;
+    const uint32_t mb_x = (entry >> 3) & 0x3ffu, mb_y = (entry >> 13) & 0x3ffu;
     const int b = (int)(entry & 7u);
-    const uint32_t mb = __umul24(mb_y, job.mb_w) + mb_x;
-    mmr_recon_mpeg2_macroblock m;
-    uint32_t mv_words[2];   // forward, backward (x, y) as packed int16
-    {
-        static_assert(sizeof(mmr_recon_mpeg2_macroblock) == 16, "macroblock record is 4 dwords");
-        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb));
-        const uint32_t w[4] = {q[0], q[1], q[2], q[3]};
-        __builtin_memcpy(&m, w, sizeof m);
-        mv_words[0] = w[1];
-        mv_words[1] = w[2];
-    }
+    const uint32_t mb_flags = entry >> 24;
+    const uint32_t mv_words[2] = {(uint32_t)ent.z, (uint32_t)ent.w};   // forward, backward (x, y) as packed int16
     // Plane of block b (Y for 0..3, Cb for 4, Cr for 5) with its reference planes.
     const int plane = b < 4 ? 0 : b - 3;
     const Mpeg2Plane pr = load_plane(job.plane + plane);
-    const bool intra = (m.flags & MMR_MPEG2_MB_INTRA) != 0;
-    const bool coded = active && (intra || ((m.cbp >> (5 - b)) & 1));
+    const bool intra = (mb_flags & MMR_MPEG2_MB_INTRA) != 0;
+    const bool coded = active && ((entry >> 23) & 1u);
     double* tmp = s_tmp + g * kIdctTmpStride;
     // Field prediction (reported by the order pass) is predicted as frame motion here, so the
     // kernel still completes.
@@ -304,13 +310,13 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     // then the reference rows (consumed after the IDCT). They are unconditional (uncoded blocks
     // read the 256-byte matrix block instead of levels), so no branch joins force an early wait.
     const int component = b < 4 ? 0 : 1;
-    const uint32_t index = m.block_offset + (intra ? (uint32_t)b : (uint32_t)__popc((m.cbp & 0x3fu) >> (6 - b)));
+    const uint32_t index = (uint32_t)ent.y & 0xffffffu;
     const int16_t* lv = coded ? job.levels + (size_t)index * 64 : reinterpret_cast<const int16_t*>(job.matrices);
     const int4 raw = load_global_int4(lv + lane * 8);
     const uint2 sraw = *reinterpret_cast<const uint2*>(kScanCoefOffset[job.alt_scan ? 1 : 0] + lane * 8);
 
     // Geometry of block b and of the row this lane predicts / stores.
-    const bool interlaced = (m.flags & MMR_MPEG2_MB_INTERLACED_DCT) != 0;
+    const bool interlaced = (mb_flags & MMR_MPEG2_MB_INTERLACED_DCT) != 0;
     const int stride = (int)pr.stride, height = (int)pr.height;
     int x0, y0, step;
     if (b < 4) {
@@ -323,8 +329,8 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
         step = 1;
     }
     const int yy = y0 + lane * step;
-    const bool has_f = !intra && (m.flags & MMR_MPEG2_MB_FORWARD) != 0;
-    const bool has_b = !intra && (m.flags & MMR_MPEG2_MB_BACKWARD) != 0;
+    const bool has_f = !intra && (mb_flags & MMR_MPEG2_MB_FORWARD) != 0;
+    const bool has_b = !intra && (mb_flags & MMR_MPEG2_MB_BACKWARD) != 0;
     uint32_t fmv = mv_words[0], bmv = mv_words[1];
     if (b >= 4) {
         fmv = halve_mv(fmv);
@@ -347,7 +353,7 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
         const uint2 wrow = *reinterpret_cast<const uint2*>(wscan + 64 * (intra ? (component ? 2 : 0) : (component ? 3 : 1)) + lane * 8);
         const int16_t* l8 = reinterpret_cast<const int16_t*>(&raw);
         const uint8_t* s8 = reinterpret_cast<const uint8_t*>(&sraw);
-        const uint32_t qs = m.qscale, non_intra = intra ? 0u : 1u;
+        const uint32_t qs = (uint32_t)ent.y >> 24, non_intra = intra ? 0u : 1u;
         int32_t val[8];
         uint32_t x = 0;
 #pragma unroll

@@ -102,7 +102,7 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
             r.status = fail(MMR_STATUS_UNSUPPORTED, "a record of this submission uses a feature the device path does not reconstruct "
                                                     "(MPEG-2 field prediction or H.264 intra macroblock); fall back to CPU reconstruction");
         } else if (st & kStatusInvalidData) {
-            r.status = fail(MMR_STATUS_INVALID_DATA, "a record of this submission is invalid (reference index out of range)");
+            r.status = fail(MMR_STATUS_INVALID_DATA, "a record of this submission is invalid (H.264 reference index or MPEG-2 level record index out of range)");
         }
         r.finished = true;
     }
@@ -519,13 +519,13 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, ordered.data(), ordered.size() * sizeof(mmr::Mpeg2Job));
     const size_t tiles_off = blob_append(blob, tile_table.data(), tile_table.size() * sizeof(uint32_t));
-    // Scratch: each picture's block order (6 entries per macroblock).
+    // Scratch: each picture's block order (6 entries of 16 bytes per macroblock).
     std::vector<size_t> order_off(ordered.size());
     size_t scratch = 0;
     uint32_t order_rows = 0;
     for (size_t k = 0; k < ordered.size(); ++k) {
         order_off[k] = scratch;
-        scratch += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * sizeof(uint32_t) + 255) & ~size_t(255);
+        scratch += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * 16 + 255) & ~size_t(255);   // 16-byte entries
         ordered[k].row_begin = per_frame ? 0 : order_rows;
         order_rows += ordered[k].mb_h;
     }

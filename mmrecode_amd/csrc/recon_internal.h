@@ -43,7 +43,7 @@ static_assert(sizeof(Mpeg2Plane) == 32, "plane record is two 16-byte loads");
 struct Mpeg2Job {
     Mpeg2Plane plane[3];          // Y, Cb, Cr
     const uint8_t* matrices;      // [4][64] natural
-    uint32_t* order;              // [6 * macroblocks] coded-first block order (submission scratch)
+    uint32_t* order;              // [6 * macroblocks][4] coded-first block entries (submission scratch)
     const mmr_recon_mpeg2_macroblock* mbs;
     const int16_t* levels;
     int32_t fwd_src;

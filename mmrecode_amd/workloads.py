@@ -133,6 +133,33 @@ def mjpeg_frame(seed, index=0, width=640, height=480, sampling=((2, 2), (1, 1), 
                 {"width": width, "height": height, "blocks": nblocks})
 
 
+def mjpeg_unit_from_frame(frame, rgb=True):
+    """A Unit holding one entropy-decoded JPEG frame (a dict of oracle.binding.jpeg_frames):
+    its levels, quantisation tables and geometry, with outputs sized like the reference's planes."""
+    width, height, sampling = frame["width"], frame["height"], frame["sampling"]
+    ncomp = len(sampling)
+    hmax = max(h for h, _ in sampling)
+    vmax = max(v for _, v in sampling)
+    comp_dims = [(-(-width * h // hmax), -(-height * v // vmax)) for h, v in sampling]
+    outputs = {f"plane{c}": ((comp_dims[c][1], comp_dims[c][0]), np.uint8, None) for c in range(ncomp)}
+    if rgb and ncomp == 3:
+        outputs["rgb"] = ((height, width, 3), np.uint8, None)
+    levels = np.ascontiguousarray(frame["levels"], np.int16)
+
+    def fill(p):
+        f = abi.new(abi.MjpegFrame, width=width, height=height, component_count=ncomp, block_count=len(levels),
+                    levels=p["levels"], quant_tables=p["quant"], rgb=p.get("rgb"), rgb_range=abi.RANGE_FULL)
+        for c, (h, v) in enumerate(sampling):
+            f.h_sampling[c] = h
+            f.v_sampling[c] = v
+            f.quant_table[c] = frame["qsel"][c]
+            f.planes[c] = p[f"plane{c}"]
+        return f
+
+    return Unit("mjpeg", {"levels": levels, "quant": np.ascontiguousarray(frame["quant"], np.uint16)}, outputs, fill,
+                {"width": width, "height": height, "blocks": len(levels)})
+
+
 # ---------------------------------------------------------------- MPEG-2 (config 1)
 GOP_DECODE_ORDER = "IPBBPBBPBBPBBBB"   # display I B B P B B P B B P B B P B B, closed
 

@@ -67,6 +67,11 @@ def lib():
             "oracle_blend_channel": (u8, [u8, u8, u8]),
             "oracle_composite": (i32, [C.POINTER(abi.CompositeJob)]),
             "oracle_yuv_blend": (i32, [C.POINTER(abi.YuvBlendJob)]),
+            "oracle_jpeg_entropy": (i32, [dp, C.c_size_t, C.POINTER(JpegLevels), C.c_char_p, C.c_size_t]),
+            "oracle_jpeg_free": (None, [C.POINTER(JpegLevels)]),
+            "oracle_jpeg_huffman_symbols": (i32, [dp, dp, i32, dp, C.c_size_t, dp, i32]),
+            "oracle_jpeg_read_bits": (i32, [dp, C.c_size_t, i32, C.POINTER(u32)]),
+            "oracle_jpeg_receive_extend": (i32, [dp, C.c_size_t, dp, i32, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -74,6 +79,43 @@ def lib():
             f.argtypes = args
         _LIB = L
     return _LIB
+
+
+class JpegLevels(C.Structure):
+    """oracle_jpeg_levels (oracle.h)."""
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("component_count", C.c_uint32),
+                ("h", C.c_uint8 * 4), ("v", C.c_uint8 * 4), ("qsel", C.c_uint8 * 4),
+                ("quant", (C.c_uint16 * 64) * 4), ("block_count", C.c_uint32),
+                ("levels", C.POINTER(C.c_int16)), ("consumed", C.c_size_t)]
+
+
+class JpegError(ValueError):
+    pass
+
+
+def jpeg_frames(data):
+    """Entropy-decodes every JPEG of a byte string (an MJPEG file is JPEGs back to back). Returns one
+    dict per frame: width, height, sampling [(h, v)], qsel, quant (uint16 [4][64] zig-zag), levels
+    (int16 [blocks][64] zig-zag, slot 0 = DC prediction)."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    frames, pos = [], 0
+    err = C.create_string_buffer(256)
+    while pos < len(buf):
+        f = JpegLevels()
+        rc = lib().oracle_jpeg_entropy(buf.ctypes.data + pos, len(buf) - pos, C.byref(f), err, 256)
+        if rc:
+            lib().oracle_jpeg_free(C.byref(f))
+            raise JpegError(err.value.decode())
+        n = f.block_count * 64
+        levels = np.ctypeslib.as_array(f.levels, shape=(n,)).reshape(f.block_count, 64).copy()
+        frames.append({"width": f.width, "height": f.height,
+                       "sampling": [(f.h[c], f.v[c]) for c in range(f.component_count)],
+                       "qsel": [f.qsel[c] for c in range(f.component_count)],
+                       "quant": np.array([[f.quant[t][k] for k in range(64)] for t in range(4)], np.uint16),
+                       "levels": levels})
+        pos += f.consumed
+        lib().oracle_jpeg_free(C.byref(f))
+    return frames
 
 
 def ptr(a):

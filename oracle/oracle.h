@@ -29,6 +29,22 @@ void oracle_mjpeg_dequantize(const int16_t levels[64], const uint16_t quant_zigz
 uint32_t oracle_mjpeg_block_count(const mmr_recon_mjpeg_frame *frame);
 int oracle_mjpeg_reconstruct(const mmr_recon_mjpeg_frame *frame);
 
+/* JPEG entropy front end (jpeg_entropy.c): one baseline frame's post-entropy buffers. */
+typedef struct oracle_jpeg_levels {
+    uint32_t width, height, component_count;
+    uint8_t h[4], v[4], qsel[4];
+    uint16_t quant[4][64];       /* zig-zag order, rows by table id */
+    uint32_t block_count;
+    int16_t *levels;             /* [block_count][64] zig-zag order, [0] = DC prediction value */
+    size_t consumed;             /* input bytes up to and including EOI */
+} oracle_jpeg_levels;
+int oracle_jpeg_entropy(const uint8_t *data, size_t len, oracle_jpeg_levels *out, char *err, size_t errcap);
+void oracle_jpeg_free(oracle_jpeg_levels *frame);
+int oracle_jpeg_huffman_symbols(const uint8_t counts[16], const uint8_t *symbols, int nsym, const uint8_t *data,
+                                size_t len, int *out, int n);
+int oracle_jpeg_read_bits(const uint8_t *data, size_t len, int nbits, uint32_t *out);
+int oracle_jpeg_receive_extend(const uint8_t *data, size_t len, const int *sizes, int n, int32_t *out);
+
 /* MPEG-2 (crates/codecs/mpeg2/src) */
 void oracle_mpeg2_inverse_dct(const int32_t coefficients[64], int16_t out[64]);
 void oracle_mpeg2_dequantize_intra(const int16_t levels[64], int component, int qscale,
