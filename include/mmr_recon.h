@@ -152,7 +152,7 @@ MMR_RECON_API mmr_status mmr_recon_mjpeg_submit(mmr_recon_context *context,
 #define MMR_MPEG2_MB_FORWARD 2u
 #define MMR_MPEG2_MB_BACKWARD 4u
 #define MMR_MPEG2_MB_INTERLACED_DCT 8u
-#define MMR_MPEG2_MB_FIELD_MOTION 16u  /* field prediction: MMR_STATUS_UNSUPPORTED */
+#define MMR_MPEG2_MB_FIELD_MOTION 16u  /* field prediction: field vectors in the picture's field_motion */
 
 /*
  * One macroblock after VLC decoding, skip resolution and motion-vector
@@ -164,10 +164,23 @@ typedef struct mmr_recon_mpeg2_macroblock {
     uint8_t cbp;            /* coded_block_pattern, bit (5-b) = block b; ignored for intra */
     uint8_t qscale;         /* quantiser_scale value (decode_quantiser_scale, decoder.rs:721-732) */
     uint8_t reserved;
-    int16_t mv[2][2];       /* [forward, backward][x, y], half-sample units */
+    int16_t mv[2][2];       /* [forward, backward][x, y], half-sample units; with FIELD_MOTION the
+                               first (top-row) field's vectors in field units */
     uint32_t block_offset;  /* index of the first coded block of this macroblock in levels
                                (below 2^24 - 6; larger values are reported INVALID_DATA) */
 } mmr_recon_mpeg2_macroblock;
+
+/*
+ * Field prediction of a frame-picture macroblock (MMR_MPEG2_MB_FIELD_MOTION; the reference's
+ * FieldMotion pairs, decode_field_motion decoder.rs:891-908, predicted by predict_field_plane /
+ * halfpel_field_sample :1053-1148): rows of parity f are predicted from field field_select[d][f]
+ * of the reference with vector f. The parity-0 vectors are the macroblock record's mv.
+ */
+typedef struct mmr_recon_mpeg2_field_motion {
+    int16_t mv[2][2];             /* [forward, backward][x, y] of the parity-1 rows, field units */
+    uint8_t field_select[2][2];   /* [forward, backward][parity]: 0 top field, 1 bottom field
+                                     (other values of a FIELD_MOTION macroblock: INVALID_DATA) */
+} mmr_recon_mpeg2_field_motion;
 
 #define MMR_MPEG2_PICTURE_I 1u
 #define MMR_MPEG2_PICTURE_P 2u
@@ -199,6 +212,10 @@ typedef struct mmr_recon_mpeg2_picture {
     const uint8_t *backward[3];     /* B: newer anchor */
     int32_t forward_source;         /* index in this submission producing forward[], or -1 */
     int32_t backward_source;        /* index in this submission producing backward[], or -1 */
+    const mmr_recon_mpeg2_field_motion *field_motion;   /* [mb_width * mb_height], read for
+                                       FIELD_MOTION macroblocks; NULL when there are none
+                                       (a FIELD_MOTION macroblock then reports INVALID_DATA). A
+                                       struct_size that ends before this member reads as NULL. */
 } mmr_recon_mpeg2_picture;
 
 MMR_RECON_API mmr_status mmr_recon_mpeg2_submit(mmr_recon_context *context,

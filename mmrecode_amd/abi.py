@@ -108,7 +108,12 @@ class Mpeg2Picture(C.Structure):
         ("backward", vp * 3),
         ("forward_source", C.c_int32),
         ("backward_source", C.c_int32),
+        ("field_motion", vp),
     ]
+
+
+class Mpeg2FieldMotion(C.Structure):
+    _fields_ = [("mv", (C.c_int16 * 2) * 2), ("field_select", (C.c_uint8 * 2) * 2)]
 
 
 class DvFrame(C.Structure):

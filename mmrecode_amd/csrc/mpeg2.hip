@@ -121,11 +121,12 @@ struct RefFetch {
 };
 
 // Issues the loads of block row `yy` (8 samples from x0) predicted with half-sample vector
-// (mvx, mvy); halfpel_sample's integer part is floor(v / 2) = v >> 1. With use == false (the
+// (mvx, mvy); halfpel_sample's integer part is floor(v / 2) = v >> 1. Rows are `pitch` bytes apart
+// and `stride` samples wide (a field of a frame plane: pitch = 2 * stride). With use == false (the
 // direction is not predicted) both dword triples come from the start of the plane. `aligned`: the
 // plane pointers of the picture are 4-byte aligned (strides are multiples of 8).
-MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int height, int x0, int yy, int mvx, int mvy, bool use,
-                           bool aligned) {
+MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int pitch, int height, int x0, int yy, int mvx, int mvy,
+                           bool use, bool aligned) {
     RefFetch f;
     const int bx = x0 + (mvx >> 1), by = yy + (mvy >> 1);
     f.fx = (mvx & 1) != 0;
@@ -133,8 +134,8 @@ MMR_DEV RefFetch fetch_ref(const uint8_t* ref, int stride, int height, int x0, i
     f.x = bx;
     f.stride = stride;
     f.fast = aligned && bx >= 0 && bx + 8 < stride;
-    const uint32_t o0 = __umul24((uint32_t)clampi(by, 0, height - 1), (uint32_t)stride);
-    const uint32_t o1 = __umul24((uint32_t)clampi(by + (f.fy ? 1 : 0), 0, height - 1), (uint32_t)stride);
+    const uint32_t o0 = __umul24((uint32_t)clampi(by, 0, height - 1), (uint32_t)pitch);
+    const uint32_t o1 = __umul24((uint32_t)clampi(by + (f.fy ? 1 : 0), 0, height - 1), (uint32_t)pitch);
     const MMR_GLOBAL uint8_t* base = as_global(ref);
     f.row0 = base + o0;
     f.row1 = base + o1;
@@ -181,7 +182,8 @@ MMR_DEV uint2 halfpel_row(const RefFetch& f) {
 // reconstruction needs from its macroblock record, so the main kernel's first load is the entry
 // itself: x = block | mb_x << 3 | mb_y << 13 | coded << 23 | macroblock flags << 24,
 // y = level record index | qscale << 24, z / w = forward / backward vector (packed int16 x, y).
-// Field prediction (not reconstructed on the device) is reported here.
+// Field-predicted macroblocks are checked here: the picture has field records and their field
+// selects are 0 or 1.
 constexpr int kOrderThreads = 256;
 
 MMR_DEV uint32_t wave_exclusive_sum(uint32_t v) {
@@ -211,14 +213,22 @@ __global__ __launch_bounds__(kOrderThreads) void mpeg2_order_kernel(const Mpeg2J
     // first dword of a record: flags | cbp << 8 | qscale << 16; bit (5 - b) set when block b is coded
     auto coded_mask = [](uint32_t w) -> uint32_t { return (w & MMR_MPEG2_MB_INTRA) ? 0x3fu : ((w >> 8) & 0x3fu); };
     uint32_t row_coded = 0;
-    bool field = false;
+    bool bad_field = false;
     for (uint32_t c0 = 0; c0 < mb_w; c0 += 64) {
         const uint32_t x = c0 + lane;
         const uint32_t w = x < mb_w ? rec[((size_t)r * mb_w + x) * kRecWords] : 0u;
         row_coded += wave_sum((uint32_t)__popc(coded_mask(w)));
-        field |= (w & MMR_MPEG2_MB_FIELD_MOTION) != 0;
+        if ((w & MMR_MPEG2_MB_FIELD_MOTION) && !(w & MMR_MPEG2_MB_INTRA)) {
+            if (!job.field_motion) {
+                bad_field = true;
+            } else {
+                const MMR_GLOBAL uint8_t* sel = reinterpret_cast<const MMR_GLOBAL uint8_t*>(
+                    as_global(job.field_motion + (size_t)r * mb_w + x)) + 8;   // field_select[2][2]
+                bad_field |= (sel[0] | sel[1] | sel[2] | sel[3]) > 1u;
+            }
+        }
     }
-    if (field) atomicOr(status, kDevStatusUnsupported);
+    if (bad_field) atomicOr(status, kDevStatusInvalidData);
     MMR_GLOBAL mmr_v4i32* order = reinterpret_cast<MMR_GLOBAL mmr_v4i32*>(as_global(job.order)) + (size_t)6 * mb_w * r;
     uint32_t coded_before = 0;   // coded blocks of the row before this chunk
     for (uint32_t c0 = 0; c0 < mb_w; c0 += 64) {
@@ -293,8 +303,6 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     const bool intra = (mb_flags & MMR_MPEG2_MB_INTRA) != 0;
     const bool coded = active && ((entry >> 23) & 1u);
     double* tmp = s_tmp + g * kIdctTmpStride;
-    // Field prediction (reported by the order pass) is predicted as frame motion here, so the
-    // kernel still completes.
 
     // Quantiser matrices in scan order, one copy per wave: wscan[m][p] = W_m[scan[p]] (all blocks
     // of a workgroup belong to one picture).
@@ -332,14 +340,40 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     const bool has_f = !intra && (mb_flags & MMR_MPEG2_MB_FORWARD) != 0;
     const bool has_b = !intra && (mb_flags & MMR_MPEG2_MB_BACKWARD) != 0;
     uint32_t fmv = mv_words[0], bmv = mv_words[1];
+    const uint8_t* fref = pr.fwd;
+    const uint8_t* bref = pr.bwd;
+    int pitch = stride, ry = yy, fh = height, bh = height;
+    if (!intra && (mb_flags & MMR_MPEG2_MB_FIELD_MOTION) && job.field_motion) {
+        // predict_field_plane / halfpel_field_sample (decoder.rs:1048-1113): the lane's row has
+        // parity yy & 1 and takes that parity's vector and field select in each direction (parity
+        // 0: the record's vectors, parity 1: the field record's). It reads field `sel` of the
+        // reference -- rows sel, sel + 2, ..., a plane of pitch 2 * stride and (height - 1 - sel) / 2
+        // + 1 rows -- at field row yy / 2 + floor(mvy / 2). The order pass reports a missing field
+        // array (then predicted as frame motion here) and selects other than 0 or 1.
+        const MMR_GLOBAL uint16_t* fm = reinterpret_cast<const MMR_GLOBAL uint16_t*>(
+            as_global(job.field_motion + (size_t)mb_y * job.mb_w + mb_x));
+        const uint32_t parity = (uint32_t)yy & 1u;
+        if (parity) {
+            fmv = (uint32_t)fm[0] | ((uint32_t)fm[1] << 16);
+            bmv = (uint32_t)fm[2] | ((uint32_t)fm[3] << 16);
+        }
+        const uint32_t fsel = ((uint32_t)fm[4] >> (parity * 8)) & 1u;   // field_select[0][parity]
+        const uint32_t bsel = ((uint32_t)fm[5] >> (parity * 8)) & 1u;   // field_select[1][parity]
+        fref += fsel * (uint32_t)stride;
+        bref += bsel * (uint32_t)stride;
+        fh = (height - 1 - (int)fsel) / 2 + 1;
+        bh = (height - 1 - (int)bsel) / 2 + 1;
+        pitch = 2 * stride;
+        ry = yy >> 1;
+    }
     if (b >= 4) {
         fmv = halve_mv(fmv);
         bmv = halve_mv(bmv);
     }
     const int fmx = (int16_t)(fmv & 0xffffu), fmy = (int16_t)(fmv >> 16);
     const int bmx = (int16_t)(bmv & 0xffffu), bmy = (int16_t)(bmv >> 16);
-    const RefFetch ff = fetch_ref(pr.fwd, stride, height, x0, yy, fmx, fmy, has_f, job.refs_aligned != 0);
-    const RefFetch fb = fetch_ref(pr.bwd, stride, height, x0, yy, bmx, bmy, has_b, job.refs_aligned != 0);
+    const RefFetch ff = fetch_ref(fref, stride, pitch, fh, x0, ry, fmx, fmy, has_f, job.refs_aligned != 0);
+    const RefFetch fb = fetch_ref(bref, stride, pitch, bh, x0, ry, bmx, bmy, has_b, job.refs_aligned != 0);
 
     // Dequantisation in scan order (decode_intra_block / decode_non_intra_block), branch-free with
     // 24-bit multiplies. Intra: |level| * W * qscale / 16; non-intra: (2 |level| + 1) * W * qscale

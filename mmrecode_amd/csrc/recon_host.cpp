@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -445,7 +446,11 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<mmr::Mpeg2Job> jobs(count);
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_mpeg2_picture& p = pics[i];
-        if (p.struct_size < sizeof(mmr_recon_mpeg2_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
+        // descriptors that end before field_motion (built without field prediction) stay valid
+        constexpr size_t kMpeg2Base = offsetof(mmr_recon_mpeg2_picture, field_motion);
+        if (p.struct_size < kMpeg2Base) return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 struct_size too small");
+        const mmr_recon_mpeg2_field_motion* field_motion =
+            p.struct_size >= kMpeg2Base + sizeof(p.field_motion) ? p.field_motion : nullptr;
         if (p.width == 0 || p.height == 0 || p.width > 16384 || p.height > 16384)
             return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u size %ux%u is invalid", i, p.width, p.height);
         if (p.picture_type < MMR_MPEG2_PICTURE_I || p.picture_type > MMR_MPEG2_PICTURE_B)
@@ -454,8 +459,9 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
             return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u scan / dc precision invalid", i);
         if (!p.matrices || !p.macroblocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
             return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u has a NULL buffer", i);
-        if (!aligned(p.levels, 16) || !aligned(p.macroblocks, 4))
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u: levels must be 16-byte aligned", i);
+        if (!aligned(p.levels, 16) || !aligned(p.macroblocks, 4) || !aligned(field_motion, 2))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "MPEG-2 picture %u: levels must be 16-byte aligned, macroblocks 4-byte, "
+                        "field_motion 2-byte", i);
         const bool needs_f = p.picture_type != MMR_MPEG2_PICTURE_I, needs_b = p.picture_type == MMR_MPEG2_PICTURE_B;
         if ((needs_f && (!p.forward[0] || !p.forward[1] || !p.forward[2])) || (needs_b && (!p.backward[0] || !p.backward[1] || !p.backward[2])))
             return fail(MMR_STATUS_INVALID_DATA, "missing %s MPEG-2 reference for picture %u", needs_b ? "forward/backward" : "forward", i);
@@ -466,6 +472,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         j.matrices = p.matrices;
         j.mbs = p.macroblocks;
         j.levels = p.levels;
+        j.field_motion = field_motion;
         for (int k = 0; k < 3; ++k) {
             mmr::Mpeg2Plane& pl = j.plane[k];
             pl.dst = p.planes[k];

@@ -46,6 +46,7 @@ struct Mpeg2Job {
     uint32_t* order;              // [6 * macroblocks][4] coded-first block entries (submission scratch)
     const mmr_recon_mpeg2_macroblock* mbs;
     const int16_t* levels;
+    const mmr_recon_mpeg2_field_motion* field_motion;   // [macroblocks] or NULL
     int32_t fwd_src;
     int32_t bwd_src;
     uint32_t mb_w;

@@ -164,9 +164,49 @@ def mjpeg_unit_from_frame(frame, rgb=True):
 GOP_DECODE_ORDER = "IPBBPBBPBBPBBBB"   # display I B B P B B P B B P B B P B B, closed
 
 
-def mpeg2_gop(seed, gop_index=0, width=1920, height=1080, gop=GOP_DECODE_ORDER, alt_scan_fraction=0.3):
+# mmr_recon_mpeg2_macroblock as a numpy record
+MPEG2_MB_DTYPE = np.dtype([("flags", "u1"), ("cbp", "u1"), ("qscale", "u1"), ("reserved", "u1"),
+                           ("mv", "<i2", (2, 2)), ("block_offset", "<u4")])
+# mmr_recon_mpeg2_field_motion
+MPEG2_FIELD_DTYPE = np.dtype([("mv", "<i2", (2, 2)), ("field_select", "u1", (2, 2))])
+
+
+def mpeg2_unit(width, height, picture_type, matrices, mbs, levels, field_motion=None, alternate_scan=0,
+               intra_dc_precision=0, fwd=None, bwd=None, meta=None):
+    """One MPEG-2 frame picture as a Unit: picture_type is MMR_MPEG2_PICTURE_*, mbs the raster-order
+    macroblock records (MPEG2_MB_DTYPE), levels int16 [blocks][64], field_motion MPEG2_FIELD_DTYPE
+    records or None; fwd / bwd are the decode-order indices of its references (unit.meta)."""
+    mb_w, mb_h = -(-width // 16), -(-height // 16)
+    W, H = mb_w * 16, mb_h * 16
+    inputs = {"matrices": np.ascontiguousarray(matrices, np.uint8), "mbs": np.ascontiguousarray(mbs),
+              "levels": np.ascontiguousarray(levels, np.int16)}
+    if field_motion is not None:
+        inputs["field_motion"] = np.ascontiguousarray(field_motion)
+    outputs = {"y": ((H, W), np.uint8, None), "cb": ((H // 2, W // 2), np.uint8, None),
+               "cr": ((H // 2, W // 2), np.uint8, None)}
+
+    def fill(p):
+        s = abi.new(abi.Mpeg2Picture, width=width, height=height, picture_type=picture_type,
+                    alternate_scan=alternate_scan, intra_dc_precision=intra_dc_precision, matrices=p["matrices"],
+                    macroblocks=p["mbs"], levels=p["levels"], field_motion=p.get("field_motion"),
+                    forward_source=p.get("forward_source", -1), backward_source=p.get("backward_source", -1))
+        for k, n in enumerate(("y", "cb", "cr")):
+            s.planes[k] = p[n]
+            s.forward[k] = p.get("fwd_" + n)
+            s.backward[k] = p.get("bwd_" + n)
+        return s
+
+    m = {"fwd": fwd, "bwd": bwd, "width": width, "height": height}
+    m.update(meta or {})
+    return Unit("mpeg2", inputs, outputs, fill, m)
+
+
+def mpeg2_gop(seed, gop_index=0, width=1920, height=1080, gop=GOP_DECODE_ORDER, alt_scan_fraction=0.3,
+              field_fraction=0.0):
     """One closed GOP in decode order. Returns a list of Units; unit.meta has 'type', 'fwd', 'bwd'
-    (indices of reference pictures inside this GOP, or None)."""
+    (indices of reference pictures inside this GOP, or None). field_fraction of the predicted
+    macroblocks use field prediction (random field vectors, some reaching past the picture edges,
+    and random field selects)."""
     mb_w, mb_h = -(-width // 16), -(-height // 16)
     n_mb = mb_w * mb_h
     W, H = mb_w * 16, mb_h * 16
@@ -198,6 +238,17 @@ def mpeg2_gop(seed, gop_index=0, width=1920, height=1080, gop=GOP_DECODE_ORDER, 
         lo_y, hi_y = -2 * mby * 16, 2 * (H - 16 - mby * 16)
         mv[:, :, 0] = np.clip(mv[:, :, 0], lo_x[:, None], hi_x[:, None])
         mv[:, :, 1] = np.clip(mv[:, :, 1], lo_y[:, None], hi_y[:, None])
+        field_motion = None
+        if field_fraction > 0 and ptype != "I":
+            field = (kind != 0) & (rng.random(n_mb) < field_fraction)
+            flags[field] |= abi.MPEG2_MB_FIELD_MOTION
+            field_motion = np.zeros(n_mb, dtype=MPEG2_FIELD_DTYPE)
+            fv = rng.integers(-48, 49, (n_mb, 2, 2, 2))          # [mb][parity][direction][x, y]
+            mv[field] = fv[field, 0]
+            field_motion["mv"] = fv[:, 1]
+            field_motion["field_select"] = rng.integers(0, 2, (n_mb, 2, 2))
+            interlaced = (kind != 0) & (rng.random(n_mb) < 0.5)   # field DCT on half of them
+            flags[interlaced] |= abi.MPEG2_MB_INTERLACED_DCT
         coded_per_mb = np.array([bin(int(c)).count("1") for c in range(64)])[cbp & 0x3F]
         offsets = np.concatenate([[0], np.cumsum(coded_per_mb)[:-1]]).astype(np.uint32)
         ncoded = int(coded_per_mb.sum())
@@ -208,8 +259,7 @@ def mpeg2_gop(seed, gop_index=0, width=1920, height=1080, gop=GOP_DECODE_ORDER, 
             # intra blocks: DC predictor value (intra_dc_precision 0: 8-bit DC)
             intra_blocks = np.repeat(kind == 0, coded_per_mb)
             levels[:ncoded][intra_blocks, 0] = rng.integers(0, 256, int(intra_blocks.sum()))
-        mbs = np.zeros(n_mb, dtype=[("flags", "u1"), ("cbp", "u1"), ("qscale", "u1"), ("reserved", "u1"),
-                                    ("mv", "<i2", (2, 2)), ("block_offset", "<u4")])
+        mbs = np.zeros(n_mb, dtype=MPEG2_MB_DTYPE)
         mbs["flags"] = flags
         mbs["cbp"] = cbp
         mbs["qscale"] = qscale
@@ -221,22 +271,8 @@ def mpeg2_gop(seed, gop_index=0, width=1920, height=1080, gop=GOP_DECODE_ORDER, 
             fwd = newer
         elif ptype == "B":
             fwd, bwd = older, newer
-        outputs = {"y": ((H, W), np.uint8, None), "cb": ((H // 2, W // 2), np.uint8, None),
-                   "cr": ((H // 2, W // 2), np.uint8, None)}
-
-        def fill(p, ptype_code=ptype_code, alt=alt):
-            s = abi.new(abi.Mpeg2Picture, width=width, height=height, picture_type=ptype_code, alternate_scan=alt,
-                        intra_dc_precision=0, matrices=p["matrices"], macroblocks=p["mbs"], levels=p["levels"],
-                        forward_source=p.get("forward_source", -1), backward_source=p.get("backward_source", -1))
-            for k, n in enumerate(("y", "cb", "cr")):
-                s.planes[k] = p[n]
-                s.forward[k] = p.get("fwd_" + n)
-                s.backward[k] = p.get("bwd_" + n)
-            return s
-
-        units.append(Unit("mpeg2", {"matrices": matrices, "mbs": mbs, "levels": levels}, outputs, fill,
-                          {"type": ptype, "fwd": fwd, "bwd": bwd, "width": width, "height": height,
-                           "coded_blocks": ncoded, "kinds": np.bincount(kind, minlength=4)}))
+        units.append(mpeg2_unit(width, height, ptype_code, matrices, mbs, levels, field_motion, alt, 0, fwd, bwd,
+                                {"type": ptype, "coded_blocks": ncoded, "kinds": np.bincount(kind, minlength=4)}))
         if ptype in "IP":
             older, newer = newer, pi
     return units

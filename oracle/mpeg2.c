@@ -125,6 +125,56 @@ static void predict_plane(uint8_t *dst, int stride, int height, int x, int y, in
         }
 }
 
+/* halfpel_field_sample + clamped_field_sample, decoder.rs:1068-1113: the destination row's field
+ * line is dest_y / 2; samples come from the field of parity `select` (rows select, select + 2, ...),
+ * clamped to that field's last line (height - 1 - select) / 2 and to the row width `stride`. */
+static uint16_t clamped_field_sample(const uint8_t *reference, int stride, int height, int x, int field_y, int select) {
+    int max_row = height > 1 + select ? (height - 1 - select) / 2 : 0;
+    if (field_y < 0) field_y = 0;
+    if (field_y > max_row) field_y = max_row;
+    if (x < 0) x = 0;
+    if (x > stride - 1) x = stride - 1;
+    return reference[(size_t)(field_y * 2 + select) * stride + x];
+}
+
+uint8_t oracle_mpeg2_halfpel_field_sample(const uint8_t *reference, int stride, int height, int x, int dest_y,
+                                          int mv_x, int mv_y, int select) {
+    int base_x = x + div_euclid2(mv_x), base_y = dest_y / 2 + div_euclid2(mv_y);
+    int fx = (mv_x - 2 * div_euclid2(mv_x)) != 0, fy = (mv_y - 2 * div_euclid2(mv_y)) != 0;
+    uint16_t a = clamped_field_sample(reference, stride, height, base_x, base_y, select);
+    if (!fx && !fy) return (uint8_t)a;
+    if (fx && !fy) return (uint8_t)((a + clamped_field_sample(reference, stride, height, base_x + 1, base_y, select) + 1) / 2);
+    if (!fx && fy) return (uint8_t)((a + clamped_field_sample(reference, stride, height, base_x, base_y + 1, select) + 1) / 2);
+    uint16_t b = clamped_field_sample(reference, stride, height, base_x + 1, base_y, select);
+    uint16_t c = clamped_field_sample(reference, stride, height, base_x, base_y + 1, select);
+    uint16_t d = clamped_field_sample(reference, stride, height, base_x + 1, base_y + 1, select);
+    return (uint8_t)((a + b + c + d + 2) / 4);
+}
+
+/* predict_field_plane, decoder.rs:1048-1066: destination rows of parity f use the field-f
+ * vector and select of each direction; fwd+bwd combine (a+b).div_ceil(2). mv[f] / sel[f]. */
+static void predict_field_plane(uint8_t *dst, int stride, int height, int x, int y, int w, int h,
+                                const int (*fwd_mv)[2], const int *fwd_sel, const int (*bwd_mv)[2], const int *bwd_sel,
+                                const uint8_t *fwd, const uint8_t *bwd) {
+    for (int row = 0; row < h; ++row) {
+        int f = row & 1, py = y + row;
+        for (int col = 0; col < w; ++col) {
+            int has_f = fwd_mv && fwd, has_b = bwd_mv && bwd;
+            uint16_t a = 0, b = 0;
+            if (has_f)
+                a = oracle_mpeg2_halfpel_field_sample(fwd, stride, height, x + col, py, fwd_mv[f][0], fwd_mv[f][1], fwd_sel[f]);
+            if (has_b)
+                b = oracle_mpeg2_halfpel_field_sample(bwd, stride, height, x + col, py, bwd_mv[f][0], bwd_mv[f][1], bwd_sel[f]);
+            uint8_t sample;
+            if (has_f && has_b) sample = (uint8_t)((a + b + 1) / 2);
+            else if (has_f) sample = (uint8_t)a;
+            else if (has_b) sample = (uint8_t)b;
+            else sample = 0;
+            dst[(size_t)py * stride + x + col] = sample;
+        }
+    }
+}
+
 /* put_block, decoder.rs:1231-1298. */
 static void put_block(const mmr_recon_mpeg2_picture *p, int stride_y, int mb_x, int mb_y, int block,
                       const int32_t c[64], int intra, int interlaced) {
@@ -154,7 +204,8 @@ static void put_block(const mmr_recon_mpeg2_picture *p, int stride_y, int mb_x, 
  * with VLC decoding replaced by the macroblock records and level buffer. Macroblocks are
  * independent inside a picture, so raster order equals slice order. Prediction follows
  * predict_macroblock (decoder.rs:944-1046): P uses the newer anchor as forward reference,
- * B uses older/newer as forward/backward; chroma MV = v / 2 truncating (chroma_motion). */
+ * B uses older/newer as forward/backward; chroma MV = v / 2 truncating (chroma_motion), per
+ * field for field prediction (chroma_field_motion, decoder.rs:1224-1229, selects unchanged). */
 int oracle_mpeg2_reconstruct(const mmr_recon_mpeg2_picture *p) {
     int mb_w = (int)((p->width + 15) / 16), mb_h = (int)((p->height + 15) / 16);
     int stride_y = mb_w * 16, height_y = mb_h * 16;
@@ -165,7 +216,6 @@ int oracle_mpeg2_reconstruct(const mmr_recon_mpeg2_picture *p) {
         int mb_x = mb % mb_w, mb_y = mb / mb_w;
         int interlaced = (m->flags & MMR_MPEG2_MB_INTERLACED_DCT) != 0;
         const int16_t *levels = p->levels + (size_t)m->block_offset * 64;
-        if (m->flags & MMR_MPEG2_MB_FIELD_MOTION) return -3;
         if (m->flags & MMR_MPEG2_MB_INTRA) {
             for (int b = 0; b < 6; ++b) {
                 int component = b < 4 ? 0 : b - 3;
@@ -181,12 +231,32 @@ int oracle_mpeg2_reconstruct(const mmr_recon_mpeg2_picture *p) {
         if (!has_f && !has_b) return -2;
         if (has_f && !p->forward[0]) return -2;
         if (has_b && !p->backward[0]) return -2;
-        int cf[2] = {fwd[0] / 2, fwd[1] / 2}, cb[2] = {bwd[0] / 2, bwd[1] / 2};
-        predict_plane(p->planes[0], stride_y, height_y, mb_x * 16, mb_y * 16, 16, 16, has_f ? fwd : 0,
-                      has_b ? bwd : 0, p->forward[0], p->backward[0]);
-        for (int k = 1; k < 3; ++k)
-            predict_plane(p->planes[k], stride_c, height_c, mb_x * 8, mb_y * 8, 8, 8, has_f ? cf : 0,
-                          has_b ? cb : 0, p->forward[k], p->backward[k]);
+        if (m->flags & MMR_MPEG2_MB_FIELD_MOTION) {
+            if (!p->field_motion) return -2;
+            const mmr_recon_mpeg2_field_motion *fm = &p->field_motion[mb];
+            int lf[2][2] = {{fwd[0], fwd[1]}, {fm->mv[0][0], fm->mv[0][1]}};
+            int lb[2][2] = {{bwd[0], bwd[1]}, {fm->mv[1][0], fm->mv[1][1]}};
+            int sf[2] = {fm->field_select[0][0], fm->field_select[0][1]};
+            int sb[2] = {fm->field_select[1][0], fm->field_select[1][1]};
+            int cfv[2][2], cbv[2][2];
+            for (int f = 0; f < 2; ++f)
+                for (int k = 0; k < 2; ++k) {
+                    cfv[f][k] = lf[f][k] / 2;
+                    cbv[f][k] = lb[f][k] / 2;
+                }
+            predict_field_plane(p->planes[0], stride_y, height_y, mb_x * 16, mb_y * 16, 16, 16, has_f ? lf : 0, sf,
+                                has_b ? lb : 0, sb, p->forward[0], p->backward[0]);
+            for (int k = 1; k < 3; ++k)
+                predict_field_plane(p->planes[k], stride_c, height_c, mb_x * 8, mb_y * 8, 8, 8, has_f ? cfv : 0, sf,
+                                    has_b ? cbv : 0, sb, p->forward[k], p->backward[k]);
+        } else {
+            int cf[2] = {fwd[0] / 2, fwd[1] / 2}, cb[2] = {bwd[0] / 2, bwd[1] / 2};
+            predict_plane(p->planes[0], stride_y, height_y, mb_x * 16, mb_y * 16, 16, 16, has_f ? fwd : 0,
+                          has_b ? bwd : 0, p->forward[0], p->backward[0]);
+            for (int k = 1; k < 3; ++k)
+                predict_plane(p->planes[k], stride_c, height_c, mb_x * 8, mb_y * 8, 8, 8, has_f ? cf : 0,
+                              has_b ? cb : 0, p->forward[k], p->backward[k]);
+        }
         int coded = 0;
         for (int b = 0; b < 6; ++b) {
             if (!(m->cbp & (1 << (5 - b)))) continue;

@@ -54,7 +54,27 @@ void oracle_mpeg2_dequantize_non_intra(const int16_t levels[64], int qscale, int
                                        const uint8_t matrix[64], int32_t coefficients[64]);
 uint8_t oracle_mpeg2_halfpel_sample(const uint8_t *reference, int stride, int height, int x,
                                     int y, int mv_x, int mv_y);
+uint8_t oracle_mpeg2_halfpel_field_sample(const uint8_t *reference, int stride, int height, int x, int dest_y,
+                                          int mv_x, int mv_y, int select);
 int oracle_mpeg2_reconstruct(const mmr_recon_mpeg2_picture *picture);
+
+/* MPEG-2 entropy front end (mpeg2_entropy.c): a parsed stream's frame pictures in decode order. */
+typedef struct oracle_mpeg2_stream oracle_mpeg2_stream;
+typedef struct oracle_mpeg2_picture {
+    uint32_t width, height, picture_type, alternate_scan, intra_dc_precision;
+    int32_t forward_index, backward_index;   /* decode-order indices of the references, -1 none */
+    uint32_t mb_count, coded_blocks, field_motion_mbs;
+    const uint8_t *matrices;                 /* [4][64] natural: intra, non-intra, chroma intra, chroma non-intra */
+    const mmr_recon_mpeg2_macroblock *macroblocks;
+    const mmr_recon_mpeg2_field_motion *field_motion;   /* [mb_count] */
+    const int16_t *levels;                   /* [coded_blocks][64] by scan position */
+    uint32_t display_index;                  /* pictures before its GOP + temporal_reference */
+    uint32_t picture_structure, frame_pred_frame_dct, progressive_frame;   /* picture coding extension */
+} oracle_mpeg2_picture;
+int oracle_mpeg2_parse(const uint8_t *data, size_t len, oracle_mpeg2_stream **out, char *err, size_t errcap);
+uint32_t oracle_mpeg2_picture_count(const oracle_mpeg2_stream *stream);
+int oracle_mpeg2_picture_info(const oracle_mpeg2_stream *stream, uint32_t index, oracle_mpeg2_picture *out);
+void oracle_mpeg2_free(oracle_mpeg2_stream *stream);
 
 /* DV (crates/codecs/dv/src) */
 void oracle_dv_inverse_88(const int32_t coefficients[64], uint8_t out[64]);

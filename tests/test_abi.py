@@ -22,6 +22,7 @@ STRUCTS = {
     "mmr_recon_mjpeg_frame": abi.MjpegFrame,
     "mmr_recon_mpeg2_macroblock": abi.Mpeg2Macroblock,
     "mmr_recon_mpeg2_picture": abi.Mpeg2Picture,
+    "mmr_recon_mpeg2_field_motion": abi.Mpeg2FieldMotion,
     "mmr_recon_dv_frame": abi.DvFrame,
     "mmr_recon_h264_macroblock": abi.H264Macroblock,
     "mmr_recon_h264_block": abi.H264Block,

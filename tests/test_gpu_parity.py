@@ -93,14 +93,36 @@ def test_mpeg2_gop_bit_exact(ctx, oracle, flags, size):
         d.free()
 
 
-def test_mpeg2_field_motion_reports_unsupported(ctx):
-    units = wl.mpeg2_gop(seed=3, width=64, height=32, gop="IP")
-    units[1].inputs["mbs"]["flags"][0] |= abi.MPEG2_MB_FIELD_MOTION
+@pytest.mark.parametrize("flags", [0, abi.SUBMIT_PER_FRAME_LAUNCH])
+@pytest.mark.parametrize("size", [(176, 144), (100, 50)])
+def test_mpeg2_field_prediction_bit_exact(ctx, oracle, flags, size):
+    """Field prediction (random field vectors reaching past the edges, random selects) and field
+    DCT mixed with frame prediction."""
+    units = wl.mpeg2_gop(seed=27, gop_index=size[1], width=size[0], height=size[1], field_fraction=0.5)
+    assert any(((u.inputs["mbs"]["flags"] & abi.MPEG2_MB_FIELD_MOTION) != 0).any() for u in units)
+    dev = _mpeg2_gpu(ctx, units, flags)
+    want = _mpeg2_oracle(oracle, units)
+    for i, d in enumerate(dev):
+        for n in ("y", "cb", "cr"):
+            _assert_same(f"picture {i} ({units[i].meta['type']}) {n}", d.download(n), want[i][n])
+    for d in dev:
+        d.free()
+
+
+@pytest.mark.parametrize("case", ["no_field_array", "bad_select"])
+def test_mpeg2_field_motion_invalid_data(ctx, case):
     import mmrecode_amd
 
+    units = wl.mpeg2_gop(seed=3, width=64, height=32, gop="IP", field_fraction=0.5)
+    field = (units[1].inputs["mbs"]["flags"] & abi.MPEG2_MB_FIELD_MOTION) != 0
+    assert field.any()
+    if case == "no_field_array":
+        del units[1].inputs["field_motion"]
+    else:
+        units[1].inputs["field_motion"]["field_select"][np.argmax(field), 1, 0] = 2
     with pytest.raises(mmrecode_amd.ReconError) as err:
         _mpeg2_gpu(ctx, units)
-    assert err.value.status == abi.STATUS_UNSUPPORTED
+    assert err.value.status == abi.STATUS_INVALID_DATA
     ctx.synchronize()   # the error was returned by wait and is not repeated
 
 
