@@ -344,7 +344,7 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     const uint8_t* bref = pr.bwd;
     int pitch = stride, ry = yy, fh = height, bh = height;
     if (!intra && (mb_flags & MMR_MPEG2_MB_FIELD_MOTION) && job.field_motion) {
-        // predict_field_plane / halfpel_field_sample (decoder.rs:1048-1113): the lane's row has
+        // predict_field_plane / halfpel_field_sample (decoder.rs:1053-1148): the lane's row has
         // parity yy & 1 and takes that parity's vector and field select in each direction (parity
         // 0: the record's vectors, parity 1: the field record's). It reads field `sel` of the
         // reference -- rows sel, sel + 2, ..., a plane of pitch 2 * stride and (height - 1 - sel) / 2

@@ -125,7 +125,7 @@ static void predict_plane(uint8_t *dst, int stride, int height, int x, int y, in
         }
 }
 
-/* halfpel_field_sample + clamped_field_sample, decoder.rs:1068-1113: the destination row's field
+/* halfpel_field_sample + clamped_field_sample, decoder.rs:1099-1148: the destination row's field
  * line is dest_y / 2; samples come from the field of parity `select` (rows select, select + 2, ...),
  * clamped to that field's last line (height - 1 - select) / 2 and to the row width `stride`. */
 static uint16_t clamped_field_sample(const uint8_t *reference, int stride, int height, int x, int field_y, int select) {
@@ -151,7 +151,7 @@ uint8_t oracle_mpeg2_halfpel_field_sample(const uint8_t *reference, int stride, 
     return (uint8_t)((a + b + c + d + 2) / 4);
 }
 
-/* predict_field_plane, decoder.rs:1048-1066: destination rows of parity f use the field-f
+/* predict_field_plane, decoder.rs:1053-1097: destination rows of parity f use the field-f
  * vector and select of each direction; fwd+bwd combine (a+b).div_ceil(2). mv[f] / sel[f]. */
 static void predict_field_plane(uint8_t *dst, int stride, int height, int x, int y, int w, int h,
                                 const int (*fwd_mv)[2], const int *fwd_sel, const int (*bwd_mv)[2], const int *bwd_sel,

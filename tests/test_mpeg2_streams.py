@@ -159,7 +159,7 @@ def test_front_end_rejects_malformed_streams():
 
 
 def test_field_sample_is_frame_sample_of_the_field_plane(oracle):
-    """halfpel_field_sample (decoder.rs:1068-1113) on a frame plane equals halfpel_sample on the
+    """halfpel_field_sample (decoder.rs:1099-1148) on a frame plane equals halfpel_sample on the
     field of that parity taken out of the plane (rows sel, sel + 2, ...), with clamping to the
     field's own last row, for every vector class and positions past each edge."""
     rng = np.random.default_rng(5)
