@@ -309,6 +309,17 @@ def dv_frame(seed, index=0, system=abi.DV_SYSTEM_625_50, mode248_fraction=0.3, z
     cls = rng.integers(0, 4, nblocks).astype(np.uint8)
     qno = rng.integers(0, 16, nblocks).astype(np.uint8)
     info = (mode | (cls << 1) | (qno << 3)).astype(np.uint8)
+    return dv_unit(system, levels, info)
+
+
+def dv_unit(system, levels, info):
+    """One DV25 frame as a Unit: levels int16 [blocks][64] by scan position ([0] = DC) and info
+    uint8 [blocks] in DIF order (sequence, slot, macroblock, block)."""
+    seqs = 12 if system == abi.DV_SYSTEM_625_50 else 10
+    nblocks = seqs * 27 * 5 * 6
+    assert levels.shape == (nblocks, 64) and info.shape == (nblocks,)
+    height = 576 if system == abi.DV_SYSTEM_625_50 else 480
+    cw, ch = (360, 288) if system == abi.DV_SYSTEM_625_50 else (180, 480)
     outputs = {"y": ((height, 720), np.uint8, None), "cb": ((ch, cw), np.uint8, None), "cr": ((ch, cw), np.uint8, None)}
 
     def fill(p):
@@ -316,8 +327,8 @@ def dv_frame(seed, index=0, system=abi.DV_SYSTEM_625_50, mode248_fraction=0.3, z
         s.planes[0], s.planes[1], s.planes[2] = p["y"], p["cb"], p["cr"]
         return s
 
-    return Unit("dv", {"levels": levels, "info": info}, outputs, fill,
-                {"system": system, "blocks": nblocks, "width": 720, "height": height})
+    return Unit("dv", {"levels": np.ascontiguousarray(levels, np.int16), "info": np.ascontiguousarray(info, np.uint8)},
+                outputs, fill, {"system": system, "blocks": nblocks, "width": 720, "height": height})
 
 
 # ---------------------------------------------------------------- H.264 (config 3)

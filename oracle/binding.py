@@ -77,6 +77,9 @@ def lib():
             "oracle_mpeg2_picture_count": (u32, [dp]),
             "oracle_mpeg2_picture_info": (i32, [dp, u32, C.POINTER(Mpeg2PictureInfo)]),
             "oracle_mpeg2_free": (None, [dp]),
+            "oracle_dv_vlc_table": (i32, [dp, dp, dp, dp, i32]),
+            "oracle_dv_entropy": (i32, [dp, C.c_size_t, i32, C.POINTER(u32), dp, dp, dp, u32, C.POINTER(u32),
+                                        C.c_char_p, C.c_size_t]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -236,3 +239,34 @@ def ycbcr_to_rgb(y, cb, cr, rng, matrix=abi.MATRIX_BT601):
     out = np.zeros(3, np.uint8)
     lib().oracle_ycbcr_to_rgb(y, cb, cr, rng, matrix, ptr(out))
     return tuple(int(v) for v in out)
+
+
+def dv_vlc_table():
+    """The DV coefficient code table as the front end builds it: arrays code, len, run, level in
+    the reference's entry order (before its sort by length)."""
+    n = 746
+    code, ln = np.zeros(n, np.uint16), np.zeros(n, np.uint8)
+    run, level = np.zeros(n, np.uint8), np.zeros(n, np.int16)
+    got = lib().oracle_dv_vlc_table(code.ctypes.data, ln.ctypes.data, run.ctypes.data, level.ctypes.data, n)
+    assert got == n, got
+    return {"code": code, "len": ln, "run": run, "level": level}
+
+
+def dv_frame_levels(data, conceal=False):
+    """Entropy-decodes one raw DV25 frame. Returns a dict: system (MMR_DV_SYSTEM_*), levels
+    (int16 [blocks][64] by scan position, [0] = DC), info (uint8 [blocks]) in DIF order, and
+    concealed [(sequence, slot)] when conceal is set (segments that failed, emitted as blocks that
+    reconstruct to Y 16 / C 128). Raises StreamError when a segment fails and conceal is off."""
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    levels = np.zeros((12 * 810, 64), np.int16)
+    info = np.zeros(12 * 810, np.uint8)
+    concealed = np.zeros(12 * 27, np.uint32)
+    system, count = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(256)
+    if lib().oracle_dv_entropy(buf.ctypes.data, len(buf), int(bool(conceal)), C.byref(system), levels.ctypes.data,
+                               info.ctypes.data, concealed.ctypes.data, len(concealed), C.byref(count), err, 256):
+        raise StreamError(err.value.decode())
+    from mmrecode_amd import abi
+    blocks = (12 if system.value == abi.DV_SYSTEM_625_50 else 10) * 810
+    return {"system": system.value, "levels": levels[:blocks].copy(), "info": info[:blocks].copy(),
+            "concealed": [(int(c) >> 8, int(c) & 0xFF) for c in concealed[:count.value]]}

@@ -76,6 +76,12 @@ uint32_t oracle_mpeg2_picture_count(const oracle_mpeg2_stream *stream);
 int oracle_mpeg2_picture_info(const oracle_mpeg2_stream *stream, uint32_t index, oracle_mpeg2_picture *out);
 void oracle_mpeg2_free(oracle_mpeg2_stream *stream);
 
+/* DV25 entropy front end (dv_entropy.c): one raw DIF frame -> mmr_recon_dv_frame buffers.
+ * levels [sequences * 810][64] and info [sequences * 810]; concealed segments as seq << 8 | slot. */
+int oracle_dv_vlc_table(uint16_t *code, uint8_t *len, uint8_t *run, int16_t *level, int capacity);
+int oracle_dv_entropy(const uint8_t *data, size_t len, int conceal, uint32_t *system, int16_t *levels, uint8_t *info,
+                      uint32_t *concealed, uint32_t concealed_cap, uint32_t *concealed_count, char *err, size_t errcap);
+
 /* DV (crates/codecs/dv/src) */
 void oracle_dv_inverse_88(const int32_t coefficients[64], uint8_t out[64]);
 void oracle_dv_inverse_248(const int32_t coefficients[64], uint8_t out[64]);
