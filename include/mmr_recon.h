@@ -253,7 +253,8 @@ MMR_RECON_API mmr_status mmr_recon_dv_submit(mmr_recon_context *context,
 
 /* ----------------------------------------------------------------- H.264 */
 
-#define MMR_H264_MB_INTRA 1u          /* deblocking bS 3/4 only; no intra prediction */
+#define MMR_H264_MB_INTRA 1u          /* intra macroblock: the picture's intra record says how it is
+                                         predicted (deblocking bS 3/4); without intra records UNSUPPORTED */
 #define MMR_H264_MB_TRANSFORM_8X8 2u
 #define MMR_H264_MB_NO_DEBLOCK 4u     /* disable_deblocking_filter_idc == 1 for this MB's slice */
 #define MMR_H264_MB_SLICE_EDGE_OFF 8u /* disable_deblocking_filter_idc == 2: no filtering across slices */
@@ -307,20 +308,47 @@ typedef struct mmr_recon_h264_reference {
 /* Level record of one coded MB (384 x int16):
  *   transform 4x4: [0,256)  16 luma blocks x 16 levels, zig-zag scan, block-index order
  *   transform 8x8: [0,256)  4 luma 8x8 blocks x 64 levels, 8x8 zig-zag scan (de-interleaved)
+ *   Intra16x16:    [16 b + 1, 16 b + 16) the 15 AC levels of luma block b (scan index 1..15),
+ *                  [16 k] the luma DC level at scan index k of the 4x4 DC array
  *   [256,264) chroma DC: Cb c0..c3, Cr c0..c3
- *   [264,384) chroma AC: Cb blocks 0..3, Cr blocks 0..3, 15 levels each (scan index 1..15) */
+ *   [264,384) chroma AC: Cb blocks 0..3, Cr blocks 0..3, 15 levels each (scan index 1..15)
+ *   I_PCM: [0,256) luma samples in raster order, [256,320) Cb, [320,384) Cr (values 0..255)
+ * Intra16x16 and I_PCM macroblocks always have a record; the others only when cbp != 0. */
 #define MMR_H264_LEVELS_PER_MB 384u
+
+#define MMR_H264_INTRA_4X4 0u
+#define MMR_H264_INTRA_8X8 1u      /* also needs MMR_H264_MB_TRANSFORM_8X8 */
+#define MMR_H264_INTRA_16X16 2u
+#define MMR_H264_INTRA_PCM 3u
+
+/*
+ * Intra prediction of one MMR_H264_MB_INTRA macroblock (predict_intra4_block, predict_intra8_block,
+ * predict_block, predict_chroma_block and their callers, crates/codecs/h264/src/decoder.rs:12224-12330,
+ * :12377-12628, :13030-13564, :14499-14712; I_PCM place_pcm_macroblock :10982-11004). Neighbours are
+ * available when they are inside the picture and in the same slice (slice_id); the top-right and
+ * top-left samples follow the reference's rules. Residuals use the intra scaling lists (4x4 lists 0-2,
+ * 8x8 list 0).
+ */
+typedef struct mmr_recon_h264_intra {
+    uint8_t type;            /* MMR_H264_INTRA_* */
+    uint8_t luma_mode;       /* Intra16x16: 0 vertical, 1 horizontal, 2 DC, 3 plane */
+    uint8_t chroma_mode;     /* 0 DC, 1 horizontal, 2 vertical, 3 plane (not I_PCM) */
+    uint8_t reserved;
+    uint8_t modes[16];       /* Intra4x4: mode 0..8 of each 4x4 block, block-index order;
+                                Intra8x8: modes[0..3] of the 8x8 blocks */
+} mmr_recon_h264_intra;
 
 #define MMR_H264_WEIGHTED_NONE 0u      /* default: average / copy */
 #define MMR_H264_WEIGHTED_TABLE 1u     /* explicit or implicit: use the weight tables */
 
 /*
- * One inter frame picture. Replaces H264 inter reconstruction:
+ * One frame picture. Replaces H264 picture reconstruction after entropy decoding:
  * predict_inter_partition / predict_bi_inter_partition
  * (crates/codecs/h264/src/decoder.rs:11602-11656, :11901-12119) with luma_qpel,
  * chroma_epel and the weights (:13619-13926), the residual paths
- * add_luma_residual_blocks / _8x8_blocks / add_chroma_residual (:12630-12789),
- * and deblock -> filter_picture (:12791-12848, crates/codecs/h264/src/deblock.rs:84-587).
+ * add_luma_residual_blocks / _8x8_blocks / add_chroma_residual (:12630-12789), intra
+ * prediction of intra macroblocks (mmr_recon_h264_intra), and deblock -> filter_picture
+ * (:12791-12848, crates/codecs/h264/src/deblock.rs:84-587).
  */
 typedef struct mmr_recon_h264_picture {
     size_t struct_size;
@@ -338,6 +366,9 @@ typedef struct mmr_recon_h264_picture {
     uint8_t *planes[3];                         /* output coded planes (stride 16*width_mbs, 8*width_mbs) */
     uint32_t ref_count[2];
     mmr_recon_h264_reference refs[2][MMR_H264_MAX_REFS];
+    const mmr_recon_h264_intra *intra;          /* [mbs], read for MMR_H264_MB_INTRA macroblocks; NULL:
+                                                   intra macroblocks report UNSUPPORTED. A struct_size
+                                                   that ends before this member reads as NULL. */
 } mmr_recon_h264_picture;
 
 MMR_RECON_API mmr_status mmr_recon_h264_submit(mmr_recon_context *context,

@@ -41,6 +41,10 @@ H264_MB_NO_DEBLOCK = 4
 H264_MB_SLICE_EDGE_OFF = 8
 H264_MAX_REFS = 16
 H264_LEVELS_PER_MB = 384
+H264_INTRA_4X4 = 0
+H264_INTRA_8X8 = 1
+H264_INTRA_16X16 = 2
+H264_INTRA_PCM = 3
 H264_WEIGHTED_NONE = 0
 H264_WEIGHTED_TABLE = 1
 
@@ -168,6 +172,16 @@ class H264BiWeight(C.Structure):
     ]
 
 
+class H264Intra(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint8),
+        ("luma_mode", C.c_uint8),
+        ("chroma_mode", C.c_uint8),
+        ("reserved", C.c_uint8),
+        ("modes", C.c_uint8 * 16),
+    ]
+
+
 class H264Reference(C.Structure):
     _fields_ = [
         ("planes", vp * 3),
@@ -193,6 +207,7 @@ class H264Picture(C.Structure):
         ("planes", vp * 3),
         ("ref_count", C.c_uint32 * 2),
         ("refs", (H264Reference * H264_MAX_REFS) * 2),
+        ("intra", vp),
     ]
 
 

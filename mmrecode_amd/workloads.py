@@ -358,10 +358,51 @@ def h264_reference_planes(seed, width_mbs, height_mbs, index):
     return plane(W, H), plane(W // 2, H // 2), plane(W // 2, H // 2)
 
 
+# mmr_recon_h264_intra
+INTRA_DTYPE = np.dtype([("type", "u1"), ("luma_mode", "u1"), ("chroma_mode", "u1"), ("reserved", "u1"),
+                        ("modes", "u1", (16,))])
+
+# Intra4x4 / Intra8x8 modes by the neighbours they need
+_NXN_MODES = {(False, False): [2], (True, False): [0, 2, 3, 7], (False, True): [1, 2, 8],
+              (True, True): list(range(9))}
+_I16_MODES = {(False, False): [2], (True, False): [0, 2], (False, True): [1, 2], (True, True): [0, 1, 2, 3]}
+_CHROMA_MODES = {(False, False): [0], (True, False): [0, 2], (False, True): [0, 1], (True, True): [0, 1, 2, 3]}
+
+
+def _intra_records(rng, intra, slice_id, width_mbs, type_weights):
+    """Intra records for the macroblocks flagged in `intra`, with modes whose neighbours exist
+    (same slice, inside the picture); 4x4 / 8x8 blocks inside the macroblock have the neighbours
+    the macroblock's own earlier blocks give."""
+    n_mb = intra.size
+    rec = np.zeros(n_mb, INTRA_DTYPE)
+    itype = rng.choice(4, n_mb, p=type_weights)
+    for a in np.nonzero(intra)[0]:
+        x, y = a % width_mbs, a // width_mbs
+        left = bool(x > 0 and slice_id[a - 1] == slice_id[a])
+        top = bool(y > 0 and slice_id[a - width_mbs] == slice_id[a])
+        t = int(itype[a])
+        rec["type"][a] = t
+        rec["chroma_mode"][a] = rng.choice(_CHROMA_MODES[(top, left)])
+        if t == abi.H264_INTRA_16X16:
+            rec["luma_mode"][a] = rng.choice(_I16_MODES[(top, left)])
+        elif t == abi.H264_INTRA_4X4:
+            for k in range(16):
+                bx, by = BLOCK_XY[k]
+                rec["modes"][a, k] = rng.choice(_NXN_MODES[(bool(by > 0 or top), bool(bx > 0 or left))])
+        elif t == abi.H264_INTRA_8X8:
+            for g in range(4):
+                rec["modes"][a, g] = rng.choice(_NXN_MODES[(bool(g >= 2 or top), bool(g % 2 == 1 or left))])
+    return rec
+
+
 def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=0.4, deblock=True,
-                 transform8_fraction=0.5, intra_fraction=0.0, mv_range=512):
-    """One inter picture. References are supplied at fill time through extra pointers
-    'ref{list}_{r}_{y|cb|cr}' (4 per list); POCs: list0 = [4, 2, 0, 6], list1 = [6, 8, 4, 2]."""
+                 transform8_fraction=0.5, intra_fraction=0.0, mv_range=512, slices=1,
+                 intra_types=(0.4, 0.25, 0.25, 0.1)):
+    """One picture. References are supplied at fill time through extra pointers
+    'ref{list}_{r}_{y|cb|cr}' (4 per list); POCs: list0 = [4, 2, 0, 6], list1 = [6, 8, 4, 2].
+    intra_fraction of the macroblocks are intra, of the types Intra4x4 / Intra8x8 / Intra16x16 /
+    I_PCM in the proportions intra_types; slices > 1 cuts the picture into that many raster
+    slices (intra neighbours end at slice boundaries)."""
     rng = np.random.Generator(np.random.PCG64([seed, 3, width_mbs, height_mbs, index]))
     n_mb = width_mbs * height_mbs
     mbs = np.zeros(n_mb, MB_DTYPE)
@@ -390,9 +431,24 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
     # residual
     cbp_luma = rng.integers(0, 16, n_mb)
     cbp_chroma = rng.integers(0, 3, n_mb)
+    intra_rec = None
+    record = np.zeros(n_mb, bool)   # level record present without a coded block pattern
+    if intra.any() or slices > 1:
+        if slices > 1:
+            cuts = np.sort(rng.choice(np.arange(1, n_mb), slices - 1, replace=False))
+            mbs["slice_id"] = np.searchsorted(cuts, np.arange(n_mb), side="right")
+        intra_rec = _intra_records(rng, intra, mbs["slice_id"], width_mbs, np.asarray(intra_types) / sum(intra_types))
+        itype = np.where(intra, intra_rec["type"], 255)
+        t8 = np.where(intra, itype == abi.H264_INTRA_8X8, t8)
+        mbs["flags"] = np.where(t8, abi.H264_MB_TRANSFORM_8X8, 0) | np.where(intra, abi.H264_MB_INTRA, 0)
+        i16 = itype == abi.H264_INTRA_16X16
+        pcm = itype == abi.H264_INTRA_PCM
+        cbp_luma = np.where(i16, np.where(cbp_luma >= 8, 15, 0), np.where(pcm, 0, cbp_luma))
+        cbp_chroma = np.where(pcm, 0, cbp_chroma)
+        record = i16 | pcm
     cbp = (cbp_luma | (cbp_chroma << 4)).astype(np.uint8)
     mbs["cbp"] = cbp
-    coded = np.nonzero(cbp)[0]
+    coded = np.nonzero((cbp != 0) | record)[0]
     mbs["coef_offset"][coded] = np.arange(coded.size)
     levels = np.zeros((max(coded.size, 1), abi.H264_LEVELS_PER_MB), np.int16)
     if coded.size:
@@ -407,6 +463,17 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
         cac *= (cbp_chroma[coded] == 2)[:, None]
         levels[:nc, 256:264] = cdc
         levels[:nc, 264:] = cac
+        if intra_rec is not None:
+            ci = intra_rec["type"][coded]
+            ii = intra[coded]
+            i16 = ii & (ci == abi.H264_INTRA_16X16)
+            if i16.any():   # luma DC levels by scan index at [16 k]
+                dcl = laplace_levels(rng, int(i16.sum()), scale0=12.0, decay=4.0, zero_fraction=0.3, first=0, length=16)
+                levels[:nc][i16, 0:256:16] = dcl
+            pcm = ii & (ci == abi.H264_INTRA_PCM)
+            if pcm.any():
+                levels[:nc][pcm] = rng.integers(0, 256, (int(pcm.sum()), 384))
+                luma[pcm] = 16          # counted as 16 nonzero coefficients per 4x4 (mark_pcm)
         # nonzero counts for bS: CAVLC total_coeff per 4x4; 8x8 transform: count of the 8x8 in all four
         nz4 = (luma != 0).sum(axis=2)
         t8c = t8[coded]
@@ -434,7 +501,8 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
     def fill(p):
         s = abi.new(abi.H264Picture, width_mbs=width_mbs, height_mbs=height_mbs, weighted_mode=mode,
                     deblock=int(deblock), macroblocks=p["mbs"], blocks=p["blocks"], levels=p["levels"],
-                    weights=p["weights"], bi_weights=p["bi_weights"], scaling_lists=None)
+                    weights=p["weights"], bi_weights=p["bi_weights"], scaling_lists=None,
+                    intra=p.get("intra"))
         s.planes[0], s.planes[1], s.planes[2] = p["y"], p["cb"], p["cr"]
         s.chroma_qp_offset[0] = s.chroma_qp_offset[1] = 0
         for lst in range(2):
@@ -447,9 +515,12 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
         return s
 
     ref_ids = ([0, 1, 2, 3], [3, 0, 1, 2])     # list -> resident reference index
-    return Unit("h264", {"mbs": mbs, "blocks": blocks, "levels": levels, "weights": weights, "bi_weights": bi_w},
+    inputs = {"mbs": mbs, "blocks": blocks, "levels": levels, "weights": weights, "bi_weights": bi_w}
+    if intra_rec is not None and intra.any():
+        inputs["intra"] = intra_rec
+    return Unit("h264", inputs,
                 outputs, fill, {"width_mbs": width_mbs, "height_mbs": height_mbs, "ref_ids": ref_ids,
-                                "coded_mbs": int(coded.size)})
+                                "coded_mbs": int(coded.size), "intra_mbs": int(intra.sum())})
 
 
 def h264_ref_ptrs(unit, refs, kind):

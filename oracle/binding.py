@@ -78,6 +78,11 @@ def lib():
             "oracle_mpeg2_picture_info": (i32, [dp, u32, C.POINTER(Mpeg2PictureInfo)]),
             "oracle_mpeg2_free": (None, [dp]),
             "oracle_dv_vlc_table": (i32, [dp, dp, dp, dp, i32]),
+            "oracle_h264_intra4_predict": (i32, [dp, i32, i32, i32, i32, i32, i32, i32, dp]),
+            "oracle_h264_intra8_predict": (i32, [dp, i32, i32, i32, i32, i32, i32, i32, dp]),
+            "oracle_h264_intra_block_predict": (i32, [dp, i32, i32, i32, i32, i32, i32, i32, dp]),
+            "oracle_h264_chroma_predict": (i32, [dp, i32, i32, i32, i32, i32, i32, dp]),
+            "oracle_h264_intra16_dc": (i32, [dp, i32, dp, dp]),
             "oracle_dv_entropy": (i32, [dp, C.c_size_t, i32, C.POINTER(u32), dp, dp, dp, u32, C.POINTER(u32),
                                         C.c_char_p, C.c_size_t]),
         }

@@ -379,12 +379,120 @@ static int add_residuals(const mmr_recon_h264_picture *p, int mb) {
     return 0;
 }
 
-/* Inter reconstruction of one frame picture followed by the loop filter when enabled
- * (decode_p_picture / decode_b_picture -> deblock, decoder.rs:1158, :1930). */
+static void place(uint8_t *plane, int stride, int ox, int oy, int n, const uint8_t *v) {
+    for (int y = 0; y < n; ++y) memcpy(plane + (size_t)(oy + y) * stride + ox, v + y * n, (size_t)n);
+}
+
+/* luma_block_index: block index of the 4x4 block at (bx, by). */
+static int luma_block_index(int bx, int by) { return (by / 2) * 8 + (bx / 2) * 4 + (by % 2) * 2 + bx % 2; }
+
+/* One intra macroblock, as the reference's intra paths do it (decoder.rs:5690-5722, :5835-5848,
+ * :5955-5961): I_PCM samples (place_pcm_macroblock :10982-11004); Intra4x4 blocks in block order,
+ * each predicted then given its residual (reconstruct_intra4_luma_block :12377-12480, list 0);
+ * Intra8x8 likewise (:12482-12578, 8x8 list 0); Intra16x16 predicted whole, then the DC
+ * transform and 16 residual blocks (predict_intra16_macroblock :12224-12279,
+ * add_intra16_luma_residual :12580-12628); then chroma prediction (predict_chroma_macroblock
+ * :12281-12330) and the chroma residual with the intra lists 1 and 2 (add_chroma_residual
+ * :12716-12789). Neighbour availability: same slice, inside the picture (:10801-10814). */
+static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
+    const mmr_recon_h264_macroblock *m = &p->macroblocks[mb];
+    const mmr_recon_h264_intra *in = &p->intra[mb];
+    const int wm = (int)p->width_mbs, W = wm * 16, CW = W / 2;
+    const int mbx = mb % wm, mby = mb / wm, qp = m->qp;
+    const int left_mb = mbx > 0 && p->macroblocks[mb - 1].slice_id == m->slice_id;
+    const int top_mb = mby > 0 && p->macroblocks[mb - wm].slice_id == m->slice_id;
+    const int has_record = m->cbp != 0 || in->type == MMR_H264_INTRA_16X16 || in->type == MMR_H264_INTRA_PCM;
+    const int16_t *lv = has_record ? p->levels + (size_t)m->coef_offset * MMR_H264_LEVELS_PER_MB : NULL;
+    int32_t levels[64], res[64];
+    uint8_t pred[256];
+    if (in->type == MMR_H264_INTRA_PCM) {
+        for (int y = 0; y < 16; ++y)
+            for (int x = 0; x < 16; ++x) p->planes[0][(size_t)(mby * 16 + y) * W + mbx * 16 + x] = (uint8_t)lv[y * 16 + x];
+        for (int c = 0; c < 2; ++c)
+            for (int y = 0; y < 8; ++y)
+                for (int x = 0; x < 8; ++x)
+                    p->planes[1 + c][(size_t)(mby * 8 + y) * CW + mbx * 8 + x] = (uint8_t)lv[256 + c * 64 + y * 8 + x];
+        return 0;
+    }
+    if (in->type == MMR_H264_INTRA_4X4) {
+        for (int k = 0; k < 16; ++k) {
+            int bx, by;
+            luma_block_position(k, &bx, &by);
+            const int ox = mbx * 16 + bx * 4, oy = mby * 16 + by * 4;
+            if (oracle_h264_intra4_predict(p->planes[0], W, ox, oy, k, in->modes[k], top_mb, left_mb, pred)) return -1;
+            place(p->planes[0], W, ox, oy, 4, pred);
+            if (m->cbp & (1 << (k / 4))) {
+                for (int i = 0; i < 16; ++i) levels[i] = lv[k * 16 + i];
+                if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 0), res)) return -2;
+                add_residual(p->planes[0], W, ox, oy, 4, res);
+            }
+        }
+    } else if (in->type == MMR_H264_INTRA_8X8) {
+        if (!(m->flags & MMR_H264_MB_TRANSFORM_8X8)) return -1;
+        for (int g = 0; g < 4; ++g) {
+            const int ox = mbx * 16 + (g % 2) * 8, oy = mby * 16 + (g / 2) * 8;
+            if (oracle_h264_intra8_predict(p->planes[0], W, ox, oy, g, in->modes[g], top_mb, left_mb, pred)) return -1;
+            place(p->planes[0], W, ox, oy, 8, pred);
+            if (m->cbp & (1 << g)) {
+                for (int i = 0; i < 64; ++i) levels[i] = lv[g * 64 + i];
+                if (oracle_h264_transform_residual_8x8(levels, qp, scaling8(p, 0), res)) return -2;
+                add_residual(p->planes[0], W, ox, oy, 8, res);
+            }
+        }
+    } else if (in->type == MMR_H264_INTRA_16X16) {
+        if (oracle_h264_intra_block_predict(p->planes[0], W, mbx * 16, mby * 16, 16, in->luma_mode, top_mb, left_mb, pred))
+            return -1;
+        place(p->planes[0], W, mbx * 16, mby * 16, 16, pred);
+        int32_t dcl[16], dc[16];
+        for (int k = 0; k < 16; ++k) dcl[k] = lv[16 * k];
+        if (oracle_h264_intra16_dc(dcl, qp, scaling4(p, 0), dc)) return -2;
+        for (int by = 0; by < 4; ++by)
+            for (int bx = 0; bx < 4; ++bx) {
+                const int k = luma_block_index(bx, by);
+                levels[0] = dc[by * 4 + bx];
+                for (int i = 1; i < 16; ++i) levels[i] = lv[k * 16 + i];
+                if (oracle_h264_transform_residual_4x4(levels, qp, 1, scaling4(p, 0), res)) return -2;
+                add_residual(p->planes[0], W, mbx * 16 + bx * 4, mby * 16 + by * 4, 4, res);
+            }
+    } else {
+        return -1;
+    }
+    for (int c = 0; c < 2; ++c) {
+        if (oracle_h264_chroma_predict(p->planes[1 + c], CW, mbx * 8, mby * 8, in->chroma_mode, top_mb, left_mb, pred))
+            return -1;
+        place(p->planes[1 + c], CW, mbx * 8, mby * 8, 8, pred);
+    }
+    if (m->cbp >> 4) {
+        const int cqp = oracle_h264_chroma_qp(qp, p->chroma_qp_offset[0]);
+        for (int comp = 0; comp < 2; ++comp) {
+            const uint8_t *sl = scaling4(p, comp + 1);
+            int32_t dc_levels[4], dc[4];
+            for (int i = 0; i < 4; ++i) dc_levels[i] = lv[256 + comp * 4 + i];
+            if (oracle_h264_transform_chroma_dc(dc_levels, cqp, sl, dc)) return -2;
+            for (int b = 0; b < 4; ++b) {
+                int32_t co[16];
+                co[0] = dc[b];
+                for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
+                if (oracle_h264_transform_residual_4x4(co, cqp, 1, sl, res)) return -2;
+                add_residual(p->planes[1 + comp], CW, mbx * 8 + (b % 2) * 4, mby * 8 + (b / 2) * 4, 4, res);
+            }
+        }
+    }
+    return 0;
+}
+
+/* Reconstruction of one frame picture in macroblock order (inter macroblocks read only the
+ * reference pictures, intra ones the reconstructed neighbours) followed by the loop filter when
+ * enabled (decode_*_picture -> deblock, decoder.rs:1158, :1930). Returns -3 for an intra
+ * macroblock without intra records. */
 int oracle_h264_reconstruct(const mmr_recon_h264_picture *p) {
     int mbs = (int)(p->width_mbs * p->height_mbs);
     for (int mb = 0; mb < mbs; ++mb) {
-        if (p->macroblocks[mb].flags & MMR_H264_MB_INTRA) return -3;
+        if (p->macroblocks[mb].flags & MMR_H264_MB_INTRA) {
+            if (!p->intra) return -3;
+            if (reconstruct_intra(p, mb)) return -4;
+            continue;
+        }
         for (int k = 0; k < 16; ++k)
             if (predict_block(p, mb, k)) return -1;
         if (add_residuals(p, mb)) return -2;

@@ -108,6 +108,16 @@ uint8_t oracle_h264_apply_bi_weight(uint8_t s0, uint8_t s1, int log2_denom, int 
                                     int o0, int o1, int explicit_weights);
 void oracle_h264_implicit_weights(int32_t poc0, int32_t poc1, int32_t current_poc, int lt0,
                                   int lt1, int32_t *w0, int32_t *w1);
+/* H.264 intra prediction (h264_intra.c); 0 or -1 when a needed neighbour is missing / bad mode. */
+int oracle_h264_intra4_predict(const uint8_t *plane, int stride, int ox, int oy, int block_index, int mode,
+                               int top_mb, int left_mb, uint8_t out[16]);
+int oracle_h264_intra8_predict(const uint8_t *plane, int stride, int ox, int oy, int group, int mode, int top_mb,
+                               int left_mb, uint8_t out[64]);
+int oracle_h264_intra_block_predict(const uint8_t *plane, int stride, int ox, int oy, int size, int mode, int has_top,
+                                    int has_left, uint8_t *out);
+int oracle_h264_chroma_predict(const uint8_t *plane, int stride, int ox, int oy, int mode, int has_top, int has_left,
+                               uint8_t out[64]);
+int oracle_h264_intra16_dc(const int32_t levels[16], int qp, const uint8_t scaling[16], int32_t out[16]);
 int oracle_h264_reconstruct(const mmr_recon_h264_picture *picture);
 int oracle_h264_deblock(const mmr_recon_h264_picture *picture);
 void oracle_h264_filter_samples(uint8_t *plane, const int p_idx[4], const int q_idx[4], int qp,

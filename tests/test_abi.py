@@ -26,6 +26,7 @@ STRUCTS = {
     "mmr_recon_dv_frame": abi.DvFrame,
     "mmr_recon_h264_macroblock": abi.H264Macroblock,
     "mmr_recon_h264_block": abi.H264Block,
+    "mmr_recon_h264_intra": abi.H264Intra,
     "mmr_recon_h264_weight": abi.H264Weight,
     "mmr_recon_h264_bi_weight": abi.H264BiWeight,
     "mmr_recon_h264_reference": abi.H264Reference,
