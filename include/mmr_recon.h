@@ -313,7 +313,8 @@ typedef struct mmr_recon_h264_reference {
  *   [256,264) chroma DC: Cb c0..c3, Cr c0..c3
  *   [264,384) chroma AC: Cb blocks 0..3, Cr blocks 0..3, 15 levels each (scan index 1..15)
  *   I_PCM: [0,256) luma samples in raster order, [256,320) Cb, [320,384) Cr (values 0..255)
- * Intra16x16 and I_PCM macroblocks always have a record; the others only when cbp != 0. */
+ * Intra16x16 and I_PCM macroblocks always have a record; the others only when cbp != 0. Levels of
+ * blocks outside the coded block pattern are zero (every block of a record is transformed). */
 #define MMR_H264_LEVELS_PER_MB 384u
 
 #define MMR_H264_INTRA_4X4 0u

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmmr_recon.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = ["recon_host.cpp", "mjpeg.hip", "mpeg2.hip", "dv.hip", "h264.hip", "deblock.hip", "color.hip"]
+SOURCES = ["recon_host.cpp", "mjpeg.hip", "mpeg2.hip", "dv.hip", "h264.hip", "h264_intra.hip", "deblock.hip", "color.hip"]
 ARCH = os.environ.get("MMR_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

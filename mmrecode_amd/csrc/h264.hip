@@ -21,101 +21,11 @@
 // Transform arithmetic is 64-bit like the reference, so results match for every level value the
 // reference accepts.
 #include "device_common.h"
+#include "h264_residual.h"
 #include "recon_internal.h"
 
 namespace mmr {
 
-__constant__ static const uint8_t kZz4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};  // row*4+col
-__constant__ static const uint8_t kZz8[64] = {
-    0, 1, 8, 16, 9, 2, 3, 10, 17, 24, 32, 25, 18, 11, 4, 5, 12, 19, 26, 33, 40, 48, 41, 34, 27, 20,
-    13, 6, 7, 14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51, 58, 59,
-    52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
-__constant__ static const int32_t kNorm4[6][3] = {{10, 16, 13}, {11, 18, 14}, {13, 20, 16}, {14, 23, 18}, {16, 25, 20}, {18, 29, 23}};
-__constant__ static const int32_t kNorm8[6][6] = {{20, 18, 32, 19, 25, 24}, {22, 19, 35, 21, 28, 26}, {26, 23, 42, 24, 33, 31},
-                                                 {28, 25, 45, 26, 35, 33}, {32, 28, 51, 30, 40, 38}, {36, 32, 58, 34, 46, 43}};
-__constant__ static const int32_t kChromaQp[22] = {29, 30, 31, 32, 32, 33, 34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
-
-MMR_DEV int chroma_qp(int luma_qp, int offset) {
-    const int index = clampi(luma_qp + offset, 0, 51);
-    return index < 30 ? index : kChromaQp[index - 30];
-}
-
-MMR_DEV int32_t level_scale_4x4(int qp, int row, int col, const MMR_GLOBAL uint8_t* scaling) {
-    const int cat = ((row & 1) == 0 && (col & 1) == 0) ? 0 : (((row & 1) && (col & 1)) ? 1 : 2);
-    return kNorm4[qp % 6][cat] * (int32_t)scaling[row * 4 + col];
-}
-
-MMR_DEV int32_t level_scale_8x8(int qp, int row, int col, const MMR_GLOBAL uint8_t* scaling) {
-    const int rm = row & 3, cm = col & 3;
-    const bool re = (rm & 1) == 0, ce = (cm & 1) == 0;
-    int cat;
-    if (!re && !ce) cat = 1;
-    else if (re != ce) cat = (rm == 2 || cm == 2) ? 5 : 3;
-    else cat = (rm == 0 && cm == 0) ? 0 : ((rm == 2 && cm == 2) ? 2 : 4);
-    return kNorm8[qp % 6][cat] * (int32_t)scaling[row * 8 + col];
-}
-
-MMR_DEV int clip255(int v) { return clampi(v, 0, 255); }
-
-// transform_residual_4x4 (decoder.rs:14185-14233) on one lane; out in raster order. T is int64_t
-// (the reference's arithmetic) or int32_t when the macroblock's levels are small enough that no
-// intermediate can leave the int32 range (then both give the same integers).
-template <typename T>
-MMR_DEV void residual_4x4(const int32_t* lv, int qp, bool dc_scaled, const MMR_GLOBAL uint8_t* scaling, int32_t out[16]) {
-    T s[16];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int rc = kZz4[i];
-        const T v = lv[i];
-        if (i == 0 && dc_scaled) {
-            s[rc] = v;
-        } else {
-            const T scale = level_scale_4x4(qp, rc >> 2, rc & 3, scaling);
-            s[rc] = qp >= 24 ? (v * scale) << (qp / 6 - 4) : (v * scale + ((T)1 << (3 - qp / 6))) >> (4 - qp / 6);
-        }
-    }
-    T h[16];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int b = r * 4;
-        const T e0 = s[b] + s[b + 2], e1 = s[b] - s[b + 2];
-        const T e2 = (s[b + 1] >> 1) - s[b + 3], e3 = s[b + 1] + (s[b + 3] >> 1);
-        h[b] = e0 + e3;
-        h[b + 1] = e1 + e2;
-        h[b + 2] = e1 - e2;
-        h[b + 3] = e0 - e3;
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const T g0 = h[c] + h[8 + c], g1 = h[c] - h[8 + c];
-        const T g2 = (h[4 + c] >> 1) - h[12 + c], g3 = h[4 + c] + (h[12 + c] >> 1);
-        out[0 * 4 + c] = (int32_t)((g0 + g3 + 32) >> 6);
-        out[1 * 4 + c] = (int32_t)((g1 + g2 + 32) >> 6);
-        out[2 * 4 + c] = (int32_t)((g1 - g2 + 32) >> 6);
-        out[3 * 4 + c] = (int32_t)((g0 - g3 + 32) >> 6);
-    }
-}
-
-// inverse_transform_8 (decoder.rs:14280-14308).
-template <typename T>
-MMR_DEV void itrans8(const T in[8], T out[8]) {
-    const T a0 = in[0] + in[4], a2 = in[0] - in[4];
-    const T a4 = (in[2] >> 1) - in[6], a6 = in[2] + (in[6] >> 1);
-    const T b0 = a0 + a6, b2 = a2 + a4, b4 = a2 - a4, b6 = a0 - a6;
-    const T a1 = -in[3] + in[5] - in[7] - (in[7] >> 1);
-    const T a3 = in[1] + in[7] - in[3] - (in[3] >> 1);
-    const T a5 = -in[1] + in[7] + in[5] + (in[5] >> 1);
-    const T a7 = in[3] + in[5] + in[1] + (in[1] >> 1);
-    const T b1 = a1 + (a7 >> 2), b7 = a7 - (a1 >> 2), b3 = a3 + (a5 >> 2), b5 = (a3 >> 2) - a5;
-    out[0] = b0 + b7;
-    out[1] = b2 + b5;
-    out[2] = b4 + b3;
-    out[3] = b6 + b1;
-    out[4] = b6 - b1;
-    out[5] = b4 - b3;
-    out[6] = b2 - b5;
-    out[7] = b0 - b7;
-}
 
 MMR_DEV int div4(int v) { return v >> 2; }   // div_euclid(4) for two's complement
 MMR_DEV int div8(int v) { return v >> 3; }
@@ -235,92 +145,6 @@ MMR_DEV uint32_t weight2(uint32_t s0, uint32_t s1, const MMR_GLOBAL mmr_recon_h2
     return (uint32_t)clip255(v + ((bw->o0 + bw->o1 + 1) >> 1));
 }
 
-struct MbShared {
-    uint8_t pred_y[256];
-    uint8_t pred_c[2][64];
-    int32_t res_y[256];
-    int32_t res_c[2][64];
-    int64_t s8[4][64];
-    int32_t dc[2][4];
-};
-
-// Residual of one macroblock (phases 2-4 of the kernel) into S.res_y / S.res_c, in integer type T.
-template <typename T>
-MMR_DEV void residual_phases(MbShared& S, const MMR_GLOBAL int16_t* lv, int qp, int cqp, bool t8, int l, const H264Job& job) {
-    // ---- phase 2
-    {
-        if (!t8) {
-            if (l < 16) {
-                int32_t lev[16], res[16];
-#pragma unroll
-                for (int i = 0; i < 16; ++i) lev[i] = lv[l * 16 + i];
-                residual_4x4<T>(lev, qp, false, as_global(job.scaling) + 3 * 16, res);
-                const int kbx = ((l >> 2) & 1) * 2 + (l & 1), kby = (l >> 3) * 2 + ((l >> 1) & 1);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) S.res_y[(kby * 4 + (i >> 2)) * 16 + kbx * 4 + (i & 3)] = res[i];
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int idx = l * 4 + t, blk8 = idx >> 6, i = idx & 63, rc = kZz8[i];
-                const T v = (T)lv[idx] * (T)level_scale_8x8(qp, rc >> 3, rc & 7, as_global(job.scaling) + 96 + 64);
-                const int shift = qp / 6;
-                T sv = shift >= 6 ? v << (shift - 6) : (v + ((T)1 << (5 - shift))) >> (6 - shift);
-                if (rc == 0) sv += 32;
-                S.s8[blk8][rc] = sv;
-            }
-        }
-        const int dcl = t8 ? l : l - 16;
-        if (dcl == 0 || dcl == 1) {
-            // transform_chroma_dc (decoder.rs:14114-14140) with inter list 4 + component.
-            const int comp = dcl;
-            const T c0 = lv[256 + comp * 4], c1 = lv[257 + comp * 4], c2 = lv[258 + comp * 4], c3 = lv[259 + comp * 4];
-            const T f[4] = {c0 + c1 + c2 + c3, c0 - c1 + c2 - c3, c0 + c1 - c2 - c3, c0 - c1 - c2 + c3};
-            const T scale = level_scale_4x4(cqp, 0, 0, as_global(job.scaling) + (4 + comp) * 16);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) S.dc[comp][i] = (int32_t)(((f[i] * scale) << (cqp / 6)) >> 5);
-        }
-    }
-    wave_lds_sync();
-
-    // ---- phase 3
-    {
-        if (t8 && l < 32) {
-            const int blk8 = l >> 3, r = l & 7;
-            T in[8], out[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][r * 8 + i];
-            itrans8(in, out);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) S.s8[blk8][r * 8 + i] = out[i];
-        }
-        if (l >= 32 && l < 40) {
-            const int comp = (l - 32) >> 2, b = (l - 32) & 3;
-            int32_t co[16], res[16];
-            co[0] = S.dc[comp][b];
-#pragma unroll
-            for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
-            residual_4x4<T>(co, cqp, true, as_global(job.scaling) + (4 + comp) * 16, res);
-#pragma unroll
-            for (int i = 0; i < 16; ++i) S.res_c[comp][((b >> 1) * 4 + (i >> 2)) * 8 + (b & 1) * 4 + (i & 3)] = res[i];
-        }
-    }
-    wave_lds_sync();
-
-    // ---- phase 4
-    if (t8 && l < 32) {
-        const int blk8 = l >> 3, c = l & 7;
-        T in[8], out[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) in[i] = S.s8[blk8][i * 8 + c];
-        itrans8(in, out);
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-            S.res_y[((blk8 >> 1) * 8 + i) * 16 + (blk8 & 1) * 8 + c] = (int32_t)(out[i] >> 6);
-    }
-    wave_lds_sync();
-
-}
 
 __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
                                                        uint32_t tile_base, uint32_t tiles, uint32_t per_xcd,
@@ -344,6 +168,9 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
         const uint32_t wd[3] = {q[0], q[1], q[2]};
         __builtin_memcpy(&m, wd, sizeof m);
     }
+    // Intra macroblocks of a picture with intra records are reconstructed by the intra pass
+    // (h264_intra.hip) after this kernel; without records they are reported below.
+    if (active && (m.flags & MMR_H264_MB_INTRA) && job.intra) return;   // uniform over the wave
     const int mby = (int)(mb / job.width_mbs), mbx = (int)(mb - (uint32_t)mby * job.width_mbs);
     const int W = (int)job.width_mbs * 16, H = (int)job.height_mbs * 16, CW = W / 2, CH = H / 2;
     const bool weighted = job.weighted == MMR_H264_WEIGHTED_TABLE;
@@ -360,8 +187,8 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
             __builtin_memcpy(&blk, wd, sizeof blk);
         }
         int r0 = blk.ref_idx[0], r1 = blk.ref_idx[1];
-        // Intra macroblocks are not predicted on the device; out-of-range reference indices are
-        // invalid data. Both are reported through the status word and clamped so the kernel
+        // Intra macroblocks without intra records are unsupported; out-of-range reference indices
+        // are invalid data. Both are reported through the status word and clamped so the kernel
         // still completes without reading outside the reference list.
         if (l == 0 && (m.flags & MMR_H264_MB_INTRA)) atomicOr(status, kDevStatusUnsupported);
         if (r0 >= (int)job.ref_count[0] || r1 >= (int)job.ref_count[1] || (r0 < 0 && r1 < 0)) {
@@ -427,8 +254,8 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
-        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, t8, l, job);
-        else residual_phases<int64_t>(S, lv, qp, cqp, t8, l, job);
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, t8, l, job, kInterLists, status);
+        else residual_phases<int64_t>(S, lv, qp, cqp, t8, l, job, kInterLists, status);
     }
     wave_lds_sync();
 

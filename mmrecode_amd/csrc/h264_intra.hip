@@ -1,0 +1,483 @@
+// H.264 intra macroblocks of frame pictures: Intra4x4 / Intra8x8 / Intra16x16 / chroma prediction,
+// I_PCM, and their residuals, after the inter macroblocks (h264.hip) and before deblocking.
+//
+// Reference (crates/codecs/h264/src/decoder.rs): predict_intra4_block and vertical-right /
+// horizontal-down :13311-13532, predict_intra8_block / filter_intra8_neighbors :13030-13309,
+// predict_block / dc_prediction / plane_prediction :14583-14698, predict_chroma_block /
+// predict_chroma_dc :14499-14581, the macroblock order of prediction and residual in
+// reconstruct_intra4_luma_block / reconstruct_intra8_luma_block / predict_intra16_macroblock /
+// add_intra16_luma_residual / predict_chroma_macroblock :12224-12628 and place_pcm_macroblock
+// :10982-11004; availability :10801-10814.
+//
+// Ordering. An intra macroblock reads the reconstructed (not yet deblocked) samples of its left,
+// top-left, top and top-right neighbours. Inter macroblocks are final when this kernel starts, so
+// only intra ones order among themselves: (x, y) may run once (x-1, y) and (x+1, y-1) are done, a
+// wavefront as in deblocking. One wave walks one macroblock row left to right (row tickets from a
+// device counter, in picture and row order, so the row a wave waits for is held by a running wave
+// or done). Before an intra macroblock x it publishes progress x (every macroblock before x is
+// final) with an agent-scope release and waits until the row above has published
+// min(x + 2, width), then acquires; after the row it publishes the width. A wave keeps its own row's
+// previous intra macroblock in LDS for the left context.
+//
+// Inside a macroblock the residuals of all blocks are computed first (in parallel, shared with
+// the inter kernel); then Intra4x4 walks its 16 blocks and Intra8x8 its four in block order, each
+// predicted from an LDS tile holding the macroblock and its context, plus its residual.
+#include "device_common.h"
+#include "h264_residual.h"
+#include "recon_internal.h"
+
+namespace mmr {
+
+constexpr uint32_t kIntraSpinLimit = 1u << 25;
+
+struct IntraShared {
+    MbShared S;
+    // Luma tile: row 0 = the row above (column 0 the top-left sample, 1..16 the top row, 17..24
+    // top-right); rows 1..16: column 0 the left column, 1..16 the macroblock.
+    uint8_t ty[17][28];
+    // Chroma tiles: row 0 (column 0 top-left, 1..8 top); rows 1..8: column 0 left, 1..8 macroblock.
+    uint8_t tc[2][9][12];
+    // Neighbours of the current 4x4 / 8x8 block: index 0 = top-left, 1.. = top (tn) or left (ln);
+    // Intra8x8 also the filtered lines (ftn, fln).
+    uint8_t tn[20];
+    uint8_t ln[12];
+    uint8_t ftn[20];
+    uint8_t fln[12];
+    // Right column of the previous macroblock of this row when it was intra.
+    uint8_t left_y[16];
+    uint8_t left_c[2][8];
+};
+
+MMR_DEV uint32_t ip_avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
+MMR_DEV uint32_t ip_f121(uint32_t a, uint32_t b, uint32_t c) { return (a + 2 * b + c + 2) >> 2; }
+MMR_DEV uint32_t ip_avg13(uint32_t a, uint32_t b) { return (a + 3 * b + 2) >> 2; }
+
+// Intra4x4 / Intra8x8 sample (x, y) of an n x n block (n = 4 or 8) from the neighbour lines
+// t[i + 1] = top i (t[0] = top-left), l[i + 1] = left i (l[0] = top-left): the formulas of
+// predict_intra4_block / predict_intra8_block with top_or_corner(i) = t[i + 1].
+MMR_DEV uint32_t pred_nxn(int mode, int n, int x, int y, const uint8_t* t, const uint8_t* l) {
+    auto T = [&](int i) -> uint32_t { return t[i + 1]; };
+    auto L = [&](int i) -> uint32_t { return l[i + 1]; };
+    switch (mode) {
+    case 0: return T(x);
+    case 1: return L(y);
+    case 3:
+        return (x == n - 1 && y == n - 1) ? ip_avg13(T(2 * n - 2), T(2 * n - 1)) : ip_f121(T(x + y), T(x + y + 1), T(x + y + 2));
+    case 4:
+        if (x > y) return ip_f121(T(x - y - 2), T(x - y - 1), T(x - y));
+        if (x < y) return ip_f121(L(y - x - 2), L(y - x - 1), L(y - x));
+        return ip_f121(T(0), T(-1), L(0));
+    case 5: {
+        const int z = 2 * x - y;
+        if (z >= 0 && (z & 1) == 0) return ip_avg2(T(x - y / 2 - 1), T(x - y / 2));
+        if (z >= 0) return ip_f121(T(x - y / 2 - 2), T(x - y / 2 - 1), T(x - y / 2));
+        if (z == -1) return ip_f121(L(0), T(-1), T(0));
+        return ip_f121(L(y - 2 * x - 1), L(y - 2 * x - 2), L(y - 2 * x - 3));
+    }
+    case 6: {
+        const int z = 2 * y - x;
+        if (z >= 0 && (z & 1) == 0) return ip_avg2(L(y - x / 2 - 1), L(y - x / 2));
+        if (z >= 0) return ip_f121(L(y - x / 2 - 2), L(y - x / 2 - 1), L(y - x / 2));
+        if (z == -1) return ip_f121(T(0), T(-1), L(0));
+        return ip_f121(T(x - 2 * y - 1), T(x - 2 * y - 2), T(x - 2 * y - 3));
+    }
+    case 7: {
+        const int s = x + y / 2;
+        return (y & 1) == 0 ? ip_avg2(T(s), T(s + 1)) : ip_f121(T(s), T(s + 1), T(s + 2));
+    }
+    case 8: {
+        const int z = x + 2 * y, last = 2 * n - 3;   // 5 for 4x4, 13 for 8x8
+        if (z < last && (z & 1) == 0) return ip_avg2(L(y + x / 2), L(y + x / 2 + 1));
+        if (z < last) return ip_f121(L(y + x / 2), L(y + x / 2 + 1), L(y + x / 2 + 2));
+        if (z == last) return ip_avg13(L(n - 2), L(n - 1));
+        return L(n - 1);
+    }
+    default: return 128;   // mode 2 (DC) is computed by the caller
+    }
+}
+
+// DC of an n x n block from the lines above (sum of n) and left (sum of n): dc_prediction.
+MMR_DEV uint32_t dc_nxn(int n, bool has_top, bool has_left, uint32_t st, uint32_t sl) {
+    if (has_top && has_left) return (st + sl + (uint32_t)n) / (uint32_t)(2 * n);
+    if (has_top) return (st + (uint32_t)n / 2) / (uint32_t)n;
+    if (has_left) return (sl + (uint32_t)n / 2) / (uint32_t)n;
+    return 128;
+}
+
+// Which neighbours a 4x4 / 8x8 mode reads: bit 0 top, bit 1 left, bit 2 top-left.
+MMR_DEV uint32_t nxn_needs(int mode) {
+    constexpr uint32_t k = 1u << (0 * 3) | 2u << (1 * 3) | 0u << (2 * 3) | 1u << (3 * 3) | 7u << (4 * 3) |
+                           7u << (5 * 3) | 7u << (6 * 3) | 1u << (7 * 3) | 2u << (8 * 3);
+    return (k >> (3 * mode)) & 7u;
+}
+
+// Plane prediction of an n x n block (16 luma / 8 chroma) at (x, y) from tile row 0 / column 0,
+// with the macroblock at tile rows / columns 1..n (plane_prediction).
+template <int STRIDE>
+MMR_DEV uint32_t plane_sample(const uint8_t (*tile)[STRIDE], int n, int x, int y) {
+    const int half = n / 2;
+    int h = 0, v = 0;
+    for (int i = 1; i <= half; ++i) {
+        h += i * ((int)tile[0][half + i] - (int)tile[0][half - i]);     // top[half-1+i] - top[half-1-i] (i = half: corner)
+        v += i * ((int)tile[half + i][0] - (int)tile[half - i][0]);
+    }
+    const int scale = n == 16 ? 5 : 17, shift = n == 16 ? 6 : 5;
+    const int a = 16 * ((int)tile[0][n] + (int)tile[n][0]);
+    const int b = (scale * h + (1 << (shift - 1))) >> shift;
+    const int c = (scale * v + (1 << (shift - 1))) >> shift;
+    return (uint32_t)clip255((a + b * (x - (half - 1)) + c * (y - (half - 1)) + 16) >> 5);
+}
+
+// Wait for the row above: relaxed agent-scope polls, bounded (a protocol error reports TIMEOUT
+// instead of hanging), then an agent-scope acquire so its samples written before the release are
+// visible (the acquire also drops this wave's stale cached lines).
+MMR_DEV void intra_wait(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
+    uint32_t spins = 0;
+    while (seen < need) {
+        seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (seen >= need) break;
+        if (++spins > kIntraSpinLimit) {
+            atomicOr(status, kDevStatusTimeout);
+            seen = need;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// One intra macroblock (x, y) of `job`, mode record `in`.
+MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int mby, const mmr_recon_h264_macroblock& m,
+                              const mmr_recon_h264_intra& in, bool left_mb, bool top_mb, bool left_in_lds, int l,
+                              uint32_t* status) {
+    MbShared& S = sh.S;
+    const int W = (int)job.width_mbs * 16, CW = W / 2;
+    const MMR_GLOBAL uint8_t* py = as_global(job.planes[0]);
+    const MMR_GLOBAL uint8_t* pc[2] = {as_global(job.planes[1]), as_global(job.planes[2])};
+    const int type = in.type;
+    const bool pcm = type == MMR_H264_INTRA_PCM, i16 = type == MMR_H264_INTRA_16X16, i8 = type == MMR_H264_INTRA_8X8;
+    if (type > MMR_H264_INTRA_PCM || i8 != ((m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0)) {
+        if (l == 0) atomicOr(status, kDevStatusInvalidData);
+    }
+    const bool has_record = pcm || i16 || m.cbp != 0;
+    const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+
+    // ---- context tile
+    const int x0 = mbx * 16, y0 = mby * 16, cx0 = mbx * 8, cy0 = mby * 8;
+    if (top_mb) {
+        if (l < 16) sh.ty[0][1 + l] = py[(size_t)(y0 - 1) * W + x0 + l];
+        else if (l < 24 && x0 + l < W) sh.ty[0][1 + l] = py[(size_t)(y0 - 1) * W + x0 + l];
+        else if (l >= 48) {
+            const int comp = (l - 48) >> 3, i = (l - 48) & 7;
+            sh.tc[comp][0][1 + i] = pc[comp][(size_t)(cy0 - 1) * CW + cx0 + i];
+        }
+    }
+    if (left_mb) {
+        if (l >= 32 && l < 48) {
+            const int r = l - 32;
+            sh.ty[1 + r][0] = left_in_lds ? sh.left_y[r] : py[(size_t)(y0 + r) * W + x0 - 1];
+        } else if (l >= 16 && l < 32) {
+            const int comp = (l - 16) >> 3, r = (l - 16) & 7;
+            sh.tc[comp][1 + r][0] = left_in_lds ? sh.left_c[comp][r] : pc[comp][(size_t)(cy0 + r) * CW + cx0 - 1];
+        }
+    }
+    if (top_mb && left_mb && l >= 24 && l < 27) {
+        if (l == 24) sh.ty[0][0] = py[(size_t)(y0 - 1) * W + x0 - 1];
+        else sh.tc[l - 25][0][0] = pc[l - 25][(size_t)(cy0 - 1) * CW + cx0 - 1];
+    }
+
+    // ---- residuals of every block (luma 4x4 / 8x8 / Intra16x16, chroma), before prediction
+    if (has_record && !pcm) {
+        uint32_t mx = 0;
+#pragma unroll
+        for (int i = 0; i < MMR_H264_LEVELS_PER_MB / 64; ++i) {
+            const int v = lv[l + 64 * i];
+            mx = max(mx, (uint32_t)(v < 0 ? -v : v));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+        const int qp = m.qp, cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both
+        const ResidualLists rl = {0, 0, 1, i16};
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+        else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+    } else {
+        for (int i = l; i < 256; i += 64) S.res_y[i] = 0;
+        for (int i = l; i < 128; i += 64) S.res_c[i >> 6][i & 63] = 0;
+    }
+    wave_lds_sync();
+
+    // ---- luma
+    if (pcm) {
+        const int r = l >> 2, c0 = (l & 3) * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.ty[1 + r][1 + c0 + c] = (uint8_t)lv[r * 16 + c0 + c];
+    } else if (i16) {
+        const int mode = in.luma_mode;
+        const bool ok = mode <= 3 && (mode != 0 || top_mb) && (mode != 1 || left_mb) && (mode != 3 || (top_mb && left_mb));
+        if (!ok && l == 0) atomicOr(status, kDevStatusInvalidData);
+        uint32_t dc = 128;
+        if (ok && mode == 2) {
+            uint32_t st = 0, sl = 0;
+            for (int i = 0; i < 16; ++i) {
+                st += top_mb ? sh.ty[0][1 + i] : 0u;
+                sl += left_mb ? sh.ty[1 + i][0] : 0u;
+            }
+            dc = dc_nxn(16, top_mb, left_mb, st, sl);
+        }
+        const int r = l >> 2, c0 = (l & 3) * 4;
+        uint8_t out[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            uint32_t p = 128;
+            if (ok) {
+                if (mode == 0) p = sh.ty[0][1 + c0 + c];
+                else if (mode == 1) p = sh.ty[1 + r][0];
+                else if (mode == 2) p = dc;
+                else p = plane_sample<28>(sh.ty, 16, c0 + c, r);
+            }
+            out[c] = (uint8_t)clip255((int)p + S.res_y[r * 16 + c0 + c]);
+        }
+        wave_lds_sync();   // plane prediction read the context; the macroblock interior is written now
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sh.ty[1 + r][1 + c0 + c] = out[c];
+    } else if (!i8) {
+        // Intra4x4: 16 blocks in order
+        for (int k = 0; k < 16; ++k) {
+            const int bx = ((k >> 2) & 1) * 2 + (k & 1), by = (k >> 3) * 2 + ((k >> 1) & 1);
+            const int ox = bx * 4, oy = by * 4;
+            const bool ht = by > 0 || top_mb, hl = bx > 0 || left_mb;
+            const int mode = in.modes[k];
+            if (l < 9) {
+                uint32_t v = 0;
+                if (l == 0) {
+                    v = ht && hl ? sh.ty[oy][ox] : 0u;
+                } else if (ht) {
+                    const int xi = l - 1;
+                    // top-right: not for blocks 3 and 11, not into the next macroblock below the
+                    // first block row, not past the picture's right edge
+                    const bool tr_ok = xi < 4 || (k != 3 && k != 11 && !(by > 0 && ox + xi >= 16) && x0 + ox + xi < W);
+                    v = sh.ty[oy][1 + ox + (tr_ok ? xi : 3)];
+                }
+                sh.tn[l] = (uint8_t)v;
+            } else if (l < 14) {
+                const int i = l - 9;
+                sh.ln[i] = (uint8_t)(i == 0 ? (ht && hl ? sh.ty[oy][ox] : 0u) : (hl ? sh.ty[oy + i][ox] : 0u));
+            }
+            wave_lds_sync();
+            const uint32_t needs = nxn_needs(mode);
+            const bool ok = mode <= 8 && (!(needs & 1) || ht) && (!(needs & 2) || hl) && (!(needs & 4) || (ht && hl));
+            if (!ok && l == 0) atomicOr(status, kDevStatusInvalidData);
+            if (l < 16) {
+                const int x = l & 3, y = l >> 2;
+                uint32_t p = 128;
+                if (ok && mode == 2) {
+                    uint32_t st = 0, sl = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        st += sh.tn[1 + i];
+                        sl += sh.ln[1 + i];
+                    }
+                    p = dc_nxn(4, ht, hl, st, sl);
+                } else if (ok) {
+                    p = pred_nxn(mode, 4, x, y, sh.tn, sh.ln);
+                }
+                sh.ty[1 + oy + y][1 + ox + x] = (uint8_t)clip255((int)p + S.res_y[(oy + y) * 16 + ox + x]);
+            }
+            wave_lds_sync();
+        }
+    } else {
+        // Intra8x8: 4 blocks in order, neighbours filtered first (filter_intra8_neighbors)
+        for (int g = 0; g < 4; ++g) {
+            const int ox = (g & 1) * 8, oy = (g >> 1) * 8;
+            const bool ht = g >= 2 || top_mb, hl = (g & 1) != 0 || left_mb, hc = ht && hl;
+            const int mode = in.modes[g];
+            if (l < 17) {
+                uint32_t v = 0;
+                if (l == 0) {
+                    v = hc ? sh.ty[oy][ox] : 0u;
+                } else if (ht) {
+                    const int xi = l - 1;   // top-right: not for the last block, not past the right edge
+                    const bool tr_ok = xi < 8 || (g != 3 && x0 + ox + xi < W);
+                    v = sh.ty[oy][1 + ox + (tr_ok ? xi : 7)];
+                }
+                sh.tn[l] = (uint8_t)v;
+            } else if (l < 26) {
+                const int i = l - 17;
+                sh.ln[i] = (uint8_t)(i == 0 ? (hc ? sh.ty[oy][ox] : 0u) : (hl ? sh.ty[oy + i][ox] : 0u));
+            }
+            wave_lds_sync();
+            if (l < 17) {
+                uint32_t v = 0;
+                if (l == 0) {
+                    v = hc ? ip_f121(sh.ln[1], sh.tn[0], sh.tn[1]) : 0u;
+                } else if (ht) {
+                    const int i = l - 1;
+                    if (i == 0) v = hc ? ip_f121(sh.tn[0], sh.tn[1], sh.tn[2]) : ip_avg13(sh.tn[2], sh.tn[1]);
+                    else if (i < 15) v = ip_f121(sh.tn[i], sh.tn[i + 1], sh.tn[i + 2]);
+                    else v = ip_avg13(sh.tn[15], sh.tn[16]);
+                }
+                sh.ftn[l] = (uint8_t)v;
+            } else if (l < 26) {
+                const int i = l - 17;
+                uint32_t v = 0;
+                if (i == 0) {
+                    v = hc ? ip_f121(sh.ln[1], sh.tn[0], sh.tn[1]) : 0u;
+                } else if (hl) {
+                    const int j = i - 1;
+                    if (j == 0) v = hc ? ip_f121(sh.ln[0], sh.ln[1], sh.ln[2]) : ip_avg13(sh.ln[2], sh.ln[1]);
+                    else if (j < 7) v = ip_f121(sh.ln[j], sh.ln[j + 1], sh.ln[j + 2]);
+                    else v = ip_avg13(sh.ln[7], sh.ln[8]);
+                }
+                sh.fln[i] = (uint8_t)v;
+            }
+            wave_lds_sync();
+            const uint32_t needs = nxn_needs(mode);
+            const bool ok = mode <= 8 && (!(needs & 1) || ht) && (!(needs & 2) || hl) && (!(needs & 4) || hc);
+            if (!ok && l == 0) atomicOr(status, kDevStatusInvalidData);
+            {
+                const int x = l & 7, y = l >> 3;
+                uint32_t p = 128;
+                if (ok && mode == 2) {
+                    uint32_t st = 0, sl = 0;
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        st += sh.ftn[1 + i];
+                        sl += sh.fln[1 + i];
+                    }
+                    p = dc_nxn(8, ht, hl, st, sl);
+                } else if (ok) {
+                    p = pred_nxn(mode, 8, x, y, sh.ftn, sh.fln);
+                }
+                sh.ty[1 + oy + y][1 + ox + x] = (uint8_t)clip255((int)p + S.res_y[(oy + y) * 16 + ox + x]);
+            }
+            wave_lds_sync();
+        }
+    }
+
+    // ---- chroma
+    if (pcm) {
+        const int comp = l >> 5, r = (l >> 2) & 7, c0 = (l & 3) * 2;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) sh.tc[comp][1 + r][1 + c0 + c] = (uint8_t)lv[256 + comp * 64 + r * 8 + c0 + c];
+    } else {
+        const int mode = in.chroma_mode;
+        const bool ok = mode <= 3 && (mode != 1 || left_mb) && (mode != 2 || top_mb) && (mode != 3 || (top_mb && left_mb));
+        if (!ok && l == 0) atomicOr(status, kDevStatusInvalidData);
+        const int comp = l >> 5, r = (l >> 2) & 7, c0 = (l & 3) * 2;
+        const uint8_t (*t)[12] = sh.tc[comp];
+        uint8_t out[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int x = c0 + c;
+            uint32_t p = 128;
+            if (ok) {
+                if (mode == 1) {
+                    p = t[1 + r][0];
+                } else if (mode == 2) {
+                    p = t[0][1 + x];
+                } else if (mode == 3) {
+                    p = plane_sample<12>(t, 8, x, r);
+                } else {
+                    // predict_chroma_dc: quadrants (0,0) and (1,1) use both lines, (1,0) prefers the
+                    // top, (0,1) the left
+                    const int qx = x >> 2, qy = r >> 2;
+                    const bool ut = top_mb && (!left_mb || qy == 0 || qx == 1);
+                    const bool ul = left_mb && (!top_mb || qx == 0 || qy == 1);
+                    uint32_t sum = 0, count = 0;
+                    if (ut) {
+                        for (int i = 0; i < 4; ++i) sum += t[0][1 + qx * 4 + i];
+                        count += 4;
+                    }
+                    if (ul) {
+                        for (int i = 0; i < 4; ++i) sum += t[1 + qy * 4 + i][0];
+                        count += 4;
+                    }
+                    p = count ? (sum + count / 2) / count : 128u;
+                }
+            }
+            out[c] = (uint8_t)clip255((int)p + S.res_c[comp][r * 8 + x]);
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < 2; ++c) sh.tc[comp][1 + r][1 + c0 + c] = out[c];
+    }
+    wave_lds_sync();
+
+    // ---- store, and keep the right column as the next macroblock's left context
+    {
+        const int r = l >> 2, c0 = (l & 3) * 4;
+        uint32_t word = 0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) word |= (uint32_t)sh.ty[1 + r][1 + c0 + c] << (8 * c);
+        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y0 + r) * W + x0 + c0) = word;
+        const int comp = l >> 5, cr = (l >> 2) & 7, cc = (l & 3) * 2;
+        const uint32_t half = (uint32_t)sh.tc[comp][1 + cr][1 + cc] | (uint32_t)sh.tc[comp][1 + cr][2 + cc] << 8;
+        *reinterpret_cast<MMR_GLOBAL uint16_t*>(as_global(job.planes[1 + comp]) + (size_t)(cy0 + cr) * CW + cx0 + cc) = (uint16_t)half;
+    }
+    if (l < 16) sh.left_y[l] = sh.ty[1 + l][16];
+    else if (l < 32) sh.left_c[(l - 16) >> 3][(l - 16) & 7] = sh.tc[(l - 16) >> 3][1 + ((l - 16) & 7)][8];
+    wave_lds_sync();
+}
+
+__global__ __launch_bounds__(64) void h264_intra_kernel(const H264Job* __restrict__ jobs, uint32_t job_count, uint32_t total_rows,
+                                                         uint32_t* __restrict__ counters, uint32_t* __restrict__ status) {
+    __shared__ IntraShared sh;
+    const int l = threadIdx.x;
+    for (;;) {
+        uint32_t t = 0;
+        if (l == 0) t = fetch_add_relaxed(&counters[0], 1);
+        t = (uint32_t)__shfl((int)t, 0, 64);
+        if (t >= total_rows) break;
+        const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.intra_row_begin; })];
+        const int mby = (int)(t - job.intra_row_begin), mw = (int)job.width_mbs;
+        uint32_t* above = mby > 0 ? &counters[1 + t - 1] : nullptr;
+        uint32_t* self = &counters[1 + t];
+        uint32_t seen = 0, published = 0;
+        bool left_in_lds = false;
+        for (int mbx = 0; mbx < mw; ++mbx) {
+            const uint32_t mb = (uint32_t)(mby * mw + mbx);
+            mmr_recon_h264_macroblock m;
+            {
+                const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb));
+                const uint32_t wd[3] = {q[0], q[1], q[2]};
+                __builtin_memcpy(&m, wd, sizeof m);
+            }
+            if (!(m.flags & MMR_H264_MB_INTRA)) {
+                left_in_lds = false;
+                continue;
+            }
+            if (published < (uint32_t)mbx) {
+                published = (uint32_t)mbx;
+                __hip_atomic_store(self, published, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (above) intra_wait(above, (uint32_t)min(mbx + 2, mw), seen, status);
+            else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            mmr_recon_h264_intra in;
+            {
+                static_assert(sizeof(mmr_recon_h264_intra) == 20, "intra record is 5 dwords");
+                const MMR_GLOBAL uint8_t* q = reinterpret_cast<const MMR_GLOBAL uint8_t*>(as_global(job.intra + mb));
+                uint8_t b[20];
+#pragma unroll
+                for (int i = 0; i < 20; ++i) b[i] = q[i];
+                __builtin_memcpy(&in, b, sizeof in);
+            }
+            // left_macroblock_available / top_macroblock_available: inside the picture, same slice
+            const bool left_mb = mbx > 0 && job.mbs[mb - 1].slice_id == m.slice_id;
+            const bool top_mb = mby > 0 && job.mbs[mb - (uint32_t)mw].slice_id == m.slice_id;
+            intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, l, status);
+            left_in_lds = true;
+        }
+        __hip_atomic_store(self, (uint32_t)mw, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+hipError_t launch_h264_intra(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters, uint32_t groups,
+                             uint32_t* d_status, hipStream_t s) {
+    if (job_count == 0 || total_rows == 0) return hipSuccess;
+    if (groups > total_rows) groups = total_rows;
+    if (groups == 0) groups = 1;
+    hipLaunchKernelGGL(h264_intra_kernel, dim3(groups), dim3(64), 0, s, d_jobs, job_count, total_rows, d_counters, d_status);
+    return hipGetLastError();
+}
+
+}  // namespace mmr

@@ -615,11 +615,14 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::H264Job> jobs(count);
     std::vector<uint32_t> tiles;
-    uint32_t bands = 0;
+    uint32_t bands = 0, intra_rows = 0;
     const uint32_t band_rows = ctx->tuning.h264_deblock_band_rows, publish = ctx->tuning.h264_deblock_publish;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
-        if (p.struct_size < sizeof(mmr_recon_h264_picture)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
+        // descriptors that end before intra (built without intra records) stay valid
+        constexpr size_t kH264Base = offsetof(mmr_recon_h264_picture, intra);
+        if (p.struct_size < kH264Base) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
+        const mmr_recon_h264_intra* intra = p.struct_size >= kH264Base + sizeof(p.intra) ? p.intra : nullptr;
         if (p.width_mbs == 0 || p.height_mbs == 0 || p.width_mbs > 1024 || p.height_mbs > 1024)
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
         if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
@@ -667,13 +670,19 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
                     if (!aligned(j.ref_planes[list][r][k], 4)) j.refs_aligned = 0;
         j.band_begin = per_frame ? 0 : bands;
         bands += j.deblock ? (p.height_mbs + band_rows - 1) / band_rows : 0;
+        j.intra = intra;
+        j.intra_row_begin = per_frame ? 0 : intra_rows;
+        intra_rows += intra ? p.height_mbs : 0;
     }
     Submission sub(ctx, stream, cap);
+    // Counters: deblocking band tickets and hand-over progress, then intra row tickets and progress
+    // (per picture with one launch per picture).
     size_t counters = 0;
     if (per_frame) {
-        for (uint32_t i = 0; i < count; ++i) counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0);
+        for (uint32_t i = 0; i < count; ++i)
+            counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0) + (jobs[i].intra ? 1 + pics[i].height_mbs : 0);
     } else {
-        counters = 1 + bands;
+        counters = 1 + bands + (intra_rows ? 1 + intra_rows : 0);
     }
     // Scratch: 16 bytes of boundary strengths per macroblock of every deblocked picture.
     std::vector<size_t> bs_off(count, 0);
@@ -694,12 +703,17 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count * ctx->tuning.h264_deblock_groups_per_cu;
+    const uint32_t intra_groups = ctx->cu_count * 8;   // one wave per macroblock row in flight
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
             const uint32_t mc_tiles = (pics[i].width_mbs * pics[i].height_mbs + 3) / 4;
             if (jobs[i].deblock) e = mmr::launch_h264_bs(dj, dt, jobs[i].tile_begin, mc_tiles, stream);
             if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, mc_tiles, sub.status_ptr(), stream);
+            if (e == hipSuccess && jobs[i].intra) {
+                e = mmr::launch_h264_intra(dj + i, 1, pics[i].height_mbs, c, intra_groups, sub.status_ptr(), stream);
+                c += 1 + pics[i].height_mbs;
+            }
             if (e == hipSuccess && jobs[i].deblock) {
                 const uint32_t nb = (pics[i].height_mbs + band_rows - 1) / band_rows;
                 e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, band_rows, publish, sub.status_ptr(), stream);
@@ -711,6 +725,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     } else {
         if (bands) e = mmr::launch_h264_bs(dj, dt, 0, (uint32_t)tiles.size(), stream);
         if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
+        if (e == hipSuccess && intra_rows)
+            e = mmr::launch_h264_intra(dj, count, intra_rows, sub.counters_ptr() + 1 + bands, intra_groups, sub.status_ptr(), stream);
         if (e == hipSuccess && bands)
             e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, band_rows, publish, sub.status_ptr(), stream);
     }

@@ -393,7 +393,9 @@ static int luma_block_index(int bx, int by) { return (by / 2) * 8 + (bx / 2) * 4
  * transform and 16 residual blocks (predict_intra16_macroblock :12224-12279,
  * add_intra16_luma_residual :12580-12628); then chroma prediction (predict_chroma_macroblock
  * :12281-12330) and the chroma residual with the intra lists 1 and 2 (add_chroma_residual
- * :12716-12789). Neighbour availability: same slice, inside the picture (:10801-10814). */
+ * :12716-12789). Neighbour availability: same slice, inside the picture (:10801-10814). As for
+ * inter macroblocks, a level record (present when cbp != 0, and always for Intra16x16 / I_PCM)
+ * holds zeros outside the coded block pattern, so every block of it is transformed. */
 static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
     const mmr_recon_h264_macroblock *m = &p->macroblocks[mb];
     const mmr_recon_h264_intra *in = &p->intra[mb];
@@ -421,7 +423,7 @@ static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
             const int ox = mbx * 16 + bx * 4, oy = mby * 16 + by * 4;
             if (oracle_h264_intra4_predict(p->planes[0], W, ox, oy, k, in->modes[k], top_mb, left_mb, pred)) return -1;
             place(p->planes[0], W, ox, oy, 4, pred);
-            if (m->cbp & (1 << (k / 4))) {
+            if (lv) {
                 for (int i = 0; i < 16; ++i) levels[i] = lv[k * 16 + i];
                 if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 0), res)) return -2;
                 add_residual(p->planes[0], W, ox, oy, 4, res);
@@ -433,7 +435,7 @@ static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
             const int ox = mbx * 16 + (g % 2) * 8, oy = mby * 16 + (g / 2) * 8;
             if (oracle_h264_intra8_predict(p->planes[0], W, ox, oy, g, in->modes[g], top_mb, left_mb, pred)) return -1;
             place(p->planes[0], W, ox, oy, 8, pred);
-            if (m->cbp & (1 << g)) {
+            if (lv) {
                 for (int i = 0; i < 64; ++i) levels[i] = lv[g * 64 + i];
                 if (oracle_h264_transform_residual_8x8(levels, qp, scaling8(p, 0), res)) return -2;
                 add_residual(p->planes[0], W, ox, oy, 8, res);
@@ -462,7 +464,7 @@ static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
             return -1;
         place(p->planes[1 + c], CW, mbx * 8, mby * 8, 8, pred);
     }
-    if (m->cbp >> 4) {
+    if (lv) {
         const int cqp = oracle_h264_chroma_qp(qp, p->chroma_qp_offset[0]);
         for (int comp = 0; comp < 2; ++comp) {
             const uint8_t *sl = scaling4(p, comp + 1);
