@@ -1,6 +1,6 @@
 """Benchmark of the MI355X reconstruction path.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mpeg2|mjpeg|dv|h264]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload mpeg2|mjpeg|dv|h264|h264i]
 
 A step is one pass of the hot path over one batch of synthetic post-entropy input, all inputs
 resident in HBM before the timed region:
@@ -8,6 +8,7 @@ resident in HBM before the timed region:
   mjpeg            120 frames of 640x480 4:2:0 + RGB8 in one launch
   dv               256 frames, 720x576 4:2:0 and 720x480 4:1:1 interleaved, one launch
   h264             16 pictures each of 1280x720, 1920x1080, 3840x2160 with deblocking, one launch
+  h264i            the same pictures with every macroblock intra (I pictures)
 Reported beside the batched value: the plain path (one picture per launch, synchronous) and the
 CPU baseline (the oracle, a scalar C restatement of the reference functions, on the host cores
 over a bounded sample of the same workload).
@@ -225,6 +226,8 @@ class H264Workload:
     kernel = "h264_mb_kernel"
     SIZES = ((80, 45), (120, 68), (240, 135))
 
+    picture_kwargs = {}
+
     def __init__(self, ctx, per_size=16, seed=2024):
         self.ctx = ctx
         self.host_refs, self.dev_refs = {}, {}
@@ -238,7 +241,7 @@ class H264Workload:
             self.dev_refs[(wmb, hmb)] = devr
             ptrs = [{n: b.ptr for n, b in r.items()} for r in devr]
             for i in range(per_size):
-                u = wl.h264_picture(seed, i, wmb, hmb)
+                u = wl.h264_picture(seed, i, wmb, hmb, **self.picture_kwargs)
                 d = wl.DeviceUnit(ctx, u, wl.h264_ref_ptrs(u, ptrs, "dev"))
                 self.units.append(u)
                 self.dev.append(d)
@@ -277,6 +280,18 @@ class H264Workload:
         mp = passes * len(picks) * metrics.mpix(1920, 1088)
         return mp / dt, len(picks), (f"{passes} pass(es) over {len(picks)} 1920x1088 pictures, one picture per thread, "
                                      f"{dt:.1f} s wall on {len(picks)} threads")
+
+
+class H264IntraWorkload(H264Workload):
+    """The H.264 sizes with every macroblock intra (I pictures: Intra4x4 / Intra8x8 / Intra16x16 /
+    I_PCM mixed 40 / 25 / 25 / 10 %, one slice, deblocking on): the intra pass's wavefront."""
+    name = "h264i"
+    kernel = "h264_intra_kernel"
+    picture_kwargs = {"intra_fraction": 1.0}
+
+    def config(self):
+        return {"workload": "H.264 High 4:2:0 intra (I pictures) + deblocking, 16 pictures each of 1280x720, "
+                            "1920x1088, 3840x2160 per launch", "pictures_per_step": self.pictures}
 
 
 # ------------------------------------------------------------------ served mixed workload
@@ -529,7 +544,8 @@ def run_served_sweep(ctx, rates, seconds, plain_limit):
             "seconds_per_rate": seconds, "rates": out}
 
 
-WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload}
+WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload,
+             "h264i": H264IntraWorkload}
 
 
 def calibrate_copy(nbytes=1 << 30, repeats=5):

@@ -367,7 +367,7 @@ typedef struct mmr_recon_h264_picture {
     uint8_t *planes[3];                         /* output coded planes (stride 16*width_mbs, 8*width_mbs) */
     uint32_t ref_count[2];
     mmr_recon_h264_reference refs[2][MMR_H264_MAX_REFS];
-    const mmr_recon_h264_intra *intra;          /* [mbs], read for MMR_H264_MB_INTRA macroblocks; NULL:
+    const mmr_recon_h264_intra *intra;          /* [mbs], 4-byte aligned, read for MMR_H264_MB_INTRA macroblocks; NULL:
                                                    intra macroblocks report UNSUPPORTED. A struct_size
                                                    that ends before this member reads as NULL. */
 } mmr_recon_h264_picture;

@@ -12,12 +12,15 @@
 // Ordering. An intra macroblock reads the reconstructed (not yet deblocked) samples of its left,
 // top-left, top and top-right neighbours. Inter macroblocks are final when this kernel starts, so
 // only intra ones order among themselves: (x, y) may run once (x-1, y) and (x+1, y-1) are done, a
-// wavefront as in deblocking. One wave walks one macroblock row left to right (row tickets from a
-// device counter, in picture and row order, so the row a wave waits for is held by a running wave
-// or done). Before an intra macroblock x it publishes progress x (every macroblock before x is
-// final) with an agent-scope release and waits until the row above has published
-// min(x + 2, width), then acquires; after the row it publishes the width. A wave keeps its own row's
-// previous intra macroblock in LDS for the left context.
+// wavefront as in deblocking. Pictures are cut into bands of kIntraRows macroblock rows; a
+// workgroup takes band tickets from a device counter (in picture and band order, so the band it
+// waits for is held by a running workgroup or done) and wave w walks row w of the band left to
+// right. Before an intra macroblock x a row publishes progress x (every macroblock before x is
+// final) and waits until the row above has published min(x + 2, width): inside the band through an
+// LDS counter with workgroup-scope release / acquire; across bands the band's last row publishes to
+// global memory with an agent-scope release every `publish` macroblocks and at its end, and the next
+// band's first row polls with relaxed agent-scope loads, then acquires at agent scope. A wave keeps
+// its own row's previous intra macroblock in LDS for the left context.
 //
 // Inside a macroblock the residuals of all blocks are computed first (in parallel, shared with
 // the inter kernel); then Intra4x4 walks its 16 blocks and Intra8x8 its four in block order, each
@@ -29,6 +32,7 @@
 namespace mmr {
 
 constexpr uint32_t kIntraSpinLimit = 1u << 25;
+constexpr int kIntraRows = 8;   // rows per band = waves per workgroup
 
 struct IntraShared {
     MbShared S;
@@ -128,12 +132,24 @@ MMR_DEV uint32_t plane_sample(const uint8_t (*tile)[STRIDE], int n, int x, int y
     return (uint32_t)clip255((a + b * (x - (half - 1)) + c * (y - (half - 1)) + 16) >> 5);
 }
 
-// Wait for the row above: relaxed agent-scope polls, bounded (a protocol error reports TIMEOUT
-// instead of hanging), then an agent-scope acquire so its samples written before the release are
-// visible (the acquire also drops this wave's stale cached lines).
-MMR_DEV void intra_wait(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
+// Waits for the row above, bounded (a protocol error reports TIMEOUT instead of hanging). Inside the
+// band: the LDS counter of the row above with a workgroup-scope acquire. Across bands: relaxed
+// agent-scope polls of the previous band's published progress, then an agent-scope acquire so the
+// samples it wrote before its release are visible (and this wave's stale cached lines dropped).
+MMR_DEV void intra_wait_lds(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
-    while (seen < need) {
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+        if (++spins > kIntraSpinLimit) {
+            atomicOr(status, kDevStatusTimeout);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+MMR_DEV void intra_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
+    if (seen >= need) return;
+    uint32_t spins = 0;
+    for (;;) {
         seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (seen >= need) break;
         if (++spins > kIntraSpinLimit) {
@@ -146,7 +162,42 @@ MMR_DEV void intra_wait(uint32_t* counter, uint32_t need, uint32_t& seen, uint32
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
-// One intra macroblock (x, y) of `job`, mode record `in`.
+// Residuals of every block of an intra macroblock (luma 4x4 / 8x8 / Intra16x16, chroma) into
+// sh.S, zero without a level record; they do not depend on the neighbours, so they are computed
+// before the wait for the row above. Also checks the type against the transform flag.
+MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon_h264_macroblock& m, const mmr_recon_h264_intra& in,
+                            int l, uint32_t* status) {
+    MbShared& S = sh.S;
+    const int type = in.type;
+    const bool pcm = type == MMR_H264_INTRA_PCM, i16 = type == MMR_H264_INTRA_16X16, i8 = type == MMR_H264_INTRA_8X8;
+    if (type > MMR_H264_INTRA_PCM || i8 != ((m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0)) {
+        if (l == 0) atomicOr(status, kDevStatusInvalidData);
+    }
+    const bool has_record = pcm || i16 || m.cbp != 0;
+    const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+    // ---- residuals of every block (luma 4x4 / 8x8 / Intra16x16, chroma), before prediction
+    if (has_record && !pcm) {
+        uint32_t mx = 0;
+#pragma unroll
+        for (int i = 0; i < MMR_H264_LEVELS_PER_MB / 64; ++i) {
+            const int v = lv[l + 64 * i];
+            mx = max(mx, (uint32_t)(v < 0 ? -v : v));
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
+        const int qp = m.qp, cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both
+        const ResidualLists rl = {0, 0, 1, i16};
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+        else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+    } else {
+        for (int i = l; i < 256; i += 64) S.res_y[i] = 0;
+        for (int i = l; i < 128; i += 64) S.res_c[i >> 6][i & 63] = 0;
+    }
+    wave_lds_sync();
+}
+
+// Prediction of one intra macroblock (x, y) of `job`, mode record `in`, plus the residual already
+// in sh.S; stores the macroblock and keeps its right column for the next one.
 MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int mby, const mmr_recon_h264_macroblock& m,
                               const mmr_recon_h264_intra& in, bool left_mb, bool top_mb, bool left_in_lds, int l,
                               uint32_t* status) {
@@ -156,10 +207,6 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
     const MMR_GLOBAL uint8_t* pc[2] = {as_global(job.planes[1]), as_global(job.planes[2])};
     const int type = in.type;
     const bool pcm = type == MMR_H264_INTRA_PCM, i16 = type == MMR_H264_INTRA_16X16, i8 = type == MMR_H264_INTRA_8X8;
-    if (type > MMR_H264_INTRA_PCM || i8 != ((m.flags & MMR_H264_MB_TRANSFORM_8X8) != 0)) {
-        if (l == 0) atomicOr(status, kDevStatusInvalidData);
-    }
-    const bool has_record = pcm || i16 || m.cbp != 0;
     const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
 
     // ---- context tile
@@ -186,25 +233,7 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
         else sh.tc[l - 25][0][0] = pc[l - 25][(size_t)(cy0 - 1) * CW + cx0 - 1];
     }
 
-    // ---- residuals of every block (luma 4x4 / 8x8 / Intra16x16, chroma), before prediction
-    if (has_record && !pcm) {
-        uint32_t mx = 0;
-#pragma unroll
-        for (int i = 0; i < MMR_H264_LEVELS_PER_MB / 64; ++i) {
-            const int v = lv[l + 64 * i];
-            mx = max(mx, (uint32_t)(v < 0 ? -v : v));
-        }
-#pragma unroll
-        for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
-        const int qp = m.qp, cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both
-        const ResidualLists rl = {0, 0, 1, i16};
-        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, job, rl, status);
-        else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, job, rl, status);
-    } else {
-        for (int i = l; i < 256; i += 64) S.res_y[i] = 0;
-        for (int i = l; i < 128; i += 64) S.res_c[i >> 6][i & 63] = 0;
-    }
-    wave_lds_sync();
+    wave_lds_sync();   // context tile
 
     // ---- luma
     if (pcm) {
@@ -419,65 +448,107 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
     wave_lds_sync();
 }
 
-__global__ __launch_bounds__(64) void h264_intra_kernel(const H264Job* __restrict__ jobs, uint32_t job_count, uint32_t total_rows,
-                                                         uint32_t* __restrict__ counters, uint32_t* __restrict__ status) {
-    __shared__ IntraShared sh;
-    const int l = threadIdx.x;
+__global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264Job* __restrict__ jobs, uint32_t job_count,
+                                                                      uint32_t total_bands, uint32_t* __restrict__ counters,
+                                                                      uint32_t publish, uint32_t* __restrict__ status) {
+    __shared__ IntraShared sh_all[kIntraRows];
+    __shared__ uint32_t prog[kIntraRows];
+    __shared__ uint32_t s_ticket;
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    IntraShared& sh = sh_all[w];
     for (;;) {
-        uint32_t t = 0;
-        if (l == 0) t = fetch_add_relaxed(&counters[0], 1);
-        t = (uint32_t)__shfl((int)t, 0, 64);
-        if (t >= total_rows) break;
-        const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.intra_row_begin; })];
-        const int mby = (int)(t - job.intra_row_begin), mw = (int)job.width_mbs;
-        uint32_t* above = mby > 0 ? &counters[1 + t - 1] : nullptr;
-        uint32_t* self = &counters[1 + t];
-        uint32_t seen = 0, published = 0;
-        bool left_in_lds = false;
-        for (int mbx = 0; mbx < mw; ++mbx) {
-            const uint32_t mb = (uint32_t)(mby * mw + mbx);
-            mmr_recon_h264_macroblock m;
-            {
-                const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb));
-                const uint32_t wd[3] = {q[0], q[1], q[2]};
-                __builtin_memcpy(&m, wd, sizeof m);
-            }
-            if (!(m.flags & MMR_H264_MB_INTRA)) {
-                left_in_lds = false;
-                continue;
-            }
-            if (published < (uint32_t)mbx) {
-                published = (uint32_t)mbx;
-                __hip_atomic_store(self, published, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (above) intra_wait(above, (uint32_t)min(mbx + 2, mw), seen, status);
-            else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            mmr_recon_h264_intra in;
-            {
-                static_assert(sizeof(mmr_recon_h264_intra) == 20, "intra record is 5 dwords");
-                const MMR_GLOBAL uint8_t* q = reinterpret_cast<const MMR_GLOBAL uint8_t*>(as_global(job.intra + mb));
-                uint8_t b[20];
+        if (threadIdx.x == 0) s_ticket = fetch_add_relaxed(&counters[0], 1);
+        if (threadIdx.x < kIntraRows) prog[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t t = s_ticket;
+        if (t >= total_bands) break;
+        const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.intra_band_begin; })];
+        const uint32_t band = t - job.intra_band_begin;
+        const int mby = (int)band * kIntraRows + w, mw = (int)job.width_mbs;
+        if (mby < (int)job.height_mbs) {
+            const bool first = w == 0, last = w == kIntraRows - 1 || mby == (int)job.height_mbs - 1;
+            uint32_t* gprev = first && band > 0 ? &counters[1 + t - 1] : nullptr;   // previous band's last row
+            uint32_t* gself = last && mby < (int)job.height_mbs - 1 ? &counters[1 + t] : nullptr;
+            uint32_t seen = 0, published = 0, gpublished = 0;
+            bool left_in_lds = false;
+            auto publish_to = [&](uint32_t x, bool force) {
+                if (x > published) {
+                    published = x;
+                    __hip_atomic_store(&prog[w], x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                if (gself && x > gpublished && (force || x - gpublished >= publish)) {
+                    gpublished = x;
+                    __hip_atomic_store(gself, x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            };
+            // Records of 64 macroblocks at a time, lane i holding macroblock chunk + i: its record
+            // (3 dwords), intra record (5 dwords) and the slice of the macroblock above; read out
+            // with readlane.
+            uint32_t rec[3] = {0, 0, 0}, irec[5] = {0, 0, 0, 0, 0}, above_slice = 0;
+            uint32_t left_slice = 0;
+            for (int mbx = 0; mbx < mw; ++mbx) {
+                const uint32_t mb = (uint32_t)(mby * mw + mbx);
+                if ((mbx & 63) == 0) {
+                    const int xl = mbx + l;
+                    if (xl < mw) {
+                        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.mbs + mb + l));
+                        rec[0] = q[0];
+                        rec[1] = q[1];
+                        rec[2] = q[2];
+                        if (rec[0] & MMR_H264_MB_INTRA) {
+                            const MMR_GLOBAL uint32_t* r = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.intra + mb + l));
 #pragma unroll
-                for (int i = 0; i < 20; ++i) b[i] = q[i];
-                __builtin_memcpy(&in, b, sizeof in);
+                            for (int i = 0; i < 5; ++i) irec[i] = r[i];
+                        }
+                        above_slice = mby > 0 ? (uint32_t)job.mbs[mb + l - (uint32_t)mw].slice_id : 0u;
+                    }
+                }
+                const int lane = mbx & 63;
+                uint32_t wd[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) wd[i] = __builtin_amdgcn_readlane(rec[i], lane);
+                mmr_recon_h264_macroblock m;
+                __builtin_memcpy(&m, wd, sizeof m);
+                const uint32_t slice = m.slice_id, prev_slice = left_slice;
+                left_slice = slice;
+                if (!(m.flags & MMR_H264_MB_INTRA)) {
+                    left_in_lds = false;
+                    continue;
+                }
+                uint32_t iw[5];
+#pragma unroll
+                for (int i = 0; i < 5; ++i) iw[i] = __builtin_amdgcn_readlane(irec[i], lane);
+                mmr_recon_h264_intra in;
+                static_assert(sizeof(mmr_recon_h264_intra) == 20, "intra record is 5 dwords");
+                __builtin_memcpy(&in, iw, sizeof in);
+                publish_to((uint32_t)mbx, false);
+                intra_residual(sh, job, m, in, l, status);
+                const uint32_t need = (uint32_t)min(mbx + 2, mw);
+                if (gprev) intra_wait_band(gprev, need, seen, status);
+                else if (!first) intra_wait_lds(&prog[w - 1], need, status);
+                // left_macroblock_available / top_macroblock_available: inside the picture, same slice
+                const bool left_mb = mbx > 0 && prev_slice == slice;
+                const bool top_mb = mby > 0 && __builtin_amdgcn_readlane(above_slice, lane) == slice;
+                intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, l, status);
+                left_in_lds = true;
             }
-            // left_macroblock_available / top_macroblock_available: inside the picture, same slice
-            const bool left_mb = mbx > 0 && job.mbs[mb - 1].slice_id == m.slice_id;
-            const bool top_mb = mby > 0 && job.mbs[mb - (uint32_t)mw].slice_id == m.slice_id;
-            intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, l, status);
-            left_in_lds = true;
+            publish_to((uint32_t)mw, true);
         }
-        __hip_atomic_store(self, (uint32_t)mw, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
     }
 }
 
-hipError_t launch_h264_intra(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters, uint32_t groups,
-                             uint32_t* d_status, hipStream_t s) {
-    if (job_count == 0 || total_rows == 0) return hipSuccess;
-    if (groups > total_rows) groups = total_rows;
+hipError_t launch_h264_intra(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters, uint32_t groups,
+                             uint32_t publish, uint32_t* d_status, hipStream_t s) {
+    if (job_count == 0 || total_bands == 0) return hipSuccess;
+    if (groups > total_bands) groups = total_bands;
     if (groups == 0) groups = 1;
-    hipLaunchKernelGGL(h264_intra_kernel, dim3(groups), dim3(64), 0, s, d_jobs, job_count, total_rows, d_counters, d_status);
+    if (publish == 0) publish = 1;
+    hipLaunchKernelGGL(h264_intra_kernel, dim3(groups), dim3(kIntraRows * 64), 0, s, d_jobs, job_count, total_bands, d_counters,
+                       publish, d_status);
     return hipGetLastError();
 }
+
+uint32_t h264_intra_band_rows() { return kIntraRows; }
 
 }  // namespace mmr

@@ -615,7 +615,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::H264Job> jobs(count);
     std::vector<uint32_t> tiles;
-    uint32_t bands = 0, intra_rows = 0;
+    uint32_t bands = 0, intra_bands = 0;
+    const uint32_t intra_rows_per_band = mmr::h264_intra_band_rows();
     const uint32_t band_rows = ctx->tuning.h264_deblock_band_rows, publish = ctx->tuning.h264_deblock_publish;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_h264_picture& p = pics[i];
@@ -623,6 +624,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         constexpr size_t kH264Base = offsetof(mmr_recon_h264_picture, intra);
         if (p.struct_size < kH264Base) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 struct_size too small");
         const mmr_recon_h264_intra* intra = p.struct_size >= kH264Base + sizeof(p.intra) ? p.intra : nullptr;
+        if (!aligned(intra, 4)) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: intra records must be 4-byte aligned", i);
         if (p.width_mbs == 0 || p.height_mbs == 0 || p.width_mbs > 1024 || p.height_mbs > 1024)
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
         if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
@@ -671,18 +673,19 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         j.band_begin = per_frame ? 0 : bands;
         bands += j.deblock ? (p.height_mbs + band_rows - 1) / band_rows : 0;
         j.intra = intra;
-        j.intra_row_begin = per_frame ? 0 : intra_rows;
-        intra_rows += intra ? p.height_mbs : 0;
+        j.intra_band_begin = per_frame ? 0 : intra_bands;
+        intra_bands += intra ? (p.height_mbs + intra_rows_per_band - 1) / intra_rows_per_band : 0;
     }
     Submission sub(ctx, stream, cap);
-    // Counters: deblocking band tickets and hand-over progress, then intra row tickets and progress
-    // (per picture with one launch per picture).
+    // Counters: deblocking band tickets and hand-over progress, then intra band tickets and
+    // hand-over progress (per picture with one launch per picture).
+    auto pic_intra_bands = [&](uint32_t i) { return (pics[i].height_mbs + intra_rows_per_band - 1) / intra_rows_per_band; };
     size_t counters = 0;
     if (per_frame) {
         for (uint32_t i = 0; i < count; ++i)
-            counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0) + (jobs[i].intra ? 1 + pics[i].height_mbs : 0);
+            counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0) + (jobs[i].intra ? 1 + pic_intra_bands(i) : 0);
     } else {
-        counters = 1 + bands + (intra_rows ? 1 + intra_rows : 0);
+        counters = 1 + bands + (intra_bands ? 1 + intra_bands : 0);
     }
     // Scratch: 16 bytes of boundary strengths per macroblock of every deblocked picture.
     std::vector<size_t> bs_off(count, 0);
@@ -703,7 +706,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     const uint32_t groups = ctx->persistent_groups ? ctx->persistent_groups : ctx->cu_count * ctx->tuning.h264_deblock_groups_per_cu;
-    const uint32_t intra_groups = ctx->cu_count * 8;   // one wave per macroblock row in flight
+    const uint32_t intra_groups = ctx->cu_count;   // one band (8 waves) per CU
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
@@ -711,8 +714,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             if (jobs[i].deblock) e = mmr::launch_h264_bs(dj, dt, jobs[i].tile_begin, mc_tiles, stream);
             if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, mc_tiles, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].intra) {
-                e = mmr::launch_h264_intra(dj + i, 1, pics[i].height_mbs, c, intra_groups, sub.status_ptr(), stream);
-                c += 1 + pics[i].height_mbs;
+                e = mmr::launch_h264_intra(dj + i, 1, pic_intra_bands(i), c, intra_groups, publish, sub.status_ptr(), stream);
+                c += 1 + pic_intra_bands(i);
             }
             if (e == hipSuccess && jobs[i].deblock) {
                 const uint32_t nb = (pics[i].height_mbs + band_rows - 1) / band_rows;
@@ -725,8 +728,9 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     } else {
         if (bands) e = mmr::launch_h264_bs(dj, dt, 0, (uint32_t)tiles.size(), stream);
         if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
-        if (e == hipSuccess && intra_rows)
-            e = mmr::launch_h264_intra(dj, count, intra_rows, sub.counters_ptr() + 1 + bands, intra_groups, sub.status_ptr(), stream);
+        if (e == hipSuccess && intra_bands)
+            e = mmr::launch_h264_intra(dj, count, intra_bands, sub.counters_ptr() + 1 + bands, intra_groups, publish,
+                                       sub.status_ptr(), stream);
         if (e == hipSuccess && bands)
             e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, band_rows, publish, sub.status_ptr(), stream);
     }

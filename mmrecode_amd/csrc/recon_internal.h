@@ -93,7 +93,7 @@ struct H264Job {
     uint32_t pad;
     uint32_t* bs;                 // [macroblocks][4] packed boundary strengths (submission scratch)
     const mmr_recon_h264_intra* intra;   // [macroblocks] or NULL (intra macroblocks unsupported)
-    uint32_t intra_row_begin;     // first intra row ticket of this picture (rows only with intra records)
+    uint32_t intra_band_begin;    // first intra band ticket of this picture (bands only with intra records)
     uint32_t intra_pad;
 };
 
@@ -130,8 +130,9 @@ hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t t
 hipError_t dv_upload_tables(hipStream_t s);
 hipError_t launch_h264_mc(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t* d_status,
                           hipStream_t s);
-hipError_t launch_h264_intra(const H264Job* d_jobs, uint32_t job_count, uint32_t total_rows, uint32_t* d_counters, uint32_t groups,
-                             uint32_t* d_status, hipStream_t s);
+hipError_t launch_h264_intra(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters, uint32_t groups,
+                             uint32_t publish, uint32_t* d_status, hipStream_t s);
+uint32_t h264_intra_band_rows();   // macroblock rows per intra band (one workgroup)
 // Deblocking in bands of `band_rows` (4, 8 or 16) macroblock rows: d_counters[0] is the band
 // ticket, d_counters[1 + t] the progress of band t's last row (zeroed before the launch), published
 // every `publish` macroblocks.

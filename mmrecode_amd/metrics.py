@@ -34,15 +34,18 @@ def dv_frame_bytes(unit):
 
 def h264_picture_bytes(unit):
     """Levels of coded macroblocks (384 x int16), 12 B per macroblock record, 12 B per 4x4 block
-    record, reference reads (1 B per predicted sample per list used), weights, output planes."""
+    record, reference reads of inter macroblocks (1 B per predicted sample per list used), 20 B
+    per intra record, weights, output planes. Intra neighbours are re-reads of output samples."""
     mbs = unit.inputs["mbs"]
     blocks = unit.inputs["blocks"]
     n_mb = mbs.size
+    inter = (mbs["flags"] & 1) == 0
     lists_used = (blocks["ref_idx"][:, :, 0] >= 0).astype(int) + (blocks["ref_idx"][:, :, 1] >= 0).astype(int)
-    ref_bytes = int(lists_used.sum()) * 24          # 16 luma + 2 x 4 chroma samples per 4x4 block per list
+    ref_bytes = int(lists_used[inter].sum()) * 24   # 16 luma + 2 x 4 chroma samples per 4x4 block per list
     out = sum(int(np.prod(s)) for s, _, _ in unit.outputs.values())
+    intra_bytes = int((~inter).sum()) * 20
     return (unit.meta["coded_mbs"] * 768 + n_mb * 12 + n_mb * 16 * 12 + unit.inputs["weights"].nbytes
-            + unit.inputs["bi_weights"].nbytes + ref_bytes + out)
+            + unit.inputs["bi_weights"].nbytes + ref_bytes + intra_bytes + out)
 
 
 def mpix(width, height):
