@@ -50,18 +50,19 @@ struct IntraShared {
     // Right column of the previous macroblock of this row when it was intra.
     uint8_t left_y[16];
     uint8_t left_c[2][8];
+    // Level record (or I_PCM samples) of the current macroblock, copied from global memory.
+    __align__(16) int16_t lvs[MMR_H264_LEVELS_PER_MB];
 };
 
 MMR_DEV uint32_t ip_avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
 MMR_DEV uint32_t ip_f121(uint32_t a, uint32_t b, uint32_t c) { return (a + 2 * b + c + 2) >> 2; }
 MMR_DEV uint32_t ip_avg13(uint32_t a, uint32_t b) { return (a + 3 * b + 2) >> 2; }
 
-// Intra4x4 / Intra8x8 sample (x, y) of an n x n block (n = 4 or 8) from the neighbour lines
-// t[i + 1] = top i (t[0] = top-left), l[i + 1] = left i (l[0] = top-left): the formulas of
-// predict_intra4_block / predict_intra8_block with top_or_corner(i) = t[i + 1].
-MMR_DEV uint32_t pred_nxn(int mode, int n, int x, int y, const uint8_t* t, const uint8_t* l) {
-    auto T = [&](int i) -> uint32_t { return t[i + 1]; };
-    auto L = [&](int i) -> uint32_t { return l[i + 1]; };
+// Intra4x4 / Intra8x8 sample (x, y) of an n x n block (n = 4 or 8) from the neighbour lines:
+// T(i) = top i, L(i) = left i, T(-1) = L(-1) = top-left -- the formulas of predict_intra4_block /
+// predict_intra8_block with top_or_corner(i) = T(i), left_or_corner(i) = L(i).
+template <class TF, class LF>
+MMR_DEV uint32_t pred_nxn(int mode, int n, int x, int y, TF T, LF L) {
     switch (mode) {
     case 0: return T(x);
     case 1: return L(y);
@@ -174,7 +175,7 @@ MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon
         if (l == 0) atomicOr(status, kDevStatusInvalidData);
     }
     const bool has_record = pcm || i16 || m.cbp != 0;
-    const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+    const int16_t* lv = sh.lvs;   // staged by the caller when has_record
     // ---- residuals of every block (luma 4x4 / 8x8 / Intra16x16, chroma), before prediction
     if (has_record && !pcm) {
         uint32_t mx = 0;
@@ -207,7 +208,7 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
     const MMR_GLOBAL uint8_t* pc[2] = {as_global(job.planes[1]), as_global(job.planes[2])};
     const int type = in.type;
     const bool pcm = type == MMR_H264_INTRA_PCM, i16 = type == MMR_H264_INTRA_16X16, i8 = type == MMR_H264_INTRA_8X8;
-    const MMR_GLOBAL int16_t* lv = as_global(job.levels) + (size_t)m.coef_offset * MMR_H264_LEVELS_PER_MB;
+    const int16_t* lv = sh.lvs;   // I_PCM samples, staged by the caller
 
     // ---- context tile
     const int x0 = mbx * 16, y0 = mby * 16, cx0 = mbx * 8, cy0 = mby * 8;
@@ -270,45 +271,38 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
 #pragma unroll
         for (int c = 0; c < 4; ++c) sh.ty[1 + r][1 + c0 + c] = out[c];
     } else if (!i8) {
-        // Intra4x4: 16 blocks in order
-        for (int k = 0; k < 16; ++k) {
-            const int bx = ((k >> 2) & 1) * 2 + (k & 1), by = (k >> 3) * 2 + ((k >> 1) & 1);
-            const int ox = bx * 4, oy = by * 4;
-            const bool ht = by > 0 || top_mb, hl = bx > 0 || left_mb;
-            const int mode = in.modes[k];
-            if (l < 9) {
-                uint32_t v = 0;
-                if (l == 0) {
-                    v = ht && hl ? sh.ty[oy][ox] : 0u;
-                } else if (ht) {
-                    const int xi = l - 1;
-                    // top-right: not for blocks 3 and 11, not into the next macroblock below the
-                    // first block row, not past the picture's right edge
-                    const bool tr_ok = xi < 4 || (k != 3 && k != 11 && !(by > 0 && ox + xi >= 16) && x0 + ox + xi < W);
-                    v = sh.ty[oy][1 + ox + (tr_ok ? xi : 3)];
-                }
-                sh.tn[l] = (uint8_t)v;
-            } else if (l < 14) {
-                const int i = l - 9;
-                sh.ln[i] = (uint8_t)(i == 0 ? (ht && hl ? sh.ty[oy][ox] : 0u) : (hl ? sh.ty[oy + i][ox] : 0u));
-            }
-            wave_lds_sync();
-            const uint32_t needs = nxn_needs(mode);
-            const bool ok = mode <= 8 && (!(needs & 1) || ht) && (!(needs & 2) || hl) && (!(needs & 4) || (ht && hl));
-            if (!ok && l == 0) atomicOr(status, kDevStatusInvalidData);
-            if (l < 16) {
-                const int x = l & 3, y = l >> 2;
+        // Intra4x4 in 10 steps: the blocks with bx + 2 by = t (up to two, lanes 0-15 and 16-31).
+        // Every sample a block reads (left, top, top-left, and the top-right ones the rules below
+        // keep, which all precede it in block order) lies in a block of a smaller t, and the two
+        // blocks of a step do not read each other, so this equals the reference's block order.
+        for (int t = 0; t < 10; ++t) {
+            const int by = (t < 3 ? 0 : (t - 2) >> 1) + (l >> 4), bx = t - 2 * by;
+            const bool active = l < 32 && by <= min(3, t >> 1) && bx >= 0 && bx <= 3;
+            if (active) {
+                const int k = (by >> 1) * 8 + (bx >> 1) * 4 + (by & 1) * 2 + (bx & 1);   // luma_block_index
+                const int ox = bx * 4, oy = by * 4;
+                const bool ht = by > 0 || top_mb, hl = bx > 0 || left_mb;
+                const int mode = in.modes[k];
+                const uint32_t needs = nxn_needs(mode);
+                const bool ok = mode <= 8 && (!(needs & 1) || ht) && (!(needs & 2) || hl) && (!(needs & 4) || (ht && hl));
+                if (!ok && (l & 15) == 0) atomicOr(status, kDevStatusInvalidData);
+                // top-right (top 4..7): not for blocks 3 and 11, not into the next macroblock below
+                // the first block row, not past the picture's right edge; else copies of top 3
+                const bool tr_ok = k != 3 && k != 11 && !(by > 0 && ox + 4 >= 16) && x0 + ox + 4 < W;
+                auto T = [&](int i) -> uint32_t { return sh.ty[oy][1 + ox + (i < 4 || tr_ok ? i : 3)]; };
+                auto L = [&](int i) -> uint32_t { return sh.ty[oy + 1 + i][ox]; };
+                const int x = l & 3, y = (l >> 2) & 3;
                 uint32_t p = 128;
                 if (ok && mode == 2) {
                     uint32_t st = 0, sl = 0;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        st += sh.tn[1 + i];
-                        sl += sh.ln[1 + i];
+                        st += ht ? T(i) : 0u;
+                        sl += hl ? L(i) : 0u;
                     }
                     p = dc_nxn(4, ht, hl, st, sl);
                 } else if (ok) {
-                    p = pred_nxn(mode, 4, x, y, sh.tn, sh.ln);
+                    p = pred_nxn(mode, 4, x, y, T, L);
                 }
                 sh.ty[1 + oy + y][1 + ox + x] = (uint8_t)clip255((int)p + S.res_y[(oy + y) * 16 + ox + x]);
             }
@@ -375,7 +369,8 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
                     }
                     p = dc_nxn(8, ht, hl, st, sl);
                 } else if (ok) {
-                    p = pred_nxn(mode, 8, x, y, sh.ftn, sh.fln);
+                    p = pred_nxn(mode, 8, x, y, [&](int i) -> uint32_t { return sh.ftn[i + 1]; },
+                                 [&](int i) -> uint32_t { return sh.fln[i + 1]; });
                 }
                 sh.ty[1 + oy + y][1 + ox + x] = (uint8_t)clip255((int)p + S.res_y[(oy + y) * 16 + ox + x]);
             }
@@ -486,6 +481,16 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
             // with readlane.
             uint32_t rec[3] = {0, 0, 0}, irec[5] = {0, 0, 0, 0, 0}, above_slice = 0;
             uint32_t left_slice = 0;
+            // Level record of the next intra macroblock of the chunk, loaded while the current one
+            // is reconstructed: lane l holds dwords 3l .. 3l+2 of the 768-byte record.
+            uint32_t pf[3] = {0, 0, 0};
+            int pf_mb = -1;
+            const MMR_GLOBAL uint32_t* levels_dw = reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.levels));
+            auto record_of = [&]() -> bool {   // this lane's macroblock is intra with a level record
+                const uint32_t type = irec[0] & 0xffu;
+                return (rec[0] & MMR_H264_MB_INTRA) &&
+                       (type == MMR_H264_INTRA_PCM || type == MMR_H264_INTRA_16X16 || ((rec[0] >> 16) & 0xffu) != 0);
+            };
             for (int mbx = 0; mbx < mw; ++mbx) {
                 const uint32_t mb = (uint32_t)(mby * mw + mbx);
                 if ((mbx & 63) == 0) {
@@ -522,6 +527,35 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
                 static_assert(sizeof(mmr_recon_h264_intra) == 20, "intra record is 5 dwords");
                 __builtin_memcpy(&in, iw, sizeof in);
                 publish_to((uint32_t)mbx, false);
+                // Stage this macroblock's level record in LDS (prefetched, or loaded now), then start
+                // loading the record of the next intra macroblock of the chunk.
+                const bool has_record = in.type == MMR_H264_INTRA_PCM || in.type == MMR_H264_INTRA_16X16 || m.cbp != 0;
+                if (has_record) {
+                    if (pf_mb != mbx) {
+                        const MMR_GLOBAL uint32_t* q = levels_dw + (size_t)m.coef_offset * (MMR_H264_LEVELS_PER_MB / 2) + 3 * l;
+                        pf[0] = q[0];
+                        pf[1] = q[1];
+                        pf[2] = q[2];
+                    }
+                    uint32_t* dst = reinterpret_cast<uint32_t*>(sh.lvs) + 3 * l;
+                    dst[0] = pf[0];
+                    dst[1] = pf[1];
+                    dst[2] = pf[2];
+                }
+                {
+                    const uint64_t later = __ballot(record_of() && mbx - lane + l < mw) & ~((2ull << lane) - 1ull);
+                    pf_mb = -1;
+                    if (later) {
+                        const int nl = __builtin_ctzll(later);
+                        const uint32_t off = __builtin_amdgcn_readlane(rec[2], nl);
+                        const MMR_GLOBAL uint32_t* q = levels_dw + (size_t)off * (MMR_H264_LEVELS_PER_MB / 2) + 3 * l;
+                        pf[0] = q[0];
+                        pf[1] = q[1];
+                        pf[2] = q[2];
+                        pf_mb = mbx - lane + nl;
+                    }
+                }
+                wave_lds_sync();   // staged record
                 intra_residual(sh, job, m, in, l, status);
                 const uint32_t need = (uint32_t)min(mbx + 2, mw);
                 if (gprev) intra_wait_band(gprev, need, seen, status);
