@@ -474,6 +474,72 @@ MMR_RECON_API mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context *context,
                                                     uint32_t job_count, uint32_t flags,
                                                     uint64_t *out_ticket);
 
+/*
+ * Overlay preparation for the YUV 4:2:0 blend. Replaces PreparedOverlay::prepare_yuv +
+ * rgb_to_yuv (crates/render/src/project_compositor.rs:1098-1185): from a straight-alpha RGBA8
+ * overlay (the cropped region from_canvas keeps, :1046-1096), BT.709 limited and full-range luma
+ * per pixel and, per 2x2 cell (the last row / column of an odd size on its own), the rounded mean
+ * alpha and the alpha-weighted mean U / V of both ranges (128 where the cell's alpha sums to 0).
+ * Outputs: y_* width x height; u_* / v_* / chroma_alpha ceil(width/2) x ceil(height/2), all
+ * tightly packed; any output may be NULL (not written).
+ */
+typedef struct mmr_recon_overlay_prepare_job {
+    size_t struct_size;
+    const uint8_t *rgba;
+    uint32_t width;
+    uint32_t height;
+    uint8_t *y_limited;
+    uint8_t *y_full;
+    uint8_t *u_limited;
+    uint8_t *v_limited;
+    uint8_t *u_full;
+    uint8_t *v_full;
+    uint8_t *chroma_alpha;
+} mmr_recon_overlay_prepare_job;
+
+MMR_RECON_API mmr_status mmr_recon_overlay_prepare_submit(mmr_recon_context *context,
+                                                          const mmr_recon_overlay_prepare_job *jobs,
+                                                          uint32_t job_count, uint32_t flags,
+                                                          uint64_t *out_ticket);
+
+/* Scale filters (FrameScaleFilter): image::imageops::FilterType::Lanczos3 / Triangle. */
+#define MMR_SCALE_FILTER_LANCZOS3 0u
+#define MMR_SCALE_FILTER_TRIANGLE 1u
+
+/* Visual scale modes (VisualScaleMode). */
+#define MMR_SCALE_FIT 0u       /* preserve aspect, pad */
+#define MMR_SCALE_FILL 1u      /* preserve aspect, crop */
+#define MMR_SCALE_STRETCH 2u   /* whole canvas */
+#define MMR_SCALE_NATIVE 3u    /* no resize, centred crop / pad */
+
+/*
+ * A planar 4:2:0 frame mapped onto an even-sized canvas. Replaces scale_yuv420_to_canvas_cpu
+ * (crates/render/src/compositor.rs:89-141) with scaled_dimensions / scaled_even (:211-254) and
+ * center_crop_or_pad (:277-305): each plane is resized with the Lanczos3 filter of
+ * image::imageops::resize (the image crate 0.25.6: a vertical pass into f32, then a horizontal pass
+ * rounding to u8, weights per output coordinate normalised to sum 1) with the Lanczos3 or Triangle
+ * filter (FrameScaleFilter, crates/render/src/frame_graph.rs:219-224) unless its scaled size equals
+ * its size, then centre-cropped / padded (luma 16 for MMR_RANGE_LIMITED else 0, chroma 128).
+ * Width, height and the canvas size are even and non-zero; chroma planes are half size.
+ */
+typedef struct mmr_recon_scale_job {
+    size_t struct_size;
+    const uint8_t *src[3];
+    uint32_t src_stride[3];
+    uint32_t width;
+    uint32_t height;
+    uint8_t *dst[3];
+    uint32_t dst_stride[3];
+    uint32_t canvas_width;
+    uint32_t canvas_height;
+    uint32_t mode;      /* MMR_SCALE_* */
+    uint32_t range;     /* MMR_RANGE_* of the frame (padding black) */
+    uint32_t filter;    /* MMR_SCALE_FILTER_* */
+} mmr_recon_scale_job;
+
+MMR_RECON_API mmr_status mmr_recon_scale_submit(mmr_recon_context *context, const mmr_recon_scale_job *jobs,
+                                                uint32_t job_count, uint32_t flags, uint64_t *out_ticket);
+
 /* Codec selectors of the prepared-batch and serving calls. */
 #define MMR_CODEC_MJPEG 1u
 #define MMR_CODEC_MPEG2 2u

@@ -154,6 +154,12 @@ class Context:
     def submit_yuv_blend(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_yuv_blend_submit, abi.YuvBlendJob, jobs, flags)
 
+    def submit_overlay_prepare(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_overlay_prepare_submit, abi.OverlayPrepareJob, jobs, flags)
+
+    def submit_scale(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_scale_submit, abi.ScaleJob, jobs, flags)
+
     # ---- tuning
     TUNING_FIELDS = [f for f, _ in abi.Tuning._fields_ if f != "struct_size"]
 

@@ -270,6 +270,47 @@ class YuvBlendJob(C.Structure):
     ]
 
 
+class OverlayPrepareJob(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("rgba", vp),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("y_limited", vp),
+        ("y_full", vp),
+        ("u_limited", vp),
+        ("v_limited", vp),
+        ("u_full", vp),
+        ("v_full", vp),
+        ("chroma_alpha", vp),
+    ]
+
+
+SCALE_FILTER_LANCZOS3 = 0
+SCALE_FILTER_TRIANGLE = 1
+SCALE_FIT = 0
+SCALE_FILL = 1
+SCALE_STRETCH = 2
+SCALE_NATIVE = 3
+
+
+class ScaleJob(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("src", vp * 3),
+        ("src_stride", C.c_uint32 * 3),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("dst", vp * 3),
+        ("dst_stride", C.c_uint32 * 3),
+        ("canvas_width", C.c_uint32),
+        ("canvas_height", C.c_uint32),
+        ("mode", C.c_uint32),
+        ("range", C.c_uint32),
+        ("filter", C.c_uint32),
+    ]
+
+
 CODEC_MJPEG = 1
 CODEC_MPEG2 = 2
 CODEC_DV = 3
@@ -381,6 +422,10 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(CompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_yuv_blend_submit", C.c_int32,
      [C.c_void_p, C.POINTER(YuvBlendJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_overlay_prepare_submit", C.c_int32,
+     [C.c_void_p, C.POINTER(OverlayPrepareJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_scale_submit", C.c_int32,
+     [C.c_void_p, C.POINTER(ScaleJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_server_create", C.c_int32, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
     ("mmr_recon_server_enqueue", C.c_int32, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]),
     ("mmr_recon_server_poll", C.c_int32,

@@ -175,6 +175,98 @@ __global__ __launch_bounds__(256) void yuv_blend_kernel(const mmr_recon_yuv_blen
     }
 }
 
+// rgb_to_yuv (crates/render/src/project_compositor.rs:1163-1185): BT.709 limited (ly, lu, lv) and
+// full range (fy, fu, fv).
+MMR_DEV void rgb_to_yuv(int r, int g, int b, uint32_t lim[3], uint32_t full[3]) {
+    lim[0] = (uint32_t)clampi(16 + ((47 * r + 157 * g + 16 * b + 128) >> 8), 16, 235);
+    lim[1] = (uint32_t)clampi(128 + ((-26 * r - 87 * g + 112 * b + 128) >> 8), 16, 240);
+    lim[2] = (uint32_t)clampi(128 + ((112 * r - 102 * g - 10 * b + 128) >> 8), 16, 240);
+    full[0] = (uint32_t)clampi((54 * r + 183 * g + 18 * b + 128) >> 8, 0, 255);
+    full[1] = (uint32_t)clampi(128 + ((-29 * r - 99 * g + 128 * b + 128) >> 8), 0, 255);
+    full[2] = (uint32_t)clampi(128 + ((128 * r - 116 * g - 12 * b + 128) >> 8), 0, 255);
+}
+
+// PreparedOverlay::prepare_yuv (project_compositor.rs:1098-1142): one thread per 2x2 cell (the
+// last row / column of an odd size alone): luma of its pixels, then the rounded mean alpha and the
+// alpha-weighted mean chroma of both ranges (128 for a transparent cell).
+__global__ __launch_bounds__(256) void overlay_prepare_kernel(const mmr_recon_overlay_prepare_job* __restrict__ jobs) {
+    const mmr_recon_overlay_prepare_job& j = jobs[blockIdx.y];
+    const uint32_t cw = (j.width + 1) / 2, ch = (j.height + 1) / 2;
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= cw * ch) return;
+    const uint32_t cy = c / cw, cx = c - cy * cw, x0 = cx * 2, y0 = cy * 2;
+    const uint32_t nw = min(2u, j.width - x0), nh = min(2u, j.height - y0);
+    uint32_t alpha_sum = 0, lu = 0, lv = 0, fu = 0, fv = 0;
+    for (uint32_t dy = 0; dy < nh; ++dy)
+        for (uint32_t dx = 0; dx < nw; ++dx) {
+            const size_t p = (size_t)(y0 + dy) * j.width + x0 + dx;
+            const uint32_t px = *reinterpret_cast<const uint32_t*>(j.rgba + p * 4);   // rgba is 4-byte aligned
+            uint32_t lim[3], full[3];
+            rgb_to_yuv((int)(px & 0xff), (int)((px >> 8) & 0xff), (int)((px >> 16) & 0xff), lim, full);
+            const uint32_t a = px >> 24;
+            if (j.y_limited) j.y_limited[p] = (uint8_t)lim[0];
+            if (j.y_full) j.y_full[p] = (uint8_t)full[0];
+            alpha_sum += a;
+            lu += lim[1] * a;
+            lv += lim[2] * a;
+            fu += full[1] * a;
+            fv += full[2] * a;
+        }
+    const uint32_t n = nw * nh;
+    auto weighted = [&](uint32_t sum) -> uint8_t { return (uint8_t)(alpha_sum ? (sum + alpha_sum / 2) / alpha_sum : 128u); };
+    if (j.chroma_alpha) j.chroma_alpha[c] = (uint8_t)((alpha_sum + n / 2) / n);
+    if (j.u_limited) j.u_limited[c] = weighted(lu);
+    if (j.v_limited) j.v_limited[c] = weighted(lv);
+    if (j.u_full) j.u_full[c] = weighted(fu);
+    if (j.v_full) j.v_full[c] = weighted(fv);
+}
+
+// Lanczos3 resize (image::imageops::resize of the image crate 0.25.6, as scale_yuv420_to_canvas_cpu
+// calls it, crates/render/src/compositor.rs:89-141), vertical pass: f32 sums, in tap order, of the
+// source samples times the row's normalised weights (taps from the host), for the scaled rows the
+// canvas window keeps. Thread (x, r) of plane blockIdx.y.
+__global__ __launch_bounds__(256) void scale_vertical_kernel(const ScalePlaneDev* __restrict__ planes) {
+    const ScalePlaneDev& p = planes[blockIdx.y];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (!p.resize || i >= p.src_w * p.copy_h) return;
+    const uint32_t r = i / p.src_w, x = i - r * p.src_w;
+    const int32_t left = p.vleft[r];
+    const uint32_t count = p.vcount[r];
+    const float* w = p.vw + (size_t)r * p.vmax;
+    const uint8_t* src = p.src + (size_t)left * p.src_stride + x;
+    float t = 0.0f;
+    for (uint32_t k = 0; k < count; ++k) t += (float)src[(size_t)k * p.src_stride] * w[k];
+    p.tmp[i] = t;
+}
+
+// Horizontal pass and placement (center_crop_or_pad, compositor.rs:277-305): thread (x, y) of the
+// canvas plane writes the background outside the window; inside, the horizontal f32 sum of the
+// intermediate row clamped to [0, 255] and rounded half away from zero, or the source sample when
+// the plane keeps its size.
+__global__ __launch_bounds__(256) void scale_output_kernel(const ScalePlaneDev* __restrict__ planes) {
+    const ScalePlaneDev& p = planes[blockIdx.y];
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.dst_w * p.dst_h) return;
+    const uint32_t y = i / p.dst_w, x = i - y * p.dst_w;
+    uint32_t v = p.background;
+    if (x >= p.dst_x && x < p.dst_x + p.copy_w && y >= p.dst_y && y < p.dst_y + p.copy_h) {
+        const uint32_t c = x - p.dst_x, r = y - p.dst_y;
+        if (p.resize) {
+            const int32_t left = p.hleft[c];
+            const uint32_t count = p.hcount[c];
+            const float* w = p.hw + (size_t)c * p.hmax;
+            const float* row = p.tmp + (size_t)r * p.src_w + left;
+            float t = 0.0f;
+            for (uint32_t k = 0; k < count; ++k) t += row[k] * w[k];
+            t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+            v = (uint32_t)roundf(t);
+        } else {
+            v = p.src[(size_t)(p.sy + r) * p.src_stride + p.sx + c];
+        }
+    }
+    p.dst[(size_t)y * p.dst_stride + x] = (uint8_t)v;
+}
+
 __global__ void fill_zero_kernel(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
@@ -196,6 +288,19 @@ hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, u
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s) {
     if (max_px == 0 || job_count == 0) return hipSuccess;
     hipLaunchKernelGGL(yuv_blend_kernel, dim3((max_px + 255) / 256, job_count), dim3(256), 0, s, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_overlay_prepare(const mmr_recon_overlay_prepare_job* d_jobs, uint32_t job_count, uint32_t max_cells, hipStream_t s) {
+    if (max_cells == 0 || job_count == 0) return hipSuccess;
+    hipLaunchKernelGGL(overlay_prepare_kernel, dim3((max_cells + 255) / 256, job_count), dim3(256), 0, s, d_jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_vertical, uint32_t max_output, hipStream_t s) {
+    if (plane_count == 0) return hipSuccess;
+    if (max_vertical) hipLaunchKernelGGL(scale_vertical_kernel, dim3((max_vertical + 255) / 256, plane_count), dim3(256), 0, s, d_planes);
+    if (max_output) hipLaunchKernelGGL(scale_output_kernel, dim3((max_output + 255) / 256, plane_count), dim3(256), 0, s, d_planes);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdarg>
 #include <cstddef>
@@ -857,6 +858,202 @@ mmr_status yuv_blend_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
     return sub.end(out_ticket);
 }
 
+mmr_status overlay_prepare_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_overlay_prepare_job* in, uint32_t count,
+                                     uint32_t flags, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    uint32_t max_cells = 0;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_overlay_prepare_job& j = in[i];
+        if (j.struct_size < sizeof(mmr_recon_overlay_prepare_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "overlay struct_size too small");
+        if (!j.rgba || j.width == 0 || j.height == 0 || j.width > 16384 || j.height > 16384)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "overlay job %u: no pixels or size out of range", i);
+        if (!aligned(j.rgba, 4)) return fail(MMR_STATUS_INVALID_ARGUMENT, "overlay job %u: rgba must be 4-byte aligned", i);
+        max_cells = std::max(max_cells, ((j.width + 1) / 2) * ((j.height + 1) / 2));
+    }
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(in, sizeof(mmr_recon_overlay_prepare_job) * count, 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr_recon_overlay_prepare_job*>(sub.jobs_ptr());
+    hipError_t e = hipSuccess;
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_overlay_prepare(dj + i, 1, ((in[i].width + 1) / 2) * ((in[i].height + 1) / 2), stream);
+    } else {
+        e = mmr::launch_overlay_prepare(dj, count, max_cells, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "overlay prepare launch");
+    return sub.end(out_ticket);
+}
+
+// Taps of output coordinate o for a pass from n to m samples: image::imageops::resize (the image
+// crate 0.25.6, called by crates/render/src/compositor.rs:117-122) -- centre (o + 0.5) n/m, support
+// s max(n/m, 1) with s = 3 for Lanczos3 and 1 for Triangle, taps clamped to the source, weights
+// kernel((i - centre + 0.5) / max(n/m, 1)) normalised by their f32 sum, all in f32 in this order.
+static float lanczos_sinc(float t) {
+    const float a = t * 3.14159265358979323846f;
+    return t == 0.0f ? 1.0f : sinf(a) / a;
+}
+static float lanczos3(float x) { return fabsf(x) < 3.0f ? lanczos_sinc(x) * lanczos_sinc(x / 3.0f) : 0.0f; }
+static float triangle(float x) { return fabsf(x) < 1.0f ? 1.0f - fabsf(x) : 0.0f; }
+static int32_t scale_taps(uint32_t filter, uint32_t n, uint32_t m, uint32_t o, std::vector<float>& w) {
+    const float ratio = (float)n / (float)m;
+    const float sratio = ratio < 1.0f ? 1.0f : ratio;
+    const float support = (filter == MMR_SCALE_FILTER_TRIANGLE ? 1.0f : 3.0f) * sratio;
+    float c = ((float)o + 0.5f) * ratio;
+    const int64_t left = std::clamp<int64_t>((int64_t)floorf(c - support), 0, (int64_t)n - 1);
+    const int64_t right = std::clamp<int64_t>((int64_t)ceilf(c + support), left + 1, (int64_t)n);
+    c = c - 0.5f;
+    w.clear();
+    float sum = 0.0f;
+    for (int64_t i = left; i < right; ++i) {
+        const float x = ((float)i - c) / sratio;
+        w.push_back(filter == MMR_SCALE_FILTER_TRIANGLE ? triangle(x) : lanczos3(x));
+        sum += w.back();
+    }
+    for (float& v : w) v /= sum;
+    return (int32_t)left;
+}
+
+// scaled_dimensions / scaled_even (crates/render/src/compositor.rs:211-254).
+static void scaled_dimensions(uint32_t sw, uint32_t sh, uint32_t cw, uint32_t ch, uint32_t mode, uint32_t& ow, uint32_t& oh) {
+    auto even = [](uint64_t source, uint64_t target, uint64_t den) -> uint32_t {
+        const uint64_t units = (source * target + den) / (2 * den);
+        return (uint32_t)std::max<uint64_t>(units * 2, 2);
+    };
+    if (mode == MMR_SCALE_STRETCH) {
+        ow = cw, oh = ch;
+    } else if (mode == MMR_SCALE_NATIVE) {
+        ow = sw, oh = sh;
+    } else {
+        const bool width_limited = (uint64_t)cw * sh <= (uint64_t)ch * sw;
+        if (mode == MMR_SCALE_FILL ? !width_limited : width_limited) ow = cw, oh = even(sh, cw, sw);
+        else ow = even(sw, ch, sh), oh = ch;
+    }
+}
+
+mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_scale_job* in, uint32_t count, uint32_t flags,
+                           uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    struct PlaneHost {
+        mmr::ScalePlaneDev d;
+        size_t vleft, vcount, vw, hleft, hcount, hw, tmp;   // offsets: blob (taps) / scratch (tmp)
+    };
+    std::vector<PlaneHost> planes;
+    std::vector<char> taps;
+    size_t scratch = 0;
+    auto put = [&taps](const void* data, size_t bytes) {
+        const size_t off = (taps.size() + 15) & ~size_t(15);
+        taps.resize(off + bytes);
+        if (bytes) memcpy(taps.data() + off, data, bytes);
+        return off;
+    };
+    std::vector<float> w;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_scale_job& j = in[i];
+        if (j.struct_size < sizeof(mmr_recon_scale_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "scale struct_size too small");
+        if (j.width == 0 || j.height == 0 || j.canvas_width == 0 || j.canvas_height == 0 || ((j.width | j.height | j.canvas_width | j.canvas_height) & 1) ||
+            j.width > 16384 || j.height > 16384 || j.canvas_width > 16384 || j.canvas_height > 16384)
+            return fail(MMR_STATUS_INVALID_DATA, "scale job %u: frame and canvas sizes must be even, non-zero and at most 16384", i);
+        if (j.mode > MMR_SCALE_NATIVE) return fail(MMR_STATUS_INVALID_ARGUMENT, "scale job %u: mode %u is invalid", i, j.mode);
+        if (j.filter > MMR_SCALE_FILTER_TRIANGLE) return fail(MMR_STATUS_INVALID_ARGUMENT, "scale job %u: filter %u is invalid", i, j.filter);
+        uint32_t sw, sh;
+        scaled_dimensions(j.width, j.height, j.canvas_width, j.canvas_height, j.mode, sw, sh);
+        if (sw > 65536 || sh > 65536) return fail(MMR_STATUS_INVALID_DATA, "scale job %u: scaled size out of range", i);
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t div = k ? 2 : 1;
+            const uint32_t pw = j.width / div, ph = j.height / div, nw = sw / div, nh = sh / div;
+            const uint32_t W = j.canvas_width / div, H = j.canvas_height / div;
+            if (!j.src[k] || !j.dst[k] || j.src_stride[k] < pw || j.dst_stride[k] < W)
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "scale job %u plane %d: NULL plane or stride below its width", i, k);
+            PlaneHost plane{};
+            mmr::ScalePlaneDev& d = plane.d;
+            d.src = j.src[k];
+            d.dst = j.dst[k];
+            d.src_w = pw, d.src_h = ph, d.src_stride = j.src_stride[k];
+            d.dst_w = W, d.dst_h = H, d.dst_stride = j.dst_stride[k];
+            d.copy_w = std::min(nw, W), d.copy_h = std::min(nh, H);
+            d.sx = (nw - d.copy_w) / 2, d.sy = (nh - d.copy_h) / 2;
+            d.dst_x = (W - d.copy_w) / 2, d.dst_y = (H - d.copy_h) / 2;
+            d.background = k ? 128u : (j.range == MMR_RANGE_LIMITED ? 16u : 0u);
+            d.resize = !(nw == pw && nh == ph);
+            if (d.resize) {
+                std::vector<int32_t> vl(d.copy_h), hl(d.copy_w);
+                std::vector<uint32_t> vc(d.copy_h), hc(d.copy_w);
+                std::vector<std::vector<float>> vws(d.copy_h), hws(d.copy_w);
+                for (uint32_t r = 0; r < d.copy_h; ++r) {
+                    vl[r] = scale_taps(j.filter, ph, nh, d.sy + r, w);
+                    vws[r] = w;
+                    vc[r] = (uint32_t)w.size();
+                    d.vmax = std::max(d.vmax, vc[r]);
+                }
+                for (uint32_t c = 0; c < d.copy_w; ++c) {
+                    hl[c] = scale_taps(j.filter, pw, nw, d.sx + c, w);
+                    hws[c] = w;
+                    hc[c] = (uint32_t)w.size();
+                    d.hmax = std::max(d.hmax, hc[c]);
+                }
+                std::vector<float> vflat((size_t)d.copy_h * d.vmax, 0.0f), hflat((size_t)d.copy_w * d.hmax, 0.0f);
+                for (uint32_t r = 0; r < d.copy_h; ++r) std::copy(vws[r].begin(), vws[r].end(), vflat.begin() + (size_t)r * d.vmax);
+                for (uint32_t c = 0; c < d.copy_w; ++c) std::copy(hws[c].begin(), hws[c].end(), hflat.begin() + (size_t)c * d.hmax);
+                plane.vleft = put(vl.data(), vl.size() * 4);
+                plane.vcount = put(vc.data(), vc.size() * 4);
+                plane.vw = put(vflat.data(), vflat.size() * 4);
+                plane.hleft = put(hl.data(), hl.size() * 4);
+                plane.hcount = put(hc.data(), hc.size() * 4);
+                plane.hw = put(hflat.data(), hflat.size() * 4);
+                plane.tmp = scratch;
+                scratch += (((size_t)pw * d.copy_h * 4) + 255) & ~size_t(255);
+            }
+            planes.push_back(plane);
+        }
+    }
+    std::vector<char> blob(planes.size() * sizeof(mmr::ScalePlaneDev));
+    const size_t taps_off = (blob.size() + 255) & ~size_t(255);
+    blob.resize(taps_off + taps.size());
+    if (!taps.empty()) memcpy(blob.data() + taps_off, taps.data(), taps.size());
+    for (size_t p = 0; p < planes.size(); ++p) memcpy(blob.data() + p * sizeof(mmr::ScalePlaneDev), &planes[p].d, sizeof(mmr::ScalePlaneDev));
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
+        char* base = static_cast<char*>(sub.jobs_ptr()) + taps_off;
+        auto* hp = reinterpret_cast<mmr::ScalePlaneDev*>(host);
+        for (size_t p = 0; p < planes.size(); ++p) {
+            if (!planes[p].d.resize) continue;
+            hp[p].vleft = reinterpret_cast<const int32_t*>(base + planes[p].vleft);
+            hp[p].vcount = reinterpret_cast<const uint32_t*>(base + planes[p].vcount);
+            hp[p].vw = reinterpret_cast<const float*>(base + planes[p].vw);
+            hp[p].hleft = reinterpret_cast<const int32_t*>(base + planes[p].hleft);
+            hp[p].hcount = reinterpret_cast<const uint32_t*>(base + planes[p].hcount);
+            hp[p].hw = reinterpret_cast<const float*>(base + planes[p].hw);
+            hp[p].tmp = reinterpret_cast<float*>(d_scratch + planes[p].tmp);
+        }
+    });
+    if (s) return s;
+    auto* dp = reinterpret_cast<const mmr::ScalePlaneDev*>(sub.jobs_ptr());
+    auto sizes = [&](size_t from, size_t to, uint32_t& mv, uint32_t& mo) {
+        mv = mo = 0;
+        for (size_t p = from; p < to; ++p) {
+            const mmr::ScalePlaneDev& d = planes[p].d;
+            if (d.resize) mv = std::max(mv, d.src_w * d.copy_h);
+            mo = std::max(mo, d.dst_w * d.dst_h);
+        }
+    };
+    hipError_t e = hipSuccess;
+    uint32_t mv = 0, mo = 0;
+    if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
+            sizes(3 * i, 3 * i + 3, mv, mo);
+            e = mmr::launch_scale(dp + 3 * i, 3, mv, mo, stream);
+        }
+    } else {
+        sizes(0, planes.size(), mv, mo);
+        e = mmr::launch_scale(dp, (uint32_t)planes.size(), mv, mo, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "scale launch");
+    return sub.end(out_ticket);
+}
+
 }  // namespace mmr_host
 
 extern "C" {
@@ -1094,6 +1291,19 @@ mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yu
     CHECK_CTX(ctx);
     LOCK_AND_SET_DEVICE(ctx);
     return mmr_host::yuv_blend_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+}
+
+mmr_status mmr_recon_overlay_prepare_submit(mmr_recon_context* ctx, const mmr_recon_overlay_prepare_job* jobs, uint32_t count, uint32_t flags,
+                                            uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    LOCK_AND_SET_DEVICE(ctx);
+    return mmr_host::overlay_prepare_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
+}
+
+mmr_status mmr_recon_scale_submit(mmr_recon_context* ctx, const mmr_recon_scale_job* jobs, uint32_t count, uint32_t flags, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    LOCK_AND_SET_DEVICE(ctx);
+    return mmr_host::scale_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
 }
 
 }  // extern "C"

@@ -144,6 +144,30 @@ hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
+
+// One plane of a scale job on the device: the canvas window (copy_w x copy_h at dst_x, dst_y)
+// shows the scaled plane's window at sx, sy; with resize, scaled rows / columns of the window have
+// their taps (first source index, count, weights at stride vmax / hmax) and the vertical pass's
+// f32 rows (src_w wide, one per window row) go to tmp; without resize sx, sy index the source.
+struct ScalePlaneDev {
+    const uint8_t* src;
+    uint8_t* dst;
+    float* tmp;
+    const int32_t* vleft;
+    const uint32_t* vcount;
+    const float* vw;
+    const int32_t* hleft;
+    const uint32_t* hcount;
+    const float* hw;
+    uint32_t src_w, src_h, src_stride;
+    uint32_t dst_w, dst_h, dst_stride;
+    uint32_t copy_w, copy_h, dst_x, dst_y;
+    uint32_t sx, sy;
+    uint32_t vmax, hmax;
+    uint32_t resize, background;
+};
+hipError_t launch_overlay_prepare(const mmr_recon_overlay_prepare_job* d_jobs, uint32_t job_count, uint32_t max_cells, hipStream_t s);
+hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_vertical, uint32_t max_output, hipStream_t s);
 hipError_t launch_fill_zero(uint32_t* d_words, uint32_t count, hipStream_t s);
 
 // Bits of the per-submission device status word.

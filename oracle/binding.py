@@ -68,6 +68,12 @@ def lib():
             "oracle_blend_channel": (u8, [u8, u8, u8]),
             "oracle_composite": (i32, [C.POINTER(abi.CompositeJob)]),
             "oracle_yuv_blend": (i32, [C.POINTER(abi.YuvBlendJob)]),
+            "oracle_rgb_to_yuv": (None, [C.c_uint8, C.c_uint8, C.c_uint8, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
+            "oracle_overlay_prepare": (i32, [C.POINTER(abi.OverlayPrepareJob)]),
+            "oracle_scale_taps": (i32, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32),
+                                        C.POINTER(C.c_float), C.c_int]),
+            "oracle_scaled_dimensions": (None, [C.c_uint32] * 5 + [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+            "oracle_scale_yuv420": (i32, [C.POINTER(abi.ScaleJob)]),
             "oracle_jpeg_entropy": (i32, [dp, C.c_size_t, C.POINTER(JpegLevels), C.c_char_p, C.c_size_t]),
             "oracle_jpeg_free": (None, [C.POINTER(JpegLevels)]),
             "oracle_jpeg_huffman_symbols": (i32, [dp, dp, i32, dp, C.c_size_t, dp, i32]),

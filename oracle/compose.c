@@ -1,5 +1,8 @@
 /* Oracle: colour conversion and blending restated from crates/cli, crates/render, crates/mmfx.
  * Test infrastructure only. */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "oracle.h"
@@ -141,5 +144,194 @@ int oracle_yuv_blend(const mmr_recon_yuv_blend_job *j) {
             *u = oracle_blend_channel(*u, j->overlay[1][s], a);
             *v = oracle_blend_channel(*v, j->overlay[2][s], a);
         }
+    return 0;
+}
+
+/* rgb_to_yuv, crates/render/src/project_compositor.rs:1163-1185: BT.709 limited and full range. */
+void oracle_rgb_to_yuv(uint8_t red, uint8_t green, uint8_t blue, uint8_t limited[3], uint8_t full[3]) {
+    const int r = red, g = green, b = blue;
+    int ly = 16 + ((47 * r + 157 * g + 16 * b + 128) >> 8);
+    int lu = 128 + ((-26 * r - 87 * g + 112 * b + 128) >> 8);
+    int lv = 128 + ((112 * r - 102 * g - 10 * b + 128) >> 8);
+    int fy = (54 * r + 183 * g + 18 * b + 128) >> 8;
+    int fu = 128 + ((-29 * r - 99 * g + 128 * b + 128) >> 8);
+    int fv = 128 + ((128 * r - 116 * g - 12 * b + 128) >> 8);
+    limited[0] = (uint8_t)(ly < 16 ? 16 : (ly > 235 ? 235 : ly));
+    limited[1] = (uint8_t)(lu < 16 ? 16 : (lu > 240 ? 240 : lu));
+    limited[2] = (uint8_t)(lv < 16 ? 16 : (lv > 240 ? 240 : lv));
+    full[0] = clamp_u8(fy);
+    full[1] = clamp_u8(fu);
+    full[2] = clamp_u8(fv);
+}
+
+/* PreparedOverlay::prepare_yuv, project_compositor.rs:1098-1142. */
+int oracle_overlay_prepare(const mmr_recon_overlay_prepare_job *j) {
+    const uint32_t w = j->width, h = j->height, cw = (w + 1) / 2;
+    for (uint32_t i = 0; i < w * h; ++i) {
+        uint8_t lim[3], full[3];
+        oracle_rgb_to_yuv(j->rgba[4 * i], j->rgba[4 * i + 1], j->rgba[4 * i + 2], lim, full);
+        if (j->y_limited) j->y_limited[i] = lim[0];
+        if (j->y_full) j->y_full[i] = full[0];
+    }
+    for (uint32_t y = 0; y < h; y += 2)
+        for (uint32_t x = 0; x < w; x += 2) {
+            uint32_t alpha_sum = 0, lu = 0, lv = 0, fu = 0, fv = 0;
+            const uint32_t nh = h - y < 2 ? h - y : 2, nw = w - x < 2 ? w - x : 2;
+            for (uint32_t dy = 0; dy < nh; ++dy)
+                for (uint32_t dx = 0; dx < nw; ++dx) {
+                    const uint8_t *p = j->rgba + ((size_t)(y + dy) * w + x + dx) * 4;
+                    uint8_t lim[3], full[3];
+                    oracle_rgb_to_yuv(p[0], p[1], p[2], lim, full);
+                    alpha_sum += p[3];
+                    lu += (uint32_t)lim[1] * p[3];
+                    lv += (uint32_t)lim[2] * p[3];
+                    fu += (uint32_t)full[1] * p[3];
+                    fv += (uint32_t)full[2] * p[3];
+                }
+            const uint32_t n = nh * nw, c = (y / 2) * cw + x / 2;
+#define WEIGHTED(sum) (uint8_t)(alpha_sum ? ((sum) + alpha_sum / 2) / alpha_sum : 128)
+            if (j->chroma_alpha) j->chroma_alpha[c] = (uint8_t)((alpha_sum + n / 2) / n);
+            if (j->u_limited) j->u_limited[c] = WEIGHTED(lu);
+            if (j->v_limited) j->v_limited[c] = WEIGHTED(lv);
+            if (j->u_full) j->u_full[c] = WEIGHTED(fu);
+            if (j->v_full) j->v_full[c] = WEIGHTED(fv);
+#undef WEIGHTED
+        }
+    return 0;
+}
+
+/* ---------------------------------------------------------------- Lanczos3 scaling
+ * image::imageops::resize with FilterType::Lanczos3 or ::Triangle (the image crate 0.25.6, a
+ * dependency of the reference that it does not vendor), as called by scale_yuv420_to_canvas_cpu
+ * (crates/render/src/compositor.rs:89-141): a vertical pass into f32, then a horizontal pass; for
+ * output coordinate o of a pass from n to m samples, ratio = n / m, sratio = max(ratio, 1),
+ * support = 3 sratio (1 sratio for Triangle), centre c = (o + 0.5) ratio, taps left = clamp(floor(c - support), 0, n - 1)
+ * .. right = clamp(ceil(c + support), left + 1, n), weight of tap i = lanczos3((i - (c - 0.5)) /
+ * sratio) divided by the sum of the weights, lanczos3(x) = sinc(x) sinc(x / 3) for |x| < 3,
+ * sinc(t) = sin(pi t) / (pi t), triangle(x) = 1 - |x| for |x| < 1; accumulation in f32 in tap order; the last pass rounds
+ * (half away from zero) the value clamped to [0, 255]. All f32, no fused multiply-add. */
+static float lz_sinc(float t) {
+    const float a = t * 3.14159265358979323846f;
+    return t == 0.0f ? 1.0f : sinf(a) / a;
+}
+static float lz_kernel(float x) { return fabsf(x) < 3.0f ? lz_sinc(x) * lz_sinc(x / 3.0f) : 0.0f; }
+/* FilterType::Triangle: 1 - |x| for |x| < 1, support 1. */
+static float tri_kernel(float x) { return fabsf(x) < 1.0f ? 1.0f - fabsf(x) : 0.0f; }
+
+/* Taps of output o of filter f (MMR_SCALE_FILTER_*): *left and the count; weights into w
+ * (capacity cap). Returns the count, -1 when it exceeds cap. */
+int oracle_scale_taps(uint32_t filter, uint32_t n, uint32_t m, uint32_t o, int32_t *left, float *w, int cap) {
+    const float ratio = (float)n / (float)m;
+    const float sratio = ratio < 1.0f ? 1.0f : ratio;
+    const float support = (filter == MMR_SCALE_FILTER_TRIANGLE ? 1.0f : 3.0f) * sratio;
+    float c = ((float)o + 0.5f) * ratio;
+    int64_t l = (int64_t)floorf(c - support);
+    if (l < 0) l = 0;
+    if (l > (int64_t)n - 1) l = (int64_t)n - 1;
+    int64_t r = (int64_t)ceilf(c + support);
+    if (r < l + 1) r = l + 1;
+    if (r > (int64_t)n) r = (int64_t)n;
+    c = c - 0.5f;
+    const int count = (int)(r - l);
+    if (count > cap) return -1;
+    float sum = 0.0f;
+    for (int i = 0; i < count; ++i) {
+        const float x = ((float)(l + i) - c) / sratio;
+        w[i] = filter == MMR_SCALE_FILTER_TRIANGLE ? tri_kernel(x) : lz_kernel(x);
+        sum += w[i];
+    }
+    for (int i = 0; i < count; ++i) w[i] /= sum;
+    *left = (int32_t)l;
+    return count;
+}
+
+static float *lz_resize_vertical(uint32_t filter, const uint8_t *src, uint32_t stride, uint32_t w, uint32_t h, uint32_t nh) {
+    float *out = (float *)malloc((size_t)w * nh * sizeof(float));
+    float wt[256];
+    if (!out) return NULL;
+    for (uint32_t oy = 0; oy < nh; ++oy) {
+        int32_t left;
+        const int count = oracle_scale_taps(filter, h, nh, oy, &left, wt, 256);
+        if (count < 0) {
+            free(out);
+            return NULL;
+        }
+        for (uint32_t x = 0; x < w; ++x) {
+            float t = 0.0f;
+            for (int i = 0; i < count; ++i) t += (float)src[(size_t)(left + i) * stride + x] * wt[i];
+            out[(size_t)oy * w + x] = t;
+        }
+    }
+    return out;
+}
+
+/* scaled_even / scaled_dimensions, compositor.rs:211-254. */
+static uint64_t lz_scaled_even(uint64_t source, uint64_t target, uint64_t denominator) {
+    uint64_t units = (source * target + denominator) / (2 * denominator);
+    return units * 2 < 2 ? 2 : units * 2;
+}
+void oracle_scaled_dimensions(uint32_t sw, uint32_t sh, uint32_t cw, uint32_t ch, uint32_t mode, uint32_t *ow, uint32_t *oh) {
+    if (mode == MMR_SCALE_STRETCH) {
+        *ow = cw;
+        *oh = ch;
+        return;
+    }
+    if (mode == MMR_SCALE_NATIVE) {
+        *ow = sw;
+        *oh = sh;
+        return;
+    }
+    const int width_limited = (uint64_t)cw * sh <= (uint64_t)ch * sw;
+    const int use_width = mode == MMR_SCALE_FILL ? !width_limited : width_limited;
+    if (use_width) {
+        *ow = cw;
+        *oh = (uint32_t)lz_scaled_even(sh, cw, sw);
+    } else {
+        *ow = (uint32_t)lz_scaled_even(sw, ch, sh);
+        *oh = ch;
+    }
+}
+
+/* scale_yuv420_to_canvas_cpu + center_crop_or_pad, compositor.rs:89-141, :277-305. */
+int oracle_scale_yuv420(const mmr_recon_scale_job *j) {
+    uint32_t sw, sh;
+    oracle_scaled_dimensions(j->width, j->height, j->canvas_width, j->canvas_height, j->mode, &sw, &sh);
+    const uint8_t luma_black = j->range == MMR_RANGE_LIMITED ? 16 : 0;
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t div = k ? 2 : 1;
+        const uint32_t pw = j->width / div, ph = j->height / div, nw = sw / div, nh = sh / div;
+        const uint32_t W = j->canvas_width / div, H = j->canvas_height / div;
+        const uint8_t bg = k ? 128 : luma_black;
+        /* the scaled plane, as u8 */
+        uint8_t *scaled = (uint8_t *)malloc((size_t)nw * nh);
+        if (!scaled) return -1;
+        if (nw == pw && nh == ph) {
+            for (uint32_t y = 0; y < ph; ++y) memcpy(scaled + (size_t)y * pw, j->src[k] + (size_t)y * j->src_stride[k], pw);
+        } else {
+            float *tmp = lz_resize_vertical(j->filter, j->src[k], j->src_stride[k], pw, ph, nh);
+            float wt[256];
+            if (!tmp) {
+                free(scaled);
+                return -1;
+            }
+            for (uint32_t ox = 0; ox < nw; ++ox) {
+                int32_t left;
+                const int count = oracle_scale_taps(j->filter, pw, nw, ox, &left, wt, 256);
+                for (uint32_t y = 0; y < nh; ++y) {
+                    float t = 0.0f;
+                    for (int i = 0; i < count; ++i) t += tmp[(size_t)y * pw + left + i] * wt[i];
+                    t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+                    scaled[(size_t)y * nw + ox] = (uint8_t)roundf(t);
+                }
+            }
+            free(tmp);
+        }
+        const uint32_t copy_w = nw < W ? nw : W, copy_h = nh < H ? nh : H;
+        const uint32_t sx = (nw - copy_w) / 2, sy = (nh - copy_h) / 2, dx = (W - copy_w) / 2, dy = (H - copy_h) / 2;
+        for (uint32_t y = 0; y < H; ++y) memset(j->dst[k] + (size_t)y * j->dst_stride[k], bg, W);
+        for (uint32_t y = 0; y < copy_h; ++y)
+            memcpy(j->dst[k] + (size_t)(dy + y) * j->dst_stride[k] + dx, scaled + (size_t)(sy + y) * nw + sx, copy_w);
+        free(scaled);
+    }
     return 0;
 }

@@ -134,6 +134,11 @@ void oracle_premultiplied_over_pixel(uint8_t destination[4], const uint8_t sourc
 uint8_t oracle_blend_channel(uint8_t destination, uint8_t source, uint8_t alpha);
 int oracle_composite(const mmr_recon_composite_job *job);
 int oracle_yuv_blend(const mmr_recon_yuv_blend_job *job);
+void oracle_rgb_to_yuv(uint8_t red, uint8_t green, uint8_t blue, uint8_t limited[3], uint8_t full[3]);
+int oracle_overlay_prepare(const mmr_recon_overlay_prepare_job *job);
+int oracle_scale_taps(uint32_t filter, uint32_t n, uint32_t m, uint32_t o, int32_t *left, float *w, int cap);
+void oracle_scaled_dimensions(uint32_t sw, uint32_t sh, uint32_t cw, uint32_t ch, uint32_t mode, uint32_t *ow, uint32_t *oh);
+int oracle_scale_yuv420(const mmr_recon_scale_job *job);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,8 @@ STRUCTS = {
     "mmr_recon_layer": abi.Layer,
     "mmr_recon_composite_job": abi.CompositeJob,
     "mmr_recon_yuv_blend_job": abi.YuvBlendJob,
+    "mmr_recon_overlay_prepare_job": abi.OverlayPrepareJob,
+    "mmr_recon_scale_job": abi.ScaleJob,
     "mmr_recon_server_config": abi.ServerConfig,
     "mmr_recon_request": abi.Request,
     "mmr_recon_frame_completion": abi.FrameCompletion,
