@@ -33,6 +33,7 @@ namespace mmr {
 
 constexpr uint32_t kIntraSpinLimit = 1u << 25;
 constexpr int kIntraRows = 8;   // rows per band = waves per workgroup
+constexpr int kIntraRing = 16;  // bottom rows of the last macroblocks a wave hands to the row below
 
 struct IntraShared {
     MbShared S;
@@ -50,9 +51,22 @@ struct IntraShared {
     // Right column of the previous macroblock of this row when it was intra.
     uint8_t left_y[16];
     uint8_t left_c[2][8];
+    // Bottom rows of this row's last kIntraRing intra macroblocks (slot x % kIntraRing), the top
+    // context of the row below in the same band.
+    uint8_t ring_y[kIntraRing][16];
+    uint8_t ring_c[kIntraRing][2][8];
     // Level record (or I_PCM samples) of the current macroblock, copied from global memory.
     __align__(16) int16_t lvs[MMR_H264_LEVELS_PER_MB];
 };
+
+// modes[k] of an intra record for a lane-varying k, by selects on the four packed words (an
+// indexed byte load would put the record in scratch memory).
+MMR_DEV uint32_t intra_mode_of(const mmr_recon_h264_intra& in, int k) {
+    uint32_t w[4];
+    __builtin_memcpy(w, in.modes, sizeof w);
+    const uint32_t lo = (k & 4) ? w[1] : w[0], hi = (k & 4) ? w[3] : w[2];
+    return (((k & 8) ? hi : lo) >> (8 * (k & 3))) & 0xffu;
+}
 
 MMR_DEV uint32_t ip_avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
 MMR_DEV uint32_t ip_f121(uint32_t a, uint32_t b, uint32_t c) { return (a + 2 * b + c + 2) >> 2; }
@@ -134,18 +148,21 @@ MMR_DEV uint32_t plane_sample(const uint8_t (*tile)[STRIDE], int n, int x, int y
 }
 
 // Waits for the row above, bounded (a protocol error reports TIMEOUT instead of hanging). Inside the
-// band: the LDS counter of the row above with a workgroup-scope acquire. Across bands: relaxed
+// band: the LDS counter of the row above, then an acquire of LDS only -- everything a row hands to
+// the row below inside the band is in LDS (its ring of bottom rows); the global samples it reads
+// there were written by an earlier kernel (inter macroblocks). Across bands: relaxed
 // agent-scope polls of the previous band's published progress, then an agent-scope acquire so the
 // samples it wrote before its release are visible (and this wave's stale cached lines dropped).
 MMR_DEV void intra_wait_lds(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
         if (++spins > kIntraSpinLimit) {
             atomicOr(status, kDevStatusTimeout);
-            return;
+            break;
         }
         __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 MMR_DEV void intra_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
     if (seen >= need) return;
@@ -198,10 +215,13 @@ MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon
 }
 
 // Prediction of one intra macroblock (x, y) of `job`, mode record `in`, plus the residual already
-// in sh.S; stores the macroblock and keeps its right column for the next one.
+// in sh.S; stores the macroblock and keeps its right column for the next one. `above` is the
+// LDS of the row above when it is in the same band (else null: the samples come from global
+// memory); bits 0 / 1 / 2 of `above_intra` say that macroblock x - 1 / x / x + 1 of the row above
+// is intra, i.e. its bottom row is in above's ring (inter macroblocks: global memory).
 MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int mby, const mmr_recon_h264_macroblock& m,
-                              const mmr_recon_h264_intra& in, bool left_mb, bool top_mb, bool left_in_lds, int l,
-                              uint32_t* status) {
+                              const mmr_recon_h264_intra& in, bool left_mb, bool top_mb, bool left_in_lds,
+                              const IntraShared* above, uint32_t above_intra, int l, uint32_t* status) {
     MbShared& S = sh.S;
     const int W = (int)job.width_mbs * 16, CW = W / 2;
     const MMR_GLOBAL uint8_t* py = as_global(job.planes[0]);
@@ -212,12 +232,16 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
 
     // ---- context tile
     const int x0 = mbx * 16, y0 = mby * 16, cx0 = mbx * 8, cy0 = mby * 8;
+    const uint32_t ring_mid = above && (above_intra & 2u) ? (uint32_t)(mbx % kIntraRing) : ~0u;
     if (top_mb) {
-        if (l < 16) sh.ty[0][1 + l] = py[(size_t)(y0 - 1) * W + x0 + l];
-        else if (l < 24 && x0 + l < W) sh.ty[0][1 + l] = py[(size_t)(y0 - 1) * W + x0 + l];
-        else if (l >= 48) {
+        if (l < 16) {
+            sh.ty[0][1 + l] = ring_mid != ~0u ? above->ring_y[ring_mid][l] : py[(size_t)(y0 - 1) * W + x0 + l];
+        } else if (l < 24 && x0 + l < W) {
+            sh.ty[0][1 + l] = above && (above_intra & 4u) ? above->ring_y[(mbx + 1) % kIntraRing][l - 16]
+                                                          : py[(size_t)(y0 - 1) * W + x0 + l];
+        } else if (l >= 48) {
             const int comp = (l - 48) >> 3, i = (l - 48) & 7;
-            sh.tc[comp][0][1 + i] = pc[comp][(size_t)(cy0 - 1) * CW + cx0 + i];
+            sh.tc[comp][0][1 + i] = ring_mid != ~0u ? above->ring_c[ring_mid][comp][i] : pc[comp][(size_t)(cy0 - 1) * CW + cx0 + i];
         }
     }
     if (left_mb) {
@@ -230,8 +254,10 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
         }
     }
     if (top_mb && left_mb && l >= 24 && l < 27) {
-        if (l == 24) sh.ty[0][0] = py[(size_t)(y0 - 1) * W + x0 - 1];
-        else sh.tc[l - 25][0][0] = pc[l - 25][(size_t)(cy0 - 1) * CW + cx0 - 1];
+        const bool ring = above && (above_intra & 1u);
+        const int slot = (mbx - 1) % kIntraRing;
+        if (l == 24) sh.ty[0][0] = ring ? above->ring_y[slot][15] : py[(size_t)(y0 - 1) * W + x0 - 1];
+        else sh.tc[l - 25][0][0] = ring ? above->ring_c[slot][l - 25][7] : pc[l - 25][(size_t)(cy0 - 1) * CW + cx0 - 1];
     }
 
     wave_lds_sync();   // context tile
@@ -282,7 +308,7 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
                 const int k = (by >> 1) * 8 + (bx >> 1) * 4 + (by & 1) * 2 + (bx & 1);   // luma_block_index
                 const int ox = bx * 4, oy = by * 4;
                 const bool ht = by > 0 || top_mb, hl = bx > 0 || left_mb;
-                const int mode = in.modes[k];
+                const int mode = (int)intra_mode_of(in, k);
                 const uint32_t needs = nxn_needs(mode);
                 const bool ok = mode <= 8 && (!(needs & 1) || ht) && (!(needs & 2) || hl) && (!(needs & 4) || (ht && hl));
                 if (!ok && (l & 15) == 0) atomicOr(status, kDevStatusInvalidData);
@@ -313,7 +339,7 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
         for (int g = 0; g < 4; ++g) {
             const int ox = (g & 1) * 8, oy = (g >> 1) * 8;
             const bool ht = g >= 2 || top_mb, hl = (g & 1) != 0 || left_mb, hc = ht && hl;
-            const int mode = in.modes[g];
+            const int mode = (int)intra_mode_of(in, g);
             if (l < 17) {
                 uint32_t v = 0;
                 if (l == 0) {
@@ -469,7 +495,8 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
             auto publish_to = [&](uint32_t x, bool force) {
                 if (x > published) {
                     published = x;
-                    __hip_atomic_store(&prog[w], x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");   // the ring, not the picture
+                    __hip_atomic_store(&prog[w], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 }
                 if (gself && x > gpublished && (force || x - gpublished >= publish)) {
                     gpublished = x;
@@ -479,7 +506,8 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
             // Records of 64 macroblocks at a time, lane i holding macroblock chunk + i: its record
             // (3 dwords), intra record (5 dwords) and the slice of the macroblock above; read out
             // with readlane.
-            uint32_t rec[3] = {0, 0, 0}, irec[5] = {0, 0, 0, 0, 0}, above_slice = 0;
+            uint32_t rec[3] = {0, 0, 0}, irec[5] = {0, 0, 0, 0, 0}, above_slice = 0, above3 = 0;
+            const IntraShared* above = first ? nullptr : &sh_all[w - 1];
             uint32_t left_slice = 0;
             // Level record of the next intra macroblock of the chunk, loaded while the current one
             // is reconstructed: lane l holds dwords 3l .. 3l+2 of the 768-byte record.
@@ -506,6 +534,12 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
                             for (int i = 0; i < 5; ++i) irec[i] = r[i];
                         }
                         above_slice = mby > 0 ? (uint32_t)job.mbs[mb + l - (uint32_t)mw].slice_id : 0u;
+                        if (above) {   // intra flags of macroblocks x - 1, x, x + 1 of the row above
+                            const mmr_recon_h264_macroblock* up = job.mbs + mb + l - (uint32_t)mw;
+                            above3 = (xl > 0 && (up[-1].flags & MMR_H264_MB_INTRA) ? 1u : 0u) |
+                                     ((up[0].flags & MMR_H264_MB_INTRA) ? 2u : 0u) |
+                                     (xl + 1 < mw && (up[1].flags & MMR_H264_MB_INTRA) ? 4u : 0u);
+                        }
                     }
                 }
                 const int lane = mbx & 63;
@@ -563,8 +597,18 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
                 // left_macroblock_available / top_macroblock_available: inside the picture, same slice
                 const bool left_mb = mbx > 0 && prev_slice == slice;
                 const bool top_mb = mby > 0 && __builtin_amdgcn_readlane(above_slice, lane) == slice;
-                intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, l, status);
+                intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, above,
+                                 __builtin_amdgcn_readlane(above3, lane), l, status);
                 left_in_lds = true;
+                if (!last) {
+                    // Hand the bottom row to the row below through the ring; its slot was last read
+                    // by the row below's macroblock mbx - kIntraRing + 1.
+                    const int free_after = mbx - kIntraRing + 2;
+                    if (free_after > 0) intra_wait_lds(&prog[w + 1], (uint32_t)free_after, status);
+                    const int slot = mbx % kIntraRing;
+                    if (l < 16) sh.ring_y[slot][l] = sh.ty[16][1 + l];
+                    else if (l < 32) sh.ring_c[slot][(l - 16) >> 3][(l - 16) & 7] = sh.tc[(l - 16) >> 3][8][1 + ((l - 16) & 7)];
+                }
             }
             publish_to((uint32_t)mw, true);
         }
