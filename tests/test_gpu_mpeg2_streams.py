@@ -1,7 +1,8 @@
-"""The reference's three MPEG-2 test streams, entropy-decoded by the oracle's front end and
-reconstructed on the device -- progressive, interlaced with field prediction, and open GOPs with
-cross-GOP references -- bit-exact against the oracle for every picture, as one submission
-(dependency-level launches) and with one launch per picture."""
+"""The reference's three MPEG-2 test streams and the video of its two transport-stream vectors,
+entropy-decoded by the oracle's front end and reconstructed on the device -- progressive,
+interlaced with field prediction, and open GOPs with cross-GOP references -- bit-exact against
+the oracle for every picture, as one submission (dependency-level launches) and with one launch per
+picture."""
 import pytest
 
 import mpeg2_streams as ms
@@ -12,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("flags", [0, abi.SUBMIT_PER_FRAME_LAUNCH])
-@pytest.mark.parametrize("name", sorted(ms.STREAMS))
+@pytest.mark.parametrize("name", sorted(ms.STREAMS) + sorted(ms.TS_STREAMS))
 def test_mpeg2_stream_bit_exact(ctx, oracle, name, flags):
     from oracle import binding
 
