@@ -294,6 +294,135 @@ class H264IntraWorkload(H264Workload):
                             "1920x1088, 3840x2160 per launch", "pictures_per_step": self.pictures}
 
 
+# ------------------------------------------------------------------ compositor neighbours
+class ScaleWorkload:
+    """The canvas scaler (scale_yuv420_to_canvas_cpu, crates/render/src/compositor.rs:89-141) with
+    Lanczos3: 16 frames 1920x1080 -> 1280x720 (FIT, the preview downscale) and 16 frames 1280x720
+    -> 1920x1080 (FIT, the export upscale) per launch. Mpix counts canvas samples (luma)."""
+    name = "scale"
+    codec = None
+    data = "synthetic (seeded random planes)"
+    metric = "scaled canvas Mpix/s (4:2:0 Lanczos3, centred crop / pad)"
+    metric_unit = "Mpix/s"
+    kernel = "scale_output_kernel"
+    dtype = "u8 in/out, f32 (Lanczos3 taps and sums) arithmetic"
+    CASES = ((1920, 1080, 1280, 720), (1280, 720, 1920, 1080))
+
+    def __init__(self, ctx, per_case=16, seed=2024):
+        self.ctx = ctx
+        rng = np.random.default_rng(seed)
+        self.host, self.bufs, self.structs = [], [], []
+        for w, h, cw, ch in self.CASES:
+            for _ in range(per_case):
+                planes = [rng.integers(0, 256, s, dtype=np.uint8) for s in ((h, w), (h // 2, w // 2), (h // 2, w // 2))]
+                src = [ctx.upload(p) for p in planes]
+                dst = [ctx.alloc(cw * ch), ctx.alloc(cw * ch // 4), ctx.alloc(cw * ch // 4)]
+                j = abi.new(abi.ScaleJob, width=w, height=h, canvas_width=cw, canvas_height=ch, mode=abi.SCALE_FIT,
+                            range=abi.RANGE_LIMITED, filter=abi.SCALE_FILTER_LANCZOS3)
+                for k in range(3):
+                    j.src[k], j.src_stride[k] = src[k].ptr, planes[k].shape[1]
+                    j.dst[k], j.dst_stride[k] = dst[k].ptr, cw >> (1 if k else 0)
+                self.host.append((planes, w, h, cw, ch))
+                self.bufs += src + dst
+                self.structs.append(j)
+        self.pictures = len(self.structs)
+        self.mpix = sum(metrics.mpix(cw, ch) for _, _, _, cw, ch in self.host)
+        self.bytes = sum(1.5 * (w * h + cw * ch) for _, w, h, cw, ch in self.host)
+        self.f64_ops = None
+
+    def submit(self, flags=0):
+        return self.ctx.submit_scale(self.structs, flags)
+
+    def plain_items(self):
+        return [[s] for s in self.structs]
+
+    def submit_items(self, items, flags):
+        return self.ctx.submit_scale(items, flags)
+
+    def config(self):
+        return {"workload": "4:2:0 canvas scaling, Lanczos3 FIT: 16 frames 1920x1080 -> 1280x720 and 16 frames "
+                            "1280x720 -> 1920x1080 per launch", "pictures_per_step": self.pictures}
+
+    def cpu_sample(self, threads):
+        from oracle import binding
+
+        L = binding.lib()
+        picks = [self.host[0], self.host[-1]] * max(1, threads // 2)
+        picks = picks[:max(1, threads)]
+        outs = []
+        for planes, w, h, cw, ch in picks:
+            out = [np.zeros((ch, cw), np.uint8), np.zeros((ch // 2, cw // 2), np.uint8), np.zeros((ch // 2, cw // 2), np.uint8)]
+            j = abi.new(abi.ScaleJob, width=w, height=h, canvas_width=cw, canvas_height=ch, mode=abi.SCALE_FIT,
+                        range=abi.RANGE_LIMITED, filter=abi.SCALE_FILTER_LANCZOS3)
+            for k in range(3):
+                j.src[k], j.src_stride[k] = planes[k].ctypes.data, planes[k].shape[1]
+                j.dst[k], j.dst_stride[k] = out[k].ctypes.data, out[k].shape[1]
+            outs.append((j, out))
+        passes, dt = cpu_passes([lambda j=j: L.oracle_scale_yuv420(C.byref(j)) for j, _ in outs])
+        mp = passes * sum(metrics.mpix(cw, ch) for _, _, _, cw, ch in picks)
+        return mp / dt, len(picks), (f"{passes} pass(es) over {len(picks)} frames (half 1080p->720p, half 720p->1080p), "
+                                     f"one frame per thread, {dt:.1f} s wall on {len(picks)} threads")
+
+
+class OverlayWorkload:
+    """Overlay preparation (PreparedOverlay::prepare_yuv, crates/render/src/project_compositor.rs:1098-1142):
+    32 RGBA overlays of 1920x1080 per launch into the seven prepared planes."""
+    name = "overlay"
+    codec = None
+    data = "synthetic (seeded random planes)"
+    metric = "prepared overlay Mpix/s (RGBA -> BT.709 limited + full Y, alpha-weighted U/V, chroma alpha)"
+    metric_unit = "Mpix/s"
+    kernel = "overlay_prepare_kernel"
+    dtype = "u8 in/out, int32 arithmetic"
+
+    def __init__(self, ctx, frames=32, width=1920, height=1080, seed=2024):
+        self.ctx = ctx
+        rng = np.random.default_rng(seed)
+        self.host, self.bufs, self.structs = [], [], []
+        cw, ch = (width + 1) // 2, (height + 1) // 2
+        for _ in range(frames):
+            rgba = rng.integers(0, 256, (height, width, 4), dtype=np.uint8)
+            d = ctx.upload(rgba)
+            outs = {n: ctx.alloc(width * height if n.startswith("y_") else cw * ch)
+                    for n in ("y_limited", "y_full", "u_limited", "v_limited", "u_full", "v_full", "chroma_alpha")}
+            self.structs.append(abi.new(abi.OverlayPrepareJob, rgba=d.ptr, width=width, height=height,
+                                        **{n: b.ptr for n, b in outs.items()}))
+            self.host.append(rgba)
+            self.bufs += [d] + list(outs.values())
+        self.pictures = frames
+        self.mpix = frames * metrics.mpix(width, height)
+        self.bytes = frames * (width * height * (4 + 2) + cw * ch * 5)
+        self.f64_ops = None
+
+    def submit(self, flags=0):
+        return self.ctx.submit_overlay_prepare(self.structs, flags)
+
+    def plain_items(self):
+        return [[s] for s in self.structs]
+
+    def submit_items(self, items, flags):
+        return self.ctx.submit_overlay_prepare(items, flags)
+
+    def config(self):
+        return {"workload": "overlay preparation, 32 RGBA overlays of 1920x1080 per launch", "pictures_per_step": self.pictures}
+
+    def cpu_sample(self, threads):
+        from oracle import binding
+
+        L = binding.lib()
+        n = max(1, min(threads, len(self.host)))
+        jobs = []
+        for rgba in self.host[:n]:
+            h, w = rgba.shape[:2]
+            outs = {k: np.zeros(w * h if k.startswith("y_") else ((w + 1) // 2) * ((h + 1) // 2), np.uint8)
+                    for k in ("y_limited", "y_full", "u_limited", "v_limited", "u_full", "v_full", "chroma_alpha")}
+            j = abi.new(abi.OverlayPrepareJob, rgba=rgba.ctypes.data, width=w, height=h, **{k: v.ctypes.data for k, v in outs.items()})
+            jobs.append((j, outs))
+        passes, dt = cpu_passes([lambda j=j: L.oracle_overlay_prepare(C.byref(j)) for j, _ in jobs])
+        mp = passes * n * metrics.mpix(self.host[0].shape[1], self.host[0].shape[0])
+        return mp / dt, n, f"{passes} pass(es) over {n} 1920x1080 overlays, one per thread, {dt:.1f} s wall on {n} threads"
+
+
 # ------------------------------------------------------------------ served mixed workload
 class ServedMix:
     """Open-loop served workload: Poisson request arrivals over 64 streams (16 per codec), each
@@ -545,7 +674,7 @@ def run_served_sweep(ctx, rates, seconds, plain_limit):
 
 
 WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload,
-             "h264i": H264IntraWorkload}
+             "h264i": H264IntraWorkload, "scale": ScaleWorkload, "overlay": OverlayWorkload}
 
 
 def calibrate_copy(nbytes=1 << 30, repeats=5):
@@ -605,7 +734,7 @@ def main():
     log(f"{w.pictures} pictures, {w.mpix:.1f} Mpix, {w.bytes / 1e9:.3f} GB algorithmic per step")
 
     tuning = {"chosen": ctx.get_tuning(), "trials": None}
-    if not args.no_autotune:
+    if not args.no_autotune and w.codec is not None:
         t0 = time.perf_counter()
         best, trials = ctx.autotune(w.codec, w.structs, repeats=3)
         tuning = {"chosen": best, "trials": [{"tuning": t, "kernel_ms": round(ms, 4)} for t, ms in trials],
@@ -628,21 +757,23 @@ def main():
     log(f"batched: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s, kernel {k_ms} ms")
 
     # The same batch prepared once (job records resident, launches replayed from a HIP graph).
-    prep = ctx.prepare(w.codec, w.structs)
-    for _ in range(max(1, args.warmup)):
-        ctx.wait(prep.launch())
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    tickets = [prep.launch() for _ in range(args.steps)]
-    for t in tickets:
-        ctx.wait(t)
-    p_elapsed = time.perf_counter() - t0
-    p_kernel_ms = float(np.mean([ctx.kernel_ms(t) for t in tickets]))
-    prep.close()
-    prepared = {"value": round(w.mpix * args.steps / p_elapsed, 3), "unit": "Mpix/s",
-                "ms_per_step": round(p_elapsed * 1e3 / args.steps, 4), "kernel_ms": round(p_kernel_ms, 4),
-                "mode": "batch prepared once, launches replayed from a HIP graph"}
-    log(f"prepared graph: {prepared['ms_per_step']:.3f} ms/step, {prepared['value']:.1f} Mpix/s")
+    prepared = None
+    if w.codec is not None:
+        prep = ctx.prepare(w.codec, w.structs)
+        for _ in range(max(1, args.warmup)):
+            ctx.wait(prep.launch())
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        tickets = [prep.launch() for _ in range(args.steps)]
+        for t in tickets:
+            ctx.wait(t)
+        p_elapsed = time.perf_counter() - t0
+        p_kernel_ms = float(np.mean([ctx.kernel_ms(t) for t in tickets]))
+        prep.close()
+        prepared = {"value": round(w.mpix * args.steps / p_elapsed, 3), "unit": "Mpix/s",
+                    "ms_per_step": round(p_elapsed * 1e3 / args.steps, 4), "kernel_ms": round(p_kernel_ms, 4),
+                    "mode": "batch prepared once, launches replayed from a HIP graph"}
+        log(f"prepared graph: {prepared['ms_per_step']:.3f} ms/step, {prepared['value']:.1f} Mpix/s")
 
     plain = None
     latencies = []
@@ -672,7 +803,7 @@ def main():
 
     achieved = w.bytes / (k_ms * 1e-3) / 1e9 if k_ms else None
     result = {
-        "metric": "reconstructed Mpix/s batched-served vs plain GPU vs CPU; p50/p99 frame latency",
+        "metric": getattr(w, "metric", "reconstructed Mpix/s batched-served vs plain GPU vs CPU; p50/p99 frame latency"),
         "value": round(value, 3),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -682,8 +813,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8/i16 in, int32 + f64 (IDCT) arithmetic",
-        "data": "synthetic (seeded post-entropy coefficient / macroblock buffers)",
+        "dtype": getattr(w, "dtype", "u8/i16 in, int32 + f64 (IDCT) arithmetic"),
+        "data": getattr(w, "data", "synthetic (seeded post-entropy coefficient / macroblock buffers)"),
         "config": w.config(),
         "plain_gpu": {"value": round(plain, 3) if plain else None, "unit": "Mpix/s",
                       "p50_frame_ms": percentile(latencies, 50), "p99_frame_ms": percentile(latencies, 99),

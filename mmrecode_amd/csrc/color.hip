@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void overlay_prepare_kernel(const mmr_recon_ov
 __global__ __launch_bounds__(256) void scale_vertical_kernel(const ScalePlaneDev* __restrict__ planes) {
     const ScalePlaneDev& p = planes[blockIdx.y];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (!p.resize || i >= p.src_w * p.copy_h) return;
+    if (p.tiled || !p.resize || i >= p.src_w * p.copy_h) return;
     const uint32_t r = i / p.src_w, x = i - r * p.src_w;
     const int32_t left = p.vleft[r];
     const uint32_t count = p.vcount[r];
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void scale_vertical_kernel(const ScalePlaneDev
 __global__ __launch_bounds__(256) void scale_output_kernel(const ScalePlaneDev* __restrict__ planes) {
     const ScalePlaneDev& p = planes[blockIdx.y];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= p.dst_w * p.dst_h) return;
+    if (p.tiled || i >= p.dst_w * p.dst_h) return;
     const uint32_t y = i / p.dst_w, x = i - y * p.dst_w;
     uint32_t v = p.background;
     if (x >= p.dst_x && x < p.dst_x + p.copy_w && y >= p.dst_y && y < p.dst_y + p.copy_h) {
@@ -265,6 +265,91 @@ __global__ __launch_bounds__(256) void scale_output_kernel(const ScalePlaneDev* 
         }
     }
     p.dst[(size_t)y * p.dst_stride + x] = (uint8_t)v;
+}
+
+// One-pass scaler: workgroup per kTileW x kTileH canvas tile of plane blockIdx.y. The vertical
+// pass of the tile's window rows over the source columns its outputs reach goes to LDS (source
+// read straight from global memory, the overlapping taps of neighbouring rows served by the
+// caches), then the horizontal pass, clamp, rounding and placement -- the same f32 operations in
+// the same order as the two-pass kernels, without the intermediate's round trip through HBM.
+__global__ __launch_bounds__(256) void scale_tile_kernel(const ScalePlaneDev* __restrict__ planes) {
+    const ScalePlaneDev& p = planes[blockIdx.y];
+    if (!p.tiled) return;
+    const uint32_t tiles_x = (p.dst_w + kScaleTileW - 1) / kScaleTileW;
+    if (blockIdx.x >= tiles_x * ((p.dst_h + kScaleTileH - 1) / kScaleTileH)) return;
+    __shared__ float tmp[kScaleTileH * kScaleTileSrc];
+    __shared__ float hw_s[kScaleTileTaps][kScaleTileW];   // the tile columns' weights, tap-major
+    __shared__ int32_t hl_s[kScaleTileW];                 // first tap, relative to the tile's source column 0
+    __shared__ uint32_t hc_s[kScaleTileW];
+    const int tx0 = (int)((blockIdx.x % tiles_x) * kScaleTileW), ty0 = (int)((blockIdx.x / tiles_x) * kScaleTileH);
+    const int tx1 = min(tx0 + kScaleTileW, (int)p.dst_w), ty1 = min(ty0 + kScaleTileH, (int)p.dst_h);
+    // The tile's part of the window, in window coordinates (empty when wx0 >= wx1 or wy0 >= wy1).
+    const int wx0 = max(tx0, (int)p.dst_x) - (int)p.dst_x, wx1 = min(tx1, (int)(p.dst_x + p.copy_w)) - (int)p.dst_x;
+    const int wy0 = max(ty0, (int)p.dst_y) - (int)p.dst_y, wy1 = min(ty1, (int)(p.dst_y + p.copy_h)) - (int)p.dst_y;
+    const bool inside = wx0 < wx1 && wy0 < wy1;
+    int sc0 = 0;
+    if (inside && p.resize) {   // uniform per workgroup
+        sc0 = p.hleft[wx0];
+        const int sw = p.hleft[wx1 - 1] + (int)p.hcount[wx1 - 1] - sc0;   // <= kScaleTileSrc (host check)
+        const int rows = wy1 - wy0, cols = wx1 - wx0, hmax = (int)p.hmax;   // hmax <= kScaleTileTaps (host check)
+        for (int i = (int)threadIdx.x; i < cols * hmax; i += 256) {
+            const int c = i / hmax, k = i - c * hmax;
+            hw_s[k][c] = p.hw[(size_t)(wx0 + c) * hmax + k];
+        }
+        for (int i = (int)threadIdx.x; i < cols; i += 256) {
+            hl_s[i] = p.hleft[wx0 + i] - sc0;
+            hc_s[i] = p.hcount[wx0 + i];
+        }
+        for (int i = (int)threadIdx.x; i < rows * sw; i += 256) {
+            const int r = i / sw, sc = i - r * sw;
+            const int y = wy0 + r;
+            const uint32_t count = p.vcount[y];
+            const float* w = p.vw + (size_t)y * p.vmax;
+            const uint8_t* src = p.src + (size_t)p.vleft[y] * p.src_stride + sc0 + sc;
+            const size_t st = p.src_stride;
+            float t = 0.0f;
+            uint32_t k = 0;
+            // four taps' loads in flight at once; the sum still takes them one by one in order
+            for (; k + 4 <= count; k += 4) {
+                const uint32_t a0 = src[k * st], a1 = src[(k + 1) * st], a2 = src[(k + 2) * st], a3 = src[(k + 3) * st];
+                t += (float)a0 * w[k];
+                t += (float)a1 * w[k + 1];
+                t += (float)a2 * w[k + 2];
+                t += (float)a3 * w[k + 3];
+            }
+            for (; k < count; ++k) t += (float)src[k * st] * w[k];
+            tmp[r * kScaleTileSrc + sc] = t;
+        }
+        __syncthreads();
+    }
+    for (int i = (int)threadIdx.x; i < kScaleTileW * kScaleTileH; i += 256) {
+        const int x = tx0 + (i % kScaleTileW), y = ty0 + (i / kScaleTileW);
+        if (x >= tx1 || y >= ty1) continue;
+        const int c = x - (int)p.dst_x, r = y - (int)p.dst_y;
+        uint32_t v = p.background;
+        if (inside && c >= wx0 && c < wx1 && r >= wy0 && r < wy1) {
+            if (p.resize) {
+                const int cc = c - wx0;
+                const uint32_t count = hc_s[cc];
+                const float* row = tmp + (r - wy0) * kScaleTileSrc + hl_s[cc];
+                float t = 0.0f;
+                uint32_t k = 0;
+                for (; k + 4 <= count; k += 4) {
+                    const float r0 = row[k], r1 = row[k + 1], r2 = row[k + 2], r3 = row[k + 3];
+                    t += r0 * hw_s[k][cc];
+                    t += r1 * hw_s[k + 1][cc];
+                    t += r2 * hw_s[k + 2][cc];
+                    t += r3 * hw_s[k + 3][cc];
+                }
+                for (; k < count; ++k) t += row[k] * hw_s[k][cc];
+                t = t < 0.0f ? 0.0f : (t > 255.0f ? 255.0f : t);
+                v = (uint32_t)roundf(t);
+            } else {
+                v = p.src[(size_t)(p.sy + r) * p.src_stride + p.sx + c];
+            }
+        }
+        p.dst[(size_t)y * p.dst_stride + x] = (uint8_t)v;
+    }
 }
 
 __global__ void fill_zero_kernel(uint32_t* p, uint32_t n) {
@@ -297,8 +382,10 @@ hipError_t launch_overlay_prepare(const mmr_recon_overlay_prepare_job* d_jobs, u
     return hipGetLastError();
 }
 
-hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_vertical, uint32_t max_output, hipStream_t s) {
+hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_tiles, uint32_t max_vertical,
+                        uint32_t max_output, hipStream_t s) {
     if (plane_count == 0) return hipSuccess;
+    if (max_tiles) hipLaunchKernelGGL(scale_tile_kernel, dim3(max_tiles, plane_count), dim3(256), 0, s, d_planes);
     if (max_vertical) hipLaunchKernelGGL(scale_vertical_kernel, dim3((max_vertical + 255) / 256, plane_count), dim3(256), 0, s, d_planes);
     if (max_output) hipLaunchKernelGGL(scale_output_kernel, dim3((max_output + 255) / 256, plane_count), dim3(256), 0, s, d_planes);
     return hipGetLastError();

@@ -18,8 +18,10 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/mmr_recon.h"
@@ -60,6 +62,15 @@ constexpr uint32_t kStatusInvalidData = mmr::kDevStatusInvalidData;
 
 }  // namespace
 
+// Resampling taps of one pass (filter, source length n, scaled length m) for every output
+// coordinate: left tap, tap count, and weights padded to `max` per coordinate.
+struct ScaleTaps {
+    std::vector<int32_t> left;
+    std::vector<uint32_t> count;
+    std::vector<float> w;
+    uint32_t max = 0;
+};
+
 struct mmr_recon_context {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -71,6 +82,8 @@ struct mmr_recon_context {
     std::map<uint64_t, TicketRecord> tickets;
     uint8_t* flat_scaling = nullptr;  // device Flat_16 scaling lists ([6][16] + [2][64])
     mmr_recon_tuning tuning{};        // launch shapes used by the next submissions
+    // Scaler taps by (filter, n, m); they depend on the geometry only (bounded: cleared when full).
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::shared_ptr<const ScaleTaps>> scale_taps;
 };
 
 namespace {
@@ -932,16 +945,44 @@ static void scaled_dimensions(uint32_t sw, uint32_t sh, uint32_t cw, uint32_t ch
     }
 }
 
+// The taps of (filter, n, m) for every output coordinate, from the context's cache.
+static std::shared_ptr<const ScaleTaps> cached_taps(mmr_recon_context* ctx, uint32_t filter, uint32_t n, uint32_t m) {
+    const auto key = std::make_tuple(filter, n, m);
+    auto it = ctx->scale_taps.find(key);
+    if (it != ctx->scale_taps.end()) return it->second;
+    if (ctx->scale_taps.size() >= 64) ctx->scale_taps.clear();
+    auto t = std::make_shared<ScaleTaps>();
+    std::vector<std::vector<float>> per(m);
+    t->left.resize(m);
+    t->count.resize(m);
+    for (uint32_t o = 0; o < m; ++o) {
+        t->left[o] = scale_taps(filter, n, m, o, per[o]);
+        t->count[o] = (uint32_t)per[o].size();
+        t->max = std::max(t->max, t->count[o]);
+    }
+    t->w.assign((size_t)m * t->max, 0.0f);
+    for (uint32_t o = 0; o < m; ++o) std::copy(per[o].begin(), per[o].end(), t->w.begin() + (size_t)o * t->max);
+    ctx->scale_taps.emplace(key, t);
+    return t;
+}
+
 mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_scale_job* in, uint32_t count, uint32_t flags,
                            uint64_t* out_ticket) {
     CHECK_CTX(ctx);
     if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    // Per plane: the device record plus byte offsets of its tap arrays in the blob (one copy per
+    // distinct geometry in the submission) and of its f32 intermediate in scratch.
     struct PlaneHost {
         mmr::ScalePlaneDev d;
-        size_t vleft, vcount, vw, hleft, hcount, hw, tmp;   // offsets: blob (taps) / scratch (tmp)
+        size_t vleft, vcount, vw, hleft, hcount, hw, tmp;
+    };
+    struct Placed {
+        size_t left, count, w;
+        uint32_t max;
     };
     std::vector<PlaneHost> planes;
     std::vector<char> taps;
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Placed> placed;
     size_t scratch = 0;
     auto put = [&taps](const void* data, size_t bytes) {
         const size_t off = (taps.size() + 15) & ~size_t(15);
@@ -949,7 +990,16 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         if (bytes) memcpy(taps.data() + off, data, bytes);
         return off;
     };
-    std::vector<float> w;
+    auto place = [&](uint32_t filter, uint32_t n, uint32_t m) -> Placed {
+        const auto key = std::make_tuple(filter, n, m);
+        auto it = placed.find(key);
+        if (it != placed.end()) return it->second;
+        auto t = cached_taps(ctx, filter, n, m);
+        Placed p{put(t->left.data(), t->left.size() * 4), put(t->count.data(), t->count.size() * 4),
+                 put(t->w.data(), t->w.size() * 4), t->max};
+        placed.emplace(key, p);
+        return p;
+    };
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_scale_job& j = in[i];
         if (j.struct_size < sizeof(mmr_recon_scale_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "scale struct_size too small");
@@ -978,33 +1028,31 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
             d.dst_x = (W - d.copy_w) / 2, d.dst_y = (H - d.copy_h) / 2;
             d.background = k ? 128u : (j.range == MMR_RANGE_LIMITED ? 16u : 0u);
             d.resize = !(nw == pw && nh == ph);
+            d.tiled = 1;
             if (d.resize) {
-                std::vector<int32_t> vl(d.copy_h), hl(d.copy_w);
-                std::vector<uint32_t> vc(d.copy_h), hc(d.copy_w);
-                std::vector<std::vector<float>> vws(d.copy_h), hws(d.copy_w);
-                for (uint32_t r = 0; r < d.copy_h; ++r) {
-                    vl[r] = scale_taps(j.filter, ph, nh, d.sy + r, w);
-                    vws[r] = w;
-                    vc[r] = (uint32_t)w.size();
-                    d.vmax = std::max(d.vmax, vc[r]);
+                // The window's rows / columns of the whole-axis tables: row r of the window is
+                // scaled row sy + r, column c is scaled column sx + c.
+                const Placed v = place(j.filter, ph, nh), h = place(j.filter, pw, nw);
+                // One-pass tiles when every run of kScaleTileW window columns reaches at most
+                // kScaleTileSrc source columns (taps are monotonic in the output coordinate) and
+                // a column has at most kScaleTileTaps taps.
+                const auto ht = cached_taps(ctx, j.filter, pw, nw);
+                if (ht->max > (uint32_t)mmr::kScaleTileTaps) d.tiled = 0;
+                for (uint32_t c = 0; c < d.copy_w && d.tiled; ++c) {
+                    const uint32_t e = d.sx + std::min(c + (uint32_t)mmr::kScaleTileW - 1, d.copy_w - 1);
+                    if ((int64_t)ht->left[e] + ht->count[e] - ht->left[d.sx + c] > mmr::kScaleTileSrc) d.tiled = 0;
                 }
-                for (uint32_t c = 0; c < d.copy_w; ++c) {
-                    hl[c] = scale_taps(j.filter, pw, nw, d.sx + c, w);
-                    hws[c] = w;
-                    hc[c] = (uint32_t)w.size();
-                    d.hmax = std::max(d.hmax, hc[c]);
+                d.vmax = v.max, d.hmax = h.max;
+                plane.vleft = v.left + (size_t)d.sy * 4;
+                plane.vcount = v.count + (size_t)d.sy * 4;
+                plane.vw = v.w + (size_t)d.sy * v.max * 4;
+                plane.hleft = h.left + (size_t)d.sx * 4;
+                plane.hcount = h.count + (size_t)d.sx * 4;
+                plane.hw = h.w + (size_t)d.sx * h.max * 4;
+                if (!d.tiled) {
+                    plane.tmp = scratch;
+                    scratch += (((size_t)pw * d.copy_h * 4) + 255) & ~size_t(255);
                 }
-                std::vector<float> vflat((size_t)d.copy_h * d.vmax, 0.0f), hflat((size_t)d.copy_w * d.hmax, 0.0f);
-                for (uint32_t r = 0; r < d.copy_h; ++r) std::copy(vws[r].begin(), vws[r].end(), vflat.begin() + (size_t)r * d.vmax);
-                for (uint32_t c = 0; c < d.copy_w; ++c) std::copy(hws[c].begin(), hws[c].end(), hflat.begin() + (size_t)c * d.hmax);
-                plane.vleft = put(vl.data(), vl.size() * 4);
-                plane.vcount = put(vc.data(), vc.size() * 4);
-                plane.vw = put(vflat.data(), vflat.size() * 4);
-                plane.hleft = put(hl.data(), hl.size() * 4);
-                plane.hcount = put(hc.data(), hc.size() * 4);
-                plane.hw = put(hflat.data(), hflat.size() * 4);
-                plane.tmp = scratch;
-                scratch += (((size_t)pw * d.copy_h * 4) + 255) & ~size_t(255);
             }
             planes.push_back(plane);
         }
@@ -1026,29 +1074,35 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
             hp[p].hleft = reinterpret_cast<const int32_t*>(base + planes[p].hleft);
             hp[p].hcount = reinterpret_cast<const uint32_t*>(base + planes[p].hcount);
             hp[p].hw = reinterpret_cast<const float*>(base + planes[p].hw);
-            hp[p].tmp = reinterpret_cast<float*>(d_scratch + planes[p].tmp);
+            hp[p].tmp = planes[p].d.tiled ? nullptr : reinterpret_cast<float*>(d_scratch + planes[p].tmp);
         }
     });
     if (s) return s;
     auto* dp = reinterpret_cast<const mmr::ScalePlaneDev*>(sub.jobs_ptr());
-    auto sizes = [&](size_t from, size_t to, uint32_t& mv, uint32_t& mo) {
-        mv = mo = 0;
+    auto sizes = [&](size_t from, size_t to, uint32_t& mt, uint32_t& mv, uint32_t& mo) {
+        mt = mv = mo = 0;
         for (size_t p = from; p < to; ++p) {
             const mmr::ScalePlaneDev& d = planes[p].d;
+            if (d.tiled) {
+                const uint32_t tiles = ((d.dst_w + mmr::kScaleTileW - 1) / mmr::kScaleTileW) *
+                                       ((d.dst_h + mmr::kScaleTileH - 1) / mmr::kScaleTileH);
+                mt = std::max(mt, tiles);
+                continue;
+            }
             if (d.resize) mv = std::max(mv, d.src_w * d.copy_h);
             mo = std::max(mo, d.dst_w * d.dst_h);
         }
     };
     hipError_t e = hipSuccess;
-    uint32_t mv = 0, mo = 0;
+    uint32_t mt = 0, mv = 0, mo = 0;
     if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            sizes(3 * i, 3 * i + 3, mv, mo);
-            e = mmr::launch_scale(dp + 3 * i, 3, mv, mo, stream);
+            sizes(3 * i, 3 * i + 3, mt, mv, mo);
+            e = mmr::launch_scale(dp + 3 * i, 3, mt, mv, mo, stream);
         }
     } else {
-        sizes(0, planes.size(), mv, mo);
-        e = mmr::launch_scale(dp, (uint32_t)planes.size(), mv, mo, stream);
+        sizes(0, planes.size(), mt, mv, mo);
+        e = mmr::launch_scale(dp, (uint32_t)planes.size(), mt, mv, mo, stream);
     }
     if (e != hipSuccess) return hip_fail(e, "scale launch");
     return sub.end(out_ticket);

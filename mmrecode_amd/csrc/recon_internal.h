@@ -165,9 +165,16 @@ struct ScalePlaneDev {
     uint32_t sx, sy;
     uint32_t vmax, hmax;
     uint32_t resize, background;
+    uint32_t tiled;   // done by scale_tile_kernel (windows within kScaleTileSrc columns, hmax <= kScaleTileTaps)
+    uint32_t pad;
 };
+// Canvas tiles of the one-pass scaler: kScaleTileW x kScaleTileH samples, the vertical pass of
+// at most kScaleTileSrc source columns per tile and horizontal taps of at most kScaleTileTaps per
+// column held in LDS.
+constexpr int kScaleTileW = 64, kScaleTileH = 16, kScaleTileSrc = 320, kScaleTileTaps = 32;
 hipError_t launch_overlay_prepare(const mmr_recon_overlay_prepare_job* d_jobs, uint32_t job_count, uint32_t max_cells, hipStream_t s);
-hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_vertical, uint32_t max_output, hipStream_t s);
+hipError_t launch_scale(const ScalePlaneDev* d_planes, uint32_t plane_count, uint32_t max_tiles, uint32_t max_vertical,
+                        uint32_t max_output, hipStream_t s);
 hipError_t launch_fill_zero(uint32_t* d_words, uint32_t count, hipStream_t s);
 
 // Bits of the per-submission device status word.

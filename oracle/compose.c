@@ -247,13 +247,18 @@ int oracle_scale_taps(uint32_t filter, uint32_t n, uint32_t m, uint32_t o, int32
 
 static float *lz_resize_vertical(uint32_t filter, const uint8_t *src, uint32_t stride, uint32_t w, uint32_t h, uint32_t nh) {
     float *out = (float *)malloc((size_t)w * nh * sizeof(float));
-    float wt[256];
-    if (!out) return NULL;
+    float *wt = (float *)malloc(((size_t)h + 1) * sizeof(float)); /* a pass has at most h taps */
+    if (!out || !wt) {
+        free(out);
+        free(wt);
+        return NULL;
+    }
     for (uint32_t oy = 0; oy < nh; ++oy) {
         int32_t left;
-        const int count = oracle_scale_taps(filter, h, nh, oy, &left, wt, 256);
+        const int count = oracle_scale_taps(filter, h, nh, oy, &left, wt, (int)h + 1);
         if (count < 0) {
             free(out);
+            free(wt);
             return NULL;
         }
         for (uint32_t x = 0; x < w; ++x) {
@@ -262,6 +267,7 @@ static float *lz_resize_vertical(uint32_t filter, const uint8_t *src, uint32_t s
             out[(size_t)oy * w + x] = t;
         }
     }
+    free(wt);
     return out;
 }
 
@@ -309,14 +315,16 @@ int oracle_scale_yuv420(const mmr_recon_scale_job *j) {
             for (uint32_t y = 0; y < ph; ++y) memcpy(scaled + (size_t)y * pw, j->src[k] + (size_t)y * j->src_stride[k], pw);
         } else {
             float *tmp = lz_resize_vertical(j->filter, j->src[k], j->src_stride[k], pw, ph, nh);
-            float wt[256];
-            if (!tmp) {
+            float *wt = (float *)malloc(((size_t)pw + 1) * sizeof(float));
+            if (!tmp || !wt) {
+                free(tmp);
+                free(wt);
                 free(scaled);
                 return -1;
             }
             for (uint32_t ox = 0; ox < nw; ++ox) {
                 int32_t left;
-                const int count = oracle_scale_taps(j->filter, pw, nw, ox, &left, wt, 256);
+                const int count = oracle_scale_taps(j->filter, pw, nw, ox, &left, wt, (int)pw + 1);
                 for (uint32_t y = 0; y < nh; ++y) {
                     float t = 0.0f;
                     for (int i = 0; i < count; ++i) t += tmp[(size_t)y * pw + left + i] * wt[i];
@@ -325,6 +333,7 @@ int oracle_scale_yuv420(const mmr_recon_scale_job *j) {
                 }
             }
             free(tmp);
+            free(wt);
         }
         const uint32_t copy_w = nw < W ? nw : W, copy_h = nh < H ? nh : H;
         const uint32_t sx = (nw - copy_w) / 2, sy = (nh - copy_h) / 2, dx = (W - copy_w) / 2, dy = (H - copy_h) / 2;

@@ -285,6 +285,8 @@ SCALE_CASES = [
     (40, 30, 24, 50, abi.SCALE_NATIVE),
     (16, 16, 16, 16, abi.SCALE_FIT),       # no resize at all
     (18, 10, 36, 10, abi.SCALE_STRETCH),   # only the width changes
+    (1024, 64, 16, 16, abi.SCALE_FIT),     # 64x downscale: a tile's taps exceed one-pass capacity
+    (640, 40, 32, 24, abi.SCALE_STRETCH),  # 20x horizontal, 1.7x vertical downscale
 ]
 
 
