@@ -370,6 +370,13 @@ typedef struct mmr_recon_h264_picture {
     const mmr_recon_h264_intra *intra;          /* [mbs], 4-byte aligned, read for MMR_H264_MB_INTRA macroblocks; NULL:
                                                    intra macroblocks report UNSUPPORTED. A struct_size
                                                    that ends before this member reads as NULL. */
+    uint32_t transform_bypass;                  /* qpprime_y_zero_transform_bypass_flag: the 4x4 luma blocks
+                                                   (Intra4x4 and inter; not 8x8 or Intra16x16) of macroblocks
+                                                   with QP 0, and the chroma blocks of macroblocks with chroma
+                                                   QP 0, skip the transform (decoder.rs:12432, :12651,
+                                                   :12739-12764). A struct_size that ends before this member
+                                                   reads as 0. */
+    uint32_t reserved;
 } mmr_recon_h264_picture;
 
 MMR_RECON_API mmr_status mmr_recon_h264_submit(mmr_recon_context *context,

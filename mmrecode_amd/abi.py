@@ -208,6 +208,8 @@ class H264Picture(C.Structure):
         ("ref_count", C.c_uint32 * 2),
         ("refs", (H264Reference * H264_MAX_REFS) * 2),
         ("intra", vp),
+        ("transform_bypass", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

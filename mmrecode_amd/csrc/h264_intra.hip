@@ -207,6 +207,14 @@ MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon
         const ResidualLists rl = {0, 0, 1, i16};
         if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, job, rl, status);
         else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+        if (job.transform_bypass && (qp == 0 || cqp == 0)) {
+            Bypass by;
+            by.luma = qp == 0 && type == MMR_H264_INTRA_4X4;
+            by.chroma = cqp == 0;
+            __builtin_memcpy(by.luma_modes, in.modes, sizeof by.luma_modes);
+            by.chroma_mode = in.chroma_mode == 1 ? 1 : (in.chroma_mode == 2 ? 0 : 2);
+            bypass_phases(S, lv, l, by);
+        }
     } else {
         for (int i = l; i < 256; i += 64) S.res_y[i] = 0;
         for (int i = l; i < 128; i += 64) S.res_c[i >> 6][i & 63] = 0;

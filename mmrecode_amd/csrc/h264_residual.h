@@ -124,6 +124,40 @@ struct ResidualLists {
 };
 constexpr ResidualLists kInterLists = {3, 1, 4, false};
 
+// Transform bypass of one macroblock (qpprime_y_zero_transform_bypass_flag): luma for the 4x4
+// blocks of an Intra4x4 or inter macroblock with QP 0, chroma when the chroma QP is 0 (the
+// reference decides chroma on the chroma QP, decoder.rs:12739). luma_modes packs the 16 DPCM
+// modes by block index (0 vertical, 1 horizontal, else none: the Intra4x4 modes, 2 for inter);
+// chroma_mode likewise (intra chroma mode 2 -> 0, 1 -> 1, else 2).
+struct Bypass {
+    bool luma, chroma;
+    uint32_t luma_modes[4];
+    int chroma_mode;
+};
+
+MMR_DEV int bypass_luma_mode(const Bypass& by, int k) {
+    const uint32_t lo = (k & 4) ? by.luma_modes[1] : by.luma_modes[0], hi = (k & 4) ? by.luma_modes[3] : by.luma_modes[2];
+    return (int)((((k & 8) ? hi : lo) >> (8 * (k & 3))) & 0xffu);
+}
+
+// transform_bypass_residual_4x4 (decoder.rs:14339-14380): levels placed by the zig-zag scan, then
+// vertical (mode 0) or horizontal (mode 1) running sums inside the block.
+MMR_DEV void bypass_residual_4x4(const int32_t* lv, int mode, int32_t out[16]) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) out[kZz4[i]] = lv[i];
+    if (mode == 0) {
+#pragma unroll
+        for (int r = 1; r < 4; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) out[r * 4 + c] += out[(r - 1) * 4 + c];
+    } else if (mode == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int c = 1; c < 4; ++c) out[r * 4 + c] += out[r * 4 + c - 1];
+    }
+}
+
 // Residual of one macroblock (phases 2-4 of the kernel) into S.res_y / S.res_c, in integer type T,
 // from its level record lv (global memory, or a copy in LDS).
 template <typename T, typename LV>
@@ -225,6 +259,47 @@ MMR_DEV void residual_phases(MbShared& S, LV lv, int qp, int cqp, bool t8, int l
 #pragma unroll
         for (int i = 0; i < 8; ++i)
             S.res_y[((blk8 >> 1) * 8 + i) * 16 + (blk8 & 1) * 8 + c] = (int32_t)(out[i] >> 6);
+    }
+    wave_lds_sync();
+}
+
+// Transform bypass after residual_phases (rare: a wave-uniform branch of its own, so the common
+// path keeps its registers): the bypassed residuals replace the transformed ones in S -- luma 4x4
+// blocks on lanes 0-15 (lane = block index), chroma blocks on lanes 32-39 (DC levels as they are
+// in coefficient 0), then continue_transform_bypass_chroma (decoder.rs:14382-14420): the lower /
+// right blocks of a component add the upper block's last row / the left block's last column.
+template <typename LV>
+MMR_DEV void bypass_phases(MbShared& S, LV lv, int l, const Bypass& by) {
+    if (by.luma && l < 16) {
+        int32_t lev[16], res[16];
+        const int kbx = ((l >> 2) & 1) * 2 + (l & 1), kby = (l >> 3) * 2 + ((l >> 1) & 1);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) lev[i] = lv[l * 16 + i];
+        bypass_residual_4x4(lev, bypass_luma_mode(by, l), res);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) S.res_y[(kby * 4 + (i >> 2)) * 16 + kbx * 4 + (i & 3)] = res[i];
+    }
+    if (by.chroma && l >= 32 && l < 40) {
+        const int comp = (l - 32) >> 2, b = (l - 32) & 3;
+        int32_t co[16], res[16];
+        co[0] = lv[256 + comp * 4 + b];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
+        bypass_residual_4x4(co, by.chroma_mode, res);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) S.res_c[comp][((b >> 1) * 4 + (i >> 2)) * 8 + (b & 1) * 4 + (i & 3)] = res[i];
+    }
+    wave_lds_sync();
+    if (by.chroma && by.chroma_mode != 2 && l < 16) {
+        const int comp = l >> 3, j = l & 7;
+        int32_t* rc = S.res_c[comp];
+        if (by.chroma_mode == 0) {
+            const int32_t carry = rc[3 * 8 + j];
+            for (int r = 4; r < 8; ++r) rc[r * 8 + j] += carry;
+        } else {
+            const int32_t carry = rc[j * 8 + 3];
+            for (int c = 4; c < 8; ++c) rc[j * 8 + c] += carry;
+        }
     }
     wave_lds_sync();
 }

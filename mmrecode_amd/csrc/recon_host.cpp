@@ -687,6 +687,8 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         j.band_begin = per_frame ? 0 : bands;
         bands += j.deblock ? (p.height_mbs + band_rows - 1) / band_rows : 0;
         j.intra = intra;
+        constexpr size_t kBypassEnd = offsetof(mmr_recon_h264_picture, transform_bypass) + sizeof(uint32_t);
+        j.transform_bypass = p.struct_size >= kBypassEnd && p.transform_bypass ? 1u : 0u;
         j.intra_band_begin = per_frame ? 0 : intra_bands;
         intra_bands += intra ? (p.height_mbs + intra_rows_per_band - 1) / intra_rows_per_band : 0;
     }

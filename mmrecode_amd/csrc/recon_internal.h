@@ -94,7 +94,7 @@ struct H264Job {
     uint32_t* bs;                 // [macroblocks][4] packed boundary strengths (submission scratch)
     const mmr_recon_h264_intra* intra;   // [macroblocks] or NULL (intra macroblocks unsupported)
     uint32_t intra_band_begin;    // first intra band ticket of this picture (bands only with intra records)
-    uint32_t intra_pad;
+    uint32_t transform_bypass;    // qpprime_y_zero_transform_bypass_flag
 };
 
 // ---------------------------------------------------------------- colour / blend

@@ -50,6 +50,7 @@ def lib():
             "oracle_dv_cos16_table": (C.POINTER(C.c_double), []),
             "oracle_dv_cos8_table": (C.POINTER(C.c_double), []),
             "oracle_h264_transform_residual_4x4": (i32, [dp, i32, i32, dp, dp]),
+            "oracle_h264_bypass_residual_4x4": (None, [dp, i32, dp]),
             "oracle_h264_transform_residual_8x8": (i32, [dp, i32, dp, dp]),
             "oracle_h264_transform_chroma_dc": (i32, [dp, i32, dp, dp]),
             "oracle_h264_chroma_qp": (i32, [i32, i32]),

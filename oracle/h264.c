@@ -333,6 +333,9 @@ static int predict_block(const mmr_recon_h264_picture *p, int mb, int k) {
     return 0;
 }
 
+void oracle_h264_bypass_residual_4x4(const int32_t levels[16], int intra_mode, int32_t out[16]);
+static void bypass_chroma(const int16_t *lv, int comp, int mode, int32_t res[4][16]);
+
 /* decode_inter_residuals' reconstruction half, decoder.rs:8097-8173: luma 4x4 blocks
  * (add_luma_residual_blocks :12630-12682, scaling list 3) or 8x8 blocks
  * (add_luma_residual_8x8_blocks :12684-12714, 8x8 list 1), then chroma
@@ -356,13 +359,22 @@ static int add_residuals(const mmr_recon_h264_picture *p, int mb) {
     } else {
         for (int k = 0; k < 16; ++k) {
             for (int i = 0; i < 16; ++i) levels[i] = lv[k * 16 + i];
-            if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 3), res)) return -2;
+            if (p->transform_bypass && qp == 0) oracle_h264_bypass_residual_4x4(levels, 2, res);
+            else if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 3), res)) return -2;
             int bx, by;
             luma_block_position(k, &bx, &by);
             add_residual(p->planes[0], W, mbx * 16 + bx * 4, mby * 16 + by * 4, 4, res);
         }
     }
     int cqp = oracle_h264_chroma_qp(qp, p->chroma_qp_offset[0]);
+    if (p->transform_bypass && cqp == 0) {
+        for (int comp = 0; comp < 2; ++comp) {
+            int32_t br[4][16];
+            bypass_chroma(lv, comp, 2, br);
+            for (int b = 0; b < 4; ++b) add_residual(p->planes[1 + comp], CW, mbx * 8 + (b % 2) * 4, mby * 8 + (b / 2) * 4, 4, br[b]);
+        }
+        return 0;
+    }
     for (int comp = 0; comp < 2; ++comp) {
         const uint8_t *sl = scaling4(p, 3 + comp + 1);
         int32_t dc_levels[4], dc[4];
@@ -377,6 +389,44 @@ static int add_residuals(const mmr_recon_h264_picture *p, int mb) {
         }
     }
     return 0;
+}
+
+/* transform_bypass_residual_4x4, decoder.rs:14339-14380: levels placed by the zig-zag scan, then
+ * for intra_mode 0 (vertical) each row adds the one above, for 1 (horizontal) each column the one
+ * to its left; 2: placement only. */
+void oracle_h264_bypass_residual_4x4(const int32_t levels[16], int intra_mode, int32_t out[16]) {
+    for (int i = 0; i < 16; ++i) out[ZZ4[i][0] * 4 + ZZ4[i][1]] = levels[i];
+    if (intra_mode == 0)
+        for (int r = 1; r < 4; ++r)
+            for (int c = 0; c < 4; ++c) out[r * 4 + c] += out[(r - 1) * 4 + c];
+    if (intra_mode == 1)
+        for (int r = 0; r < 4; ++r)
+            for (int c = 1; c < 4; ++c) out[r * 4 + c] += out[r * 4 + c - 1];
+}
+
+/* The chroma residual of one component with the transform bypassed (add_chroma_residual's bypass
+ * branch, decoder.rs:12739-12764): DC levels taken as they are, each 4x4 block through
+ * oracle_h264_bypass_residual_4x4 with the mode of the chroma intra prediction (1 horizontal -> 1,
+ * 2 vertical -> 0, otherwise and for inter 2), then continue_transform_bypass_chroma
+ * (:14382-14420): the lower blocks add the upper block's last row (vertical), the right blocks
+ * the left block's last column (horizontal). res[b] in raster order per block. */
+static void bypass_chroma(const int16_t *lv, int comp, int mode, int32_t res[4][16]) {
+    for (int b = 0; b < 4; ++b) {
+        int32_t co[16];
+        co[0] = lv[256 + comp * 4 + b];
+        for (int i = 1; i < 16; ++i) co[i] = lv[264 + (comp * 4 + b) * 15 + (i - 1)];
+        oracle_h264_bypass_residual_4x4(co, mode, res[b]);
+    }
+    if (mode == 0)
+        for (int b = 2; b < 4; ++b)
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) res[b][r * 4 + c] += res[b - 2][12 + c];
+    if (mode == 1)
+        for (int b = 1; b < 4; b += 2)
+            for (int r = 0; r < 4; ++r) {
+                const int32_t carry = res[b - 1][r * 4 + 3];
+                for (int c = 0; c < 4; ++c) res[b][r * 4 + c] += carry;
+            }
 }
 
 static void place(uint8_t *plane, int stride, int ox, int oy, int n, const uint8_t *v) {
@@ -425,7 +475,8 @@ static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
             place(p->planes[0], W, ox, oy, 4, pred);
             if (lv) {
                 for (int i = 0; i < 16; ++i) levels[i] = lv[k * 16 + i];
-                if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 0), res)) return -2;
+                if (p->transform_bypass && qp == 0) oracle_h264_bypass_residual_4x4(levels, in->modes[k], res);
+                else if (oracle_h264_transform_residual_4x4(levels, qp, 0, scaling4(p, 0), res)) return -2;
                 add_residual(p->planes[0], W, ox, oy, 4, res);
             }
         }
@@ -466,6 +517,16 @@ static int reconstruct_intra(const mmr_recon_h264_picture *p, int mb) {
     }
     if (lv) {
         const int cqp = oracle_h264_chroma_qp(qp, p->chroma_qp_offset[0]);
+        if (p->transform_bypass && cqp == 0) {
+            const int mode = in->chroma_mode == 1 ? 1 : (in->chroma_mode == 2 ? 0 : 2);
+            for (int comp = 0; comp < 2; ++comp) {
+                int32_t br[4][16];
+                bypass_chroma(lv, comp, mode, br);
+                for (int b = 0; b < 4; ++b)
+                    add_residual(p->planes[1 + comp], CW, mbx * 8 + (b % 2) * 4, mby * 8 + (b / 2) * 4, 4, br[b]);
+            }
+            return 0;
+        }
         for (int comp = 0; comp < 2; ++comp) {
             const uint8_t *sl = scaling4(p, comp + 1);
             int32_t dc_levels[4], dc[4];

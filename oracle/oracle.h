@@ -94,6 +94,7 @@ const double *oracle_dv_cos16_table(void); /* [x*8+u] = cos((2x+1)u pi/16) */
 const double *oracle_dv_cos8_table(void);  /* [y*4+v] = cos((2y+1)v pi/8) */
 
 /* H.264 (crates/codecs/h264/src/decoder.rs, deblock.rs) */
+void oracle_h264_bypass_residual_4x4(const int32_t levels[16], int intra_mode, int32_t out[16]);
 int oracle_h264_transform_residual_4x4(const int32_t levels[16], int qp, int dc_already_scaled,
                                        const uint8_t scaling[16], int32_t out[16]);
 int oracle_h264_transform_residual_8x8(const int32_t levels[64], int qp,
