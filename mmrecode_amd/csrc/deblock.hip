@@ -272,17 +272,22 @@ hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uin
 }
 
 // Bounded wait on a workgroup progress counter: gives up (and reports) after about a second, so a
-// protocol error surfaces as a status instead of a hung kernel.
+// protocol error surfaces as a status instead of a hung kernel. Rows of a band hand over only
+// through LDS (the ring and these counters): a row reads no sample another row of its band wrote
+// in global memory (its own macroblocks come from earlier kernels, the band above's rows through
+// the agent-scope hand-over), so the acquire and the matching release cover LDS only and the
+// release does not wait for the write-back of the picture.
 constexpr uint32_t kDbSpinLimit = 1u << 25;
 MMR_DEV bool db_wait(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
         if (++spins > kDbSpinLimit) {
             atomicOr(status, kDevStatusTimeout);
             return false;
         }
         __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     return true;
 }
 
@@ -477,7 +482,8 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             }
         }
         wave_sync();
-        if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (gself && ((x + 1) % publish == 0 || x == MW - 1))
             __hip_atomic_store(gself, (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         // shift the window left by one macroblock (its right 4 columns become the left context)
