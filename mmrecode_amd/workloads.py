@@ -397,13 +397,15 @@ def _intra_records(rng, intra, slice_id, width_mbs, type_weights):
 
 def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=0.4, deblock=True,
                  transform8_fraction=0.5, intra_fraction=0.0, mv_range=512, slices=1,
-                 intra_types=(0.4, 0.25, 0.25, 0.1), transform_bypass=False, qp0_fraction=0.0):
+                 intra_types=(0.4, 0.25, 0.25, 0.1), transform_bypass=False, qp0_fraction=0.0,
+                 scaling_lists=False):
     """One picture. References are supplied at fill time through extra pointers
     'ref{list}_{r}_{y|cb|cr}' (4 per list); POCs: list0 = [4, 2, 0, 6], list1 = [6, 8, 4, 2].
     intra_fraction of the macroblocks are intra, of the types Intra4x4 / Intra8x8 / Intra16x16 /
     I_PCM in the proportions intra_types; slices > 1 cuts the picture into that many raster
     slices (intra neighbours end at slice boundaries). transform_bypass sets
-    qpprime_y_zero_transform_bypass_flag and qp0_fraction of the macroblocks get QP 0."""
+    qpprime_y_zero_transform_bypass_flag and qp0_fraction of the macroblocks get QP 0;
+    scaling_lists gives the picture random scaling lists ([6][16] then [2][64], values 4..63)."""
     rng = np.random.Generator(np.random.PCG64([seed, 3, width_mbs, height_mbs, index]))
     n_mb = width_mbs * height_mbs
     mbs = np.zeros(n_mb, MB_DTYPE)
@@ -504,7 +506,7 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
     def fill(p):
         s = abi.new(abi.H264Picture, width_mbs=width_mbs, height_mbs=height_mbs, weighted_mode=mode,
                     deblock=int(deblock), macroblocks=p["mbs"], blocks=p["blocks"], levels=p["levels"],
-                    weights=p["weights"], bi_weights=p["bi_weights"], scaling_lists=None,
+                    weights=p["weights"], bi_weights=p["bi_weights"], scaling_lists=p.get("scaling"),
                     intra=p.get("intra"), transform_bypass=int(transform_bypass))
         s.planes[0], s.planes[1], s.planes[2] = p["y"], p["cb"], p["cr"]
         s.chroma_qp_offset[0] = s.chroma_qp_offset[1] = 0
@@ -521,6 +523,8 @@ def h264_picture(seed, index, width_mbs, height_mbs, weighted=True, bi_fraction=
     inputs = {"mbs": mbs, "blocks": blocks, "levels": levels, "weights": weights, "bi_weights": bi_w}
     if intra_rec is not None and intra.any():
         inputs["intra"] = intra_rec
+    if scaling_lists:   # drawn last, so the other draws do not depend on it
+        inputs["scaling"] = rng.integers(4, 64, 6 * 16 + 2 * 64, dtype=np.uint8)
     return Unit("h264", inputs,
                 outputs, fill, {"width_mbs": width_mbs, "height_mbs": height_mbs, "ref_ids": ref_ids,
                                 "coded_mbs": int(coded.size), "intra_mbs": int(intra.sum())})

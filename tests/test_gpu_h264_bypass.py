@@ -1,7 +1,8 @@
 """H.264 transform bypass on the device: the lossless round-trip pictures reconstruct to their
 targets (inter 4x4 luma and chroma; Intra4x4 with the DPCM of vertical / horizontal modes and the
 chroma continuation), and random pictures with QP 0 macroblocks of every type, with and without
-the flag, are bit-exact against the oracle."""
+the flag, are bit-exact against the oracle. Also pictures with their own scaling lists (every
+other test uses Flat_16)."""
 import ctypes as C
 
 import numpy as np
@@ -58,13 +59,16 @@ def test_lossless_round_trip_on_device(ctx, oracle, kind):
     dict(wmb=9, hmb=7, intra=1.0, flag=True, deblock=False, types=(1, 0, 0, 0)),
     dict(wmb=12, hmb=5, intra=0.0, flag=True, deblock=True),
     dict(wmb=9, hmb=7, intra=0.5, flag=False, deblock=True),
+    dict(wmb=9, hmb=7, intra=0.5, flag=False, deblock=True, scaling=True, qp0=0.0),
+    dict(wmb=9, hmb=7, intra=1.0, flag=True, deblock=True, scaling=True),
 ])
 def test_bypass_pictures_bit_exact(ctx, oracle, case):
     wmb, hmb = case["wmb"], case["hmb"]
     host_refs, dev_refs = _h264_refs(ctx, wmb, hmb)
     units = [wl.h264_picture(seed=61, index=i, width_mbs=wmb, height_mbs=hmb, intra_fraction=case["intra"],
                              deblock=case["deblock"], intra_types=case.get("types", (0.4, 0.25, 0.25, 0.1)),
-                             transform_bypass=case["flag"], qp0_fraction=0.5) for i in range(3)]
+                             transform_bypass=case["flag"], qp0_fraction=case.get("qp0", 0.5),
+                             scaling_lists=case.get("scaling", False)) for i in range(3)]
     dptrs = [{n: b.ptr for n, b in r.items()} for r in dev_refs]
     dev = [wl.DeviceUnit(ctx, u, wl.h264_ref_ptrs(u, dptrs, "dev")) for u in units]
     try:
