@@ -6,7 +6,10 @@
 // through an event per ticket. Errors follow crates/capi: status codes plus a thread-local
 // message (crates/capi/src/lib.rs:46-48, :305-321). Kernels report unsupported or invalid records
 // found on the device through a per-ticket status word that mmr_recon_wait / mmr_recon_poll
-// return.
+// return. No C++ exception leaves the library: every export runs its body inside `guarded`, which
+// turns an exception (std::bad_alloc from a container, std::system_error from a thread) into
+// MMR_STATUS_INTERNAL_ERROR with the message set, as crates/capi wraps every export in
+// catch_unwind (crates/capi/src/lib.rs:572-588).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,20 +32,43 @@
 
 namespace {
 
-thread_local std::string g_last_error;
+// The thread's last error message: a fixed buffer, so that recording an error never allocates
+// (and never throws, also while an out-of-memory exception is being turned into a status).
+thread_local char g_last_error[512];
 
 mmr_status fail(mmr_status status, const char* fmt, ...) {
-    char buf[512];
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    vsnprintf(g_last_error, sizeof g_last_error, fmt, ap);
     va_end(ap);
-    g_last_error = buf;
     return status;
 }
 
 mmr_status hip_fail(hipError_t e, const char* what) {
     return fail(MMR_STATUS_INTERNAL_ERROR, "%s: %s", what, hipGetErrorString(e));
+}
+
+// Runs an export's body; an exception becomes MMR_STATUS_INTERNAL_ERROR with the message set.
+template <typename F>
+mmr_status guarded(const char* what, F&& body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        return fail(MMR_STATUS_INTERNAL_ERROR, "%s: out of host memory", what);
+    } catch (const std::exception& e) {
+        return fail(MMR_STATUS_INTERNAL_ERROR, "%s: internal error: %s", what, e.what());
+    } catch (...) {
+        return fail(MMR_STATUS_INTERNAL_ERROR, "%s: internal error (unknown exception)", what);
+    }
+}
+
+// Same for the exports without a status (destroy calls): the error is only recorded.
+template <typename F>
+void guarded_void(const char* what, F&& body) noexcept {
+    (void)guarded(what, [&]() -> mmr_status {
+        body();
+        return MMR_STATUS_OK;
+    });
 }
 
 struct TicketRecord {
@@ -1117,249 +1143,289 @@ extern "C" {
 uint32_t mmr_recon_abi_version(void) { return MMR_RECON_ABI_VERSION; }
 
 size_t mmr_recon_last_error_message(char* buffer, size_t capacity) {
-    const size_t need = g_last_error.size() + 1;
+    const size_t len = strnlen(g_last_error, sizeof g_last_error - 1);
     if (buffer && capacity) {
-        const size_t n = std::min(capacity - 1, g_last_error.size());
-        memcpy(buffer, g_last_error.data(), n);
+        const size_t n = std::min(capacity - 1, len);
+        memcpy(buffer, g_last_error, n);
         buffer[n] = 0;
     }
-    return need;
+    return len + 1;
 }
 
 mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_context** out_context) {
-    if (!out_context) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_context is NULL");
-    *out_context = nullptr;
-    mmr_recon_config cfg;
-    memset(&cfg, 0, sizeof cfg);
-    if (config) {
-        if (config->struct_size < sizeof(mmr_recon_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "config struct_size too small");
-        cfg = *config;
-    }
-    int count = 0;
-    hipError_t e = hipGetDeviceCount(&count);
-    if (e != hipSuccess || count == 0) return fail(MMR_STATUS_UNSUPPORTED, "no HIP device is available");
-    if (cfg.device_ordinal < 0 || cfg.device_ordinal >= count)
-        return fail(MMR_STATUS_INVALID_ARGUMENT, "device ordinal %d out of range (%d devices)", cfg.device_ordinal, count);
-    auto* ctx = new mmr_recon_context();
-    ctx->device = cfg.device_ordinal;
-    ctx->queue_depth = cfg.queue_depth ? cfg.queue_depth : 256;
-    ctx->persistent_groups = cfg.persistent_groups;
-    ctx->tuning = mmr_host::default_tuning();
-    e = hipSetDevice(ctx->device);
-    hipDeviceProp_t prop;
-    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
-    if (e == hipSuccess) ctx->cu_count = (uint32_t)prop.multiProcessorCount;
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = mmr::dv_upload_tables(ctx->stream);
-    if (e == hipSuccess) e = hipMalloc((void**)&ctx->flat_scaling, 96 + 128);
-    if (e == hipSuccess) {
-        uint8_t flat[224];
-        memset(flat, 16, sizeof flat);
-        e = hipMemcpy(ctx->flat_scaling, flat, sizeof flat, hipMemcpyHostToDevice);
-    }
-    if (e != hipSuccess) {
-        mmr_status s = hip_fail(e, "context setup");
-        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-        if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
-        delete ctx;
-        return s;
-    }
-    *out_context = ctx;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_context_create", [&]() -> mmr_status {
+        if (!out_context) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_context is NULL");
+        *out_context = nullptr;
+        mmr_recon_config cfg;
+        memset(&cfg, 0, sizeof cfg);
+        if (config) {
+            if (config->struct_size < sizeof(mmr_recon_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "config struct_size too small");
+            cfg = *config;
+        }
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) return fail(MMR_STATUS_UNSUPPORTED, "no HIP device is available");
+        if (cfg.device_ordinal < 0 || cfg.device_ordinal >= count)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "device ordinal %d out of range (%d devices)", cfg.device_ordinal, count);
+        auto* ctx = new mmr_recon_context();
+        ctx->device = cfg.device_ordinal;
+        ctx->queue_depth = cfg.queue_depth ? cfg.queue_depth : 256;
+        ctx->persistent_groups = cfg.persistent_groups;
+        ctx->tuning = mmr_host::default_tuning();
+        e = hipSetDevice(ctx->device);
+        hipDeviceProp_t prop;
+        if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
+        if (e == hipSuccess) ctx->cu_count = (uint32_t)prop.multiProcessorCount;
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = mmr::dv_upload_tables(ctx->stream);
+        if (e == hipSuccess) e = hipMalloc((void**)&ctx->flat_scaling, 96 + 128);
+        if (e == hipSuccess) {
+            uint8_t flat[224];
+            memset(flat, 16, sizeof flat);
+            e = hipMemcpy(ctx->flat_scaling, flat, sizeof flat, hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) {
+            mmr_status s = hip_fail(e, "context setup");
+            if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+            if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+            delete ctx;
+            return s;
+        }
+        *out_context = ctx;
+        return MMR_STATUS_OK;
+    });
 }
 
 void mmr_recon_context_destroy(mmr_recon_context* ctx) {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->tickets) {
-        if (kv.second.start) (void)hipEventDestroy(kv.second.start);
-        if (kv.second.kernels_done) (void)hipEventDestroy(kv.second.kernels_done);
-        if (kv.second.done) (void)hipEventDestroy(kv.second.done);
-        if (kv.second.host_status) (void)hipHostFree(kv.second.host_status);
-    }
-    if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    guarded_void("mmr_recon_context_destroy", [&]() {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto& kv : ctx->tickets) {
+            if (kv.second.start) (void)hipEventDestroy(kv.second.start);
+            if (kv.second.kernels_done) (void)hipEventDestroy(kv.second.kernels_done);
+            if (kv.second.done) (void)hipEventDestroy(kv.second.done);
+            if (kv.second.host_status) (void)hipHostFree(kv.second.host_status);
+        }
+        if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
 }
 
 mmr_status mmr_recon_device_alloc(mmr_recon_context* ctx, size_t bytes, void** out) {
-    CHECK_CTX(ctx);
-    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out pointer is NULL");
-    LOCK_AND_SET_DEVICE(ctx);
-    hipError_t e = hipMalloc(out, bytes ? bytes : 1);
-    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipMalloc");
+    return guarded("mmr_recon_device_alloc", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out pointer is NULL");
+        LOCK_AND_SET_DEVICE(ctx);
+        hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+        return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipMalloc");
+    });
 }
 
 mmr_status mmr_recon_device_free(mmr_recon_context* ctx, void* p) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e == hipSuccess && p) e = hipFree(p);
-    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipFree");
+    return guarded("mmr_recon_device_free", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e == hipSuccess && p) e = hipFree(p);
+        return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "hipFree");
+    });
 }
 
 mmr_status mmr_recon_copy_to_device(mmr_recon_context* ctx, void* dst, const void* src, size_t bytes) {
-    CHECK_CTX(ctx);
-    if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
-    LOCK_AND_SET_DEVICE(ctx);
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to device");
+    return guarded("mmr_recon_copy_to_device", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
+        LOCK_AND_SET_DEVICE(ctx);
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to device");
+    });
 }
 
 mmr_status mmr_recon_copy_to_host(mmr_recon_context* ctx, void* dst, const void* src, size_t bytes) {
-    CHECK_CTX(ctx);
-    if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
-    LOCK_AND_SET_DEVICE(ctx);
-    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to host");
+    return guarded("mmr_recon_copy_to_host", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if ((!dst || !src) && bytes) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL copy pointer");
+        LOCK_AND_SET_DEVICE(ctx);
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        return e == hipSuccess ? MMR_STATUS_OK : hip_fail(e, "copy to host");
+    });
 }
 
 mmr_status mmr_recon_wait(mmr_recon_context* ctx, uint64_t ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    auto it = ctx->tickets.find(ticket);
-    if (it == ctx->tickets.end()) {
-        if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
-        return MMR_STATUS_OK;  // already retired
-    }
-    mmr_status s = finish(ctx, it->second);
-    it->second.reported = true;
-    return s;
+    return guarded("mmr_recon_wait", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        auto it = ctx->tickets.find(ticket);
+        if (it == ctx->tickets.end()) {
+            if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+            return MMR_STATUS_OK;  // already retired
+        }
+        mmr_status s = finish(ctx, it->second);
+        it->second.reported = true;
+        return s;
+    });
 }
 
 mmr_status mmr_recon_poll(mmr_recon_context* ctx, uint64_t ticket, int32_t* out_done) {
-    CHECK_CTX(ctx);
-    if (!out_done) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_done is NULL");
-    LOCK_AND_SET_DEVICE(ctx);
-    auto it = ctx->tickets.find(ticket);
-    if (it == ctx->tickets.end()) {
-        if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+    return guarded("mmr_recon_poll", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out_done) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_done is NULL");
+        LOCK_AND_SET_DEVICE(ctx);
+        auto it = ctx->tickets.find(ticket);
+        if (it == ctx->tickets.end()) {
+            if (ticket == 0 || ticket >= ctx->next_ticket) return fail(MMR_STATUS_INVALID_ARGUMENT, "unknown ticket %llu", (unsigned long long)ticket);
+            *out_done = 1;
+            return MMR_STATUS_OK;
+        }
+        hipError_t q = hipEventQuery(it->second.done);
+        if (q == hipErrorNotReady) {
+            *out_done = 0;
+            return MMR_STATUS_OK;
+        }
         *out_done = 1;
-        return MMR_STATUS_OK;
-    }
-    hipError_t q = hipEventQuery(it->second.done);
-    if (q == hipErrorNotReady) {
-        *out_done = 0;
-        return MMR_STATUS_OK;
-    }
-    *out_done = 1;
-    it->second.reported = true;
-    return finish(ctx, it->second);
+        it->second.reported = true;
+        return finish(ctx, it->second);
+    });
 }
 
 mmr_status mmr_recon_synchronize(mmr_recon_context* ctx) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    // Errors already returned by wait / poll (or carried by server completions) are not repeated.
-    mmr_status worst = MMR_STATUS_OK;
-    for (auto& kv : ctx->tickets) {
-        mmr_status s = finish(ctx, kv.second);
-        if (s != MMR_STATUS_OK && worst == MMR_STATUS_OK && !kv.second.reported) worst = s;
-        kv.second.reported = true;
-    }
-    reap(ctx);
-    return worst;
+    return guarded("mmr_recon_synchronize", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        // Errors already returned by wait / poll (or carried by server completions) are not repeated.
+        mmr_status worst = MMR_STATUS_OK;
+        for (auto& kv : ctx->tickets) {
+            mmr_status s = finish(ctx, kv.second);
+            if (s != MMR_STATUS_OK && worst == MMR_STATUS_OK && !kv.second.reported) worst = s;
+            kv.second.reported = true;
+        }
+        reap(ctx);
+        return worst;
+    });
 }
 
 mmr_status mmr_recon_last_kernel_ms(mmr_recon_context* ctx, uint64_t ticket, double* out_ms) {
-    CHECK_CTX(ctx);
-    if (!out_ms) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_ms is NULL");
-    LOCK_AND_SET_DEVICE(ctx);
-    auto it = ctx->tickets.find(ticket);
-    if (it == ctx->tickets.end()) return fail(MMR_STATUS_INVALID_STATE, "ticket %llu is not tracked", (unsigned long long)ticket);
-    mmr_status s = finish(ctx, it->second);
-    *out_ms = it->second.kernel_ms;
-    return s;
+    return guarded("mmr_recon_last_kernel_ms", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out_ms) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_ms is NULL");
+        LOCK_AND_SET_DEVICE(ctx);
+        auto it = ctx->tickets.find(ticket);
+        if (it == ctx->tickets.end()) return fail(MMR_STATUS_INVALID_STATE, "ticket %llu is not tracked", (unsigned long long)ticket);
+        mmr_status s = finish(ctx, it->second);
+        *out_ms = it->second.kernel_ms;
+        return s;
+    });
 }
 
 // ------------------------------------------------------------------ MJPEG
 mmr_status mmr_recon_mjpeg_submit(mmr_recon_context* ctx, const mmr_recon_mjpeg_frame* frames, uint32_t count, uint32_t flags,
                                   uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::mjpeg_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
+    return guarded("mmr_recon_mjpeg_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::mjpeg_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
+    });
 }
 
 // ------------------------------------------------------------------ MPEG-2
 mmr_status mmr_recon_mpeg2_submit(mmr_recon_context* ctx, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
                                   uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::mpeg2_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
+    return guarded("mmr_recon_mpeg2_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::mpeg2_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
+    });
 }
 
 // ------------------------------------------------------------------ DV
 mmr_status mmr_recon_dv_submit(mmr_recon_context* ctx, const mmr_recon_dv_frame* frames, uint32_t count, uint32_t flags,
                                uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::dv_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
+    return guarded("mmr_recon_dv_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::dv_submit_on(ctx, ctx->stream, frames, count, flags, out_ticket);
+    });
 }
 
 // ------------------------------------------------------------------ H.264
 mmr_status mmr_recon_h264_implicit_weights(int32_t poc0, int32_t poc1, int32_t cur, int32_t lt0, int32_t lt1, int32_t* w0,
                                            int32_t* w1) {
-    // implicit_b_prediction_weights, crates/codecs/h264/src/decoder.rs:3577-3620.
-    if (!w0 || !w1) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL output");
-    int64_t td = std::clamp<int64_t>((int64_t)poc1 - poc0, -128, 127);
-    int32_t weight1 = 32;
-    if (!(lt0 || lt1 || td == 0)) {
-        const int64_t tb = std::clamp<int64_t>((int64_t)cur - poc0, -128, 127);
-        const int64_t tx = (16384 + (td < 0 ? -td : td) / 2) / td;
-        const int64_t scale = std::clamp<int64_t>((tb * tx + 32) >> 6, -1024, 1023);
-        const int64_t candidate = scale >> 2;
-        weight1 = (candidate >= -64 && candidate <= 128) ? (int32_t)candidate : 32;
-    }
-    *w0 = 64 - weight1;
-    *w1 = weight1;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_h264_implicit_weights", [&]() -> mmr_status {
+        // implicit_b_prediction_weights, crates/codecs/h264/src/decoder.rs:3577-3620.
+        if (!w0 || !w1) return fail(MMR_STATUS_INVALID_ARGUMENT, "NULL output");
+        int64_t td = std::clamp<int64_t>((int64_t)poc1 - poc0, -128, 127);
+        int32_t weight1 = 32;
+        if (!(lt0 || lt1 || td == 0)) {
+            const int64_t tb = std::clamp<int64_t>((int64_t)cur - poc0, -128, 127);
+            const int64_t tx = (16384 + (td < 0 ? -td : td) / 2) / td;
+            const int64_t scale = std::clamp<int64_t>((tb * tx + 32) >> 6, -1024, 1023);
+            const int64_t candidate = scale >> 2;
+            weight1 = (candidate >= -64 && candidate <= 128) ? (int32_t)candidate : 32;
+        }
+        *w0 = 64 - weight1;
+        *w1 = weight1;
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_h264_submit(mmr_recon_context* ctx, const mmr_recon_h264_picture* pics, uint32_t count, uint32_t flags,
                                  uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::h264_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
+    return guarded("mmr_recon_h264_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::h264_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
+    });
 }
 
 // ------------------------------------------------------------------ colour / blend
 mmr_status mmr_recon_yuv_to_rgb_submit(mmr_recon_context* ctx, const mmr_recon_yuv_to_rgb_job* in, uint32_t count, uint32_t flags,
                                        uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::yuv_to_rgb_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    return guarded("mmr_recon_yuv_to_rgb_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::yuv_to_rgb_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
 }
 
 mmr_status mmr_recon_composite_submit(mmr_recon_context* ctx, const mmr_recon_composite_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::composite_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    return guarded("mmr_recon_composite_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::composite_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
 }
 
 mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::yuv_blend_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    return guarded("mmr_recon_yuv_blend_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::yuv_blend_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
 }
 
 mmr_status mmr_recon_overlay_prepare_submit(mmr_recon_context* ctx, const mmr_recon_overlay_prepare_job* jobs, uint32_t count, uint32_t flags,
                                             uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::overlay_prepare_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
+    return guarded("mmr_recon_overlay_prepare_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::overlay_prepare_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
+    });
 }
 
 mmr_status mmr_recon_scale_submit(mmr_recon_context* ctx, const mmr_recon_scale_job* jobs, uint32_t count, uint32_t flags, uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    LOCK_AND_SET_DEVICE(ctx);
-    return mmr_host::scale_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
+    return guarded("mmr_recon_scale_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::scale_submit_on(ctx, ctx->stream, jobs, count, flags, out_ticket);
+    });
 }
 
 }  // extern "C"
@@ -1381,93 +1447,99 @@ extern "C" {
 
 mmr_status mmr_recon_prepare(mmr_recon_context* ctx, uint32_t codec, const void* descriptors, uint32_t count, uint32_t flags,
                              mmr_recon_prepared** out) {
-    CHECK_CTX(ctx);
-    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_prepared is NULL");
-    *out = nullptr;
-    if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
-    LOCK_AND_SET_DEVICE(ctx);
-    auto* p = new mmr_recon_prepared();
-    p->codec = codec;
-    hipError_t e = hipStreamCreateWithFlags(&p->cap.stream, hipStreamNonBlocking);   // default priority
-    if (e != hipSuccess) {
-        delete p;
-        return hip_fail(e, "capture stream");
-    }
-    mmr_status s;
-    switch (codec) {
-    case MMR_CODEC_MJPEG:
-        s = mmr_host::mjpeg_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, nullptr, &p->cap);
-        break;
-    case MMR_CODEC_MPEG2:
-        s = mmr_host::mpeg2_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, nullptr, &p->cap);
-        break;
-    case MMR_CODEC_DV:
-        s = mmr_host::dv_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, nullptr, &p->cap);
-        break;
-    case MMR_CODEC_H264:
-        s = mmr_host::h264_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, nullptr, &p->cap);
-        break;
-    default:
-        s = fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
-    }
-    if (s != MMR_STATUS_OK || !p->cap.exec) {
-        if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
-        if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
-        if (p->cap.d_block) (void)hipFree(p->cap.d_block);
-        (void)hipStreamDestroy(p->cap.stream);
-        delete p;
-        return s != MMR_STATUS_OK ? s : fail(MMR_STATUS_INTERNAL_ERROR, "graph capture produced no graph");
-    }
-    *out = p;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_prepare", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_prepared is NULL");
+        *out = nullptr;
+        if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
+        LOCK_AND_SET_DEVICE(ctx);
+        auto* p = new mmr_recon_prepared();
+        p->codec = codec;
+        hipError_t e = hipStreamCreateWithFlags(&p->cap.stream, hipStreamNonBlocking);   // default priority
+        if (e != hipSuccess) {
+            delete p;
+            return hip_fail(e, "capture stream");
+        }
+        mmr_status s;
+        switch (codec) {
+        case MMR_CODEC_MJPEG:
+            s = mmr_host::mjpeg_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, nullptr, &p->cap);
+            break;
+        case MMR_CODEC_MPEG2:
+            s = mmr_host::mpeg2_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, nullptr, &p->cap);
+            break;
+        case MMR_CODEC_DV:
+            s = mmr_host::dv_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, nullptr, &p->cap);
+            break;
+        case MMR_CODEC_H264:
+            s = mmr_host::h264_submit_on(ctx, p->cap.stream, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, nullptr, &p->cap);
+            break;
+        default:
+            s = fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
+        }
+        if (s != MMR_STATUS_OK || !p->cap.exec) {
+            if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
+            if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
+            if (p->cap.d_block) (void)hipFree(p->cap.d_block);
+            (void)hipStreamDestroy(p->cap.stream);
+            delete p;
+            return s != MMR_STATUS_OK ? s : fail(MMR_STATUS_INTERNAL_ERROR, "graph capture produced no graph");
+        }
+        *out = p;
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_prepared_launch(mmr_recon_context* ctx, mmr_recon_prepared* p, uint64_t* out_ticket) {
-    CHECK_CTX(ctx);
-    if (!p) return fail(MMR_STATUS_INVALID_ARGUMENT, "prepared batch is NULL");
-    LOCK_AND_SET_DEVICE(ctx);
-    if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
-    if (ctx->tickets.size() >= ctx->queue_depth) return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
-    TicketRecord rec;
-    hipError_t e = hipEventCreate(&rec.start);
-    if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
-    if (e == hipSuccess) e = hipEventCreate(&rec.done);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
-    if (e == hipSuccess) {
-        *rec.host_status = 0;
-        e = hipEventRecord(rec.start, ctx->stream);
-    }
-    if (e == hipSuccess) e = hipGraphLaunch(p->cap.exec, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(rec.kernels_done, ctx->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, p->cap.status, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(rec.done, ctx->stream);
-    if (e != hipSuccess) {
-        // the events may be in flight: wait for the stream before releasing them
-        (void)hipStreamSynchronize(ctx->stream);
-        if (rec.start) (void)hipEventDestroy(rec.start);
-        if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
-        if (rec.done) (void)hipEventDestroy(rec.done);
-        if (rec.host_status) (void)hipHostFree(rec.host_status);
-        return hip_fail(e, "prepared batch launch");
-    }
-    const uint64_t id = ctx->next_ticket++;
-    ctx->tickets.emplace(id, std::move(rec));
-    if (out_ticket) *out_ticket = id;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_prepared_launch", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!p) return fail(MMR_STATUS_INVALID_ARGUMENT, "prepared batch is NULL");
+        LOCK_AND_SET_DEVICE(ctx);
+        if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
+        if (ctx->tickets.size() >= ctx->queue_depth) return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
+        TicketRecord rec;
+        hipError_t e = hipEventCreate(&rec.start);
+        if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
+        if (e == hipSuccess) e = hipEventCreate(&rec.done);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
+        if (e == hipSuccess) {
+            *rec.host_status = 0;
+            e = hipEventRecord(rec.start, ctx->stream);
+        }
+        if (e == hipSuccess) e = hipGraphLaunch(p->cap.exec, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(rec.kernels_done, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, p->cap.status, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(rec.done, ctx->stream);
+        if (e != hipSuccess) {
+            // the events may be in flight: wait for the stream before releasing them
+            (void)hipStreamSynchronize(ctx->stream);
+            if (rec.start) (void)hipEventDestroy(rec.start);
+            if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
+            if (rec.done) (void)hipEventDestroy(rec.done);
+            if (rec.host_status) (void)hipHostFree(rec.host_status);
+            return hip_fail(e, "prepared batch launch");
+        }
+        const uint64_t id = ctx->next_ticket++;
+        ctx->tickets.emplace(id, std::move(rec));
+        if (out_ticket) *out_ticket = id;
+        return MMR_STATUS_OK;
+    });
 }
 
 void mmr_recon_prepared_destroy(mmr_recon_context* ctx, mmr_recon_prepared* p) {
-    if (!p) return;
-    if (ctx) {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        (void)hipSetDevice(ctx->device);
-        (void)hipStreamSynchronize(ctx->stream);   // no launch of this batch may still be running
-    }
-    if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
-    if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
-    if (p->cap.d_block) (void)hipFree(p->cap.d_block);
-    if (p->cap.stream) (void)hipStreamDestroy(p->cap.stream);
-    delete p;
+    guarded_void("mmr_recon_prepared_destroy", [&]() {
+        if (!p) return;
+        if (ctx) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            (void)hipSetDevice(ctx->device);
+            (void)hipStreamSynchronize(ctx->stream);   // no launch of this batch may still be running
+        }
+        if (p->cap.exec) (void)hipGraphExecDestroy(p->cap.exec);
+        if (p->cap.graph) (void)hipGraphDestroy(p->cap.graph);
+        if (p->cap.d_block) (void)hipFree(p->cap.d_block);
+        if (p->cap.stream) (void)hipStreamDestroy(p->cap.stream);
+        delete p;
+    });
 }
 
 }  // extern "C"
@@ -1477,109 +1549,115 @@ void mmr_recon_prepared_destroy(mmr_recon_context* ctx, mmr_recon_prepared* p) {
 extern "C" {
 
 mmr_status mmr_recon_get_tuning(mmr_recon_context* ctx, mmr_recon_tuning* out) {
-    CHECK_CTX(ctx);
-    if (!out || out->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    *out = ctx->tuning;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_get_tuning", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out || out->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        *out = ctx->tuning;
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_set_tuning(mmr_recon_context* ctx, const mmr_recon_tuning* in) {
-    CHECK_CTX(ctx);
-    if (!in || in->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
-    mmr_status s = mmr_host::check_tuning(*in);
-    if (s) return s;
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->tuning = *in;
-    ctx->tuning.struct_size = sizeof(mmr_recon_tuning);
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_set_tuning", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!in || in->struct_size < sizeof(mmr_recon_tuning)) return fail(MMR_STATUS_INVALID_ARGUMENT, "tuning struct_size too small");
+        mmr_status s = mmr_host::check_tuning(*in);
+        if (s) return s;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->tuning = *in;
+        ctx->tuning.struct_size = sizeof(mmr_recon_tuning);
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_autotune(mmr_recon_context* ctx, uint32_t codec, const void* descriptors, uint32_t count, uint32_t flags,
                               uint32_t repeats, mmr_recon_tuning* out_best, mmr_recon_tuning_trial* trials, uint32_t trial_capacity,
                               uint32_t* out_trials) {
-    CHECK_CTX(ctx);
-    if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
-    if (repeats == 0) repeats = 3;
-    mmr_recon_tuning base;
-    {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        base = ctx->tuning;
-    }
-    // Candidates: the codec's own fields varied, the others as they are.
-    std::vector<mmr_recon_tuning> cands;
-    const uint32_t groups[3] = {16, 32, 64};
-    switch (codec) {
-    case MMR_CODEC_MJPEG:
-    case MMR_CODEC_MPEG2:
-    case MMR_CODEC_DV:
-        for (uint32_t g : groups) {
-            mmr_recon_tuning t = base;
-            (codec == MMR_CODEC_MJPEG ? t.mjpeg_group_blocks : codec == MMR_CODEC_MPEG2 ? t.mpeg2_group_blocks : t.dv_group_blocks) = g;
-            cands.push_back(t);
-        }
-        break;
-    case MMR_CODEC_H264:
-        for (uint32_t rows : {4u, 8u, 16u})
-            for (uint32_t per_cu : {1u, 2u})
-                for (uint32_t publish : {2u, 8u}) {
-                    mmr_recon_tuning t = base;
-                    t.h264_deblock_band_rows = rows;
-                    t.h264_deblock_groups_per_cu = per_cu;
-                    t.h264_deblock_publish = publish;
-                    cands.push_back(t);
-                }
-        break;
-    default:
-        return fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
-    }
-    auto submit = [&](uint64_t* ticket) -> mmr_status {
-        switch (codec) {
-        case MMR_CODEC_MJPEG: return mmr_recon_mjpeg_submit(ctx, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, ticket);
-        case MMR_CODEC_MPEG2: return mmr_recon_mpeg2_submit(ctx, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, ticket);
-        case MMR_CODEC_DV: return mmr_recon_dv_submit(ctx, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, ticket);
-        default: return mmr_recon_h264_submit(ctx, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, ticket);
-        }
-    };
-    auto restore = [&]() {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        ctx->tuning = base;
-    };
-    size_t best = 0;
-    std::vector<double> medians(cands.size());
-    for (size_t c = 0; c < cands.size(); ++c) {
+    return guarded("mmr_recon_autotune", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!descriptors || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no descriptors");
+        if (repeats == 0) repeats = 3;
+        mmr_recon_tuning base;
         {
             std::lock_guard<std::mutex> lock(ctx->mu);
-            ctx->tuning = cands[c];
+            base = ctx->tuning;
         }
-        std::vector<double> ms;
-        for (uint32_t r = 0; r <= repeats; ++r) {
-            uint64_t t = 0;
-            mmr_status s = submit(&t);
-            if (s == MMR_STATUS_OK) s = mmr_recon_wait(ctx, t);
-            double k = 0.0;
-            if (s == MMR_STATUS_OK) s = mmr_recon_last_kernel_ms(ctx, t, &k);
-            if (s != MMR_STATUS_OK) {
-                restore();
-                return s;
+        // Candidates: the codec's own fields varied, the others as they are.
+        std::vector<mmr_recon_tuning> cands;
+        const uint32_t groups[3] = {16, 32, 64};
+        switch (codec) {
+        case MMR_CODEC_MJPEG:
+        case MMR_CODEC_MPEG2:
+        case MMR_CODEC_DV:
+            for (uint32_t g : groups) {
+                mmr_recon_tuning t = base;
+                (codec == MMR_CODEC_MJPEG ? t.mjpeg_group_blocks : codec == MMR_CODEC_MPEG2 ? t.mpeg2_group_blocks : t.dv_group_blocks) = g;
+                cands.push_back(t);
             }
-            if (r > 0) ms.push_back(k);   // the first run is untimed
+            break;
+        case MMR_CODEC_H264:
+            for (uint32_t rows : {4u, 8u, 16u})
+                for (uint32_t per_cu : {1u, 2u})
+                    for (uint32_t publish : {2u, 8u}) {
+                        mmr_recon_tuning t = base;
+                        t.h264_deblock_band_rows = rows;
+                        t.h264_deblock_groups_per_cu = per_cu;
+                        t.h264_deblock_publish = publish;
+                        cands.push_back(t);
+                    }
+            break;
+        default:
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "codec %u invalid", codec);
         }
-        std::sort(ms.begin(), ms.end());
-        medians[c] = ms[ms.size() / 2];
-        if (medians[c] < medians[best]) best = c;
-        if (trials && c < trial_capacity) {
-            trials[c].tuning = cands[c];
-            trials[c].kernel_ms = medians[c];
+        auto submit = [&](uint64_t* ticket) -> mmr_status {
+            switch (codec) {
+            case MMR_CODEC_MJPEG: return mmr_recon_mjpeg_submit(ctx, static_cast<const mmr_recon_mjpeg_frame*>(descriptors), count, flags, ticket);
+            case MMR_CODEC_MPEG2: return mmr_recon_mpeg2_submit(ctx, static_cast<const mmr_recon_mpeg2_picture*>(descriptors), count, flags, ticket);
+            case MMR_CODEC_DV: return mmr_recon_dv_submit(ctx, static_cast<const mmr_recon_dv_frame*>(descriptors), count, flags, ticket);
+            default: return mmr_recon_h264_submit(ctx, static_cast<const mmr_recon_h264_picture*>(descriptors), count, flags, ticket);
+            }
+        };
+        auto restore = [&]() {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = base;
+        };
+        size_t best = 0;
+        std::vector<double> medians(cands.size());
+        for (size_t c = 0; c < cands.size(); ++c) {
+            {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                ctx->tuning = cands[c];
+            }
+            std::vector<double> ms;
+            for (uint32_t r = 0; r <= repeats; ++r) {
+                uint64_t t = 0;
+                mmr_status s = submit(&t);
+                if (s == MMR_STATUS_OK) s = mmr_recon_wait(ctx, t);
+                double k = 0.0;
+                if (s == MMR_STATUS_OK) s = mmr_recon_last_kernel_ms(ctx, t, &k);
+                if (s != MMR_STATUS_OK) {
+                    restore();
+                    return s;
+                }
+                if (r > 0) ms.push_back(k);   // the first run is untimed
+            }
+            std::sort(ms.begin(), ms.end());
+            medians[c] = ms[ms.size() / 2];
+            if (medians[c] < medians[best]) best = c;
+            if (trials && c < trial_capacity) {
+                trials[c].tuning = cands[c];
+                trials[c].kernel_ms = medians[c];
+            }
         }
-    }
-    {
-        std::lock_guard<std::mutex> lock(ctx->mu);
-        ctx->tuning = cands[best];
-    }
-    if (out_best) *out_best = cands[best];
-    if (out_trials) *out_trials = (uint32_t)cands.size();
-    return MMR_STATUS_OK;
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = cands[best];
+        }
+        if (out_best) *out_best = cands[best];
+        if (out_trials) *out_trials = (uint32_t)cands.size();
+        return MMR_STATUS_OK;
+    });
 }
 
 }  // extern "C"
@@ -1745,10 +1823,33 @@ mmr_status launch_batch(mmr_recon_server* s, int slot, InflightBatch& b, std::ve
     return MMR_STATUS_OK;
 }
 
-void server_loop(mmr_recon_server* s) {
-    std::unique_lock<std::mutex> lk(s->mu);
-    size_t rr = 0;
-    for (;;) {
+// Fails every frame of the active and pending requests that no batch carries yet (after an
+// exception inside the loop); frames already in flight complete normally.
+void fail_unplaced(mmr_recon_server* s, mmr_status st) noexcept {
+    try {
+        for (auto& r : s->active) {
+            while (r->next < r->frame_count) {
+                s->completions.push_back(mmr_recon_frame_completion{r->id, r->tag, r->next, st, 0.0});
+                r->next++;
+                r->done++;
+                if (!r->status) r->status = st;
+            }
+        }
+        for (auto& r : s->pending) {
+            while (r->next < r->frame_count) {
+                s->completions.push_back(mmr_recon_frame_completion{r->id, r->tag, r->next, st, 0.0});
+                r->next++;
+                r->done++;
+            }
+        }
+    } catch (...) {
+        // no memory even for the completions: the requests stay and the next step retries
+    }
+}
+
+// One scheduler step (retire, admit, launch, wait); returns true when the server stops.
+bool server_step(mmr_recon_server* s, std::unique_lock<std::mutex>& lk, size_t& rr) {
+    {
         bool progressed = false;
         // 1. retire completed batches (in launch order per slot; any order across slots)
         for (auto it = s->inflight.begin(); it != s->inflight.end();) {
@@ -1854,7 +1955,7 @@ void server_loop(mmr_recon_server* s) {
                     auto sl = s->stream_last.find(r->stream_id);
                     if (sl != s->stream_last.end()) r->last = sl->second;
                 }
-            mmr_status rc = launch_batch(s, slot, b, reqs);
+            mmr_status rc = guarded("server batch", [&]() -> mmr_status { return launch_batch(s, slot, b, reqs); });
             if (rc) {
                 for (auto& f : b.frames) {
                     mmr_recon_frame_completion c{f.first->id, f.first->tag, f.second, rc, 0.0};
@@ -1877,13 +1978,29 @@ void server_loop(mmr_recon_server* s) {
         s->stats.active = (uint32_t)s->active.size();
         if (s->pending.empty() && s->active.empty() && s->inflight.empty()) {
             s->idle_cv.notify_all();
-            if (s->stop) break;
+            if (s->stop) return true;
             s->cv.wait(lk);
-            continue;
+            return false;
         }
         if (!progressed) {
             if (s->inflight.empty()) s->cv.wait_for(lk, std::chrono::milliseconds(1));
             else s->cv.wait_for(lk, std::chrono::microseconds(20));
+        }
+    }
+    return false;
+}
+
+// The scheduler thread. An exception inside a step (host memory exhausted) fails the frames no
+// batch carries yet instead of ending the process; batches in flight still complete.
+void server_loop(mmr_recon_server* s) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    size_t rr = 0;
+    for (;;) {
+        try {
+            if (server_step(s, lk, rr)) break;
+        } catch (...) {
+            fail_unplaced(s, MMR_STATUS_INTERNAL_ERROR);
+            s->cv.wait_for(lk, std::chrono::milliseconds(1));
         }
     }
 }
@@ -1893,117 +2010,135 @@ void server_loop(mmr_recon_server* s) {
 extern "C" {
 
 mmr_status mmr_recon_server_create(mmr_recon_context* ctx, const mmr_recon_server_config* config, mmr_recon_server** out) {
-    CHECK_CTX(ctx);
-    if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_server is NULL");
-    *out = nullptr;
-    mmr_recon_server_config cfg{};
-    if (config) {
-        if (config->struct_size < sizeof(mmr_recon_server_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "server config struct_size too small");
-        cfg = *config;
-    }
-    if (!cfg.slots) cfg.slots = 4;
-    if (cfg.slots > 32) return fail(MMR_STATUS_INVALID_ARGUMENT, "at most 32 slots");
-    if (!cfg.max_batch_frames) cfg.max_batch_frames = 64;
-    if (!cfg.max_frames_per_request_per_batch) cfg.max_frames_per_request_per_batch = 8;
-    if (!cfg.max_pending) cfg.max_pending = 4096;
-    auto* s = new mmr_recon_server();
-    s->ctx = ctx;
-    s->cfg = cfg;
-    hipError_t e = hipSetDevice(ctx->device);
-    for (uint32_t i = 0; i < cfg.slots && e == hipSuccess; ++i) {
-        hipStream_t st = nullptr;
-        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
-        if (e == hipSuccess) s->streams.push_back(st);
-    }
-    if (e != hipSuccess) {
-        for (auto st : s->streams) (void)hipStreamDestroy(st);
-        delete s;
-        return hip_fail(e, "server streams");
-    }
-    s->slot_busy.assign(cfg.slots, false);
-    s->stats.struct_size = sizeof(mmr_recon_server_stats);
-    s->thread = std::thread(mmr_host::server_loop, s);
-    *out = s;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_server_create", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_server is NULL");
+        *out = nullptr;
+        mmr_recon_server_config cfg{};
+        if (config) {
+            if (config->struct_size < sizeof(mmr_recon_server_config)) return fail(MMR_STATUS_INVALID_ARGUMENT, "server config struct_size too small");
+            cfg = *config;
+        }
+        if (!cfg.slots) cfg.slots = 4;
+        if (cfg.slots > 32) return fail(MMR_STATUS_INVALID_ARGUMENT, "at most 32 slots");
+        if (!cfg.max_batch_frames) cfg.max_batch_frames = 64;
+        if (!cfg.max_frames_per_request_per_batch) cfg.max_frames_per_request_per_batch = 8;
+        if (!cfg.max_pending) cfg.max_pending = 4096;
+        auto* s = new mmr_recon_server();
+        s->ctx = ctx;
+        s->cfg = cfg;
+        hipError_t e = hipSetDevice(ctx->device);
+        for (uint32_t i = 0; i < cfg.slots && e == hipSuccess; ++i) {
+            hipStream_t st = nullptr;
+            e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);   // default priority
+            if (e == hipSuccess) s->streams.push_back(st);
+        }
+        if (e != hipSuccess) {
+            for (auto st : s->streams) (void)hipStreamDestroy(st);
+            delete s;
+            return hip_fail(e, "server streams");
+        }
+        s->slot_busy.assign(cfg.slots, false);
+        s->stats.struct_size = sizeof(mmr_recon_server_stats);
+        try {
+            s->thread = std::thread(mmr_host::server_loop, s);
+        } catch (...) {
+            for (auto st : s->streams) (void)hipStreamDestroy(st);
+            delete s;
+            throw;
+        }
+        *out = s;
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_server_enqueue(mmr_recon_server* s, const mmr_recon_request* req, uint64_t* out_id) {
-    if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
-    if (!req || req->struct_size < sizeof(mmr_recon_request)) return fail(MMR_STATUS_INVALID_ARGUMENT, "request invalid");
-    const size_t sz = mmr_host::codec_struct_size(req->codec);
-    if (!sz) return fail(MMR_STATUS_INVALID_ARGUMENT, "request codec %u invalid", req->codec);
-    if (!req->frame_count || !req->frames) return fail(MMR_STATUS_INVALID_ARGUMENT, "request has no frames");
-    auto r = std::make_unique<mmr_host::ServerRequest>();
-    r->codec = req->codec;
-    r->stream_id = req->stream_id;
-    r->frame_count = req->frame_count;
-    r->hbm = req->hbm_bytes;
-    r->tag = req->user_tag;
-    r->frames.assign((const char*)req->frames, (const char*)req->frames + sz * req->frame_count);
-    if (req->codec == MMR_CODEC_MPEG2) {
-        auto* pics = reinterpret_cast<const mmr_recon_mpeg2_picture*>(req->frames);
-        for (uint32_t i = 0; i < req->frame_count; ++i)
-            if (pics[i].forward_source >= (int32_t)i || pics[i].backward_source >= (int32_t)i)
-                return fail(MMR_STATUS_INVALID_ARGUMENT, "request picture %u references a later picture", i);
-    }
-    if (req->convert) r->convert.assign(req->convert, req->convert + req->frame_count);
-    if (req->composite) r->composite.assign(req->composite, req->composite + req->frame_count);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) return fail(MMR_STATUS_INVALID_STATE, "server is shutting down");
-        if (s->pending.size() >= s->cfg.max_pending) return fail(MMR_STATUS_BUSY, "server queue full (%u pending)", s->cfg.max_pending);
-        r->id = s->next_id++;
-        r->t0 = mmr_host::Clock::now();
-        if (out_id) *out_id = r->id;
-        s->pending.push_back(std::move(r));
-        s->stats.requests_enqueued++;
-    }
-    s->cv.notify_one();
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_server_enqueue", [&]() -> mmr_status {
+        if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
+        if (!req || req->struct_size < sizeof(mmr_recon_request)) return fail(MMR_STATUS_INVALID_ARGUMENT, "request invalid");
+        const size_t sz = mmr_host::codec_struct_size(req->codec);
+        if (!sz) return fail(MMR_STATUS_INVALID_ARGUMENT, "request codec %u invalid", req->codec);
+        if (!req->frame_count || !req->frames) return fail(MMR_STATUS_INVALID_ARGUMENT, "request has no frames");
+        auto r = std::make_unique<mmr_host::ServerRequest>();
+        r->codec = req->codec;
+        r->stream_id = req->stream_id;
+        r->frame_count = req->frame_count;
+        r->hbm = req->hbm_bytes;
+        r->tag = req->user_tag;
+        r->frames.assign((const char*)req->frames, (const char*)req->frames + sz * req->frame_count);
+        if (req->codec == MMR_CODEC_MPEG2) {
+            auto* pics = reinterpret_cast<const mmr_recon_mpeg2_picture*>(req->frames);
+            for (uint32_t i = 0; i < req->frame_count; ++i)
+                if (pics[i].forward_source >= (int32_t)i || pics[i].backward_source >= (int32_t)i)
+                    return fail(MMR_STATUS_INVALID_ARGUMENT, "request picture %u references a later picture", i);
+        }
+        if (req->convert) r->convert.assign(req->convert, req->convert + req->frame_count);
+        if (req->composite) r->composite.assign(req->composite, req->composite + req->frame_count);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (s->stop) return fail(MMR_STATUS_INVALID_STATE, "server is shutting down");
+            if (s->pending.size() >= s->cfg.max_pending) return fail(MMR_STATUS_BUSY, "server queue full (%u pending)", s->cfg.max_pending);
+            r->id = s->next_id++;
+            r->t0 = mmr_host::Clock::now();
+            if (out_id) *out_id = r->id;
+            s->pending.push_back(std::move(r));
+            s->stats.requests_enqueued++;
+        }
+        s->cv.notify_one();
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_server_poll(mmr_recon_server* s, mmr_recon_frame_completion* out, uint32_t capacity, uint32_t* out_count) {
-    if (!s || !out_count || (capacity && !out)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid poll arguments");
-    std::lock_guard<std::mutex> lk(s->mu);
-    uint32_t n = 0;
-    while (n < capacity && !s->completions.empty()) {
-        out[n++] = s->completions.front();
-        s->completions.pop_front();
-    }
-    *out_count = n;
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_server_poll", [&]() -> mmr_status {
+        if (!s || !out_count || (capacity && !out)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid poll arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        uint32_t n = 0;
+        while (n < capacity && !s->completions.empty()) {
+            out[n++] = s->completions.front();
+            s->completions.pop_front();
+        }
+        *out_count = n;
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_server_drain(mmr_recon_server* s) {
-    if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->cv.notify_all();
-    s->idle_cv.wait(lk, [s] { return s->pending.empty() && s->active.empty() && s->inflight.empty(); });
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_server_drain", [&]() -> mmr_status {
+        if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.notify_all();
+        s->idle_cv.wait(lk, [s] { return s->pending.empty() && s->active.empty() && s->inflight.empty(); });
+        return MMR_STATUS_OK;
+    });
 }
 
 mmr_status mmr_recon_server_stats_get(mmr_recon_server* s, mmr_recon_server_stats* out) {
-    if (!s || !out || out->struct_size < sizeof(mmr_recon_server_stats)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid stats arguments");
-    std::lock_guard<std::mutex> lk(s->mu);
-    *out = s->stats;
-    out->struct_size = sizeof(mmr_recon_server_stats);
-    return MMR_STATUS_OK;
+    return guarded("mmr_recon_server_stats_get", [&]() -> mmr_status {
+        if (!s || !out || out->struct_size < sizeof(mmr_recon_server_stats)) return fail(MMR_STATUS_INVALID_ARGUMENT, "invalid stats arguments");
+        std::lock_guard<std::mutex> lk(s->mu);
+        *out = s->stats;
+        out->struct_size = sizeof(mmr_recon_server_stats);
+        return MMR_STATUS_OK;
+    });
 }
 
 void mmr_recon_server_destroy(mmr_recon_server* s) {
-    if (!s) return;
-    {
-        std::unique_lock<std::mutex> lk(s->mu);
-        s->pending.clear();
-        s->stop = true;
-    }
-    s->cv.notify_all();
-    s->thread.join();
-    for (auto st : s->streams) {
-        (void)hipStreamSynchronize(st);
-        (void)hipStreamDestroy(st);
-    }
-    delete s;
+    guarded_void("mmr_recon_server_destroy", [&]() {
+        if (!s) return;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->pending.clear();
+            s->stop = true;
+        }
+        s->cv.notify_all();
+        s->thread.join();
+        for (auto st : s->streams) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+        delete s;
+    });
 }
 
 }  // extern "C"

@@ -1,0 +1,117 @@
+"""Checks that run in a child process on the model library (tests/test_served_replay.py).
+
+python tests/served_replay/checks.py planted     -- descriptors with planted mistakes (a level buffer
+    one record short, a freed output plane, output planes too small, a DV level buffer cut short)
+    are each reported by the kernel address models, and correct submissions are not
+python tests/served_replay/checks.py exceptions  -- with the address space limited, a submission
+    whose job table cannot be allocated returns MMR_STATUS_INTERNAL_ERROR with a message instead
+    of letting std::bad_alloc escape through the C ABI
+Prints one JSON line.
+"""
+import ctypes as C
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from replay import load_model_library, violations  # noqa: E402
+
+
+def planted():
+    lib = load_model_library()
+    import numpy as np
+
+    import mmrecode_amd
+    from mmrecode_amd import abi
+    from mmrecode_amd import workloads as wl
+
+    out = {}
+    ctx = mmrecode_amd.Context(device=0)
+
+    def run(name, submit):
+        before, _ = violations(lib)
+        status = 0
+        try:
+            ctx.wait(submit())
+        except mmrecode_amd.ReconError as e:
+            status = e.status
+        ctx.synchronize()
+        n, msgs = violations(lib)
+        out[name] = {"new": n - before, "status": status, "last": msgs[-1] if n > before and msgs else ""}
+
+    # MPEG-2: a GOP reconstructed correctly, then its P picture with a level buffer one record short
+    units = wl.mpeg2_gop(seed=3, width=176, height=144, gop="IPB")
+    dev = []
+    for unit in units:
+        extra = {}
+        for tag in ("fwd", "bwd"):
+            if unit.meta[tag] is not None:
+                for n in ("y", "cb", "cr"):
+                    extra[f"{tag}_{n}"] = dev[unit.meta[tag]].ptr(n)
+        dev.append(wl.DeviceUnit(ctx, unit, extra))
+    run("mpeg2_ok", lambda: ctx.submit_mpeg2([d.struct for d in dev], abi.SUBMIT_PER_FRAME_LAUNCH))
+    p = units[1]
+    short = ctx.upload(p.inputs["levels"][:-1])
+    s = p.fill({**{k: b.ptr for k, b in dev[1].bufs.items()}, "levels": short.ptr,
+                **{f"fwd_{n}": dev[0].ptr(n) for n in ("y", "cb", "cr")}})
+    run("mpeg2_levels_short", lambda: ctx.submit_mpeg2([s]))
+    # a freed output plane
+    victim = dev[2].bufs["cr"]
+    s2 = dev[2].struct
+    victim.free()
+    run("mpeg2_plane_freed", lambda: ctx.submit_mpeg2([s2], abi.SUBMIT_PER_FRAME_LAUNCH))
+
+    # H.264: planes of a smaller picture
+    pic = wl.h264_picture(5, 0, 8, 6, intra_fraction=0.0)
+    refs = [dict(zip(("y", "cb", "cr"), wl.h264_reference_planes(5, 8, 6, i))) for i in range(4)]
+    rdev = [{n: ctx.upload(a).ptr for n, a in r.items()} for r in refs]
+    hd = wl.DeviceUnit(ctx, pic, wl.h264_ref_ptrs(pic, rdev, "dev"))
+    run("h264_ok", lambda: ctx.submit_h264([hd.struct]))
+    small = {n: ctx.alloc(b.nbytes // 2) for n, b in hd.bufs.items() if n in ("y", "cb", "cr")}
+    hs = pic.fill({**{k: b.ptr for k, b in hd.bufs.items()}, **{n: b.ptr for n, b in small.items()},
+                   **wl.h264_ref_ptrs(pic, rdev, "dev")})
+    run("h264_planes_small", lambda: ctx.submit_h264([hs]))
+
+    # DV: levels one block short
+    f = wl.dv_frame(9, 0, system=abi.DV_SYSTEM_525_60)
+    dd = wl.DeviceUnit(ctx, f)
+    run("dv_ok", lambda: ctx.submit_dv([dd.struct]))
+    cut = ctx.upload(f.inputs["levels"][:-1])
+    ds = f.fill({**{k: b.ptr for k, b in dd.bufs.items()}, "levels": cut.ptr})
+    run("dv_levels_short", lambda: ctx.submit_dv([ds]))
+    ctx.close()
+    _ = np
+    return out
+
+
+def exceptions():
+    import resource
+
+    lib = load_model_library()
+    import mmrecode_amd
+    from mmrecode_amd import abi
+
+    ctx = mmrecode_amd.Context(device=0)
+    soft, hard = resource.getrlimit(resource.RLIMIT_AS)
+    resource.setrlimit(resource.RLIMIT_AS, (12 << 30, hard))
+    frames = (abi.MjpegFrame * 1)()
+    ticket = C.c_uint64()
+    # 2^27 frames: the job table alone is about 16 GiB, beyond the address-space limit
+    status = lib.mmr_recon_mjpeg_submit(ctx.handle, frames, 1 << 27, 0, C.byref(ticket))
+    message = mmrecode_amd.last_error()
+    resource.setrlimit(resource.RLIMIT_AS, (soft, hard))
+    # the library is still usable afterwards
+    d = ctx.alloc(16)
+    ctx.write(d, bytes(range(16)))
+    ok = bytes(ctx.download(d)) == bytes(range(16))
+    ctx.close()
+    return {"status": status, "message": message, "usable_after": ok}
+
+
+if __name__ == "__main__":
+    what = sys.argv[1]
+    print(json.dumps({"planted": planted, "exceptions": exceptions}[what]()))

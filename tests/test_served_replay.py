@@ -1,0 +1,55 @@
+"""The served path on the CPU: the library's real host code (submission, tickets, the
+continuous-batching server) on a host model of the HIP runtime with address models of every kernel
+(tests/served_replay/). Each check runs in a child process, which loads the model library instead
+of libmmr_recon.so."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "served_replay")
+
+
+def _run(args, timeout):
+    sys.path.insert(0, HERE)
+    import build
+
+    build.build()
+    r = subprocess.run([sys.executable, os.path.join(HERE, args[0]), *args[1:]], capture_output=True, text=True, timeout=timeout)
+    assert r.stdout.strip(), r.stderr[-3000:]
+    return r.returncode, json.loads(r.stdout.strip().splitlines()[-1]), r.stderr
+
+
+def test_model_reports_planted_mistakes():
+    rc, out, err = _run(["checks.py", "planted"], 600)
+    assert rc == 0, err[-3000:]
+    for ok in ("mpeg2_ok", "h264_ok", "dv_ok"):
+        assert out[ok]["new"] == 0, (ok, out[ok])
+    assert out["mpeg2_levels_short"]["new"] >= 1 and "mpeg2 levels" in out["mpeg2_levels_short"]["last"]
+    assert out["mpeg2_plane_freed"]["new"] >= 1 and "already freed" in out["mpeg2_plane_freed"]["last"]
+    assert out["h264_planes_small"]["new"] >= 1 and "overrun" in out["h264_planes_small"]["last"]
+    assert out["dv_levels_short"]["new"] >= 1 and "dv levels" in out["dv_levels_short"]["last"]
+
+
+def test_exception_becomes_internal_error_status():
+    """crates/capi/src/lib.rs:572-588 wraps every export in catch_unwind; here an allocation failure
+    inside a submit call returns MMR_STATUS_INTERNAL_ERROR (100) with a message."""
+    rc, out, err = _run(["checks.py", "exceptions"], 300)
+    assert rc == 0, err[-3000:]
+    assert out["status"] == 100, out
+    assert "out of host memory" in out["message"]
+    assert out["usable_after"]
+
+
+@pytest.mark.timeout(900)
+def test_served_mix_replay_full_shape():
+    """bench.py's served trace at full shapes (64 streams; MJPEG 640x480, MPEG-2 1080p GOP-15, DV,
+    H.264 720p / 1080p / 2160p; ring slots; BT.709 RGBA + 3-layer composite) through the server
+    and the plain path: every kernel access inside a live allocation, every frame completed OK."""
+    rc, out, err = _run(["replay.py", "--rates", "300", "--seconds", "0.4", "--plain-seconds", "2"], 900)
+    assert out["violations"] == 0, out["messages"]
+    run = out["runs"][0]
+    assert run["requests"] > 50 and run["served_frames"] == run["frames"] and run["bad_status"] == 0, run
+    assert rc == 0
