@@ -223,6 +223,28 @@ MMR_RECON_API mmr_status mmr_recon_mpeg2_submit(mmr_recon_context *context,
                                                 uint32_t picture_count, uint32_t flags,
                                                 uint64_t *out_ticket);
 
+/* Device scratch of the allocation-free entry point below: 64 MiB, held by every GPU the library
+ * runs on until the process ends. A submission needs 256-byte-aligned room for its job records and
+ * tile table, its counters and status word, and 96 bytes of block order per macroblock (a 1080p
+ * picture takes about 0.8 MB, so a closed GOP of 15 1080p pictures fits). */
+#define MMR_RECON_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/*
+ * No counterpart in the reference. For callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc): the same reconstruction as mmr_recon_mpeg2_submit
+ * (same descriptors, flags and results), run synchronously on the context's stream with its job
+ * records, counters and block order in a fixed device array of MMR_RECON_NOALLOC_SCRATCH_BYTES
+ * instead of a per-submission allocation. It allocates no device, pinned or event memory and
+ * returns once the pictures are reconstructed, with their status (INVALID_DATA / UNSUPPORTED as
+ * mmr_recon_wait reports them). A submission larger than the array fails with
+ * MMR_STATUS_INVALID_ARGUMENT before anything runs. Every call on one GPU uses that GPU's one array:
+ * calls from one context are serialised, but calls that overlap on one GPU from different contexts
+ * race and must be serialised by the caller.
+ */
+MMR_RECON_API mmr_status mmr_recon_mpeg2_reconstruct_noalloc(mmr_recon_context *context,
+                                                             const mmr_recon_mpeg2_picture *pictures,
+                                                             uint32_t picture_count, uint32_t flags);
+
 /* -------------------------------------------------------------------- DV */
 
 #define MMR_DV_SYSTEM_525_60 0u  /* 720x480, 4:1:1, 10 DIF sequences */

@@ -139,6 +139,12 @@ class Context:
     def submit_mpeg2(self, pictures, flags=0):
         return self._submit(self.lib.mmr_recon_mpeg2_submit, abi.Mpeg2Picture, pictures, flags)
 
+    def reconstruct_mpeg2_noalloc(self, pictures, flags=0):
+        """mmr_recon_mpeg2_reconstruct_noalloc: the pictures reconstructed synchronously inside the
+        library's fixed device scratch (no device allocation); raises ReconError on a non-OK status."""
+        arr = (abi.Mpeg2Picture * len(pictures))(*pictures)
+        check(self.lib.mmr_recon_mpeg2_reconstruct_noalloc(self.handle, arr, len(pictures), flags))
+
     def submit_dv(self, frames, flags=0):
         return self._submit(self.lib.mmr_recon_dv_submit, abi.DvFrame, frames, flags)
 

@@ -412,6 +412,7 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(MjpegFrame), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_mpeg2_submit", C.c_int32,
      [C.c_void_p, C.POINTER(Mpeg2Picture), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_mpeg2_reconstruct_noalloc", C.c_int32, [C.c_void_p, C.POINTER(Mpeg2Picture), C.c_uint32, C.c_uint32]),
     ("mmr_recon_dv_submit", C.c_int32,
      [C.c_void_p, C.POINTER(DvFrame), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_h264_submit", C.c_int32,

@@ -97,8 +97,36 @@ struct ScaleTaps {
     uint32_t max = 0;
 };
 
+// Device scratch of the allocation-free entry point (mmr_recon_mpeg2_reconstruct_noalloc): the
+// submission's job records, tile table, counters, status word and block order live here instead of
+// a stream-ordered allocation. Every GPU the library runs on holds its own copy, from the load of
+// the code object until the process ends. On the host the name is only a stand-in for the symbol,
+// not a device address: host code uses the per-device pointer that noalloc_scratch() obtains once
+// per device with hipGetSymbolAddress, never the name itself.
+static __device__ __align__(256) char scratch[MMR_RECON_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_ptr;   // device ordinal -> that device's copy of `scratch`
+
+// The current device's copy of `scratch`, resolved on the first call for the device.
+hipError_t noalloc_scratch(int device, char** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        it = g_scratch_ptr.emplace(device, static_cast<char*>(p)).first;
+    }
+    *out = it->second;
+    return hipSuccess;
+}
+}  // namespace
+
 struct mmr_recon_context {
     int device = 0;
+    char* noalloc_scratch = nullptr;  // this device's copy of `scratch`
     hipStream_t stream = nullptr;
     uint32_t queue_depth = 256;
     uint32_t cu_count = 256;
@@ -130,21 +158,25 @@ void reap(mmr_recon_context* ctx) {
     }
 }
 
+// The status a submission's device status word stands for (message set when not OK).
+mmr_status status_from_word(uint32_t st) {
+    if (st & mmr::kDevStatusTimeout)
+        return fail(MMR_STATUS_INTERNAL_ERROR, "a device-side wait timed out (internal error); outputs are invalid");
+    if (st & kStatusUnsupported)
+        return fail(MMR_STATUS_UNSUPPORTED, "a record of this submission uses a feature the device path does not reconstruct "
+                                            "(MPEG-2 field prediction or H.264 intra macroblock); fall back to CPU reconstruction");
+    if (st & kStatusInvalidData)
+        return fail(MMR_STATUS_INVALID_DATA, "a record of this submission is invalid (H.264 reference index or MPEG-2 level record index out of range)");
+    return MMR_STATUS_OK;
+}
+
 mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
     if (!r.finished) {
         hipError_t e = hipEventSynchronize(r.done);
         if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.start, r.kernels_done) == hipSuccess) r.kernel_ms = ms;
-        const uint32_t st = r.host_status ? *r.host_status : 0u;
-        if (st & mmr::kDevStatusTimeout) {
-            r.status = fail(MMR_STATUS_INTERNAL_ERROR, "a device-side wait timed out (internal error); outputs are invalid");
-        } else if (st & kStatusUnsupported) {
-            r.status = fail(MMR_STATUS_UNSUPPORTED, "a record of this submission uses a feature the device path does not reconstruct "
-                                                    "(MPEG-2 field prediction or H.264 intra macroblock); fall back to CPU reconstruction");
-        } else if (st & kStatusInvalidData) {
-            r.status = fail(MMR_STATUS_INVALID_DATA, "a record of this submission is invalid (H.264 reference index or MPEG-2 level record index out of range)");
-        }
+        r.status = status_from_word(r.host_status ? *r.host_status : 0u);
         r.finished = true;
     }
     (void)ctx;
@@ -163,10 +195,18 @@ struct PreparedCapture {
     hipGraphExec_t exec = nullptr;
 };
 
+// Device memory a submission may use instead of allocating its block: the allocation-free entry
+// point passes the device's copy of the file-scope `scratch` array (see below).
+struct Workspace {
+    char* base = nullptr;
+    size_t capacity = 0;
+};
+
 struct Submission {
     mmr_recon_context* ctx;
     hipStream_t stream;
     PreparedCapture* cap;
+    const Workspace* ws;      // non-null: synchronous submission inside this memory, no allocation
     bool capturing = false;
     TicketRecord rec;
     char* d_block = nullptr;
@@ -174,7 +214,36 @@ struct Submission {
     size_t counter_words = 0;
     size_t counters_bytes = 0;
 
-    Submission(mmr_recon_context* c, hipStream_t s, PreparedCapture* capture = nullptr) : ctx(c), stream(s), cap(capture) {}
+    Submission(mmr_recon_context* c, hipStream_t s, PreparedCapture* capture = nullptr, const Workspace* workspace = nullptr)
+        : ctx(c), stream(s), cap(capture), ws(workspace) {}
+
+    // Workspace mode: the block is the workspace (checked against its capacity), the job records
+    // are copied from pageable host memory, the counters and status word cleared by the fill
+    // kernel; no device, pinned or event allocation.
+    mmr_status begin_workspace(const void* jobs, size_t bytes, size_t scratch_bytes,
+                               const std::function<void(char* host_jobs, char* scratch)>& patch) {
+        const size_t need = jobs_bytes + counters_bytes + scratch_bytes;
+        if (need > ws->capacity)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "the submission needs %zu bytes of device scratch, the allocation-free entry point "
+                        "has %zu (MMR_RECON_NOALLOC_SCRATCH_BYTES); submit fewer or smaller pictures", need, ws->capacity);
+        d_block = ws->base;
+        rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
+        if (patch) patch(rec.host_jobs.data(), scratch_ptr());
+        hipError_t e = hipSuccess;
+        if (bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, stream);
+        if (e != hipSuccess) return hip_fail(e, "job upload");
+        return MMR_STATUS_OK;
+    }
+    // Waits for the submission and returns its status.
+    mmr_status end_workspace() {
+        uint32_t word = 0;
+        hipError_t e = hipMemcpyAsync(&word, status_ptr(), sizeof word, hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        d_block = nullptr;
+        if (e != hipSuccess) return hip_fail(e, "allocation-free submission");
+        return status_from_word(word);
+    }
 
     mmr_status begin_capture(const void* jobs, size_t bytes, size_t scratch_bytes,
                              const std::function<void(char* host_jobs, char* scratch)>& patch) {
@@ -209,11 +278,11 @@ struct Submission {
     // records can point into the block's scratch.
     mmr_status begin(const void* jobs, size_t bytes, size_t counters, size_t scratch_bytes = 0,
                      const std::function<void(char* host_jobs, char* scratch)>& patch = nullptr) {
-        if (cap) {
+        if (cap || ws) {
             jobs_bytes = (bytes + 255) & ~size_t(255);
             counter_words = counters + 1;
             counters_bytes = (counter_words * sizeof(uint32_t) + 255) & ~size_t(255);
-            return begin_capture(jobs, bytes, scratch_bytes, patch);
+            return cap ? begin_capture(jobs, bytes, scratch_bytes, patch) : begin_workspace(jobs, bytes, scratch_bytes, patch);
         }
         if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
         if (ctx->tickets.size() >= ctx->queue_depth)
@@ -244,6 +313,7 @@ struct Submission {
 
     mmr_status end(uint64_t* out_ticket) {
         if (cap) return end_capture();
+        if (ws) return end_workspace();
         hipError_t e = hipEventRecord(rec.kernels_done, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipEventRecord(rec.done, stream);
@@ -261,9 +331,11 @@ struct Submission {
             hipGraph_t g = nullptr;
             if (hipStreamEndCapture(stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
         }
-        if (d_block) {
+        if (d_block && !ws) {
             if (cap) (void)hipFree(d_block);
             else (void)hipFreeAsync(d_block, stream);
+        } else if (d_block) {
+            (void)hipStreamSynchronize(stream);   // a failed workspace submission: the next call may reuse the memory
         }
         if (rec.start) (void)hipEventDestroy(rec.start);
         if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
@@ -478,7 +550,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
 }
 
 mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_mpeg2_picture* pics, uint32_t count, uint32_t flags,
-                                  uint64_t* out_ticket, PreparedCapture* cap = nullptr) {
+                                  uint64_t* out_ticket, PreparedCapture* cap = nullptr, const Workspace* ws = nullptr) {
     CHECK_CTX(ctx);
     if (!pics || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no pictures");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
@@ -568,16 +640,16 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     const size_t tiles_off = blob_append(blob, tile_table.data(), tile_table.size() * sizeof(uint32_t));
     // Scratch: each picture's block order (6 entries of 16 bytes per macroblock).
     std::vector<size_t> order_off(ordered.size());
-    size_t scratch = 0;
+    size_t scratch_total = 0;
     uint32_t order_rows = 0;
     for (size_t k = 0; k < ordered.size(); ++k) {
-        order_off[k] = scratch;
-        scratch += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * 16 + 255) & ~size_t(255);   // 16-byte entries
+        order_off[k] = scratch_total;
+        scratch_total += ((size_t)ordered[k].mb_w * ordered[k].mb_h * 6 * 16 + 255) & ~size_t(255);   // 16-byte entries
         ordered[k].row_begin = per_frame ? 0 : order_rows;
         order_rows += ordered[k].mb_h;
     }
-    Submission sub(ctx, stream, cap);
-    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
+    Submission sub(ctx, stream, cap, ws);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch_total, [&](char* host, char* d_scratch) {
         auto* hj = reinterpret_cast<mmr::Mpeg2Job*>(host + jobs_off);
         for (size_t k = 0; k < ordered.size(); ++k) {
             hj[k].order = reinterpret_cast<uint32_t*>(d_scratch + order_off[k]);
@@ -731,15 +803,15 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     }
     // Scratch: 16 bytes of boundary strengths per macroblock of every deblocked picture.
     std::vector<size_t> bs_off(count, 0);
-    size_t scratch = 0;
+    size_t scratch_total = 0;
     for (uint32_t i = 0; i < count; ++i) {
-        bs_off[i] = scratch;
-        if (jobs[i].deblock) scratch += ((size_t)pics[i].width_mbs * pics[i].height_mbs * 16 + 255) & ~size_t(255);
+        bs_off[i] = scratch_total;
+        if (jobs[i].deblock) scratch_total += ((size_t)pics[i].width_mbs * pics[i].height_mbs * 16 + 255) & ~size_t(255);
     }
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::H264Job));
     const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
-    mmr_status s = sub.begin(blob.data(), blob.size(), counters, scratch, [&](char* host, char* d_scratch) {
+    mmr_status s = sub.begin(blob.data(), blob.size(), counters, scratch_total, [&](char* host, char* d_scratch) {
         auto* hj = reinterpret_cast<mmr::H264Job*>(host + jobs_off);
         for (uint32_t i = 0; i < count; ++i) hj[i].bs = jobs[i].deblock ? reinterpret_cast<uint32_t*>(d_scratch + bs_off[i]) : nullptr;
     });
@@ -1011,7 +1083,7 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     std::vector<PlaneHost> planes;
     std::vector<char> taps;
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, Placed> placed;
-    size_t scratch = 0;
+    size_t scratch_total = 0;
     auto put = [&taps](const void* data, size_t bytes) {
         const size_t off = (taps.size() + 15) & ~size_t(15);
         taps.resize(off + bytes);
@@ -1078,8 +1150,8 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
                 plane.hcount = h.count + (size_t)d.sx * 4;
                 plane.hw = h.w + (size_t)d.sx * h.max * 4;
                 if (!d.tiled) {
-                    plane.tmp = scratch;
-                    scratch += (((size_t)pw * d.copy_h * 4) + 255) & ~size_t(255);
+                    plane.tmp = scratch_total;
+                    scratch_total += (((size_t)pw * d.copy_h * 4) + 255) & ~size_t(255);
                 }
             }
             planes.push_back(plane);
@@ -1091,7 +1163,7 @@ mmr_status scale_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     if (!taps.empty()) memcpy(blob.data() + taps_off, taps.data(), taps.size());
     for (size_t p = 0; p < planes.size(); ++p) memcpy(blob.data() + p * sizeof(mmr::ScalePlaneDev), &planes[p].d, sizeof(mmr::ScalePlaneDev));
     Submission sub(ctx, stream);
-    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch, [&](char* host, char* d_scratch) {
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0, scratch_total, [&](char* host, char* d_scratch) {
         char* base = static_cast<char*>(sub.jobs_ptr()) + taps_off;
         auto* hp = reinterpret_cast<mmr::ScalePlaneDev*>(host);
         for (size_t p = 0; p < planes.size(); ++p) {
@@ -1176,7 +1248,8 @@ mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_co
         hipDeviceProp_t prop;
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
         if (e == hipSuccess) ctx->cu_count = (uint32_t)prop.multiProcessorCount;
-        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = noalloc_scratch(ctx->device, &ctx->noalloc_scratch);   // with the device current
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = mmr::dv_upload_tables(ctx->stream);
         if (e == hipSuccess) e = hipMalloc((void**)&ctx->flat_scaling, 96 + 128);
         if (e == hipSuccess) {
@@ -1340,6 +1413,17 @@ mmr_status mmr_recon_mpeg2_submit(mmr_recon_context* ctx, const mmr_recon_mpeg2_
         CHECK_CTX(ctx);
         LOCK_AND_SET_DEVICE(ctx);
         return mmr_host::mpeg2_submit_on(ctx, ctx->stream, pics, count, flags, out_ticket);
+    });
+}
+
+mmr_status mmr_recon_mpeg2_reconstruct_noalloc(mmr_recon_context* ctx, const mmr_recon_mpeg2_picture* pics, uint32_t count,
+                                               uint32_t flags) {
+    return guarded("mmr_recon_mpeg2_reconstruct_noalloc", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        if (!ctx->noalloc_scratch) return fail(MMR_STATUS_INTERNAL_ERROR, "the device scratch of this context is not resolved");
+        const Workspace ws{ctx->noalloc_scratch, MMR_RECON_NOALLOC_SCRATCH_BYTES};
+        return mmr_host::mpeg2_submit_on(ctx, ctx->stream, pics, count, flags, nullptr, nullptr, &ws);
     });
 }
 

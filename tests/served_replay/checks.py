@@ -112,6 +112,55 @@ def exceptions():
     return {"status": status, "message": message, "usable_after": ok}
 
 
+def noalloc():
+    """mmr_recon_mpeg2_reconstruct_noalloc: no device / pinned allocation and no event during two
+    calls (the second smaller), no access outside live allocations, an oversized call rejected."""
+    from replay import model_stats
+
+    lib = load_model_library()
+    import mmrecode_amd
+    from mmrecode_amd import abi
+    from mmrecode_amd import workloads as wl
+
+    ctx = mmrecode_amd.Context(device=0)
+    sets = []
+    for seed, (w, h, gop) in ((41, (720, 480, "IPBBPBB")), (42, (176, 144, "IPB"))):
+        units = wl.mpeg2_gop(seed=seed, gop_index=seed, width=w, height=h, gop=gop, field_fraction=0.3)
+        dev = []
+        for u in units:
+            extra = {"forward_source": -1 if u.meta["fwd"] is None else u.meta["fwd"],
+                     "backward_source": -1 if u.meta["bwd"] is None else u.meta["bwd"]}
+            for tag in ("fwd", "bwd"):
+                if u.meta[tag] is not None:
+                    for n in ("y", "cb", "cr"):
+                        extra[f"{tag}_{n}"] = dev[u.meta[tag]].ptr(n)
+            dev.append(wl.DeviceUnit(ctx, u, extra))
+        sets.append(dev)
+    before = model_stats(lib)
+    for dev in sets:
+        ctx.reconstruct_mpeg2_noalloc([d.struct for d in dev])
+    ctx.reconstruct_mpeg2_noalloc([d.struct for d in sets[1]], abi.SUBMIT_PER_FRAME_LAUNCH)
+    calls = model_stats(lib)
+    # the oversized case of tests/test_zz_no_alloc.py: 120 1080p descriptors on the buffers of a
+    # 64x32 picture (120 x 8160 macroblocks x 96 B of block order > 64 MiB) must be rejected before
+    # any kernel runs, or those kernels would read far past the small buffers
+    u = wl.mpeg2_gop(seed=43, gop_index=3, width=64, height=32, gop="I")[0]
+    d = wl.DeviceUnit(ctx, u)
+    after = model_stats(lib)
+    big = abi.new(abi.Mpeg2Picture, width=1920, height=1080, picture_type=abi.MPEG2_PICTURE_I, matrices=d.ptr("matrices"),
+                  macroblocks=d.ptr("mbs"), levels=d.ptr("levels"), forward_source=-1, backward_source=-1)
+    for k, n in enumerate(("y", "cb", "cr")):
+        big.planes[k] = d.ptr(n)
+    status = lib.mmr_recon_mpeg2_reconstruct_noalloc(ctx.handle, (abi.Mpeg2Picture * 120)(*([big] * 120)), 120, 0)
+    message = mmrecode_amd.last_error()
+    ran = model_stats(lib)["kernels"] - after["kernels"]
+    n, msgs = violations(lib)
+    ctx.close()
+    return {"allocations": calls["allocations"] - before["allocations"], "events": calls["events"] - before["events"],
+            "kernels": calls["kernels"] - before["kernels"], "violations": n, "messages": msgs[:5],
+            "oversized_status": status, "oversized_message": message, "oversized_kernels": ran}
+
+
 if __name__ == "__main__":
     what = sys.argv[1]
-    print(json.dumps({"planted": planted, "exceptions": exceptions}[what]()))
+    print(json.dumps({"planted": planted, "exceptions": exceptions, "noalloc": noalloc}[what]()))

@@ -14,6 +14,7 @@
 #define __host__
 #define __global__
 #define __forceinline__ inline
+#define __align__(n) __attribute__((aligned(n)))
 
 typedef int hipError_t;
 enum : int {

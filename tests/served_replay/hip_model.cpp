@@ -80,7 +80,6 @@ std::vector<std::string> g_violations;
 size_t g_violation_count = 0;
 Stats g_stats;
 hipError_t g_last = hipSuccess;
-char g_symbol_space[1 << 20] __attribute__((aligned(256)));
 
 void reserve_arena() {
     if (g_arena) return;
@@ -98,6 +97,7 @@ void reserve_arena() {
 
 void* take(size_t bytes, int kind) {
     reserve_arena();
+    g_stats.allocations++;
     const size_t need = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
     const size_t off = (g_arena_used + 255) & ~size_t(255);
     if (off + need + 4096 > g_arena_size) return nullptr;
@@ -244,8 +244,6 @@ Stats& stats() { return g_stats; }
 
 const std::vector<std::string>& violations() { return g_violations; }
 size_t violation_count() { return g_violation_count; }
-
-void* symbol_space() { return g_symbol_space; }
 
 }  // namespace model
 
@@ -408,7 +406,9 @@ hipError_t hipGetSymbolAddress(void** out, const void* symbol) {
     // A symbol's device copy: one block of the model's address range per symbol address.
     static std::map<const void*, void*> copies;
     auto it = copies.find(symbol);
-    if (it == copies.end()) it = copies.emplace(symbol, take(sizeof g_symbol_space, 0)).first;
+    // (the model cannot see a symbol's size; 256 MiB covers every array of the library and,
+    // untouched, costs no memory)
+    if (it == copies.end()) it = copies.emplace(symbol, take(size_t(256) << 20, 0)).first;
     *out = it->second;
     return *out ? hipSuccess : ret(hipErrorMemoryAllocation);
 }
@@ -450,6 +450,7 @@ hipError_t hipStreamWaitEvent(hipStream_t s, hipEvent_t e, unsigned) {
 
 hipError_t hipEventCreate(hipEvent_t* e) {
     LOCK;
+    g_stats.events++;
     *e = new ModelEvent();   // never freed: a use after destroy stays detectable
     return hipSuccess;
 }
@@ -538,12 +539,15 @@ extern "C" __attribute__((visibility("default"))) size_t mmr_model_violations(ch
     return g_violation_count;
 }
 
-extern "C" __attribute__((visibility("default"))) void mmr_model_stats(uint64_t* out4) {
+// kernels run, checks, stream operations run, jobs checked, device / pinned allocations, events created
+extern "C" __attribute__((visibility("default"))) void mmr_model_stats(uint64_t* out6) {
     LOCK;
-    out4[0] = g_stats.kernels;
-    out4[1] = g_stats.checks;
-    out4[2] = g_stats.ops_run;
-    out4[3] = g_stats.jobs;
+    out6[0] = g_stats.kernels;
+    out6[1] = g_stats.checks;
+    out6[2] = g_stats.ops_run;
+    out6[3] = g_stats.jobs;
+    out6[4] = g_stats.allocations;
+    out6[5] = g_stats.events;
 }
 
 extern "C" __attribute__((visibility("default"))) void mmr_model_seed(uint64_t seed) {

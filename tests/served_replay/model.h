@@ -13,7 +13,7 @@
 namespace model {
 
 struct Stats {
-    uint64_t kernels = 0, checks = 0, ops_run = 0, jobs = 0;
+    uint64_t kernels = 0, checks = 0, ops_run = 0, jobs = 0, allocations = 0, events = 0;
 };
 
 std::recursive_mutex& mutex();

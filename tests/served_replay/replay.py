@@ -52,9 +52,9 @@ def violations(lib):
 
 
 def model_stats(lib):
-    out = (C.c_uint64 * 4)()
+    out = (C.c_uint64 * 6)()
     lib.mmr_model_stats(out)
-    return {"kernels": out[0], "checks": out[1], "ops": out[2], "jobs": out[3]}
+    return {"kernels": out[0], "checks": out[1], "ops": out[2], "jobs": out[3], "allocations": out[4], "events": out[5]}
 
 
 def replay(rates, seconds, plain_seconds, seed=7, slots=4, log=print):

@@ -53,3 +53,16 @@ def test_served_mix_replay_full_shape():
     run = out["runs"][0]
     assert run["requests"] > 50 and run["served_frames"] == run["frames"] and run["bad_status"] == 0, run
     assert rc == 0
+
+
+def test_noalloc_entry_point_allocates_nothing():
+    """mmr_recon_mpeg2_reconstruct_noalloc makes no device, pinned or event allocation over three
+    calls (a 7-picture 720x480 GOP, then a smaller 3-picture one, then that one per picture), every
+    kernel access stays inside live memory, and a submission beyond its scratch is rejected before
+    any kernel runs."""
+    rc, out, err = _run(["checks.py", "noalloc"], 300)
+    assert rc == 0, err[-3000:]
+    assert out["allocations"] == 0 and out["events"] == 0, out
+    assert out["kernels"] > 0 and out["violations"] == 0, out
+    assert out["oversized_status"] == 1 and out["oversized_kernels"] == 0, out
+    assert "MMR_RECON_NOALLOC_SCRATCH_BYTES" in out["oversized_message"]
