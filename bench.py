@@ -38,9 +38,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_passes(jobs, min_seconds=3.0, max_passes=64):
+def cpu_passes(jobs, min_cpu_seconds=12.0, min_seconds=1.0, max_passes=2000):
     """Runs `jobs` (callables, one thread each; the oracle calls release the GIL) as passes until
-    at least `min_seconds` of wall time have passed. Returns (passes, seconds)."""
+    at least `min_cpu_seconds` of thread time (wall time x threads) and `min_seconds` of wall time
+    have passed. Returns (passes, seconds)."""
     passes, t = 0, time.time()
     while True:
         ts = [threading.Thread(target=j) for j in jobs]
@@ -50,12 +51,30 @@ def cpu_passes(jobs, min_seconds=3.0, max_passes=64):
             th.join()
         passes += 1
         dt = time.time() - t
-        if dt >= min_seconds or passes >= max_passes:
+        if (dt * len(jobs) >= min_cpu_seconds and dt >= min_seconds) or passes >= max_passes:
             return passes, dt
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """(threads this process may run on, os.cpu_count(), CPU model string)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return usable, os.cpu_count(), model
 
 
 def percentile(values, q):
@@ -68,6 +87,7 @@ class Mpeg2Workload:
     codec = abi.CODEC_MPEG2
     metric_unit = "Mpix/s"
     kernel = "mpeg2_kernel"
+    phases = ("mpeg2_order_kernel", "mpeg2_kernel")
 
     def __init__(self, ctx, gops=8, width=1920, height=1080, seed=2024):
         self.ctx, self.width, self.height = ctx, width, height
@@ -98,6 +118,7 @@ class Mpeg2Workload:
         self.pictures = len(self.structs)
         self.mpix = self.pictures * metrics.mpix(width, height)
         self.bytes = sum(metrics.mpeg2_picture_bytes(self.gops[g][i]) for g, i in self.order)
+        self.kernel_bytes = {"mpeg2_kernel": self.bytes}   # the order kernel re-reads records only
         # f64 IDCT operation count at the reference's order (dense: 1024 mul + 1024 add per block)
         self.f64_ops = sum(self.gops[g][i].meta["coded_blocks"] for g, i in self.order) * 2048
 
@@ -116,21 +137,23 @@ class Mpeg2Workload:
                 "resolution": f"{self.width}x{self.height}", "gop": wl.GOP_DECODE_ORDER}
 
     def cpu_sample(self, threads):
-        """One GOP per thread through the oracle (threads <= GOPs), Mpix/s."""
+        """One GOP per thread through the oracle, Mpix/s: the step's GOPs plus further GOPs of the
+        same configuration (seeded after them) when there are more threads than GOPs."""
         from oracle import binding
 
         L = binding.lib()
-        n = max(1, min(threads, len(self.gops)))
+        n = max(1, threads)
+        gops = self.gops + [wl.mpeg2_gop(2024, g, self.width, self.height) for g in range(len(self.gops), n)]
 
-        outs = [{i: u.host_outputs() for i, u in enumerate(self.gops[g])} for g in range(n)]
+        outs = [{i: u.host_outputs() for i, u in enumerate(gops[g])} for g in range(n)]
 
         def run_gop(g):
-            for i, u in enumerate(self.gops[g]):
+            for i, u in enumerate(gops[g]):
                 s = u.host_struct(outs[g][i], wl.mpeg2_ref_ptrs(outs[g], u, "host"))
                 assert L.oracle_mpeg2_reconstruct(C.byref(s)) == 0
 
         passes, dt = cpu_passes([lambda g=g: run_gop(g) for g in range(n)])
-        mp = passes * sum(len(self.gops[g]) for g in range(n)) * metrics.mpix(self.width, self.height)
+        mp = passes * sum(len(gops[g]) for g in range(n)) * metrics.mpix(self.width, self.height)
         return mp / dt, n, (f"{passes} pass(es) over {n} GOP(s) x 15 pictures {self.width}x{self.height}, one GOP per "
                             f"thread, {dt:.1f} s wall on {n} threads")
 
@@ -140,6 +163,7 @@ class MjpegWorkload:
     codec = abi.CODEC_MJPEG
     metric_unit = "Mpix/s"
     kernel = "mjpeg_blocks_kernel"
+    phases = ("mjpeg_blocks_kernel", "yuv_to_rgb_kernel")
 
     def __init__(self, ctx, frames=120, seed=2024):
         self.ctx = ctx
@@ -149,6 +173,8 @@ class MjpegWorkload:
         self.pictures = frames
         self.mpix = frames * metrics.mpix(640, 480)
         self.bytes = sum(metrics.mjpeg_frame_bytes(u) for u in self.units)
+        self.kernel_bytes = {"mjpeg_blocks_kernel": sum(metrics.mjpeg_block_kernel_bytes(u) for u in self.units),
+                             "yuv_to_rgb_kernel": sum(metrics.mjpeg_rgb_kernel_bytes(u) for u in self.units)}
         self.f64_ops = sum(u.meta["blocks"] for u in self.units) * 2048
 
     def submit(self, flags=0):
@@ -225,6 +251,11 @@ class H264Workload:
     metric_unit = "Mpix/s"
     kernel = "h264_mb_kernel"
     SIZES = ((80, 45), (120, 68), (240, 135))
+
+    def phase_names(self, n):
+        if n == 4:
+            return ("h264_bs_kernel", "h264_mb_kernel", "h264_intra_kernel", "h264_deblock_kernel")
+        return ("h264_bs_kernel", "h264_mb_kernel", "h264_deblock_kernel")[:n]
 
     picture_kwargs = {}
 
@@ -695,6 +726,14 @@ def calibrate_copy(nbytes=1 << 30, repeats=5):
     ctx.close()
 
 
+def phase_names(w, n):
+    """Names of the library's launch phases of one batched submission of workload w."""
+    if hasattr(w, "phase_names"):
+        return w.phase_names(n)
+    names = getattr(w, "phases", (w.kernel,))
+    return tuple(names[:n]) if n <= len(names) else tuple(names) + tuple(f"phase{i}" for i in range(len(names), n))
+
+
 def measured_traffic(workload):
     try:
         with open(TRAFFIC_FILE) as f:
@@ -743,18 +782,29 @@ def main():
     for _ in range(args.warmup):
         ctx.wait(w.submit())
     ctx.synchronize()
-    kernel_ms = []
+    # Frame latency of the batched path: one batch in flight, submit -> completion observed on the
+    # host; every picture of a batch completes with it.
+    batch_lat = []
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        ctx.wait(w.submit())
+        batch_lat.append((time.perf_counter() - ts) * 1e3)
     t0 = time.perf_counter()
     tickets = [w.submit() for _ in range(args.steps)]
     for t in tickets:
         ctx.wait(t)
     elapsed = time.perf_counter() - t0
-    for t in tickets:
-        kernel_ms.append(ctx.kernel_ms(t))
+    # Device time of each launch phase (HIP events the library records on the submission stream
+    # between phases; they add up to the submission's kernel span).
+    span_ms = float(np.mean([ctx.kernel_ms(t) for t in tickets]))
+    phase_lists = [ctx.kernel_phases_ms(t) for t in tickets]
+    names = phase_names(w, len(phase_lists[0]))
+    phase_ms = {n: float(np.mean([p[i] for p in phase_lists])) for i, n in enumerate(names)}
+    k_ms = phase_ms.get(w.kernel, span_ms)
     ms_per_step = elapsed * 1e3 / args.steps
     value = w.mpix * args.steps / elapsed
-    k_ms = float(np.mean(kernel_ms)) if kernel_ms else None
-    log(f"batched: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s, kernel {k_ms} ms")
+    log(f"batched: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s, kernels {span_ms:.4f} ms "
+        f"({', '.join(f'{n} {v:.4f}' for n, v in phase_ms.items())})")
 
     # The same batch prepared once (job records resident, launches replayed from a HIP graph).
     prepared = None
@@ -791,9 +841,12 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        usable, nproc, model = host_cpu()
+        # every thread this process may use, up to the host share of one GPU on these boxes (16)
+        threads = args.cpu_threads or min(16, usable)
         v, cores, sample = w.cpu_sample(threads)
-        cpu = {"value": round(v, 3), "unit": "Mpix/s", "cores": cores, "kind": "port", "sample": sample}
+        cpu = {"value": round(v, 3), "unit": "Mpix/s", "cores": cores, "kind": "port", "sample": sample,
+               "host": {"nproc": nproc, "usable_threads": usable, "cpu_model": model}}
         log(f"cpu baseline: {v:.2f} Mpix/s on {cores} threads")
 
     served = None
@@ -801,7 +854,15 @@ def main():
         rates = [float(r) for r in args.served_rates.split(",") if r]
         served = run_served_sweep(ctx, rates, args.served_seconds, args.plain_seconds)
 
-    achieved = w.bytes / (k_ms * 1e-3) / 1e9 if k_ms else None
+    kernel_bytes = getattr(w, "kernel_bytes", {w.kernel: w.bytes})
+    achieved = kernel_bytes[w.kernel] / (k_ms * 1e-3) / 1e9 if k_ms else None
+    phases = []
+    for n, ms in phase_ms.items():
+        ph = {"kernel": n, "ms": round(ms, 4)}
+        if n in kernel_bytes and ms > 0:
+            ph["algorithmic_bytes"] = int(kernel_bytes[n])
+            ph["achieved_gbs"] = round(kernel_bytes[n] / (ms * 1e-3) / 1e9, 2)
+        phases.append(ph)
     result = {
         "metric": getattr(w, "metric", "reconstructed Mpix/s batched-served vs plain GPU vs CPU; p50/p99 frame latency"),
         "value": round(value, 3),
@@ -819,22 +880,28 @@ def main():
         "plain_gpu": {"value": round(plain, 3) if plain else None, "unit": "Mpix/s",
                       "p50_frame_ms": percentile(latencies, 50), "p99_frame_ms": percentile(latencies, 99),
                       "mode": "one picture per launch, synchronous"},
-        "batched_frame_latency_ms": {"p50": round(ms_per_step, 4), "p99": round(ms_per_step, 4),
-                                     "note": "every picture of a batch completes with its launch"},
+        "batched_frame_latency_ms": {"p50": round(percentile(batch_lat, 50), 4), "p99": round(percentile(batch_lat, 99), 4),
+                                     "samples": len(batch_lat),
+                                     "note": "submit -> completion of one batch with nothing else in flight; every "
+                                             "picture of a batch completes with it"},
         "speedup_vs_plain": round(value / plain, 2) if plain else None,
         "prepared_graph": prepared,
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2) if achieved else None, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-                     "traffic": None, "kernel": w.kernel, "kernel_ms": k_ms,
-                     "algorithmic_bytes_per_launch": int(w.bytes)},
+                     "traffic": None, "kernel": w.kernel, "kernel_ms": round(k_ms, 4),
+                     "algorithmic_bytes_per_launch": int(kernel_bytes[w.kernel]),
+                     "per": "step: every launch of the kernel in one step (bytes and device time summed over them)",
+                     "phases": phases, "kernels_ms_per_step": round(span_ms, 4),
+                     "timing": "HIP events recorded by the library on the submission stream around each launch phase"},
         "cpu_baseline": cpu,
         "served": served,
     }
     tr = measured_traffic(args.workload)
-    if tr:
+    if tr and tr.get("kernel") == w.kernel:
         result["roofline"]["traffic"] = int(tr["hbm_bytes_per_step"])
         result["roofline"]["traffic_source"] = tr["source"]
+        result["roofline"]["launches_per_step"] = tr["dispatches_per_step"]
     if w.f64_ops and k_ms:
         result["roofline"]["fp64_dense_tflops_equiv"] = round(w.f64_ops / (k_ms * 1e-3) / 1e12, 3)
         result["roofline"]["fp64_peak_tflops"] = FP64_PEAK_TFLOPS

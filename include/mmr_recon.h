@@ -111,6 +111,22 @@ MMR_RECON_API mmr_status mmr_recon_synchronize(mmr_recon_context *context);
 MMR_RECON_API mmr_status mmr_recon_last_kernel_ms(mmr_recon_context *context, uint64_t ticket,
                                                   double *out_ms);
 
+/* Device time (milliseconds) of each launch phase of a completed ticket, from HIP events recorded
+ * on the submission stream between the phases; the phases add up to mmr_recon_last_kernel_ms.
+ * Batched submissions (flags without MMR_SUBMIT_PER_FRAME_LAUNCH) have these phases:
+ *   MJPEG   [block kernel] or [block kernel, YUV->RGB] when a frame asks for RGB
+ *   MPEG-2  [macroblock order kernel, all dependency-level launches of the block kernel]
+ *   H.264   [boundary strengths (when deblocking), inter macroblocks, intra bands (when
+ *           present), deblocking (when present)]
+ *   others  one phase.
+ * Per-frame launches report one phase. At most MMR_RECON_MAX_PHASES values are written to
+ * out_ms (capacity entries available); *out_count receives the number of phases. No counterpart
+ * in the reference (measurement of this library's own launches). */
+#define MMR_RECON_MAX_PHASES 4u
+MMR_RECON_API mmr_status mmr_recon_kernel_phases_ms(mmr_recon_context *context, uint64_t ticket,
+                                                    double *out_ms, uint32_t capacity,
+                                                    uint32_t *out_count);
+
 /* ----------------------------------------------------------------- MJPEG */
 
 /*
@@ -424,10 +440,10 @@ MMR_RECON_API mmr_status mmr_recon_h264_implicit_weights(int32_t poc0, int32_t p
  */
 typedef struct mmr_recon_yuv_to_rgb_job {
     size_t struct_size;
-    uint32_t width;                 /* luma width = output width */
-    uint32_t height;
-    uint32_t chroma_width;
-    uint32_t chroma_height;
+    uint32_t width;                 /* luma width = output width, 1..65535 */
+    uint32_t height;                /* 1..65535 */
+    uint32_t chroma_width;          /* 1..width */
+    uint32_t chroma_height;         /* 1..height */
     uint32_t luma_stride;
     uint32_t chroma_stride;
     uint32_t range;                 /* MMR_RANGE_* */

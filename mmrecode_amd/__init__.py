@@ -218,6 +218,13 @@ class Context:
         check(self.lib.mmr_recon_last_kernel_ms(self.handle, ticket, C.byref(ms)))
         return ms.value
 
+    def kernel_phases_ms(self, ticket):
+        """Device milliseconds of each launch phase of a ticket (mmr_recon_kernel_phases_ms)."""
+        out = (C.c_double * abi.MAX_PHASES)()
+        n = C.c_uint32()
+        check(self.lib.mmr_recon_kernel_phases_ms(self.handle, ticket, out, abi.MAX_PHASES, C.byref(n)))
+        return [out[i] for i in range(min(n.value, abi.MAX_PHASES))]
+
 
 class Prepared:
     """mmr_recon_prepare / mmr_recon_prepared_launch: a batch whose job records are uploaded once and

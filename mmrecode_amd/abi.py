@@ -6,6 +6,7 @@ sizes and field offsets against a C compiler's view of the header.
 import ctypes as C
 
 ABI_VERSION = 1
+MAX_PHASES = 4            # MMR_RECON_MAX_PHASES
 
 STATUS_OK = 0
 STATUS_INVALID_ARGUMENT = 1
@@ -408,6 +409,8 @@ ENTRY_POINTS = [
     ("mmr_recon_poll", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
     ("mmr_recon_synchronize", C.c_int32, [C.c_void_p]),
     ("mmr_recon_last_kernel_ms", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double)]),
+    ("mmr_recon_kernel_phases_ms", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double), C.c_uint32,
+                                               C.POINTER(C.c_uint32)]),
     ("mmr_recon_mjpeg_submit", C.c_int32,
      [C.c_void_p, C.POINTER(MjpegFrame), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_mpeg2_submit", C.c_int32,

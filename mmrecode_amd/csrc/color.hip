@@ -10,94 +10,169 @@
 
 namespace mmr {
 
-MMR_DEV void ycbcr_to_rgb(int y, int cb, int cr, uint32_t range, uint32_t matrix, int rgb[3]) {
-    const int b = cb - 128, r = cr - 128;
-    int R, G, B;
-    if (range == MMR_RANGE_LIMITED) {
-        const int yy = max(y - 16, 0);
-        if (matrix == MMR_MATRIX_BT709) {
-            R = (298 * yy + 459 * r + 128) >> 8;
-            G = (298 * yy - 55 * b - 136 * r + 128) >> 8;
-            B = (298 * yy + 541 * b + 128) >> 8;
-        } else {
-            R = (298 * yy + 409 * r + 128) >> 8;
-            G = (298 * yy - 100 * b - 208 * r + 128) >> 8;
-            B = (298 * yy + 516 * b + 128) >> 8;
-        }
-    } else {
-        if (matrix == MMR_MATRIX_BT709) {
-            R = y + ((403 * r + 128) >> 8);
-            G = y - ((48 * b + 120 * r + 128) >> 8);
-            B = y + ((475 * b + 128) >> 8);
-        } else {
-            R = y + ((359 * r + 128) >> 8);
-            G = y - ((88 * b + 183 * r + 128) >> 8);
-            B = y + ((454 * b + 128) >> 8);
-        }
-    }
-    rgb[0] = clampi(R, 0, 255);
-    rgb[1] = clampi(G, 0, 255);
-    rgb[2] = clampi(B, 0, 255);
+// floor(n / d) for n < 2^32, d >= 1, from inv = 1.0 / d (rounded): n * inv is within
+// 2^-51 * n / d < 1 of n / d, so the truncated estimate is floor(n / d) or one off; the remainder
+// settles which.
+MMR_DEV uint32_t div_floor(uint32_t n, uint32_t d, double inv) {
+    uint32_t q = (uint32_t)((double)n * inv);
+    const int64_t r = (int64_t)n - (int64_t)q * d;
+    if (r < 0) --q;
+    else if (r >= (int64_t)d) ++q;
+    return q;
 }
 
-// One thread per 4 horizontally adjacent pixels (12 RGB bytes / 16 RGBA bytes).
-__global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restrict__ jobs, uint32_t job_count, uint32_t total_quads,
-                                                         uint32_t per_xcd) {
-    const uint32_t gq = xcd_tile(blockIdx.x, per_xcd) * 256 + threadIdx.x;
-    if (gq >= total_quads) return;
-    const uint32_t ji = find_job(jobs, job_count, gq, [](const RgbJob& j) { return j.quad_begin; });
-    const RgbJob& j = jobs[ji];
-    const uint32_t q = gq - j.quad_begin;
-    const uint32_t y = q / j.quads_per_row, x0 = (q % j.quads_per_row) * 4;
-    uint32_t cy = (uint32_t)(((uint64_t)y * j.ch) / j.height);
-    cy = min(cy, j.ch - 1);
-    const uint8_t* Y = j.planes[0] + (size_t)y * j.luma_stride;
-    const uint8_t* U = j.planes[1] + (size_t)cy * j.chroma_stride;
-    const uint8_t* V = j.planes[2] + (size_t)cy * j.chroma_stride;
-    uint8_t px[16];
-    const int n = (int)min(4u, j.width - x0);
+// Chroma terms of one (Cb, Cr) sample in binary32 (exact; RgbJob::kcb / kcr / k0); full range
+// floors them here, limited range after adding the luma term.
+struct ChromaTerms {
+    float r, g, b;
+};
+
+template <bool kLimited>
+MMR_DEV ChromaTerms chroma_terms(const RgbJob& j, uint32_t cb, uint32_t cr) {
+    const float fb = (float)cb, fr = (float)cr;
+    ChromaTerms t;
+    t.r = __builtin_fmaf(j.kcr[0], fr, j.k0[0]);
+    t.g = __builtin_fmaf(j.kcb[1], fb, __builtin_fmaf(j.kcr[1], fr, j.k0[1]));
+    t.b = __builtin_fmaf(j.kcb[2], fb, j.k0[2]);
+    if (!kLimited) {
+        t.r = __builtin_floorf(t.r);
+        t.g = __builtin_floorf(t.g);
+        t.b = __builtin_floorf(t.b);
+    }
+    return t;
+}
+
+MMR_DEV float clamp_u8f(float v) { return __builtin_fminf(__builtin_fmaxf(v, 0.0f), 255.0f); }
+
+// R, G, B of luma sample y under chroma terms t, each an integer in [0, 255] held as binary32.
+template <bool kLimited>
+MMR_DEV void rgb_of(uint32_t y, const ChromaTerms& t, float out[3]) {
+    const float fy = (float)y;
+    if (kLimited) {
+        const float yy = __builtin_fmaxf(fy - 16.0f, 0.0f);
+        out[0] = __builtin_floorf(__builtin_fmaf(298.0f / 256.0f, yy, t.r));
+        out[1] = __builtin_floorf(__builtin_fmaf(298.0f / 256.0f, yy, t.g));
+        out[2] = __builtin_floorf(__builtin_fmaf(298.0f / 256.0f, yy, t.b));
+    } else {
+        out[0] = fy + t.r;
+        out[1] = fy - t.g;
+        out[2] = fy + t.b;
+    }
+    out[0] = clamp_u8f(out[0]);
+    out[1] = clamp_u8f(out[1]);
+    out[2] = clamp_u8f(out[2]);
+}
+
+// Byte k (0..3) of dword d replaced by the integer v in [0, 255] held as binary32.
+MMR_DEV uint32_t put_byte(uint32_t d, float v, uint32_t k) { return __builtin_amdgcn_cvt_pk_u8_f32(v, k, d); }
+
+// One wave converts 256 pixels of one row, 4 horizontally adjacent pixels per thread (one luma
+// dword, 12 RGB bytes / 16 RGBA bytes); four waves per workgroup tile. The host's tile -> job
+// table gives the tile's job with one scalar load; the row, its chroma row and the job's constants
+// are wave-uniform. The chroma sample of pixel x is floor(x cw / width) (video_frame_image): when
+// 2 cw == width that is x / 2 and pixels 2k, 2k + 1 share one set of chroma terms; otherwise the
+// first from div_floor, the next three by carrying the remainder (cw <= width, so each step moves
+// at most one chroma sample).
+template <bool kLimited>
+MMR_DEV void yuv_to_rgb_wave(const RgbJob& j, uint32_t y, uint32_t x0) {
+    const uint32_t cy = min(div_floor(y * j.ch, j.height, j.inv_height), j.ch - 1);
+    const MMR_GLOBAL uint8_t* Y = as_global(j.planes[0]) + (size_t)y * j.luma_stride;
+    const MMR_GLOBAL uint8_t* U = as_global(j.planes[1]) + (size_t)cy * j.chroma_stride;
+    const MMR_GLOBAL uint8_t* V = as_global(j.planes[2]) + (size_t)cy * j.chroma_stride;
+    const uint32_t n = min(4u, j.width - x0);
+    uint32_t ys;
+    if (n == 4 && ((reinterpret_cast<uintptr_t>(Y + x0) & 3) == 0)) {
+        ys = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(Y + x0);
+    } else {
+        ys = 0;
+        for (uint32_t i = 0; i < n; ++i) ys |= (uint32_t)Y[x0 + i] << (8 * i);
+    }
+    float px[4][3];
+    if (j.half_chroma) {
+        // x0 is a multiple of 4: chroma samples x0 / 2 and x0 / 2 + 1 (the second only when pixel
+        // x0 + 2 exists, so it never passes cw - 1).
+        const uint32_t c0 = x0 >> 1;
+        const uint32_t c1 = n > 2 ? c0 + 1 : c0;
+        const ChromaTerms t0 = chroma_terms<kLimited>(j, U[c0], V[c0]);
+        const ChromaTerms t1 = chroma_terms<kLimited>(j, U[c1], V[c1]);
+        rgb_of<kLimited>(ys & 0xffu, t0, px[0]);
+        rgb_of<kLimited>((ys >> 8) & 0xffu, t0, px[1]);
+        rgb_of<kLimited>((ys >> 16) & 0xffu, t1, px[2]);
+        rgb_of<kLimited>(ys >> 24, t1, px[3]);
+    } else {
+        uint32_t cx = div_floor(x0 * j.cw, j.width, j.inv_width);
+        uint32_t rem = x0 * j.cw - cx * j.width;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (i < n) {
-            const uint32_t x = x0 + i;
-            uint32_t cx = (uint32_t)(((uint64_t)x * j.cw) / j.width);
-            cx = min(cx, j.cw - 1);
-            int rgb[3];
-            ycbcr_to_rgb(Y[x], U[cx], V[cx], j.range, j.matrix, rgb);
-            px[i * 4 + 0] = (uint8_t)rgb[0];
-            px[i * 4 + 1] = (uint8_t)rgb[1];
-            px[i * 4 + 2] = (uint8_t)rgb[2];
-            px[i * 4 + 3] = 255;
+        for (uint32_t i = 0; i < 4; ++i) {
+            const uint32_t c = min(cx, j.cw - 1);
+            rgb_of<kLimited>((ys >> (8 * i)) & 0xffu, chroma_terms<kLimited>(j, U[c], V[c]), px[i]);
+            rem += j.cw;
+            if (rem >= j.width) {
+                rem -= j.width;
+                ++cx;
+            }
         }
     }
     const size_t p = (size_t)y * j.width + x0;
     if (j.rgb) {
         MMR_GLOBAL uint8_t* o = as_global(j.rgb) + p * 3;
         if (n == 4 && (((uintptr_t)o) & 3) == 0) {   // 12 bytes as three dwords
-            uint32_t d[3];
-            d[0] = px[0] | (px[1] << 8) | (px[2] << 16) | ((uint32_t)px[4] << 24);
-            d[1] = px[5] | (px[6] << 8) | (px[8] << 16) | ((uint32_t)px[9] << 24);
-            d[2] = px[10] | (px[12] << 8) | (px[13] << 16) | ((uint32_t)px[14] << 24);
+            uint32_t d0 = put_byte(0u, px[0][0], 0), d1 = put_byte(0u, px[1][1], 0), d2 = put_byte(0u, px[2][2], 0);
+            d0 = put_byte(d0, px[0][1], 1);
+            d0 = put_byte(d0, px[0][2], 2);
+            d0 = put_byte(d0, px[1][0], 3);
+            d1 = put_byte(d1, px[1][2], 1);
+            d1 = put_byte(d1, px[2][0], 2);
+            d1 = put_byte(d1, px[2][1], 3);
+            d2 = put_byte(d2, px[3][0], 1);
+            d2 = put_byte(d2, px[3][1], 2);
+            d2 = put_byte(d2, px[3][2], 3);
             MMR_GLOBAL uint32_t* o32 = reinterpret_cast<MMR_GLOBAL uint32_t*>(o);
-            o32[0] = d[0];
-            o32[1] = d[1];
-            o32[2] = d[2];
+            o32[0] = d0;
+            o32[1] = d1;
+            o32[2] = d2;
         } else {
-            for (int i = 0; i < n; ++i) {
-                o[i * 3 + 0] = px[i * 4 + 0];
-                o[i * 3 + 1] = px[i * 4 + 1];
-                o[i * 3 + 2] = px[i * 4 + 2];
+            for (uint32_t i = 0; i < n; ++i) {
+                o[i * 3 + 0] = (uint8_t)px[i][0];
+                o[i * 3 + 1] = (uint8_t)px[i][1];
+                o[i * 3 + 2] = (uint8_t)px[i][2];
             }
         }
     }
     if (j.rgba) {
-        uint8_t* o = j.rgba + p * 4;
+        MMR_GLOBAL uint8_t* o = as_global(j.rgba) + p * 4;
         if (n == 4 && (((uintptr_t)o) & 15) == 0) {
-            *reinterpret_cast<uint4*>(o) = *reinterpret_cast<const uint4*>(px);
+            mmr_v4i32 v;
+            for (int i = 0; i < 4; ++i) {
+                uint32_t d = put_byte(0xff000000u, px[i][0], 0);
+                d = put_byte(d, px[i][1], 1);
+                v[i] = (int)put_byte(d, px[i][2], 2);
+            }
+            *reinterpret_cast<MMR_GLOBAL mmr_v4i32*>(o) = v;
         } else {
-            for (int i = 0; i < n * 4; ++i) o[i] = px[i];
+            for (uint32_t i = 0; i < n; ++i) {
+                o[i * 4 + 0] = (uint8_t)px[i][0];
+                o[i * 4 + 1] = (uint8_t)px[i][1];
+                o[i * 4 + 2] = (uint8_t)px[i][2];
+                o[i * 4 + 3] = 255;
+            }
         }
     }
+}
+
+__global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                                         uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const RgbJob& j = jobs[tile_job[tile]];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
+    const uint32_t y = wave / j.waves_per_row;
+    if (y >= j.height) return;   // the job's last tile may hold fewer waves
+    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
+    if (x0 >= j.width) return;
+    if (j.limited) yuv_to_rgb_wave<true>(j, y, x0);
+    else yuv_to_rgb_wave<false>(j, y, x0);
 }
 
 MMR_DEV void blend_rgba_pixel(uint32_t d[4], const uint32_t s[4]) {
@@ -356,10 +431,10 @@ __global__ void fill_zero_kernel(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s) {
-    if (total_quads == 0) return hipSuccess;
-    const uint32_t per = xcd_per((total_quads + 255) / 256);
-    hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_quads, per);
+hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
     return hipGetLastError();
 }
 

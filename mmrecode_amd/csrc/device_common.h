@@ -171,6 +171,74 @@ MMR_DEV void idct8x8_column_pass(const double* __restrict__ tmp, int x, uint32_t
     }
 }
 
+// Fast form of the same two passes: the same sums over the same basis, each term fused into its
+// sum (one rounding per step instead of two), in half the instructions. It is not the reference's
+// arithmetic, so its results are used only where they provably round like the reference's:
+// both evaluations are within (2 g8 + g8^2) A of the exact sum S (g8 = 8u / (1 - 8u), u = 2^-53;
+// A = sum_v |M[y][v]| sum_u |M[x][u]| |c[v][u]| <= max|M|^2 sum|c|, max|M| = 0.49039264020161522),
+// so they differ by at most 32.0001 u A. For an MPEG-2 block (63 coefficients saturated to
+// [-2048, 2047], the intra DC an int16 shifted left by at most 3) sum|c| <= 391168, so the
+// difference is below 3.4e-10, and a fast value farther than kIdctTieWindow = 2^-30 (9.3e-10) from
+// every half-integer rounds (half away from zero) exactly as the reference's does. A fast value
+// inside the window is reported (idct_near_half) and the caller recomputes in reference order.
+constexpr double kIdctTieWindow = 0x1p-30;
+
+// Blocks whose coefficients are not saturated (MJPEG: level x quantiser) keep within that sum|c|
+// when every row's sum of |c| is at most kIdctFastRowAbsSum; a row past it sends the block to the
+// reference order. (MJPEG's + 128 rounds both evaluations once more, by at most 2^-45 each.)
+constexpr uint32_t kIdctFastRowAbsSum = 391168 / 8;
+
+MMR_DEV bool idct_row_within_fast_bound(const int32_t c[8]) {
+    uint32_t s = 0;
+    bool ok = true;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const uint32_t a = (uint32_t)(c[u] < 0 ? -(int64_t)c[u] : c[u]);
+        ok &= a <= kIdctFastRowAbsSum;
+        s += ok ? a : 0u;
+    }
+    return ok && s <= kIdctFastRowAbsSum;
+}
+
+MMR_DEV void idct8x8_row_pass_fast(const int32_t c[8], double* __restrict__ tmp_row) {
+    double cd[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cd[u] = (double)c[u];
+    double t[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        double s = dmul(kIdctBasis[x * 8], cd[0]);
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = __builtin_fma(kIdctBasis[x * 8 + u], cd[u], s);
+        t[x] = s;
+    }
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) *reinterpret_cast<double2*>(tmp_row + x) = make_double2(t[x], t[x + 1]);
+}
+
+MMR_DEV void idct8x8_column_pass_fast(const double* __restrict__ tmp, int x, uint32_t rows, double out[8]) {
+    const double c0 = tmp[x];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) out[y] = dmul(kIdctBasis[y * 8], c0);
+    uint32_t rest = rows & ~1u;
+    while (rest) {
+        const int v = __builtin_ctz(rest);
+        rest &= rest - 1;
+        const double cv = tmp[v * 8 + x];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) out[y] = __builtin_fma(kIdctBasis[y * 8 + v], cv, out[y]);
+    }
+}
+
+// True when one of the 8 values lies within kIdctTieWindow of a half-integer: |fract(|v|) - 1/2|
+// (fract and the subtraction are exact near 1/2).
+MMR_DEV bool idct_near_half(const double v[8]) {
+    bool near = false;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) near |= __builtin_fabs(__builtin_amdgcn_fract(__builtin_fabs(v[i])) - 0.5) <= kIdctTieWindow;
+    return near;
+}
+
 // round() (half away from zero) of a value with |v| < 2^31, as an int: v + copysign(h, v) with
 // h the largest double below 0.5, truncated. For |v| < 2^52 the sum rounds to at least n + 1 exactly
 // when |v| >= n + 0.5 and stays below n + 1 otherwise (h is 2^-54 short of 0.5, less than half a

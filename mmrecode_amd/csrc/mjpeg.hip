@@ -8,7 +8,8 @@
 // workgroup; a host-built tile -> frame table gives each workgroup its frame with one scalar
 // load. Lane j loads scan positions 8j..8j+7 (16-byte levels, 16-byte quantisers, 8-byte natural
 // positions), dequantises them and scatters to natural order in LDS. Lane v then takes natural
-// row v for the IDCT row pass, lane x runs column x of the column pass, and after an LDS
+// row v for the IDCT row pass, lane x runs column x of the column pass (the fused form, with the
+// reference order where it could round differently), and after an LDS
 // transpose lane r stores row r of the block with one 8-byte store (write_block's right / bottom
 // clipping handled per row). The lanes of a block share a wave, so the exchanges need no
 // workgroup barrier.
@@ -54,15 +55,34 @@ __global__ __launch_bounds__(BPG * 8) void mjpeg_blocks_kernel(const MjpegJob* _
     const int32_t c[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
     const uint32_t rows = wave_nonzero_rows(active && (lo.x | lo.y | lo.z | lo.w | hi.x | hi.y | hi.z | hi.w) != 0);
     wave_lds_sync();   // every row is in registers before the row pass overwrites the area
-    if (active) idct8x8_row_pass(c, tmp + lane * 8);
+    // Fast fused IDCT; the reference order again for the whole wave if a row is past the fast
+    // form's coefficient bound or a sample (after + 128) lies within the tie window of a
+    // half-integer (device_common.h).
+    if (active) idct8x8_row_pass_fast(c, tmp + lane * 8);
     wave_lds_sync();
     double col[8];
-    if (active) idct8x8_column_pass(tmp, lane, rows, col);
+    bool near = false;
+    if (active) {
+        idct8x8_column_pass_fast(tmp, lane, rows, col);
+#pragma unroll
+        for (int y = 0; y < 8; ++y) col[y] = dadd(col[y], 128.0);
+        near = !idct_row_within_fast_bound(c) || idct_near_half(col);
+    }
+    if (__ballot(near) != 0) {   // wave-uniform
+        wave_lds_sync();   // the fast column reads are done before the row pass rewrites the area
+        if (active) idct8x8_row_pass(c, tmp + lane * 8);
+        wave_lds_sync();
+        if (active) {
+            idct8x8_column_pass(tmp, lane, rows, col);
+#pragma unroll
+            for (int y = 0; y < 8; ++y) col[y] = dadd(col[y], 128.0);
+        }
+    }
     wave_lds_sync();
     uint8_t* pix = reinterpret_cast<uint8_t*>(tmp);
     if (active) {
 #pragma unroll
-        for (int y = 0; y < 8; ++y) pix[y * 8 + lane] = (uint8_t)clampi(round_half_away_i32(dadd(col[y], 128.0)), 0, 255);
+        for (int y = 0; y < 8; ++y) pix[y * 8 + lane] = (uint8_t)clampi(round_half_away_i32(col[y]), 0, 255);
     }
     wave_lds_sync();
     if (!active) return;

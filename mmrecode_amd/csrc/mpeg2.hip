@@ -274,8 +274,10 @@ __constant__ static const uint8_t kScanCoefOffset[2][64] = {
      28, 60, 92, 124, 152, 184, 216, 248, 156, 188, 220, 252}};
 
 // BPG 8x8 blocks (8 lanes each) per workgroup.
+// At least 7 waves per SIMD (72 VGPRs; the LDS of 32-block workgroups allows 8): measured 3 % faster
+// than the 6 waves the unconstrained allocation gives, although 4 VGPRs then spill.
 template <int BPG>
-__global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
+__global__ __launch_bounds__(BPG * 8, 7) void mpeg2_kernel(const Mpeg2Job* __restrict__ jobs,
                                                         const uint32_t* __restrict__ tile_job, uint32_t tile_base,
                                                         uint32_t tiles, uint32_t per_xcd, uint32_t* __restrict__ status) {
     // Per block group: natural-order coefficients, then the IDCT row-pass output, then the residual
@@ -314,9 +316,10 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
 #pragma unroll
         for (int mi = 0; mi < 4; ++mi) wscan[mi * 64 + p] = Wg[mi * 64 + sp];
     }
-    // Loads, in the order they are consumed: levels (scan positions 8*lane..), the scan positions,
-    // then the reference rows (consumed after the IDCT). They are unconditional (uncoded blocks
-    // read the 256-byte matrix block instead of levels), so no branch joins force an early wait.
+    // Loads: levels (scan positions 8*lane..) and the scan positions here, unconditional (uncoded
+    // blocks read the 256-byte matrix block instead of levels) so that no branch join forces an
+    // early wait; the reference rows are fetched after the IDCT (issued before it, they kept ~18
+    // more registers live across it and cost a wave per SIMD of occupancy, which measured slower).
     const int component = b < 4 ? 0 : 1;
     const uint32_t index = (uint32_t)ent.y & 0xffffffu;
     const int16_t* lv = coded ? job.levels + (size_t)index * 64 : reinterpret_cast<const int16_t*>(job.matrices);
@@ -372,8 +375,6 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     }
     const int fmx = (int16_t)(fmv & 0xffffu), fmy = (int16_t)(fmv >> 16);
     const int bmx = (int16_t)(bmv & 0xffffu), bmy = (int16_t)(bmv >> 16);
-    const RefFetch ff = fetch_ref(fref, stride, pitch, fh, x0, ry, fmx, fmy, has_f, job.refs_aligned != 0);
-    const RefFetch fb = fetch_ref(bref, stride, pitch, bh, x0, ry, bmx, bmy, has_b, job.refs_aligned != 0);
 
     // Dequantisation in scan order (decode_intra_block / decode_non_intra_block), branch-free with
     // 24-bit multiplies. Intra: |level| * W * qscale / 16; non-intra: (2 |level| + 1) * W * qscale
@@ -419,10 +420,24 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     }
     const uint32_t rows = wave_nonzero_rows((c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) != 0);
     wave_lds_sync();   // every row is in registers before the row pass overwrites the area
-    if (coded) idct8x8_row_pass(c, tmp + lane * 8);
+    // inverse_dct (transform.rs:90-110): the fused fast form, then the reference's order for the
+    // whole wave if any of its samples lies within the tie window (device_common.h: the AC
+    // coefficients are saturated to [-2048, 2047] above and the DC is an int16 << dc_shift <= 3,
+    // the block sizes the window's bound covers).
+    if (coded) idct8x8_row_pass_fast(c, tmp + lane * 8);
     wave_lds_sync();
     double col[8];
-    if (coded) idct8x8_column_pass(tmp, lane, rows, col);
+    bool near = false;
+    if (coded) {
+        idct8x8_column_pass_fast(tmp, lane, rows, col);
+        near = idct_near_half(col);
+    }
+    if (__ballot(near) != 0) {   // wave-uniform
+        wave_lds_sync();   // the fast column reads are done before the row pass rewrites the area
+        if (coded) idct8x8_row_pass(c, tmp + lane * 8);
+        wave_lds_sync();
+        if (coded) idct8x8_column_pass(tmp, lane, rows, col);
+    }
     wave_lds_sync();   // the column reads are done before the residual overwrites the area
     int16_t* blk = reinterpret_cast<int16_t*>(tmp);
     if (coded) {
@@ -436,6 +451,8 @@ __global__ __launch_bounds__(BPG * 8) void mpeg2_kernel(const Mpeg2Job* __restri
     // Row `lane` of the block: prediction (0 for intra) + residual, clamped to 0..255. Each
     // direction is formed only where a lane of the wave predicts from it; the average of both is
     // lerp(f, b, 1), a single direction is lerp(x, x, 0).
+    const RefFetch ff = fetch_ref(fref, stride, pitch, fh, x0, ry, fmx, fmy, has_f, job.refs_aligned != 0);
+    const RefFetch fb = fetch_ref(bref, stride, pitch, bh, x0, ry, bmx, bmy, has_b, job.refs_aligned != 0);
     uint32_t pf[2] = {0u, 0u}, pb[2] = {0u, 0u};
     if (has_f) {
         const uint2 t = halfpel_row(ff);

@@ -75,6 +75,9 @@ struct TicketRecord {
     hipEvent_t start = nullptr;       // before the first kernel (after the job upload)
     hipEvent_t kernels_done = nullptr;  // after the last kernel
     hipEvent_t done = nullptr;        // after the status word is copied back
+    hipEvent_t marks[MMR_RECON_MAX_PHASES - 1] = {};   // between launch phases
+    uint32_t mark_count = 0;
+    std::vector<double> phase_ms;     // start -> marks[0] -> ... -> kernels_done
     std::vector<char> host_jobs;      // kept alive until completion (async upload source)
     uint32_t* host_status = nullptr;  // pinned; device status word copied here
     bool finished = false;
@@ -153,6 +156,7 @@ void reap(mmr_recon_context* ctx) {
         if (r.start) (void)hipEventDestroy(r.start);
         if (r.kernels_done) (void)hipEventDestroy(r.kernels_done);
         if (r.done) (void)hipEventDestroy(r.done);
+        for (uint32_t m = 0; m < r.mark_count; ++m) (void)hipEventDestroy(r.marks[m]);
         if (r.host_status) (void)hipHostFree(r.host_status);
         ctx->tickets.erase(it);
     }
@@ -176,6 +180,13 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
         if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, r.start, r.kernels_done) == hipSuccess) r.kernel_ms = ms;
+        hipEvent_t from = r.start;
+        for (uint32_t m = 0; m <= r.mark_count; ++m) {
+            hipEvent_t to = m < r.mark_count ? r.marks[m] : r.kernels_done;
+            float seg = 0.f;
+            r.phase_ms.push_back(hipEventElapsedTime(&seg, from, to) == hipSuccess ? seg : -1.0);
+            from = to;
+        }
         r.status = status_from_word(r.host_status ? *r.host_status : 0u);
         r.finished = true;
     }
@@ -306,6 +317,18 @@ struct Submission {
         if (e != hipSuccess) return hip_fail(e, "job upload");
         return MMR_STATUS_OK;
     }
+    // Ends a launch phase: an event on the stream between the kernels before and after it (timed
+    // submissions only; prepared graphs and the allocation-free entry point keep one phase).
+    mmr_status mark() {
+        if (cap || ws || rec.mark_count >= MMR_RECON_MAX_PHASES - 1) return MMR_STATUS_OK;
+        hipEvent_t ev = nullptr;
+        hipError_t e = hipEventCreate(&ev);
+        if (e != hipSuccess) return hip_fail(e, "hipEventCreate (phase)");
+        rec.marks[rec.mark_count++] = ev;
+        e = hipEventRecord(ev, stream);
+        if (e != hipSuccess) return hip_fail(e, "hipEventRecord (phase)");
+        return MMR_STATUS_OK;
+    }
     void* jobs_ptr() const { return d_block; }
     uint32_t* counters_ptr() const { return reinterpret_cast<uint32_t*>(d_block + jobs_bytes); }
     uint32_t* status_ptr() const { return counters_ptr() + (counter_words - 1); }
@@ -340,6 +363,7 @@ struct Submission {
         if (rec.start) (void)hipEventDestroy(rec.start);
         if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
         if (rec.done) (void)hipEventDestroy(rec.done);
+        for (uint32_t m = 0; m < rec.mark_count; ++m) (void)hipEventDestroy(rec.marks[m]);
         if (rec.host_status) (void)hipHostFree(rec.host_status);
     }
 };
@@ -433,6 +457,8 @@ mmr_status build_mjpeg(const mmr_recon_mjpeg_frame& f, mmr::MjpegJob& j, uint32_
     rgb_quads = 0;
     if (f.rgb) {
         if (f.component_count != 3) return fail(MMR_STATUS_UNSUPPORTED, "RGB output needs 3 components");
+        if (f.h_sampling[0] != hmax || f.v_sampling[0] != vmax)
+            return fail(MMR_STATUS_UNSUPPORTED, "RGB output needs the luma component at the largest sampling factors");
         rgb_quads = ((f.width + 3) / 4) * f.height;
     }
     return MMR_STATUS_OK;
@@ -453,7 +479,10 @@ mmr::RgbJob rgb_job_of(const mmr_recon_mjpeg_frame& f, const mmr::MjpegJob& j) {
     r.chroma_stride = j.plane_w[1];
     r.range = f.rgb_range ? f.rgb_range : MMR_RANGE_FULL;
     r.matrix = MMR_MATRIX_BT601;
-    r.quads_per_row = (f.width + 3) / 4;
+    r.waves_per_row = (f.width + 255) / 256;
+    r.inv_width = 1.0 / f.width;
+    r.inv_height = 1.0 / f.height;
+    mmr::set_rgb_terms(r);
     return r;
 }
 
@@ -461,6 +490,8 @@ mmr_status check_rgb_job(const mmr_recon_yuv_to_rgb_job& s) {
     if (s.struct_size < sizeof(mmr_recon_yuv_to_rgb_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB struct_size too small");
     if (!s.width || !s.height || !s.chroma_width || !s.chroma_height)
         return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB job has zero dimensions");
+    if (s.width > 65535 || s.height > 65535 || s.chroma_width > s.width || s.chroma_height > s.height)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB job: sizes above 65535 or chroma larger than luma");
     if (s.luma_stride < s.width || s.chroma_stride < s.chroma_width)
         return fail(MMR_STATUS_INVALID_ARGUMENT, "YUV->RGB stride smaller than width");
     if (!s.planes[0] || !s.planes[1] || !s.planes[2] || (!s.rgb && !s.rgba))
@@ -505,8 +536,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     if (!frames || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no frames");
     std::vector<mmr::MjpegJob> jobs(count);
     std::vector<mmr::RgbJob> rgb;
-    std::vector<uint32_t> tiles;
-    uint32_t total_quads = 0;
+    std::vector<uint32_t> tiles, rgb_tiles;
     const uint32_t bpg = ctx->tuning.mjpeg_group_blocks;
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t quads = 0;
@@ -515,8 +545,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         jobs[i].tile_begin = add_tiles(tiles, i, jobs[i].block_count, bpg);
         if (quads) {
             mmr::RgbJob r = rgb_job_of(frames[i], jobs[i]);
-            r.quad_begin = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) ? 0 : total_quads;
-            total_quads += quads;
+            r.tile_begin = add_tiles(rgb_tiles, (uint32_t)rgb.size(), mmr::rgb_job_tiles(r), 1);
             rgb.push_back(r);
         }
     }
@@ -524,6 +553,7 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::MjpegJob));
     const size_t rgb_off = blob_append(blob, rgb.data(), rgb.size() * sizeof(mmr::RgbJob));
     const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
+    const size_t rgb_tiles_off = blob_append(blob, rgb_tiles.data(), rgb_tiles.size() * sizeof(uint32_t));
     Submission sub(ctx, stream, cap);
     mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
@@ -531,19 +561,23 @@ mmr_status mjpeg_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
     auto* dj = reinterpret_cast<const mmr::MjpegJob*>(base + jobs_off);
     auto* dr = reinterpret_cast<const mmr::RgbJob*>(base + rgb_off);
     auto* dt = reinterpret_cast<const uint32_t*>(base + tiles_off);
+    auto* drt = reinterpret_cast<const uint32_t*>(base + rgb_tiles_off);
     hipError_t e = hipSuccess;
     if (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) {
         size_t r = 0;
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
             e = mmr::launch_mjpeg(dj, dt, jobs[i].tile_begin, (jobs[i].block_count + bpg - 1) / bpg, bpg, stream);
             if (e == hipSuccess && frames[i].rgb) {
-                e = mmr::launch_yuv_to_rgb(dr + r, 1, ((frames[i].width + 3) / 4) * frames[i].height, stream);
+                e = mmr::launch_yuv_to_rgb(dr, drt, rgb[r].tile_begin, mmr::rgb_job_tiles(rgb[r]), stream);
                 ++r;
             }
         }
     } else {
         e = mmr::launch_mjpeg(dj, dt, 0, (uint32_t)tiles.size(), bpg, stream);
-        if (e == hipSuccess && !rgb.empty()) e = mmr::launch_yuv_to_rgb(dr, (uint32_t)rgb.size(), total_quads, stream);
+        if (e == hipSuccess && !rgb.empty()) {
+            if ((s = sub.mark())) return s;
+            e = mmr::launch_yuv_to_rgb(dr, drt, 0, (uint32_t)rgb_tiles.size(), stream);
+        }
     }
     if (e != hipSuccess) return hip_fail(e, "MJPEG launch");
     return sub.end(out_ticket);
@@ -668,6 +702,7 @@ mmr_status mpeg2_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr
         }
     } else {
         e = mmr::launch_mpeg2_order(dj, (uint32_t)ordered.size(), order_rows, sub.status_ptr(), stream);
+        if (e == hipSuccess && (s = sub.mark())) return s;
         for (uint32_t lv = 0; lv < levels && e == hipSuccess; ++lv)
             e = mmr::launch_mpeg2(dj, dt, level_tile_first[lv], level_tile_first[lv + 1] - level_tile_first[lv], bpg,
                                   sub.status_ptr(), stream);
@@ -840,13 +875,20 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             }
         }
     } else {
-        if (bands) e = mmr::launch_h264_bs(dj, dt, 0, (uint32_t)tiles.size(), stream);
+        if (bands) {
+            e = mmr::launch_h264_bs(dj, dt, 0, (uint32_t)tiles.size(), stream);
+            if (e == hipSuccess && (s = sub.mark())) return s;
+        }
         if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, 0, (uint32_t)tiles.size(), sub.status_ptr(), stream);
-        if (e == hipSuccess && intra_bands)
+        if (e == hipSuccess && intra_bands) {
+            if ((s = sub.mark())) return s;
             e = mmr::launch_h264_intra(dj, count, intra_bands, sub.counters_ptr() + 1 + bands, intra_groups, publish,
                                        sub.status_ptr(), stream);
-        if (e == hipSuccess && bands)
+        }
+        if (e == hipSuccess && bands) {
+            if ((s = sub.mark())) return s;
             e = mmr::launch_h264_deblock(dj, count, bands, sub.counters_ptr(), groups, band_rows, publish, sub.status_ptr(), stream);
+        }
     }
     if (e != hipSuccess) return hip_fail(e, "H.264 launch");
     return sub.end(out_ticket);
@@ -858,7 +900,7 @@ mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, cons
     if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::RgbJob> jobs(count);
-    uint32_t quads = 0;
+    std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
         mmr_status s = check_rgb_job(in[i]);
         if (s) return s;
@@ -875,20 +917,26 @@ mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, cons
         r.chroma_stride = in[i].chroma_stride;
         r.range = in[i].range;
         r.matrix = in[i].matrix;
-        r.quads_per_row = (in[i].width + 3) / 4;
-        r.quad_begin = per_frame ? 0 : quads;
-        quads += r.quads_per_row * r.height;
+        r.waves_per_row = (in[i].width + 255) / 256;
+        r.inv_width = 1.0 / in[i].width;
+        r.inv_height = 1.0 / in[i].height;
+        mmr::set_rgb_terms(r);
+        r.tile_begin = add_tiles(tiles, i, mmr::rgb_job_tiles(r), 1);
     }
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::RgbJob));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
     Submission sub(ctx, stream);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::RgbJob), 0);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::RgbJob*>(sub.jobs_ptr());
+    auto* dj = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     if (per_frame) {
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
-            e = mmr::launch_yuv_to_rgb(dj + i, 1, jobs[i].quads_per_row * jobs[i].height, stream);
+            e = mmr::launch_yuv_to_rgb(dj, dt, jobs[i].tile_begin, mmr::rgb_job_tiles(jobs[i]), stream);
     } else {
-        e = mmr::launch_yuv_to_rgb(dj, count, quads, stream);
+        e = mmr::launch_yuv_to_rgb(dj, dt, 0, (uint32_t)tiles.size(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "YUV->RGB launch");
     return sub.end(out_ticket);
@@ -1392,6 +1440,22 @@ mmr_status mmr_recon_last_kernel_ms(mmr_recon_context* ctx, uint64_t ticket, dou
         if (it == ctx->tickets.end()) return fail(MMR_STATUS_INVALID_STATE, "ticket %llu is not tracked", (unsigned long long)ticket);
         mmr_status s = finish(ctx, it->second);
         *out_ms = it->second.kernel_ms;
+        return s;
+    });
+}
+
+mmr_status mmr_recon_kernel_phases_ms(mmr_recon_context* ctx, uint64_t ticket, double* out_ms, uint32_t capacity,
+                                      uint32_t* out_count) {
+    return guarded("mmr_recon_kernel_phases_ms", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out_count || (capacity && !out_ms)) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_ms / out_count is NULL");
+        LOCK_AND_SET_DEVICE(ctx);
+        auto it = ctx->tickets.find(ticket);
+        if (it == ctx->tickets.end()) return fail(MMR_STATUS_INVALID_STATE, "ticket %llu is not tracked", (unsigned long long)ticket);
+        mmr_status s = finish(ctx, it->second);
+        const std::vector<double>& ph = it->second.phase_ms;
+        *out_count = (uint32_t)ph.size();
+        for (uint32_t i = 0; i < capacity && i < ph.size(); ++i) out_ms[i] = ph[i];
         return s;
     });
 }

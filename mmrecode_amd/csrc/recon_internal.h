@@ -103,9 +103,50 @@ struct RgbJob {
     uint8_t* rgb;
     uint8_t* rgba;
     uint32_t width, height, cw, ch, luma_stride, chroma_stride, range, matrix;
-    uint32_t quad_begin;          // first global 4-pixel quad of this job
-    uint32_t quads_per_row;
+    uint32_t tile_begin;          // first workgroup tile of this job in the launch's tile -> job table
+    uint32_t waves_per_row;       // ceil(width / 256): a wave covers 256 pixels of one row
+    double inv_width, inv_height; // 1.0 / width, 1.0 / height (chroma coordinate estimates)
+    // Chroma terms X_c = kcb[c] * Cb + kcr[c] * Cr + k0[c] (c = R, G, B), each the integer form's
+    // (A_c (Cb - 128) + B_c (Cr - 128) + 128) / 256 as an exact binary32 value; set_rgb_terms().
+    float kcb[3], kcr[3], k0[3];
+    uint32_t limited;             // range == MMR_RANGE_LIMITED
+    uint32_t half_chroma;         // 2 cw == width: pixels 2k and 2k + 1 share chroma sample k
 };
+// Workgroup tiles of the YUV->RGB kernel: 4 waves (256 threads, 4 pixels each), never straddling
+// two jobs; job j has ceil(waves_per_row * height / 4) tiles.
+inline uint32_t rgb_job_tiles(const RgbJob& j) { return (j.waves_per_row * j.height + 3) / 4; }
+
+// Fills the chroma-term constants, limited and half_chroma of j from range, matrix, width and cw.
+// Integer forms (b = Cb - 128, r = Cr - 128, yy = max(Y - 16, 0)), ycbcr_to_rgb
+// (crates/cli/src/terminal_preview.rs:7159-7238) for BT.601:
+//   limited  R = (298 yy + 409 r + 128) >> 8, G = (298 yy - 100 b - 208 r + 128) >> 8,
+//            B = (298 yy + 516 b + 128) >> 8            (BT.709: 459; 55, 136; 541)
+//   full     R = Y + ((359 r + 128) >> 8), G = Y - ((88 b + 183 r + 128) >> 8),
+//            B = Y + ((454 b + 128) >> 8)                (BT.709: 403; 48, 120; 475)
+// With X = (A b + B r + 128) / 256 = (A / 256) Cb + (B / 256) Cr + 1/2 - (A + B) / 2, limited
+// channels are floor(298 / 256 yy + X_c) (X_G carrying the minus signs) and full-range ones
+// Y + floor(X_R), Y - floor(X_G), Y + floor(X_B). Every constant is a multiple of 1/256 below
+// 2^10 and every partial sum one of magnitude below 2^10, so binary32 holds them all exactly.
+inline void set_rgb_terms(RgbJob& j) {
+    const bool lim = j.range == MMR_RANGE_LIMITED, bt709 = j.matrix == MMR_MATRIX_BT709;
+    int A[3], B[3];
+    if (lim) {
+        A[0] = 0;                 B[0] = bt709 ? 459 : 409;
+        A[1] = bt709 ? -55 : -100; B[1] = bt709 ? -136 : -208;
+        A[2] = bt709 ? 541 : 516; B[2] = 0;
+    } else {
+        A[0] = 0;                 B[0] = bt709 ? 403 : 359;
+        A[1] = bt709 ? 48 : 88;   B[1] = bt709 ? 120 : 183;
+        A[2] = bt709 ? 475 : 454; B[2] = 0;
+    }
+    for (int c = 0; c < 3; ++c) {
+        j.kcb[c] = (float)A[c] / 256.0f;
+        j.kcr[c] = (float)B[c] / 256.0f;
+        j.k0[c] = 0.5f - (float)(A[c] + B[c]) / 2.0f;
+    }
+    j.limited = lim ? 1u : 0u;
+    j.half_chroma = (uint64_t)j.cw * 2 == j.width ? 1u : 0u;
+}
 
 struct CompositeJobDev {
     uint8_t* target;
@@ -141,7 +182,7 @@ bool h264_deblock_rows_supported(uint32_t rows);
 hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
                                uint32_t groups, uint32_t band_rows, uint32_t publish, uint32_t* d_status, hipStream_t s);
-hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s);
+hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
 

@@ -27,6 +27,22 @@ def mjpeg_frame_bytes(unit):
     return unit.meta["blocks"] * 128 + 2 * 64 * 2 + out
 
 
+def mjpeg_plane_bytes(unit):
+    return sum(int(np.prod(s)) for n, (s, _, _) in unit.outputs.items() if n != "rgb")
+
+
+def mjpeg_block_kernel_bytes(unit):
+    """The block kernel's share: levels, quantisation tables, component planes written once."""
+    return unit.meta["blocks"] * 128 + 2 * 64 * 2 + mjpeg_plane_bytes(unit)
+
+
+def mjpeg_rgb_kernel_bytes(unit):
+    """The YUV->RGB kernel's share: the three planes read once, packed RGB written once."""
+    if "rgb" not in unit.outputs:
+        return 0
+    return mjpeg_plane_bytes(unit) + int(np.prod(unit.outputs["rgb"][0]))
+
+
 def dv_frame_bytes(unit):
     out = sum(int(np.prod(s)) for s, _, _ in unit.outputs.values())
     return unit.meta["blocks"] * (128 + 1) + out

@@ -277,26 +277,34 @@ hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint
     return hipSuccess;
 }
 
-// color.hip: yuv_to_rgb_kernel. Quad gq maps to the last job with quad_begin <= gq; its row
-// y = q / quads_per_row reads Y row y, chroma row min(y ch / height, ch - 1) and writes the packed
-// RGB / RGBA row y (width pixels).
-hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, uint32_t job_count, uint32_t total_quads, hipStream_t s) {
-    if (job_count == 0 || total_quads == 0) return hipSuccess;
+// color.hip: yuv_to_rgb_kernel. Tile t belongs to job tile_job[t]; its four waves are wave numbers
+// (t - tile_begin) * 4 .. + 3 of the job, wave w covers row w / waves_per_row, pixels
+// (w % waves_per_row) * 256 .. + 255 clipped to the width. Row y reads Y row y, chroma row
+// min(floor(y ch / height), ch - 1), chroma samples up to min(floor((width - 1) cw / width), cw - 1),
+// and writes packed RGB / RGBA row y.
+hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
     model::enqueue(s, "yuv_to_rgb", [=] {
-        if (!check(d_jobs, sizeof(RgbJob) * job_count, "yuv_to_rgb job records")) return;
-        for (uint32_t i = 0; i < job_count; ++i) {
-            const RgbJob& j = d_jobs[i];
+        std::map<uint32_t, std::vector<uint32_t>> offs;
+        std::map<uint32_t, RgbJob> recs;
+        if (!walk_tiles("yuv_to_rgb job / tile table", d_jobs, d_tile_job, tile_base, tiles, offs, recs)) return;
+        for (auto& [i, v] : offs) {
+            const RgbJob& j = recs[i];
             model::stats().jobs++;
-            const uint64_t begin = j.quad_begin, end = i + 1 < job_count ? d_jobs[i + 1].quad_begin : total_quads;
-            if (end < begin) {
-                violation("yuv_to_rgb: quad_begin not ascending at job %u", i);
+            if (j.waves_per_row != (j.width + 255) / 256)
+                violation("yuv_to_rgb: job %u has %u waves per row for width %u", i, j.waves_per_row, j.width);
+            check_tiles("yuv_to_rgb", i, v, rgb_job_tiles(j));
+            if (!j.width || !j.height || !j.cw || !j.ch) {
+                violation("yuv_to_rgb: job %u has a zero dimension", i);
                 continue;
             }
-            if (end - begin != (uint64_t)j.quads_per_row * j.height)
-                violation("yuv_to_rgb: job %u gets %llu quads, its picture has %llu", i, (unsigned long long)(end - begin),
-                          (unsigned long long)j.quads_per_row * j.height);
-            if (end == begin || !j.quads_per_row || !j.height || !j.width) continue;
-            const uint64_t ymax = (end - 1 - begin) / j.quads_per_row;
+            if (j.cw > j.width || j.width > 65535 || j.height > 65535)
+                violation("yuv_to_rgb: job %u sizes outside the kernel's range", i);
+            if (j.inv_width != 1.0 / j.width || j.inv_height != 1.0 / j.height)
+                violation("yuv_to_rgb: job %u reciprocals do not match its sizes", i);
+            if (j.half_chroma != ((uint64_t)j.cw * 2 == j.width ? 1u : 0u) || j.limited != (j.range == MMR_RANGE_LIMITED ? 1u : 0u))
+                violation("yuv_to_rgb: job %u chroma sharing / range flags do not match its sizes and range", i);
+            const uint64_t ymax = j.height - 1;
             const uint64_t cy = std::min<uint64_t>(ymax * j.ch / j.height, j.ch - 1);
             const uint64_t cx = std::min<uint64_t>((uint64_t)(j.width - 1) * j.cw / j.width, j.cw - 1);
             check(j.planes[0], ymax * j.luma_stride + j.width, "yuv_to_rgb luma plane");
