@@ -495,6 +495,75 @@ MMR_RECON_API mmr_status mmr_recon_composite_submit(mmr_recon_context *context,
                                                     uint32_t job_count, uint32_t flags,
                                                     uint64_t *out_ticket);
 
+/* -------------------------------------------- MMFX linear light (u16 per channel) */
+
+/*
+ * Surfaces of premultiplied linear-light RGBA with one uint16 per channel, in the order red,
+ * green, blue, alpha (8 bytes per pixel): MMFX's Surface of LinearPixel
+ * (crates/mmfx/src/render.rs:190-280, :1780-1880). Conversions use the reference's
+ * srgb_to_linear_u16 / linear_u16_to_srgb, evaluated on the host once per context.
+ */
+#define MMR_LINEAR_SOURCE_SURFACE 1u  /* a linear u16 surface (uint16[4] per pixel) */
+#define MMR_LINEAR_SOURCE_SRGBA8 2u   /* straight-alpha sRGBA8 (uint8[4] per pixel), LinearPixel::from_rgba */
+
+typedef struct mmr_recon_linear_layer {
+    const void *pixels;    /* MMR_LINEAR_SOURCE_SURFACE: uint16[4]; MMR_LINEAR_SOURCE_SRGBA8: uint8[4] */
+    uint32_t format;       /* MMR_LINEAR_SOURCE_* */
+    uint32_t stride;       /* pixels per row */
+    int32_t x;             /* target position of the layer's pixel (0, 0) */
+    int32_t y;
+    uint32_t width;        /* layer size in pixels */
+    uint32_t height;
+    int32_t clip_left;     /* target-space clip (Clip), right and bottom exclusive */
+    int32_t clip_top;
+    int32_t clip_right;
+    int32_t clip_bottom;
+    uint32_t opacity;      /* 0..65535 */
+} mmr_recon_linear_layer;
+
+/*
+ * Layers in order over a linear target in place: for every target pixel inside the layer's clip,
+ * the target and the layer, target = source.over(target, opacity) (LinearPixel::over,
+ * crates/mmfx/src/render.rs:1826-1854, with multiply_channel / add_saturating :1847-1857).
+ * A surface layer at (0, 0) of the target's size is Surface::blend_surface (:269-277). A sRGBA8
+ * layer is execute_draw_image (:1410-1444) for a destination at integer coordinates of the
+ * image's own size (its nearest-sample lookup then reads the image 1:1), with the product of
+ * the coverage masks (or 65535) as opacity. Sizes up to 65535; at most MMR_RECON_MAX_LAYERS.
+ */
+typedef struct mmr_recon_linear_composite_job {
+    size_t struct_size;
+    uint16_t *target;      /* uint16[4] per pixel */
+    uint32_t width;
+    uint32_t height;
+    uint32_t stride;       /* pixels per row */
+    uint32_t layer_count;
+    mmr_recon_linear_layer layers[MMR_RECON_MAX_LAYERS];
+} mmr_recon_linear_composite_job;
+
+MMR_RECON_API mmr_status mmr_recon_linear_composite_submit(mmr_recon_context *context,
+                                                           const mmr_recon_linear_composite_job *jobs,
+                                                           uint32_t job_count, uint32_t flags,
+                                                           uint64_t *out_ticket);
+
+#define MMR_LINEAR_TO_SRGBA8 1u    /* Surface::to_rgba8 (:233-254): unpremultiply, linear_u16_to_srgb */
+#define MMR_LINEAR_FROM_SRGBA8 2u  /* LinearPixel::from_rgba (:1796-1808) for every pixel */
+
+typedef struct mmr_recon_linear_convert_job {
+    size_t struct_size;
+    uint32_t direction;    /* MMR_LINEAR_TO_SRGBA8 / MMR_LINEAR_FROM_SRGBA8 */
+    uint32_t width;        /* 1..65535 */
+    uint32_t height;       /* 1..65535 */
+    uint16_t *linear;      /* uint16[4] per pixel */
+    uint32_t linear_stride;  /* pixels per row */
+    uint32_t rgba_stride;    /* bytes per row */
+    uint8_t *rgba;         /* straight-alpha sRGBA8 */
+} mmr_recon_linear_convert_job;
+
+MMR_RECON_API mmr_status mmr_recon_linear_convert_submit(mmr_recon_context *context,
+                                                         const mmr_recon_linear_convert_job *jobs,
+                                                         uint32_t job_count, uint32_t flags,
+                                                         uint64_t *out_ticket);
+
 /*
  * Blend a preconverted YUV 4:2:0 overlay with alpha into a YUV 4:2:0 target.
  * Replaces blend_yuv420 / blend_channel (crates/render/src/frame_graph.rs:548-671).

@@ -157,6 +157,12 @@ class Context:
     def submit_composite(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_composite_submit, abi.CompositeJob, jobs, flags)
 
+    def submit_linear_composite(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_linear_composite_submit, abi.LinearCompositeJob, jobs, flags)
+
+    def submit_linear_convert(self, jobs, flags=0):
+        return self._submit(self.lib.mmr_recon_linear_convert_submit, abi.LinearConvertJob, jobs, flags)
+
     def submit_yuv_blend(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_yuv_blend_submit, abi.YuvBlendJob, jobs, flags)
 

@@ -52,6 +52,10 @@ H264_WEIGHTED_TABLE = 1
 BLEND_SOURCE_OVER = 1
 BLEND_PREMULTIPLIED_OVER = 2
 MAX_LAYERS = 8
+LINEAR_SOURCE_SURFACE = 1
+LINEAR_SOURCE_SRGBA8 = 2
+LINEAR_TO_SRGBA8 = 1
+LINEAR_FROM_SRGBA8 = 2
 
 P8 = C.POINTER(C.c_uint8)
 vp = C.c_void_p
@@ -256,6 +260,48 @@ class CompositeJob(C.Structure):
     ]
 
 
+class LinearLayer(C.Structure):
+    _fields_ = [
+        ("pixels", vp),
+        ("format", C.c_uint32),
+        ("stride", C.c_uint32),
+        ("x", C.c_int32),
+        ("y", C.c_int32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("clip_left", C.c_int32),
+        ("clip_top", C.c_int32),
+        ("clip_right", C.c_int32),
+        ("clip_bottom", C.c_int32),
+        ("opacity", C.c_uint32),
+    ]
+
+
+class LinearCompositeJob(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("target", vp),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("stride", C.c_uint32),
+        ("layer_count", C.c_uint32),
+        ("layers", LinearLayer * MAX_LAYERS),
+    ]
+
+
+class LinearConvertJob(C.Structure):
+    _fields_ = [
+        ("struct_size", C.c_size_t),
+        ("direction", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("linear", vp),
+        ("linear_stride", C.c_uint32),
+        ("rgba_stride", C.c_uint32),
+        ("rgba", vp),
+    ]
+
+
 class YuvBlendJob(C.Structure):
     _fields_ = [
         ("struct_size", C.c_size_t),
@@ -426,6 +472,10 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(YuvToRgbJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_composite_submit", C.c_int32,
      [C.c_void_p, C.POINTER(CompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_linear_composite_submit", C.c_int32,
+     [C.c_void_p, C.POINTER(LinearCompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_linear_convert_submit", C.c_int32,
+     [C.c_void_p, C.POINTER(LinearConvertJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_yuv_blend_submit", C.c_int32,
      [C.c_void_p, C.POINTER(YuvBlendJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_overlay_prepare_submit", C.c_int32,

@@ -175,6 +175,124 @@ __global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restric
     else yuv_to_rgb_wave<false>(j, y, x0);
 }
 
+// ---------------------------------------------------------------- MMFX linear light (u16)
+// LinearPixel (crates/mmfx/src/render.rs:1780-1880): premultiplied linear RGBA, CHANNEL_MAX 65535.
+
+// multiply_channel (:1847-1850): (l * r + 32767) / 65535; the product is below 2^32 and the
+// quotient at most 65535, so it needs no saturation.
+MMR_DEV uint32_t linear_mul(uint32_t l, uint32_t r) { return (l * r + 32767u) / 65535u; }
+
+// LinearPixel::over (:1826-1845) with add_saturating (:1852-1854), in place on d.
+MMR_DEV void linear_over(const uint32_t s[4], uint32_t d[4], uint32_t opacity) {
+    const uint32_t sa = linear_mul(s[3], opacity);
+    const uint32_t inverse = 65535u - sa;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) d[c] = min(linear_mul(s[c], opacity) + linear_mul(d[c], inverse), 65535u);
+    d[3] = min(sa + linear_mul(d[3], inverse), 65535u);
+}
+
+// LinearPixel::from_rgba (:1796-1808) through the srgb_to_linear_u16 table.
+MMR_DEV void linear_from_rgba(uint32_t rgba, const uint16_t* __restrict__ to_linear, uint32_t out[4]) {
+    const uint32_t alpha = (rgba >> 24) * 257u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = linear_mul(to_linear[(rgba >> (8 * c)) & 0xffu], alpha);
+    out[3] = alpha;
+}
+
+MMR_DEV void unpack_linear(mmr_v2u32 v, uint32_t p[4]) {
+    p[0] = v.x & 0xffffu;
+    p[1] = v.x >> 16;
+    p[2] = v.y & 0xffffu;
+    p[3] = v.y >> 16;
+}
+
+MMR_DEV mmr_v2u32 pack_linear(const uint32_t p[4]) {
+    mmr_v2u32 v;
+    v.x = p[0] | (p[1] << 16);
+    v.y = p[2] | (p[3] << 16);
+    return v;
+}
+
+// The wave's row and this lane's pixel of a 64-pixel-per-wave linear job; false past the job.
+MMR_DEV bool linear_position(uint32_t tile, uint32_t tile_begin, uint32_t waves_per_row, uint32_t width, uint32_t height,
+                             uint32_t* y, uint32_t* x) {
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - tile_begin) * 4 + (threadIdx.x >> 6)));
+    *y = wave / waves_per_row;
+    *x = (wave - *y * waves_per_row) * 64 + (threadIdx.x & 63);
+    return *y < height && *x < width;
+}
+
+// One lane per target pixel: the layers in order, each where the pixel lies inside its clip and
+// its extent (Surface::blend_surface :269-277; execute_draw_image :1410-1444 at integer
+// destinations of the image's size).
+__global__ __launch_bounds__(256) void linear_composite_kernel(const LinearCompositeDev* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                               uint32_t tiles, const uint16_t* __restrict__ to_linear,
+                                                               uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const LinearCompositeDev& j = jobs[tile_job[tile]];
+    uint32_t y, x;
+    if (!linear_position(tile, j.tile_begin, j.waves_per_row, j.width, j.height, &y, &x)) return;
+    MMR_GLOBAL mmr_v2u32* t = reinterpret_cast<MMR_GLOBAL mmr_v2u32*>(as_global(j.target) + ((size_t)y * j.stride + x) * 4);
+    uint32_t d[4];
+    unpack_linear(*t, d);
+    for (uint32_t l = 0; l < j.layer_count; ++l) {
+        const mmr_recon_linear_layer& L = j.layers[l];
+        if ((int64_t)y < L.clip_top || (int64_t)y >= L.clip_bottom || (int64_t)x < L.clip_left || (int64_t)x >= L.clip_right)
+            continue;
+        const int64_t sx = (int64_t)x - L.x, sy = (int64_t)y - L.y;
+        if (sx < 0 || sy < 0 || sx >= (int64_t)L.width || sy >= (int64_t)L.height) continue;
+        const size_t at = (size_t)sy * L.stride + (size_t)sx;
+        uint32_t src[4];
+        if (L.format == MMR_LINEAR_SOURCE_SRGBA8) {
+            linear_from_rgba(as_global(reinterpret_cast<const uint32_t*>(L.pixels))[at], to_linear, src);
+        } else {
+            unpack_linear(as_global(reinterpret_cast<const mmr_v2u32*>(L.pixels))[at], src);
+        }
+        linear_over(src, d, L.opacity);
+    }
+    *t = pack_linear(d);
+}
+
+// Surface::to_rgba8 (:233-254) or LinearPixel::from_rgba per pixel; tables: srgb_to_linear_u16
+// (256 x u16) then linear_u16_to_srgb (65536 x u8).
+__global__ __launch_bounds__(256) void linear_convert_kernel(const LinearConvertDev* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ tile_job, uint32_t tile_base,
+                                                             uint32_t tiles, const uint8_t* __restrict__ tables,
+                                                             uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const LinearConvertDev& j = jobs[tile_job[tile]];
+    uint32_t y, x;
+    if (!linear_position(tile, j.tile_begin, j.waves_per_row, j.width, j.height, &y, &x)) return;
+    const uint16_t* to_linear = reinterpret_cast<const uint16_t*>(tables);
+    const MMR_GLOBAL uint8_t* to_srgb = as_global(tables + 512);
+    MMR_GLOBAL mmr_v2u32* lin = reinterpret_cast<MMR_GLOBAL mmr_v2u32*>(as_global(j.linear) + ((size_t)y * j.linear_stride + x) * 4);
+    MMR_GLOBAL uint32_t* rgba = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(j.rgba) + (size_t)y * j.rgba_stride) + x;
+    if (j.direction == MMR_LINEAR_FROM_SRGBA8) {
+        uint32_t p[4];
+        linear_from_rgba(*rgba, to_linear, p);
+        *lin = pack_linear(p);
+        return;
+    }
+    uint32_t p[4];
+    unpack_linear(*lin, p);
+    const uint32_t alpha = p[3];
+    uint32_t out = 0;
+    if (alpha != 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const uint32_t linear = min((p[c] * 65535u + alpha / 2) / alpha, 65535u);
+            out |= (uint32_t)to_srgb[linear] << (8 * c);
+        }
+        out |= min((alpha + 128u) / 257u, 255u) << 24;
+    }
+    *rgba = out;
+}
+
 MMR_DEV void blend_rgba_pixel(uint32_t d[4], const uint32_t s[4]) {
     const uint32_t sa = s[3];
     if (sa == 0) return;
@@ -435,6 +553,23 @@ hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, u
     if (tiles == 0) return hipSuccess;
     const uint32_t per = xcd_per(tiles);
     hipLaunchKernelGGL(yuv_to_rgb_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_linear_composite(const LinearCompositeDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                  const uint16_t* d_to_linear, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(linear_composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, d_to_linear,
+                       per);
+    return hipGetLastError();
+}
+
+hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                const uint8_t* d_tables, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(linear_convert_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, d_tables, per);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <functional>
 #include <map>
 #include <memory>
@@ -78,13 +79,18 @@ struct TicketRecord {
     hipEvent_t marks[MMR_RECON_MAX_PHASES - 1] = {};   // between launch phases
     uint32_t mark_count = 0;
     std::vector<double> phase_ms;     // start -> marks[0] -> ... -> kernels_done
-    std::vector<char> host_jobs;      // kept alive until completion (async upload source)
+    std::vector<char> host_jobs;      // host copy of the job records (prepared / allocation-free modes)
+    char* pinned = nullptr;           // pinned staging: status word at 0, job records at kPinnedJobs
+    size_t pinned_bytes = 0;
     uint32_t* host_status = nullptr;  // pinned; device status word copied here
     bool finished = false;
     bool reported = false;            // status returned to the caller (wait / poll / server completion)
     mmr_status status = MMR_STATUS_OK;
     double kernel_ms = -1.0;
 };
+
+// Offset of the job records in a ticket's pinned staging block (the status word sits at 0).
+constexpr size_t kPinnedJobs = 256;
 
 constexpr uint32_t kStatusUnsupported = mmr::kDevStatusUnsupported;
 constexpr uint32_t kStatusInvalidData = mmr::kDevStatusInvalidData;
@@ -138,12 +144,149 @@ struct mmr_recon_context {
     uint64_t next_ticket = 1;
     std::map<uint64_t, TicketRecord> tickets;
     uint8_t* flat_scaling = nullptr;  // device Flat_16 scaling lists ([6][16] + [2][64])
+    uint8_t* linear_tables = nullptr; // device srgb_to_linear_u16 + linear_u16_to_srgb, on first use
     mmr_recon_tuning tuning{};        // launch shapes used by the next submissions
     // Scaler taps by (filter, n, m); they depend on the geometry only (bounded: cleared when full).
     std::map<std::tuple<uint32_t, uint32_t, uint32_t>, std::shared_ptr<const ScaleTaps>> scale_taps;
+    // Ticket resources kept for reuse once a ticket has completed: timing events and pinned
+    // staging blocks by size (powers of two), so a submission neither creates events nor pins
+    // memory in the steady state. Bounded; the surplus is released.
+    std::vector<hipEvent_t> free_events;
+    std::multimap<size_t, char*> free_pinned;
+    size_t free_pinned_bytes = 0;
+    // Pinned staging ring: one pinned allocation made on the first submission, carved in
+    // submission order and released as tickets complete (in any order; the tail advances over
+    // released spans). A submission that does not fit takes a pooled block instead. Allocating
+    // pinned memory per submission would stall the stream pipeline.
+    // Device block of the submissions on the context's own stream: the stream runs them in order,
+    // so each submission's upload into it starts after the previous submission's kernels have
+    // finished and the one block serves all of them (grown, stream-ordered, when one needs more).
+    char* stream_block = nullptr;
+    size_t stream_block_cap = 0;
+    char* ring = nullptr;
+    size_t ring_cap = 0, ring_head = 0, ring_tail = 0;
+    struct RingSpan {
+        size_t off, bytes;
+        bool released;
+    };
+    std::deque<RingSpan> ring_live;
 };
 
 namespace {
+
+constexpr size_t kMaxPooledEvents = 4096;
+constexpr size_t kMaxPooledPinnedBytes = size_t(256) << 20;
+
+hipError_t take_event(mmr_recon_context* ctx, hipEvent_t* ev) {
+    if (!ctx->free_events.empty()) {
+        *ev = ctx->free_events.back();
+        ctx->free_events.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(ev);
+}
+
+void give_event(mmr_recon_context* ctx, hipEvent_t& ev) {
+    if (!ev) return;
+    if (ctx->free_events.size() < kMaxPooledEvents) ctx->free_events.push_back(ev);
+    else (void)hipEventDestroy(ev);
+    ev = nullptr;
+}
+
+constexpr size_t kPinnedRingBytes = size_t(32) << 20;
+
+// `bytes` (a multiple of 256) from the ring, or nullptr when it does not fit now.
+char* ring_take(mmr_recon_context* ctx, size_t bytes) {
+    if (!ctx->ring) {
+        if (hipHostMalloc((void**)&ctx->ring, kPinnedRingBytes, hipHostMallocDefault) != hipSuccess) {
+            ctx->ring = nullptr;
+            return nullptr;
+        }
+        ctx->ring_cap = kPinnedRingBytes;
+    }
+    if (ctx->ring_live.empty()) ctx->ring_head = ctx->ring_tail = 0;
+    size_t off;
+    if (ctx->ring_live.empty() || ctx->ring_head > ctx->ring_tail) {   // free: [head, cap) and [0, tail)
+        if (ctx->ring_cap - ctx->ring_head >= bytes) off = ctx->ring_head;
+        else if (!ctx->ring_live.empty() && ctx->ring_tail > bytes) off = 0;
+        else if (ctx->ring_live.empty() && ctx->ring_cap >= bytes) off = 0;
+        else return nullptr;
+    } else {                                                           // free: [head, tail)
+        if (ctx->ring_tail - ctx->ring_head > bytes) off = ctx->ring_head;
+        else return nullptr;
+    }
+    ctx->ring_head = off + bytes;
+    ctx->ring_live.push_back({off, bytes, false});
+    return ctx->ring + off;
+}
+
+bool ring_give(mmr_recon_context* ctx, char* p) {
+    if (!ctx->ring || p < ctx->ring || p >= ctx->ring + ctx->ring_cap) return false;
+    const size_t off = (size_t)(p - ctx->ring);
+    for (auto& sp : ctx->ring_live)
+        if (sp.off == off && !sp.released) {
+            sp.released = true;
+            break;
+        }
+    while (!ctx->ring_live.empty() && ctx->ring_live.front().released) {
+        ctx->ring_tail = ctx->ring_live.front().off + ctx->ring_live.front().bytes;
+        ctx->ring_live.pop_front();
+    }
+    if (ctx->ring_live.empty()) ctx->ring_head = ctx->ring_tail = 0;
+    else ctx->ring_tail = ctx->ring_live.front().off;
+    return true;
+}
+
+// Pinned staging of at least `bytes`: from the ring, else a pooled block (a power of two, at least
+// 4 KiB).
+hipError_t take_pinned(mmr_recon_context* ctx, size_t bytes, char** out, size_t* out_bytes) {
+    const size_t rounded = (bytes + 255) & ~size_t(255);
+    if (char* r = ring_take(ctx, rounded)) {
+        *out = r;
+        *out_bytes = rounded;
+        return hipSuccess;
+    }
+    size_t cap = 4096;
+    while (cap < bytes) cap <<= 1;
+    auto it = ctx->free_pinned.find(cap);
+    if (it != ctx->free_pinned.end()) {
+        *out = it->second;
+        ctx->free_pinned_bytes -= cap;
+        ctx->free_pinned.erase(it);
+    } else {
+        hipError_t e = hipHostMalloc((void**)out, cap, hipHostMallocDefault);
+        if (e != hipSuccess) return e;
+    }
+    *out_bytes = cap;
+    return hipSuccess;
+}
+
+void give_pinned(mmr_recon_context* ctx, char*& p, size_t bytes) {
+    if (!p) return;
+    if (ring_give(ctx, p)) {
+        p = nullptr;
+        return;
+    }
+    if (ctx->free_pinned_bytes + bytes <= kMaxPooledPinnedBytes) {
+        ctx->free_pinned.emplace(bytes, p);
+        ctx->free_pinned_bytes += bytes;
+    } else {
+        (void)hipHostFree(p);
+    }
+    p = nullptr;
+}
+
+// Returns a ticket's events and pinned block to the pools. Only for tickets whose work has
+// completed (or never reached the stream).
+void release_ticket(mmr_recon_context* ctx, TicketRecord& r) {
+    give_event(ctx, r.start);
+    give_event(ctx, r.kernels_done);
+    give_event(ctx, r.done);
+    for (uint32_t m = 0; m < r.mark_count; ++m) give_event(ctx, r.marks[m]);
+    r.mark_count = 0;
+    give_pinned(ctx, r.pinned, r.pinned_bytes);
+    r.host_status = nullptr;
+}
 
 // Retire finished tickets from the front so the map stays bounded.
 void reap(mmr_recon_context* ctx) {
@@ -152,12 +295,7 @@ void reap(mmr_recon_context* ctx) {
         if (!it->second.finished) {
             if (hipEventQuery(it->second.done) != hipSuccess) break;
         }
-        TicketRecord& r = it->second;
-        if (r.start) (void)hipEventDestroy(r.start);
-        if (r.kernels_done) (void)hipEventDestroy(r.kernels_done);
-        if (r.done) (void)hipEventDestroy(r.done);
-        for (uint32_t m = 0; m < r.mark_count; ++m) (void)hipEventDestroy(r.marks[m]);
-        if (r.host_status) (void)hipHostFree(r.host_status);
+        release_ticket(ctx, it->second);
         ctx->tickets.erase(it);
     }
 }
@@ -189,8 +327,8 @@ mmr_status finish(mmr_recon_context* ctx, TicketRecord& r) {
         }
         r.status = status_from_word(r.host_status ? *r.host_status : 0u);
         r.finished = true;
+        release_ticket(ctx, r);   // the ticket's work is complete: its events and staging are free
     }
-    (void)ctx;
     return r.status;
 }
 
@@ -219,6 +357,8 @@ struct Submission {
     PreparedCapture* cap;
     const Workspace* ws;      // non-null: synchronous submission inside this memory, no allocation
     bool capturing = false;
+    bool enqueued = false;    // stream work of this submission may be in flight
+    bool owns_block = true;   // d_block is this submission's own allocation (not the context's stream block)
     TicketRecord rec;
     char* d_block = nullptr;
     size_t jobs_bytes = 0;
@@ -301,17 +441,39 @@ struct Submission {
         jobs_bytes = (bytes + 255) & ~size_t(255);
         counter_words = counters + 1;  // + status word
         counters_bytes = (counter_words * sizeof(uint32_t) + 255) & ~size_t(255);
-        rec.host_jobs.assign((const char*)jobs, (const char*)jobs + bytes);
-        hipError_t e = hipEventCreate(&rec.start);
-        if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
-        if (e == hipSuccess) e = hipEventCreate(&rec.done);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
+        hipError_t e = take_event(ctx, &rec.start);
+        if (e == hipSuccess) e = take_event(ctx, &rec.kernels_done);
+        if (e == hipSuccess) e = take_event(ctx, &rec.done);
+        if (e == hipSuccess) e = take_pinned(ctx, kPinnedJobs + bytes, &rec.pinned, &rec.pinned_bytes);
         if (e != hipSuccess) return hip_fail(e, "ticket setup");
+        rec.host_status = reinterpret_cast<uint32_t*>(rec.pinned);
         *rec.host_status = 0;
-        e = hipMallocAsync((void**)&d_block, jobs_bytes + counters_bytes + scratch_bytes, stream);
-        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
-        if (patch) patch(rec.host_jobs.data(), scratch_ptr());
-        if (bytes) e = hipMemcpyAsync(d_block, rec.host_jobs.data(), bytes, hipMemcpyHostToDevice, stream);
+        char* staged = rec.pinned + kPinnedJobs;
+        if (bytes) memcpy(staged, jobs, bytes);
+        const size_t need = jobs_bytes + counters_bytes + scratch_bytes;
+        if (stream == ctx->stream) {
+            if (need > ctx->stream_block_cap) {
+                if (ctx->stream_block) (void)hipFreeAsync(ctx->stream_block, stream);
+                ctx->stream_block = nullptr;
+                ctx->stream_block_cap = 0;
+                const size_t grown = std::max(need, need + need / 4);
+                e = hipMallocAsync((void**)&ctx->stream_block, grown, stream);
+                if (e != hipSuccess) {
+                    ctx->stream_block = nullptr;
+                    return hip_fail(e, "hipMallocAsync");
+                }
+                ctx->stream_block_cap = grown;
+            }
+            d_block = ctx->stream_block;
+            owns_block = false;
+        } else {
+            e = hipMallocAsync((void**)&d_block, need, stream);
+            if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+            owns_block = true;
+        }
+        enqueued = true;
+        if (patch) patch(staged, scratch_ptr());
+        if (bytes) e = hipMemcpyAsync(d_block, staged, bytes, hipMemcpyHostToDevice, stream);
         if (e == hipSuccess) e = mmr::launch_fill_zero(counters_ptr(), (uint32_t)counter_words, stream);
         if (e == hipSuccess) e = hipEventRecord(rec.start, stream);
         if (e != hipSuccess) return hip_fail(e, "job upload");
@@ -322,7 +484,7 @@ struct Submission {
     mmr_status mark() {
         if (cap || ws || rec.mark_count >= MMR_RECON_MAX_PHASES - 1) return MMR_STATUS_OK;
         hipEvent_t ev = nullptr;
-        hipError_t e = hipEventCreate(&ev);
+        hipError_t e = take_event(ctx, &ev);
         if (e != hipSuccess) return hip_fail(e, "hipEventCreate (phase)");
         rec.marks[rec.mark_count++] = ev;
         e = hipEventRecord(ev, stream);
@@ -340,12 +502,13 @@ struct Submission {
         hipError_t e = hipEventRecord(rec.kernels_done, stream);
         if (e == hipSuccess) e = hipMemcpyAsync(rec.host_status, status_ptr(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipEventRecord(rec.done, stream);
-        if (e == hipSuccess) e = hipFreeAsync(d_block, stream);
+        if (e == hipSuccess && owns_block) e = hipFreeAsync(d_block, stream);
         d_block = nullptr;
         if (e != hipSuccess) return hip_fail(e, "submission end");
         const uint64_t id = ctx->next_ticket++;
         ctx->tickets.emplace(id, std::move(rec));
         rec = TicketRecord();
+        enqueued = false;
         if (out_ticket) *out_ticket = id;
         return MMR_STATUS_OK;
     }
@@ -356,15 +519,14 @@ struct Submission {
         }
         if (d_block && !ws) {
             if (cap) (void)hipFree(d_block);
-            else (void)hipFreeAsync(d_block, stream);
+            else if (owns_block) (void)hipFreeAsync(d_block, stream);
         } else if (d_block) {
             (void)hipStreamSynchronize(stream);   // a failed workspace submission: the next call may reuse the memory
         }
-        if (rec.start) (void)hipEventDestroy(rec.start);
-        if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
-        if (rec.done) (void)hipEventDestroy(rec.done);
-        for (uint32_t m = 0; m < rec.mark_count; ++m) (void)hipEventDestroy(rec.marks[m]);
-        if (rec.host_status) (void)hipHostFree(rec.host_status);
+        // A submission that failed after enqueueing work: the upload may still read the pinned
+        // staging and the events may still be recorded, so wait before they go back to the pools.
+        if (enqueued && (rec.pinned || rec.start)) (void)hipStreamSynchronize(stream);
+        release_ticket(ctx, rec);
     }
 };
 
@@ -990,6 +1152,144 @@ mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
     return sub.end(out_ticket);
 }
 
+// srgb_to_linear_u16 / linear_u16_to_srgb (crates/mmfx/src/render.rs:1856-1880) as device tables,
+// built on the host with the C library's pow (what Rust's f64::powf calls on Linux) on first use.
+mmr_status ensure_linear_tables(mmr_recon_context* ctx) {
+    if (ctx->linear_tables) return MMR_STATUS_OK;
+    std::vector<uint8_t> host(mmr::kLinearTableBytes);
+    uint16_t* to_linear = reinterpret_cast<uint16_t*>(host.data());
+    for (int v = 0; v < 256; ++v) {
+        const double encoded = (double)v / 255.0;
+        const double linear = encoded <= 0.04045 ? encoded / 12.92 : pow((encoded + 0.055) / 1.055, 2.4);
+        to_linear[v] = (uint16_t)round(linear * 65535.0);
+    }
+    uint8_t* to_srgb = host.data() + 512;
+    for (uint32_t v = 0; v < 65536; ++v) {
+        const double linear = (double)v / 65535.0;
+        const double encoded = linear <= 0.0031308 ? linear * 12.92 : 1.055 * pow(linear, 1.0 / 2.4) - 0.055;
+        const double r = round(encoded * 255.0);
+        to_srgb[v] = (uint8_t)(r < 0.0 ? 0.0 : (r > 255.0 ? 255.0 : r));
+    }
+    uint8_t* d = nullptr;
+    hipError_t e = hipMalloc((void**)&d, host.size());
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc (linear-light tables)");
+    e = hipMemcpy(d, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "linear-light table upload");
+    }
+    ctx->linear_tables = d;
+    return MMR_STATUS_OK;
+}
+
+mmr_status linear_composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_linear_composite_job* in,
+                                      uint32_t count, uint32_t flags, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::LinearCompositeDev> jobs(count);
+    std::vector<uint32_t> tiles;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_linear_composite_job& c = in[i];
+        if (c.struct_size < sizeof(mmr_recon_linear_composite_job))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "linear composite struct_size too small");
+        if (!c.target || !c.width || !c.height || c.width > 65535 || c.height > 65535 || c.stride < c.width ||
+            c.layer_count > MMR_RECON_MAX_LAYERS)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "linear composite job %u is invalid", i);
+        if (!aligned(c.target, 8)) return fail(MMR_STATUS_INVALID_ARGUMENT, "linear composite target must be 8-byte aligned");
+        for (uint32_t l = 0; l < c.layer_count; ++l) {
+            const mmr_recon_linear_layer& L = c.layers[l];
+            if (L.format != MMR_LINEAR_SOURCE_SURFACE && L.format != MMR_LINEAR_SOURCE_SRGBA8)
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "linear composite job %u layer %u: format %u invalid", i, l, L.format);
+            if (!L.pixels || L.width > 65535 || L.height > 65535 || L.stride < L.width ||
+                !aligned(L.pixels, L.format == MMR_LINEAR_SOURCE_SURFACE ? 8 : 4))
+                return fail(MMR_STATUS_INVALID_ARGUMENT, "linear composite job %u layer %u storage is invalid", i, l);
+            if (L.opacity > 65535) return fail(MMR_STATUS_INVALID_ARGUMENT, "linear opacity must be 0..65535");
+        }
+        mmr::LinearCompositeDev& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.target = c.target;
+        j.width = c.width;
+        j.height = c.height;
+        j.stride = c.stride;
+        j.layer_count = c.layer_count;
+        for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
+        j.waves_per_row = (c.width + 63) / 64;
+        j.tile_begin = add_tiles(tiles, i, mmr::linear_job_tiles(j.waves_per_row, j.height), 1);
+    }
+    mmr_status s = ensure_linear_tables(ctx);
+    if (s) return s;
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::LinearCompositeDev));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
+    Submission sub(ctx, stream);
+    s = sub.begin(blob.data(), blob.size(), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::LinearCompositeDev*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
+    const uint16_t* to_linear = reinterpret_cast<const uint16_t*>(ctx->linear_tables);
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_linear_composite(dj, dt, jobs[i].tile_begin, mmr::linear_job_tiles(jobs[i].waves_per_row, jobs[i].height),
+                                             to_linear, stream);
+    } else {
+        e = mmr::launch_linear_composite(dj, dt, 0, (uint32_t)tiles.size(), to_linear, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "linear composite launch");
+    return sub.end(out_ticket);
+}
+
+mmr_status linear_convert_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_linear_convert_job* in,
+                                    uint32_t count, uint32_t flags, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::LinearConvertDev> jobs(count);
+    std::vector<uint32_t> tiles;
+    for (uint32_t i = 0; i < count; ++i) {
+        const mmr_recon_linear_convert_job& c = in[i];
+        if (c.struct_size < sizeof(mmr_recon_linear_convert_job))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "linear convert struct_size too small");
+        if (c.direction != MMR_LINEAR_TO_SRGBA8 && c.direction != MMR_LINEAR_FROM_SRGBA8)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "linear convert job %u: direction %u invalid", i, c.direction);
+        if (!c.linear || !c.rgba || !c.width || !c.height || c.width > 65535 || c.height > 65535 || c.linear_stride < c.width ||
+            c.rgba_stride < (uint64_t)c.width * 4 || c.rgba_stride % 4 || !aligned(c.linear, 8) || !aligned(c.rgba, 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "linear convert job %u storage is invalid", i);
+        mmr::LinearConvertDev& j = jobs[i];
+        memset(&j, 0, sizeof j);
+        j.linear = c.linear;
+        j.rgba = c.rgba;
+        j.direction = c.direction;
+        j.width = c.width;
+        j.height = c.height;
+        j.linear_stride = c.linear_stride;
+        j.rgba_stride = c.rgba_stride;
+        j.waves_per_row = (c.width + 63) / 64;
+        j.tile_begin = add_tiles(tiles, i, mmr::linear_job_tiles(j.waves_per_row, j.height), 1);
+    }
+    mmr_status s = ensure_linear_tables(ctx);
+    if (s) return s;
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::LinearConvertDev));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
+    Submission sub(ctx, stream);
+    s = sub.begin(blob.data(), blob.size(), 0);
+    if (s) return s;
+    auto* dj = reinterpret_cast<const mmr::LinearConvertDev*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_linear_convert(dj, dt, jobs[i].tile_begin, mmr::linear_job_tiles(jobs[i].waves_per_row, jobs[i].height),
+                                           ctx->linear_tables, stream);
+    } else {
+        e = mmr::launch_linear_convert(dj, dt, 0, (uint32_t)tiles.size(), ctx->linear_tables, stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "linear convert launch");
+    return sub.end(out_ticket);
+}
+
 mmr_status yuv_blend_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
     CHECK_CTX(ctx);
@@ -1297,7 +1597,17 @@ mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_co
         if (e == hipSuccess) e = hipGetDeviceProperties(&prop, ctx->device);
         if (e == hipSuccess) ctx->cu_count = (uint32_t)prop.multiProcessorCount;
         if (e == hipSuccess) e = noalloc_scratch(ctx->device, &ctx->noalloc_scratch);   // with the device current
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) {
+            // Submission blocks come from the device's stream-ordered pool; keeping freed memory in
+            // the pool (instead of returning it at each synchronisation) lets a submission allocate
+            // without waiting for the device while earlier ones are in flight.
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess && pool) {
+                uint64_t keep = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            }
+        }
         if (e == hipSuccess) e = mmr::dv_upload_tables(ctx->stream);
         if (e == hipSuccess) e = hipMalloc((void**)&ctx->flat_scaling, 96 + 128);
         if (e == hipSuccess) {
@@ -1309,6 +1619,7 @@ mmr_status mmr_recon_context_create(const mmr_recon_config* config, mmr_recon_co
             mmr_status s = hip_fail(e, "context setup");
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
             if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+            if (ctx->linear_tables) (void)hipFree(ctx->linear_tables);
             delete ctx;
             return s;
         }
@@ -1322,13 +1633,16 @@ void mmr_recon_context_destroy(mmr_recon_context* ctx) {
         if (!ctx) return;
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        for (auto& kv : ctx->tickets) {
-            if (kv.second.start) (void)hipEventDestroy(kv.second.start);
-            if (kv.second.kernels_done) (void)hipEventDestroy(kv.second.kernels_done);
-            if (kv.second.done) (void)hipEventDestroy(kv.second.done);
-            if (kv.second.host_status) (void)hipHostFree(kv.second.host_status);
+        for (auto& kv : ctx->tickets) release_ticket(ctx, kv.second);
+        for (hipEvent_t ev : ctx->free_events) (void)hipEventDestroy(ev);
+        for (auto& kv : ctx->free_pinned) (void)hipHostFree(kv.second);
+        if (ctx->ring) (void)hipHostFree(ctx->ring);
+        if (ctx->stream_block) {
+            (void)hipFreeAsync(ctx->stream_block, ctx->stream);
+            (void)hipStreamSynchronize(ctx->stream);
         }
         if (ctx->flat_scaling) (void)hipFree(ctx->flat_scaling);
+        if (ctx->linear_tables) (void)hipFree(ctx->linear_tables);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
     });
@@ -1402,7 +1716,7 @@ mmr_status mmr_recon_poll(mmr_recon_context* ctx, uint64_t ticket, int32_t* out_
             *out_done = 1;
             return MMR_STATUS_OK;
         }
-        hipError_t q = hipEventQuery(it->second.done);
+        const hipError_t q = it->second.finished ? hipSuccess : hipEventQuery(it->second.done);
         if (q == hipErrorNotReady) {
             *out_done = 0;
             return MMR_STATUS_OK;
@@ -1550,6 +1864,24 @@ mmr_status mmr_recon_composite_submit(mmr_recon_context* ctx, const mmr_recon_co
     });
 }
 
+mmr_status mmr_recon_linear_composite_submit(mmr_recon_context* ctx, const mmr_recon_linear_composite_job* in, uint32_t count,
+                                             uint32_t flags, uint64_t* out_ticket) {
+    return guarded("mmr_recon_linear_composite_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::linear_composite_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
+}
+
+mmr_status mmr_recon_linear_convert_submit(mmr_recon_context* ctx, const mmr_recon_linear_convert_job* in, uint32_t count,
+                                           uint32_t flags, uint64_t* out_ticket) {
+    return guarded("mmr_recon_linear_convert_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::linear_convert_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
+}
+
 mmr_status mmr_recon_yuv_blend_submit(mmr_recon_context* ctx, const mmr_recon_yuv_blend_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
     return guarded("mmr_recon_yuv_blend_submit", [&]() -> mmr_status {
@@ -1646,11 +1978,12 @@ mmr_status mmr_recon_prepared_launch(mmr_recon_context* ctx, mmr_recon_prepared*
         if (ctx->tickets.size() >= ctx->queue_depth) reap(ctx);
         if (ctx->tickets.size() >= ctx->queue_depth) return fail(MMR_STATUS_BUSY, "queue full: %u tickets in flight", ctx->queue_depth);
         TicketRecord rec;
-        hipError_t e = hipEventCreate(&rec.start);
-        if (e == hipSuccess) e = hipEventCreate(&rec.kernels_done);
-        if (e == hipSuccess) e = hipEventCreate(&rec.done);
-        if (e == hipSuccess) e = hipHostMalloc((void**)&rec.host_status, sizeof(uint32_t), hipHostMallocDefault);
+        hipError_t e = take_event(ctx, &rec.start);
+        if (e == hipSuccess) e = take_event(ctx, &rec.kernels_done);
+        if (e == hipSuccess) e = take_event(ctx, &rec.done);
+        if (e == hipSuccess) e = take_pinned(ctx, kPinnedJobs, &rec.pinned, &rec.pinned_bytes);
         if (e == hipSuccess) {
+            rec.host_status = reinterpret_cast<uint32_t*>(rec.pinned);
             *rec.host_status = 0;
             e = hipEventRecord(rec.start, ctx->stream);
         }
@@ -1661,10 +1994,7 @@ mmr_status mmr_recon_prepared_launch(mmr_recon_context* ctx, mmr_recon_prepared*
         if (e != hipSuccess) {
             // the events may be in flight: wait for the stream before releasing them
             (void)hipStreamSynchronize(ctx->stream);
-            if (rec.start) (void)hipEventDestroy(rec.start);
-            if (rec.kernels_done) (void)hipEventDestroy(rec.kernels_done);
-            if (rec.done) (void)hipEventDestroy(rec.done);
-            if (rec.host_status) (void)hipHostFree(rec.host_status);
+            release_ticket(ctx, rec);
             return hip_fail(e, "prepared batch launch");
         }
         const uint64_t id = ctx->next_ticket++;
@@ -1826,7 +2156,6 @@ mmr_status mmr_recon_autotune(mmr_recon_context* ctx, uint32_t codec, const void
 
 #include <chrono>
 #include <condition_variable>
-#include <deque>
 #include <list>
 #include <memory>
 #include <thread>

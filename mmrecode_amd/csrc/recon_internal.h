@@ -148,6 +148,24 @@ inline void set_rgb_terms(RgbJob& j) {
     j.half_chroma = (uint64_t)j.cw * 2 == j.width ? 1u : 0u;
 }
 
+// MMFX linear-light jobs (u16 premultiplied RGBA): one lane per pixel, one wave per 64 pixels of
+// a row, four waves per workgroup tile; job j has ceil(waves_per_row * height / 4) tiles.
+struct LinearCompositeDev {
+    uint16_t* target;
+    uint32_t width, height, stride, layer_count;
+    uint32_t tile_begin, waves_per_row;
+    mmr_recon_linear_layer layers[MMR_RECON_MAX_LAYERS];
+};
+struct LinearConvertDev {
+    uint16_t* linear;
+    uint8_t* rgba;
+    uint32_t direction, width, height, linear_stride, rgba_stride;
+    uint32_t tile_begin, waves_per_row, pad;
+};
+inline uint32_t linear_job_tiles(uint32_t waves_per_row, uint32_t height) { return (waves_per_row * height + 3) / 4; }
+// Device copy of srgb_to_linear_u16 (256 x u16) followed by linear_u16_to_srgb (65536 x u8).
+constexpr size_t kLinearTableBytes = 256 * 2 + 65536;
+
 struct CompositeJobDev {
     uint8_t* target;
     uint32_t width, height, stride, layer_count;
@@ -182,6 +200,10 @@ bool h264_deblock_rows_supported(uint32_t rows);
 hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_h264_deblock(const H264Job* d_jobs, uint32_t job_count, uint32_t total_bands, uint32_t* d_counters,
                                uint32_t groups, uint32_t band_rows, uint32_t publish, uint32_t* d_status, hipStream_t s);
+hipError_t launch_linear_composite(const LinearCompositeDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                  const uint16_t* d_to_linear, hipStream_t s);
+hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                const uint8_t* d_tables, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);

@@ -30,6 +30,14 @@ def lib():
         L = C.CDLL(path)
         i32, u8, u32, dp = C.c_int32, C.c_uint8, C.c_uint32, C.c_void_p
         sig = {
+            "oracle_linear_multiply_channel": (C.c_uint16, [C.c_uint16, C.c_uint16]),
+            "oracle_srgb_to_linear_u16": (C.c_uint16, [u8]),
+            "oracle_linear_u16_to_srgb": (u8, [C.c_uint16]),
+            "oracle_linear_from_rgba": (None, [dp, dp]),
+            "oracle_linear_over": (None, [dp, dp, C.c_uint16]),
+            "oracle_linear_to_rgba8": (None, [dp, dp]),
+            "oracle_linear_composite": (i32, [C.POINTER(abi.LinearCompositeJob)]),
+            "oracle_linear_convert": (i32, [C.POINTER(abi.LinearConvertJob)]),
             "oracle_mjpeg_inverse_dct": (None, [dp, dp]),
             "oracle_mjpeg_dequantize": (None, [dp, dp, dp]),
             "oracle_mjpeg_block_count": (u32, [C.POINTER(abi.MjpegFrame)]),

@@ -344,3 +344,109 @@ int oracle_scale_yuv420(const mmr_recon_scale_job *j) {
     }
     return 0;
 }
+
+/* ---------------------------------------------------------------- MMFX linear light (u16)
+ * LinearPixel (crates/mmfx/src/render.rs:1780-1880): premultiplied linear-light RGBA with
+ * CHANNEL_MAX = 65535 (:23). */
+#define LINEAR_MAX 65535u
+
+/* multiply_channel (:1847-1850): (left * right + CHANNEL_MAX / 2) / CHANNEL_MAX, saturated to u16. */
+uint16_t oracle_linear_multiply_channel(uint16_t left, uint16_t right) {
+    const uint32_t q = ((uint32_t)left * right + LINEAR_MAX / 2) / LINEAR_MAX;
+    return (uint16_t)(q > 65535u ? 65535u : q);
+}
+
+/* add_saturating (:1852-1854). */
+static uint16_t linear_add_saturating(uint16_t a, uint16_t b) {
+    const uint32_t s = (uint32_t)a + b;
+    return (uint16_t)(s > 65535u ? 65535u : s);
+}
+
+/* srgb_to_linear_u16 (:1856-1867): f64, the C library's pow (Rust's powf on Linux), round half away. */
+uint16_t oracle_srgb_to_linear_u16(uint8_t value) {
+    const double encoded = (double)value / 255.0;
+    const double linear = encoded <= 0.04045 ? encoded / 12.92 : pow((encoded + 0.055) / 1.055, 2.4);
+    return (uint16_t)round(linear * 65535.0);
+}
+
+/* linear_u16_to_srgb (:1869-1880). */
+uint8_t oracle_linear_u16_to_srgb(uint16_t value) {
+    const double linear = (double)value / 65535.0;
+    const double encoded = linear <= 0.0031308 ? linear * 12.92 : 1.055 * pow(linear, 1.0 / 2.4) - 0.055;
+    double r = round(encoded * 255.0);
+    r = r < 0.0 ? 0.0 : (r > 255.0 ? 255.0 : r);
+    return (uint8_t)r;
+}
+
+/* LinearPixel::from_color / from_rgba (:1796-1808): alpha = a * 257, channels premultiplied. */
+void oracle_linear_from_rgba(const uint8_t rgba[4], uint16_t out[4]) {
+    const uint16_t alpha = (uint16_t)(rgba[3] * 257u);
+    for (int c = 0; c < 3; ++c) out[c] = oracle_linear_multiply_channel(oracle_srgb_to_linear_u16(rgba[c]), alpha);
+    out[3] = alpha;
+}
+
+/* LinearPixel::over (:1826-1845): source over destination with an extra opacity. */
+void oracle_linear_over(const uint16_t src[4], uint16_t dst[4], uint16_t opacity) {
+    const uint16_t source_alpha = oracle_linear_multiply_channel(src[3], opacity);
+    const uint16_t inverse = (uint16_t)(65535u - source_alpha);
+    for (int c = 0; c < 3; ++c)
+        dst[c] = linear_add_saturating(oracle_linear_multiply_channel(src[c], opacity),
+                                       oracle_linear_multiply_channel(dst[c], inverse));
+    dst[3] = linear_add_saturating(source_alpha, oracle_linear_multiply_channel(dst[3], inverse));
+}
+
+/* Surface::to_rgba8 (:233-254) for one pixel. */
+void oracle_linear_to_rgba8(const uint16_t px[4], uint8_t out[4]) {
+    const uint32_t alpha = px[3];
+    if (alpha == 0) {
+        out[0] = out[1] = out[2] = out[3] = 0;
+        return;
+    }
+    for (int c = 0; c < 3; ++c) {
+        uint32_t linear = ((uint32_t)px[c] * LINEAR_MAX + alpha / 2) / alpha;
+        if (linear > LINEAR_MAX) linear = LINEAR_MAX;
+        out[c] = oracle_linear_u16_to_srgb((uint16_t)linear);
+    }
+    const uint32_t a8 = (alpha + 128) / 257;
+    out[3] = (uint8_t)(a8 > 255 ? 255 : a8);
+}
+
+/* Layers in order: Surface::blend_surface (:269-277) for surface layers, execute_draw_image
+ * (:1410-1444) for sRGBA8 layers at integer destinations of the image's size (every target
+ * pixel in the clip and the layer takes source.over(target, opacity)). */
+int oracle_linear_composite(const mmr_recon_linear_composite_job *j) {
+    for (uint32_t l = 0; l < j->layer_count; ++l) {
+        const mmr_recon_linear_layer *L = &j->layers[l];
+        const int64_t top = L->clip_top > 0 ? L->clip_top : 0;
+        const int64_t left = L->clip_left > 0 ? L->clip_left : 0;
+        const int64_t bottom = L->clip_bottom < (int64_t)j->height ? L->clip_bottom : (int64_t)j->height;
+        const int64_t right = L->clip_right < (int64_t)j->width ? L->clip_right : (int64_t)j->width;
+        for (int64_t y = top; y < bottom; ++y)
+            for (int64_t x = left; x < right; ++x) {
+                const int64_t sx = x - L->x, sy = y - L->y;
+                if (sx < 0 || sy < 0 || sx >= (int64_t)L->width || sy >= (int64_t)L->height) continue;
+                uint16_t src[4];
+                if (L->format == MMR_LINEAR_SOURCE_SRGBA8) {
+                    oracle_linear_from_rgba((const uint8_t *)L->pixels + ((size_t)sy * L->stride + (size_t)sx) * 4, src);
+                } else if (L->format == MMR_LINEAR_SOURCE_SURFACE) {
+                    memcpy(src, (const uint16_t *)L->pixels + ((size_t)sy * L->stride + (size_t)sx) * 4, sizeof src);
+                } else {
+                    return -1;
+                }
+                oracle_linear_over(src, j->target + ((size_t)y * j->stride + (size_t)x) * 4, (uint16_t)L->opacity);
+            }
+    }
+    return 0;
+}
+
+int oracle_linear_convert(const mmr_recon_linear_convert_job *j) {
+    for (uint32_t y = 0; y < j->height; ++y)
+        for (uint32_t x = 0; x < j->width; ++x) {
+            uint16_t *lin = j->linear + ((size_t)y * j->linear_stride + x) * 4;
+            uint8_t *rgba = j->rgba + (size_t)y * j->rgba_stride + (size_t)x * 4;
+            if (j->direction == MMR_LINEAR_TO_SRGBA8) oracle_linear_to_rgba8(lin, rgba);
+            else if (j->direction == MMR_LINEAR_FROM_SRGBA8) oracle_linear_from_rgba(rgba, lin);
+            else return -1;
+        }
+    return 0;
+}

@@ -134,6 +134,14 @@ void oracle_premultiplied_over_pixel(uint8_t destination[4], const uint8_t sourc
                                      uint32_t opacity);
 uint8_t oracle_blend_channel(uint8_t destination, uint8_t source, uint8_t alpha);
 int oracle_composite(const mmr_recon_composite_job *job);
+uint16_t oracle_linear_multiply_channel(uint16_t left, uint16_t right);
+uint16_t oracle_srgb_to_linear_u16(uint8_t value);
+uint8_t oracle_linear_u16_to_srgb(uint16_t value);
+void oracle_linear_from_rgba(const uint8_t rgba[4], uint16_t out[4]);
+void oracle_linear_over(const uint16_t src[4], uint16_t dst[4], uint16_t opacity);
+void oracle_linear_to_rgba8(const uint16_t px[4], uint8_t out[4]);
+int oracle_linear_composite(const mmr_recon_linear_composite_job *job);
+int oracle_linear_convert(const mmr_recon_linear_convert_job *job);
 int oracle_yuv_blend(const mmr_recon_yuv_blend_job *job);
 void oracle_rgb_to_yuv(uint8_t red, uint8_t green, uint8_t blue, uint8_t limited[3], uint8_t full[3]);
 int oracle_overlay_prepare(const mmr_recon_overlay_prepare_job *job);

@@ -31,6 +31,8 @@ typedef struct ModelEvent* hipEvent_t;
 typedef struct ModelGraph* hipGraph_t;
 typedef struct ModelGraph* hipGraphExec_t;
 typedef struct ModelGraphNode* hipGraphNode_t;
+typedef struct ModelMemPool* hipMemPool_t;
+enum hipMemPoolAttr { hipMemPoolAttrReleaseThreshold = 4 };
 
 enum hipMemcpyKind { hipMemcpyHostToHost = 0, hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3, hipMemcpyDefault = 4 };
 enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1, hipStreamCaptureModeRelaxed = 2 };
@@ -63,6 +65,8 @@ hipError_t hipMemcpy(void* dst, const void* src, size_t bytes, hipMemcpyKind kin
 hipError_t hipMemcpyAsync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s);
 hipError_t hipMemsetAsync(void* dst, int value, size_t bytes, hipStream_t s);
 hipError_t hipGetSymbolAddress(void** out, const void* symbol);
+hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int device);
+hipError_t hipMemPoolSetAttribute(hipMemPool_t pool, hipMemPoolAttr attr, void* value);
 
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned flags);
 hipError_t hipStreamDestroy(hipStream_t s);

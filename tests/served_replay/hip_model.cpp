@@ -413,6 +413,13 @@ hipError_t hipGetSymbolAddress(void** out, const void* symbol) {
     return *out ? hipSuccess : ret(hipErrorMemoryAllocation);
 }
 
+// The stream-ordered pool's settings do not change what the model checks.
+hipError_t hipDeviceGetDefaultMemPool(hipMemPool_t* pool, int) {
+    *pool = reinterpret_cast<hipMemPool_t>(uintptr_t(1));
+    return hipSuccess;
+}
+hipError_t hipMemPoolSetAttribute(hipMemPool_t, hipMemPoolAttr, void*) { return hipSuccess; }
+
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
     LOCK;
     *s = new ModelStream();

@@ -317,6 +317,63 @@ hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, u
     return hipSuccess;
 }
 
+// color.hip: linear_composite_kernel. Tile t belongs to job tile_job[t]; wave w of the job covers
+// row w / waves_per_row, pixels (w % waves_per_row) * 64 .. + 63 clipped to the width. Each layer
+// reads the pixels of the intersection of its clip, the target and its own extent.
+hipError_t launch_linear_composite(const LinearCompositeDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                  const uint16_t* d_to_linear, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    model::enqueue(s, "linear_composite", [=] {
+        std::map<uint32_t, std::vector<uint32_t>> offs;
+        std::map<uint32_t, LinearCompositeDev> recs;
+        if (!walk_tiles("linear_composite job / tile table", d_jobs, d_tile_job, tile_base, tiles, offs, recs)) return;
+        bool srgb = false;
+        for (auto& [i, v] : offs) {
+            const LinearCompositeDev& j = recs[i];
+            model::stats().jobs++;
+            if (j.waves_per_row != (j.width + 63) / 64) violation("linear_composite: job %u waves per row", i);
+            check_tiles("linear_composite", i, v, linear_job_tiles(j.waves_per_row, j.height));
+            if (!j.width || !j.height) continue;
+            check(j.target, ((uint64_t)(j.height - 1) * j.stride + j.width) * 8, "linear_composite target");
+            for (uint32_t l = 0; l < j.layer_count && l < MMR_RECON_MAX_LAYERS; ++l) {
+                const mmr_recon_linear_layer& L = j.layers[l];
+                const int64_t y0 = std::max<int64_t>({0, L.clip_top, L.y});
+                const int64_t y1 = std::min<int64_t>({(int64_t)j.height, L.clip_bottom, (int64_t)L.y + L.height});
+                const int64_t x0 = std::max<int64_t>({0, L.clip_left, L.x});
+                const int64_t x1 = std::min<int64_t>({(int64_t)j.width, L.clip_right, (int64_t)L.x + L.width});
+                if (y0 >= y1 || x0 >= x1) continue;
+                const uint64_t bpp = L.format == MMR_LINEAR_SOURCE_SRGBA8 ? 4 : 8;
+                srgb |= L.format == MMR_LINEAR_SOURCE_SRGBA8;
+                check(L.pixels, ((uint64_t)(y1 - 1 - L.y) * L.stride + (uint64_t)(x1 - 1 - L.x) + 1) * bpp, "linear_composite layer");
+            }
+        }
+        if (srgb) check(d_to_linear, 512, "linear_composite srgb_to_linear table");
+    });
+    return hipSuccess;
+}
+
+// color.hip: linear_convert_kernel, the same pixel walk; reads / writes one linear and one sRGBA8
+// pixel per lane and the tables.
+hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
+                                const uint8_t* d_tables, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    model::enqueue(s, "linear_convert", [=] {
+        std::map<uint32_t, std::vector<uint32_t>> offs;
+        std::map<uint32_t, LinearConvertDev> recs;
+        if (!walk_tiles("linear_convert job / tile table", d_jobs, d_tile_job, tile_base, tiles, offs, recs)) return;
+        check(d_tables, kLinearTableBytes, "linear_convert tables");
+        for (auto& [i, v] : offs) {
+            const LinearConvertDev& j = recs[i];
+            model::stats().jobs++;
+            check_tiles("linear_convert", i, v, linear_job_tiles(j.waves_per_row, j.height));
+            if (!j.width || !j.height) continue;
+            check(j.linear, ((uint64_t)(j.height - 1) * j.linear_stride + j.width) * 8, "linear_convert linear surface");
+            check(j.rgba, (uint64_t)(j.height - 1) * j.rgba_stride + (uint64_t)j.width * 4, "linear_convert sRGBA8 image");
+        }
+    });
+    return hipSuccess;
+}
+
 // color.hip: composite_kernel. Pixel gp maps to the last job with px_begin <= gp; row
 // y = p / width of the target (stride bytes apart) and of every layer that covers it.
 hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s) {
