@@ -34,6 +34,9 @@ STRUCTS = {
     "mmr_recon_yuv_to_rgb_job": abi.YuvToRgbJob,
     "mmr_recon_layer": abi.Layer,
     "mmr_recon_composite_job": abi.CompositeJob,
+    "mmr_recon_linear_layer": abi.LinearLayer,
+    "mmr_recon_linear_composite_job": abi.LinearCompositeJob,
+    "mmr_recon_linear_convert_job": abi.LinearConvertJob,
     "mmr_recon_yuv_blend_job": abi.YuvBlendJob,
     "mmr_recon_overlay_prepare_job": abi.OverlayPrepareJob,
     "mmr_recon_scale_job": abi.ScaleJob,
