@@ -730,6 +730,81 @@ MMR_RECON_API mmr_status mmr_recon_autotune(mmr_recon_context *context, uint32_t
                                             mmr_recon_tuning_trial *trials, uint32_t trial_capacity,
                                             uint32_t *out_trials);
 
+/* ------------------------------------------------- resident frame cache */
+
+/*
+ * Library-owned, byte-budgeted LRU cache of device frame storage with generation protection.
+ * Replaces DeviceResourceCache (crates/render/src/device_cache.rs:53-287): a semantic key
+ * (FrameResourceKey, crates/render/src/frame_graph.rs:28-43) maps to one device allocation
+ * that the cache creates on the first retain and reuses afterwards. Entries retained or fetched
+ * in the current generation are protected: a retain that would have to evict one of them fails
+ * with MMR_STATUS_INVALID_STATE and leaves the cache within budget. Evicted and removed storage
+ * is released stream-ordered on the context's stream, so submissions of the context enqueued
+ * before the release finish with it; work on other streams must be complete before the call
+ * that evicts. Calls on one cache are serialised by the context.
+ */
+typedef struct mmr_recon_frame_cache mmr_recon_frame_cache;
+
+typedef struct mmr_recon_frame_key {
+    uint32_t namespace_id;         /* FrameResourceNamespace */
+    uint32_t variant;              /* backend-independent variant discriminator */
+    uint64_t owner;                /* stable media identifier, 0 for caller-owned targets */
+    uint64_t revision;             /* deterministic content revision */
+    int64_t local_frame;           /* source-local frame, -1 for a time-invariant resource */
+    uint32_t width;
+    uint32_t height;
+} mmr_recon_frame_key;
+
+typedef struct mmr_recon_frame_descriptor {   /* FrameDescriptor: compared on every hit */
+    uint32_t width;
+    uint32_t height;
+    uint32_t format;
+    uint32_t color;
+    uint32_t alpha_mode;
+} mmr_recon_frame_descriptor;
+
+#define MMR_CACHE_INSERTED 1u
+#define MMR_CACHE_REUSED 2u
+
+typedef struct mmr_recon_frame_cache_stats {
+    size_t struct_size;
+    uint64_t resources;            /* retained entries */
+    uint64_t retained_bytes;
+    uint64_t byte_budget;
+    uint64_t reuses;
+    uint64_t insertions;
+    uint64_t evictions;
+    uint64_t generation;
+} mmr_recon_frame_cache_stats;
+
+/* byte_budget > 0. */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_create(mmr_recon_context *context, uint64_t byte_budget,
+                                                      mmr_recon_frame_cache **out_cache);
+MMR_RECON_API void mmr_recon_frame_cache_destroy(mmr_recon_frame_cache *cache);
+/* begin_frame: the next generation; what is retained or fetched after it is protected until the next. */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_begin_frame(mmr_recon_frame_cache *cache);
+/* retain_with: the entry's storage (reused: *out_status MMR_CACHE_REUSED) or a new device allocation
+ * of `bytes` (MMR_CACHE_INSERTED) after evicting least recently used entries of earlier
+ * generations. Errors: a descriptor different from the retained one under the same key
+ * (INVALID_STATE), bytes 0 or above the budget (INVALID_DATA), not enough evictable bytes
+ * (INVALID_STATE, nothing inserted). */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_retain(mmr_recon_frame_cache *cache, const mmr_recon_frame_key *key,
+                                                      const mmr_recon_frame_descriptor *descriptor, uint64_t bytes,
+                                                      void **out_device_ptr, uint32_t *out_status);
+/* get: the entry's storage marked used in this generation, or *out_device_ptr = NULL when absent. */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_get(mmr_recon_frame_cache *cache, const mmr_recon_frame_key *key,
+                                                   const mmr_recon_frame_descriptor *descriptor, void **out_device_ptr);
+/* remove: release one entry (*out_removed 1) or nothing (0). */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_remove(mmr_recon_frame_cache *cache, const mmr_recon_frame_key *key,
+                                                      int32_t *out_removed);
+/* release_idle: release entries idle for more than max_idle_generations completed generations
+ * (never those of the current generation); *out_released receives their number. */
+MMR_RECON_API mmr_status mmr_recon_frame_cache_release_idle(mmr_recon_frame_cache *cache, uint64_t max_idle_generations,
+                                                            uint64_t *out_released);
+MMR_RECON_API mmr_status mmr_recon_frame_cache_clear(mmr_recon_frame_cache *cache);
+MMR_RECON_API mmr_status mmr_recon_frame_cache_stats_get(mmr_recon_frame_cache *cache,
+                                                         mmr_recon_frame_cache_stats *out_stats);
+
 /* --------------------------------------------------------------- serving */
 
 /*
@@ -769,7 +844,9 @@ typedef struct mmr_recon_request {
                                        MPEG-2 forward_source / backward_source index this array */
     const mmr_recon_yuv_to_rgb_job *convert;     /* NULL or frame_count jobs */
     const mmr_recon_composite_job *composite;    /* NULL or frame_count jobs (after convert) */
-    uint64_t hbm_bytes;             /* device memory the request holds while admitted */
+    uint64_t hbm_bytes;             /* device memory the request holds while admitted; admission
+                                       counts at least the bytes of the planes, RGB / RGBA images and
+                                       composite targets its descriptors name */
     uint64_t user_tag;
 } mmr_recon_request;
 

@@ -232,6 +232,65 @@ class Context:
         return [out[i] for i in range(min(n.value, abi.MAX_PHASES))]
 
 
+class FrameCache:
+    """mmr_recon_frame_cache_*: the library-owned resident frame cache (DeviceResourceCache,
+    crates/render/src/device_cache.rs:53-287). Keys and descriptors are abi.FrameKey /
+    abi.FrameDescriptor or keyword dicts of their fields."""
+
+    def __init__(self, ctx, byte_budget):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(ctx.lib.mmr_recon_frame_cache_create(ctx.handle, byte_budget, C.byref(h)))
+        self.handle = h
+
+    @staticmethod
+    def _key(k):
+        return k if isinstance(k, abi.FrameKey) else abi.FrameKey(**k)
+
+    @staticmethod
+    def _desc(d):
+        return d if isinstance(d, abi.FrameDescriptor) else abi.FrameDescriptor(**d)
+
+    def begin_frame(self):
+        check(self.ctx.lib.mmr_recon_frame_cache_begin_frame(self.handle))
+
+    def retain(self, key, descriptor, nbytes):
+        """(device pointer, abi.CACHE_INSERTED or abi.CACHE_REUSED)."""
+        p, st = C.c_void_p(), C.c_uint32()
+        check(self.ctx.lib.mmr_recon_frame_cache_retain(self.handle, C.byref(self._key(key)), C.byref(self._desc(descriptor)),
+                                                        nbytes, C.byref(p), C.byref(st)))
+        return p.value, st.value
+
+    def get(self, key, descriptor):
+        p = C.c_void_p()
+        check(self.ctx.lib.mmr_recon_frame_cache_get(self.handle, C.byref(self._key(key)), C.byref(self._desc(descriptor)),
+                                                     C.byref(p)))
+        return p.value
+
+    def remove(self, key):
+        r = C.c_int32()
+        check(self.ctx.lib.mmr_recon_frame_cache_remove(self.handle, C.byref(self._key(key)), C.byref(r)))
+        return bool(r.value)
+
+    def release_idle(self, max_idle_generations):
+        n = C.c_uint64()
+        check(self.ctx.lib.mmr_recon_frame_cache_release_idle(self.handle, max_idle_generations, C.byref(n)))
+        return n.value
+
+    def clear(self):
+        check(self.ctx.lib.mmr_recon_frame_cache_clear(self.handle))
+
+    def stats(self):
+        st = abi.new(abi.FrameCacheStats)
+        check(self.ctx.lib.mmr_recon_frame_cache_stats_get(self.handle, C.byref(st)))
+        return {n: getattr(st, n) for n, _ in abi.FrameCacheStats._fields_ if n != "struct_size"}
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.mmr_recon_frame_cache_destroy(self.handle)
+            self.handle = None
+
+
 class Prepared:
     """mmr_recon_prepare / mmr_recon_prepared_launch: a batch whose job records are uploaded once and
     whose kernel launches are replayed from a HIP graph."""

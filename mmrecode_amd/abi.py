@@ -56,6 +56,8 @@ LINEAR_SOURCE_SURFACE = 1
 LINEAR_SOURCE_SRGBA8 = 2
 LINEAR_TO_SRGBA8 = 1
 LINEAR_FROM_SRGBA8 = 2
+CACHE_INSERTED = 1
+CACHE_REUSED = 2
 
 P8 = C.POINTER(C.c_uint8)
 vp = C.c_void_p
@@ -302,6 +304,27 @@ class LinearConvertJob(C.Structure):
     ]
 
 
+class FrameKey(C.Structure):
+    _fields_ = [
+        ("namespace_id", C.c_uint32),
+        ("variant", C.c_uint32),
+        ("owner", C.c_uint64),
+        ("revision", C.c_uint64),
+        ("local_frame", C.c_int64),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+    ]
+
+
+class FrameDescriptor(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("width", "height", "format", "color", "alpha_mode")]
+
+
+class FrameCacheStats(C.Structure):
+    _fields_ = [("struct_size", C.c_size_t)] + [(n, C.c_uint64) for n in (
+        "resources", "retained_bytes", "byte_budget", "reuses", "insertions", "evictions", "generation")]
+
+
 class YuvBlendJob(C.Structure):
     _fields_ = [
         ("struct_size", C.c_size_t),
@@ -476,6 +499,17 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(LinearCompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_linear_convert_submit", C.c_int32,
      [C.c_void_p, C.POINTER(LinearConvertJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_frame_cache_create", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    ("mmr_recon_frame_cache_destroy", None, [C.c_void_p]),
+    ("mmr_recon_frame_cache_begin_frame", C.c_int32, [C.c_void_p]),
+    ("mmr_recon_frame_cache_retain", C.c_int32, [C.c_void_p, C.POINTER(FrameKey), C.POINTER(FrameDescriptor), C.c_uint64,
+                                                 C.POINTER(C.c_void_p), C.POINTER(C.c_uint32)]),
+    ("mmr_recon_frame_cache_get", C.c_int32, [C.c_void_p, C.POINTER(FrameKey), C.POINTER(FrameDescriptor),
+                                              C.POINTER(C.c_void_p)]),
+    ("mmr_recon_frame_cache_remove", C.c_int32, [C.c_void_p, C.POINTER(FrameKey), C.POINTER(C.c_int32)]),
+    ("mmr_recon_frame_cache_release_idle", C.c_int32, [C.c_void_p, C.c_uint64, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_frame_cache_clear", C.c_int32, [C.c_void_p]),
+    ("mmr_recon_frame_cache_stats_get", C.c_int32, [C.c_void_p, C.POINTER(FrameCacheStats)]),
     ("mmr_recon_yuv_blend_submit", C.c_int32,
      [C.c_void_p, C.POINTER(YuvBlendJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_overlay_prepare_submit", C.c_int32,

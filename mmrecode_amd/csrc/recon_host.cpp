@@ -1690,6 +1690,217 @@ mmr_status mmr_recon_copy_to_host(mmr_recon_context* ctx, void* dst, const void*
     });
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------------ resident frame cache
+// DeviceResourceCache (crates/render/src/device_cache.rs:53-287) holding device allocations.
+// Keys order as FrameResourceKey's derived Ord does (namespace, owner, revision, local_frame,
+// width, height, variant), which breaks ties between equally old eviction candidates.
+struct mmr_recon_frame_cache {
+    using Key = std::tuple<uint32_t, uint64_t, uint64_t, int64_t, uint32_t, uint32_t, uint32_t>;
+    struct Entry {
+        mmr_recon_frame_descriptor descriptor;
+        void* storage;
+        uint64_t bytes;
+        uint64_t last_used;
+    };
+    mmr_recon_context* ctx = nullptr;
+    uint64_t budget = 0, retained = 0, generation = 0, reuses = 0, insertions = 0, evictions = 0;
+    std::map<Key, Entry> entries;
+};
+
+namespace {
+
+mmr_recon_frame_cache::Key cache_key(const mmr_recon_frame_key& k) {
+    return {k.namespace_id, k.owner, k.revision, k.local_frame, k.width, k.height, k.variant};
+}
+
+bool same_descriptor(const mmr_recon_frame_descriptor& a, const mmr_recon_frame_descriptor& b) {
+    return a.width == b.width && a.height == b.height && a.format == b.format && a.color == b.color &&
+           a.alpha_mode == b.alpha_mode;
+}
+
+// Stream-ordered release on the context's stream: submissions enqueued before it finish first.
+void cache_release(mmr_recon_frame_cache* c, mmr_recon_frame_cache::Entry& e) {
+    if (e.storage) (void)hipFreeAsync(e.storage, c->ctx->stream);
+    e.storage = nullptr;
+    c->retained -= std::min(c->retained, e.bytes);
+}
+
+// evict_for (device_cache.rs:263-287): least recently used first, never the current generation.
+mmr_status cache_evict_for(mmr_recon_frame_cache* c, uint64_t additional) {
+    while (c->retained + additional > c->budget) {
+        auto victim = c->entries.end();
+        for (auto it = c->entries.begin(); it != c->entries.end(); ++it)
+            if (it->second.last_used < c->generation && (victim == c->entries.end() || it->second.last_used < victim->second.last_used))
+                victim = it;   // map order: the smallest key wins among equally old entries
+        if (victim == c->entries.end())
+            return fail(MMR_STATUS_INVALID_STATE, "device resource budget cannot fit %llu more bytes without releasing a resource "
+                        "used in generation %llu", (unsigned long long)additional, (unsigned long long)c->generation);
+        cache_release(c, victim->second);
+        c->entries.erase(victim);
+        ++c->evictions;
+    }
+    return MMR_STATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+mmr_status mmr_recon_frame_cache_create(mmr_recon_context* ctx, uint64_t byte_budget, mmr_recon_frame_cache** out) {
+    return guarded("mmr_recon_frame_cache_create", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        if (!out) return fail(MMR_STATUS_INVALID_ARGUMENT, "out_cache is NULL");
+        if (byte_budget == 0) return fail(MMR_STATUS_INVALID_DATA, "device resource cache byte budget must be positive");
+        auto* c = new mmr_recon_frame_cache();
+        c->ctx = ctx;
+        c->budget = byte_budget;
+        *out = c;
+        return MMR_STATUS_OK;
+    });
+}
+
+void mmr_recon_frame_cache_destroy(mmr_recon_frame_cache* c) {
+    guarded_void("mmr_recon_frame_cache_destroy", [&]() {
+        if (!c) return;
+        {
+            std::lock_guard<std::mutex> lock(c->ctx->mu);
+            (void)hipSetDevice(c->ctx->device);
+            for (auto& kv : c->entries) cache_release(c, kv.second);
+            (void)hipStreamSynchronize(c->ctx->stream);
+        }
+        delete c;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_begin_frame(mmr_recon_frame_cache* c) {
+    return guarded("mmr_recon_frame_cache_begin_frame", [&]() -> mmr_status {
+        if (!c) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache is NULL");
+        std::lock_guard<std::mutex> lock(c->ctx->mu);
+        if (c->generation != UINT64_MAX) ++c->generation;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_retain(mmr_recon_frame_cache* c, const mmr_recon_frame_key* key, const mmr_recon_frame_descriptor* d,
+                                        uint64_t bytes, void** out_ptr, uint32_t* out_status) {
+    return guarded("mmr_recon_frame_cache_retain", [&]() -> mmr_status {
+        if (!c || !key || !d || !out_ptr) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache / key / descriptor / out pointer is NULL");
+        LOCK_AND_SET_DEVICE(c->ctx);
+        *out_ptr = nullptr;
+        const auto k = cache_key(*key);
+        auto it = c->entries.find(k);
+        if (it != c->entries.end()) {
+            if (!same_descriptor(it->second.descriptor, *d))
+                return fail(MMR_STATUS_INVALID_STATE, "stable device resource key was reused with a different descriptor");
+            it->second.last_used = c->generation;
+            ++c->reuses;
+            *out_ptr = it->second.storage;
+            if (out_status) *out_status = MMR_CACHE_REUSED;
+            return MMR_STATUS_OK;
+        }
+        if (bytes == 0 || bytes > c->budget)
+            return fail(MMR_STATUS_INVALID_DATA, "device resource requires %llu bytes with a %llu byte budget", (unsigned long long)bytes,
+                        (unsigned long long)c->budget);
+        void* p = nullptr;
+        hipError_t e = hipMallocAsync(&p, bytes, c->ctx->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMallocAsync (frame cache)");
+        mmr_status s = cache_evict_for(c, bytes);
+        if (s) {
+            (void)hipFreeAsync(p, c->ctx->stream);   // as the reference drops the created resource
+            return s;
+        }
+        c->entries.emplace(k, mmr_recon_frame_cache::Entry{*d, p, bytes, c->generation});
+        c->retained += bytes;
+        ++c->insertions;
+        *out_ptr = p;
+        if (out_status) *out_status = MMR_CACHE_INSERTED;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_get(mmr_recon_frame_cache* c, const mmr_recon_frame_key* key, const mmr_recon_frame_descriptor* d,
+                                     void** out_ptr) {
+    return guarded("mmr_recon_frame_cache_get", [&]() -> mmr_status {
+        if (!c || !key || !d || !out_ptr) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache / key / descriptor / out pointer is NULL");
+        std::lock_guard<std::mutex> lock(c->ctx->mu);
+        *out_ptr = nullptr;
+        auto it = c->entries.find(cache_key(*key));
+        if (it == c->entries.end()) return MMR_STATUS_OK;
+        if (!same_descriptor(it->second.descriptor, *d))
+            return fail(MMR_STATUS_INVALID_STATE, "stable device resource key was reused with a different descriptor");
+        it->second.last_used = c->generation;
+        ++c->reuses;
+        *out_ptr = it->second.storage;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_remove(mmr_recon_frame_cache* c, const mmr_recon_frame_key* key, int32_t* out_removed) {
+    return guarded("mmr_recon_frame_cache_remove", [&]() -> mmr_status {
+        if (!c || !key) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache / key is NULL");
+        LOCK_AND_SET_DEVICE(c->ctx);
+        auto it = c->entries.find(cache_key(*key));
+        const bool found = it != c->entries.end();
+        if (found) {
+            cache_release(c, it->second);
+            c->entries.erase(it);
+        }
+        if (out_removed) *out_removed = found ? 1 : 0;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_release_idle(mmr_recon_frame_cache* c, uint64_t max_idle, uint64_t* out_released) {
+    return guarded("mmr_recon_frame_cache_release_idle", [&]() -> mmr_status {
+        if (!c) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache is NULL");
+        LOCK_AND_SET_DEVICE(c->ctx);
+        uint64_t released = 0;
+        for (auto it = c->entries.begin(); it != c->entries.end();) {
+            const uint64_t idle = c->generation - std::min(c->generation, it->second.last_used);
+            if (idle == 0 || idle <= max_idle) {
+                ++it;
+                continue;
+            }
+            cache_release(c, it->second);
+            it = c->entries.erase(it);
+            ++released;
+        }
+        c->evictions += released;
+        if (out_released) *out_released = released;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_clear(mmr_recon_frame_cache* c) {
+    return guarded("mmr_recon_frame_cache_clear", [&]() -> mmr_status {
+        if (!c) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache is NULL");
+        LOCK_AND_SET_DEVICE(c->ctx);
+        for (auto& kv : c->entries) cache_release(c, kv.second);
+        c->entries.clear();
+        c->retained = 0;
+        return MMR_STATUS_OK;
+    });
+}
+
+mmr_status mmr_recon_frame_cache_stats_get(mmr_recon_frame_cache* c, mmr_recon_frame_cache_stats* out) {
+    return guarded("mmr_recon_frame_cache_stats_get", [&]() -> mmr_status {
+        if (!c || !out) return fail(MMR_STATUS_INVALID_ARGUMENT, "cache / out_stats is NULL");
+        if (out->struct_size < sizeof(mmr_recon_frame_cache_stats))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "frame cache stats struct_size too small");
+        std::lock_guard<std::mutex> lock(c->ctx->mu);
+        out->resources = c->entries.size();
+        out->retained_bytes = c->retained;
+        out->byte_budget = c->budget;
+        out->reuses = c->reuses;
+        out->insertions = c->insertions;
+        out->evictions = c->evictions;
+        out->generation = c->generation;
+        return MMR_STATUS_OK;
+    });
+}
+
 mmr_status mmr_recon_wait(mmr_recon_context* ctx, uint64_t ticket) {
     return guarded("mmr_recon_wait", [&]() -> mmr_status {
         CHECK_CTX(ctx);
@@ -2529,6 +2740,61 @@ mmr_status mmr_recon_server_create(mmr_recon_context* ctx, const mmr_recon_serve
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Device bytes a request's descriptors write: every output plane, RGB / RGBA image and composite
+// target its frames name, from their sizes. Admission counts at least this much for a request,
+// whatever hbm_bytes it declares.
+uint64_t request_footprint(const mmr_recon_request& req) {
+    uint64_t bytes = 0;
+    for (uint32_t i = 0; i < req.frame_count; ++i) {
+        switch (req.codec) {
+        case MMR_CODEC_MJPEG: {
+            const auto& f = reinterpret_cast<const mmr_recon_mjpeg_frame*>(req.frames)[i];
+            uint32_t hmax = 1, vmax = 1;
+            for (uint32_t c = 0; c < f.component_count && c < 4; ++c) {
+                hmax = std::max<uint32_t>(hmax, f.h_sampling[c]);
+                vmax = std::max<uint32_t>(vmax, f.v_sampling[c]);
+            }
+            for (uint32_t c = 0; c < f.component_count && c < 4; ++c)
+                bytes += (uint64_t)((f.width * f.h_sampling[c] + hmax - 1) / hmax) * ((f.height * f.v_sampling[c] + vmax - 1) / vmax);
+            if (f.rgb) bytes += (uint64_t)f.width * f.height * 3;
+            break;
+        }
+        case MMR_CODEC_MPEG2: {
+            const auto& p = reinterpret_cast<const mmr_recon_mpeg2_picture*>(req.frames)[i];
+            const uint64_t mbs = (uint64_t)((p.width + 15) / 16) * ((p.height + 15) / 16);
+            bytes += mbs * 384;
+            break;
+        }
+        case MMR_CODEC_DV: {
+            const auto& f = reinterpret_cast<const mmr_recon_dv_frame*>(req.frames)[i];
+            bytes += f.system == MMR_DV_SYSTEM_625_50 ? 720 * 576 + 2 * 360 * 288 : 720 * 480 + 2 * 180 * 480;
+            break;
+        }
+        case MMR_CODEC_H264: {
+            const auto& p = reinterpret_cast<const mmr_recon_h264_picture*>(req.frames)[i];
+            bytes += (uint64_t)p.width_mbs * p.height_mbs * 384;
+            break;
+        }
+        default:
+            break;
+        }
+        if (req.convert) {
+            const auto& c = req.convert[i];
+            bytes += (uint64_t)c.width * c.height * ((c.rgb ? 3 : 0) + (c.rgba ? 4 : 0));
+        }
+        if (req.composite) bytes += (uint64_t)req.composite[i].stride * req.composite[i].height;
+    }
+    return bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
 mmr_status mmr_recon_server_enqueue(mmr_recon_server* s, const mmr_recon_request* req, uint64_t* out_id) {
     return guarded("mmr_recon_server_enqueue", [&]() -> mmr_status {
         if (!s) return fail(MMR_STATUS_INVALID_ARGUMENT, "server is NULL");
@@ -2540,7 +2806,7 @@ mmr_status mmr_recon_server_enqueue(mmr_recon_server* s, const mmr_recon_request
         r->codec = req->codec;
         r->stream_id = req->stream_id;
         r->frame_count = req->frame_count;
-        r->hbm = req->hbm_bytes;
+        r->hbm = std::max<uint64_t>(req->hbm_bytes, request_footprint(*req));
         r->tag = req->user_tag;
         r->frames.assign((const char*)req->frames, (const char*)req->frames + sz * req->frame_count);
         if (req->codec == MMR_CODEC_MPEG2) {

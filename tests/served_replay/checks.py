@@ -6,6 +6,8 @@ python tests/served_replay/checks.py planted     -- descriptors with planted mis
 python tests/served_replay/checks.py exceptions  -- with the address space limited, a submission
     whose job table cannot be allocated returns MMR_STATUS_INTERNAL_ERROR with a message instead
     of letting std::bad_alloc escape through the C ABI
+python tests/served_replay/checks.py cache       -- the resident frame cache's LRU, generation
+    protection, budget and stream-ordered release
 Prints one JSON line.
 """
 import ctypes as C
@@ -161,6 +163,121 @@ def noalloc():
             "oversized_status": status, "oversized_message": message, "oversized_kernels": ran}
 
 
+def cache():
+    """The resident frame cache on the model library: the reference's DeviceResourceCache tests
+    (crates/render/src/device_cache.rs:316-392) with device allocations as the resources, plus
+    budget errors, remove / clear accounting, and an eviction while a kernel that reads the evicted
+    storage is still queued (the release is stream-ordered, so the kernel's reads stay inside a live
+    allocation)."""
+    lib = load_model_library()
+    import numpy as np
+
+    import mmrecode_amd
+    from mmrecode_amd import abi
+
+    ctx = mmrecode_amd.Context(device=0)
+    out = {}
+
+    def handle(owner):   # device_cache.rs:300-314: MmfxCanvas, revision 1, time-invariant, 2x2, rgba8
+        return dict(namespace_id=3, owner=owner, revision=1, local_frame=-1, width=2, height=2, variant=0)
+
+    rgba8_2x2 = dict(width=2, height=2, format=1, color=1, alpha_mode=1)
+
+    def status_of(fn):
+        try:
+            fn()
+            return 0, ""
+        except mmrecode_amd.ReconError as e:
+            return e.status, e.message
+
+    # stable_key_reuses_one_backend_resource
+    c = mmrecode_amd.FrameCache(ctx, 64)
+    p1, s1 = c.retain(handle(1), rgba8_2x2, 16)
+    p2, s2 = c.retain(handle(1), rgba8_2x2, 16)
+    st = c.stats()
+    out["reuse"] = {"inserted": s1 == abi.CACHE_INSERTED, "reused": s2 == abi.CACHE_REUSED, "same": p1 == p2 and p1 is not None,
+                    "resources": st["resources"], "reuses": st["reuses"]}
+    c.close()
+
+    # evicts_oldest_resource_but_protects_current_graph_generation
+    c = mmrecode_amd.FrameCache(ctx, 32)
+    c.retain(handle(1), rgba8_2x2, 16)
+    c.retain(handle(2), rgba8_2x2, 16)
+    c.begin_frame()
+    second = c.get(handle(2), rgba8_2x2)
+    c.retain(handle(3), rgba8_2x2, 16)
+    out["evict"] = {"second_before": second is not None, "first": c.get(handle(1), rgba8_2x2),
+                    "second": c.get(handle(2), rgba8_2x2) is not None, "third": c.get(handle(3), rgba8_2x2) is not None,
+                    "evictions": c.stats()["evictions"], "retained": c.stats()["retained_bytes"]}
+    c.close()
+
+    # refuses_to_evict_a_resource_used_by_the_current_graph
+    c = mmrecode_amd.FrameCache(ctx, 16)
+    c.retain(handle(1), rgba8_2x2, 16)
+    status, message = status_of(lambda: c.retain(handle(2), rgba8_2x2, 16))
+    out["protect"] = {"status": status, "message": message, "resources": c.stats()["resources"],
+                      "retained": c.stats()["retained_bytes"]}
+    c.close()
+
+    # rejects_descriptor_collisions (a foreign backend has no counterpart: caches are per context)
+    c = mmrecode_amd.FrameCache(ctx, 64)
+    c.retain(handle(1), rgba8_2x2, 16)
+    status, message = status_of(lambda: c.get(handle(1), dict(rgba8_2x2, width=4, height=4)))
+    status2, _ = status_of(lambda: c.retain(handle(1), dict(rgba8_2x2, width=4, height=4), 16))
+    out["collision"] = {"get_status": status, "message": message, "retain_status": status2}
+    c.close()
+
+    # releases_only_idle_generations
+    c = mmrecode_amd.FrameCache(ctx, 64)
+    c.retain(handle(1), rgba8_2x2, 16)
+    c.begin_frame()
+    c.retain(handle(2), rgba8_2x2, 16)
+    c.begin_frame()
+    c.get(handle(2), rgba8_2x2)
+    released = c.release_idle(1)
+    out["idle"] = {"released": released, "first": c.get(handle(1), rgba8_2x2), "second": c.get(handle(2), rgba8_2x2) is not None}
+    c.close()
+
+    # budget errors and accounting
+    status_zero, _ = status_of(lambda: mmrecode_amd.FrameCache(ctx, 0))
+    c = mmrecode_amd.FrameCache(ctx, 64)
+    status_big, _ = status_of(lambda: c.retain(handle(9), rgba8_2x2, 65))
+    status_empty, _ = status_of(lambda: c.retain(handle(9), rgba8_2x2, 0))
+    c.retain(handle(1), rgba8_2x2, 16)
+    c.retain(handle(2), rgba8_2x2, 32)
+    removed = c.remove(handle(1))
+    removed_again = c.remove(handle(1))
+    after_remove = c.stats()["retained_bytes"]
+    c.clear()
+    out["budget"] = {"zero_budget": status_zero, "over_budget": status_big, "zero_bytes": status_empty, "removed": removed,
+                     "removed_again": removed_again, "after_remove": after_remove, "after_clear": c.stats()["retained_bytes"],
+                     "resources_after_clear": c.stats()["resources"]}
+    c.close()
+
+    # eviction while a queued kernel still reads the storage
+    before, _ = violations(lib)
+    W, H = 64, 16
+    c = mmrecode_amd.FrameCache(ctx, W * H * 8)
+    key = dict(namespace_id=1, owner=7, revision=1, local_frame=0, width=W, height=H, variant=0)
+    desc = dict(width=W, height=H, format=2, color=2, alpha_mode=2)
+    surface, _ = c.retain(key, desc, W * H * 8)
+    ctx.write(mmrecode_amd.DeviceBuffer(ctx, surface, W * H * 8), np.zeros(W * H * 4, np.uint16))
+    rgba = ctx.alloc(W * H * 4)
+    job = abi.new(abi.LinearConvertJob, direction=abi.LINEAR_TO_SRGBA8, width=W, height=H, linear=surface, linear_stride=W,
+                  rgba_stride=W * 4, rgba=rgba.ptr)
+    ticket = ctx.submit_linear_convert([job])
+    c.begin_frame()
+    c.retain(dict(key, owner=8), desc, W * H * 8)   # evicts the surface the queued kernel reads
+    evictions = c.stats()["evictions"]
+    ctx.wait(ticket)
+    after, msgs = violations(lib)
+    out["ordered_release"] = {"evictions": evictions, "new_violations": after - before, "messages": msgs[-3:]}
+    c.close()
+    rgba.free()
+    ctx.close()
+    return out
+
+
 if __name__ == "__main__":
     what = sys.argv[1]
-    print(json.dumps({"planted": planted, "exceptions": exceptions, "noalloc": noalloc}[what]()))
+    print(json.dumps({"planted": planted, "exceptions": exceptions, "noalloc": noalloc, "cache": cache}[what]()))

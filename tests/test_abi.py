@@ -37,6 +37,9 @@ STRUCTS = {
     "mmr_recon_linear_layer": abi.LinearLayer,
     "mmr_recon_linear_composite_job": abi.LinearCompositeJob,
     "mmr_recon_linear_convert_job": abi.LinearConvertJob,
+    "mmr_recon_frame_key": abi.FrameKey,
+    "mmr_recon_frame_descriptor": abi.FrameDescriptor,
+    "mmr_recon_frame_cache_stats": abi.FrameCacheStats,
     "mmr_recon_yuv_blend_job": abi.YuvBlendJob,
     "mmr_recon_overlay_prepare_job": abi.OverlayPrepareJob,
     "mmr_recon_scale_job": abi.ScaleJob,

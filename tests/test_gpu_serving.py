@@ -182,6 +182,9 @@ def test_admission_respects_hbm_budget(ctx):
         server.close()
     assert sorted(c[1] for c in comps) == list(range(12))
     assert st["peak_admitted_bytes"] <= 3000
+    # each request declares 1000 bytes but its frame writes 1024 + 2 x 256 bytes of planes: admission
+    # counts the larger figure, so at most one request is admitted at a time
+    assert st["peak_admitted_bytes"] == 1536
     assert st["requests_completed"] == 12 and st["admitted_bytes"] == 0
     for d in devs:
         d.free()

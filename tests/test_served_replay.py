@@ -66,3 +66,22 @@ def test_noalloc_entry_point_allocates_nothing():
     assert out["kernels"] > 0 and out["violations"] == 0, out
     assert out["oversized_status"] == 1 and out["oversized_kernels"] == 0, out
     assert "MMR_RECON_NOALLOC_SCRATCH_BYTES" in out["oversized_message"]
+
+
+def test_frame_cache_follows_device_resource_cache():
+    """mmr_recon_frame_cache_* against the reference's DeviceResourceCache tests
+    (crates/render/src/device_cache.rs:316-392) on the model library, plus budget errors,
+    accounting, and a stream-ordered release under a queued kernel (checks.py cache)."""
+    rc, out, err = _run(["checks.py", "cache"], 300)
+    assert rc == 0, err[-3000:]
+    assert out["reuse"] == {"inserted": True, "reused": True, "same": True, "resources": 1, "reuses": 1}
+    assert out["evict"] == {"second_before": True, "first": None, "second": True, "third": True, "evictions": 1, "retained": 32}
+    assert out["protect"]["status"] == 4 and "used in generation" in out["protect"]["message"]
+    assert out["protect"]["resources"] == 1 and out["protect"]["retained"] == 16
+    assert out["collision"]["get_status"] == 4 and out["collision"]["retain_status"] == 4
+    assert out["idle"] == {"released": 1, "first": None, "second": True}
+    b = out["budget"]
+    assert b["zero_budget"] == 2 and b["over_budget"] == 2 and b["zero_bytes"] == 2
+    assert b["removed"] and not b["removed_again"] and b["after_remove"] == 32
+    assert b["after_clear"] == 0 and b["resources_after_clear"] == 0
+    assert out["ordered_release"]["evictions"] == 1 and out["ordered_release"]["new_violations"] == 0, out["ordered_release"]
