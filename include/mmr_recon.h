@@ -403,7 +403,7 @@ typedef struct mmr_recon_h264_picture {
     const mmr_recon_h264_bi_weight *bi_weights; /* [MAX_REFS][MAX_REFS][3] */
     const uint8_t *scaling_lists;               /* NULL: Flat_16; else [6][16] 4x4 then [2][64] 8x8, raster */
     uint8_t *planes[3];                         /* output coded planes (stride 16*width_mbs, 8*width_mbs) */
-    uint32_t ref_count[2];
+    uint32_t ref_count[2];                      /* list 0, list 1; {0, 0} for an all-intra (I / IDR) picture */
     mmr_recon_h264_reference refs[2][MMR_H264_MAX_REFS];
     const mmr_recon_h264_intra *intra;          /* [mbs], 4-byte aligned, read for MMR_H264_MB_INTRA macroblocks; NULL:
                                                    intra macroblocks report UNSUPPORTED. A struct_size

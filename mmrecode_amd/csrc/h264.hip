@@ -195,6 +195,7 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
             atomicOr(status, kDevStatusInvalidData);
             r0 = job.ref_count[0] ? min(max(r0, 0), (int)job.ref_count[0] - 1) : -1;
             r1 = (r0 < 0 && job.ref_count[1]) ? min(max(r1, 0), (int)job.ref_count[1] - 1) : -1;
+            if (r0 < 0 && r1 < 0) r0 = 0;   // a picture without references: slot 0 holds an in-bounds stand-in
         }
         const int ox = mbx * 16 + bx * 4, oy = mby * 16 + by * 4 + (l & 3);
         const int cxs = mbx * 8 + bx * 2 + (l & 1), cys = mby * 8 + by * 2 + ((l >> 1) & 1);

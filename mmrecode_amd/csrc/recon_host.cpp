@@ -938,8 +938,12 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u size invalid", i);
         if (!p.macroblocks || !p.blocks || !p.levels || !p.planes[0] || !p.planes[1] || !p.planes[2])
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u has a NULL buffer", i);
-        if (p.ref_count[0] > MMR_H264_MAX_REFS || p.ref_count[1] > MMR_H264_MAX_REFS || p.ref_count[0] + p.ref_count[1] == 0)
+        if (p.ref_count[0] > MMR_H264_MAX_REFS || p.ref_count[1] > MMR_H264_MAX_REFS)
             return fail(MMR_STATUS_INVALID_DATA, "H.264 picture %u reference counts invalid", i);
+        // An all-intra picture (I or IDR) has no references. Every reference slot then points at the
+        // picture's own planes, an in-bounds stand-in that only an inter block of invalid data reads
+        // (and the device reports such a block as invalid data).
+        const bool no_refs = p.ref_count[0] + p.ref_count[1] == 0;
         if (p.weighted_mode > MMR_H264_WEIGHTED_TABLE) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 weighted_mode invalid");
         if (p.weighted_mode == MMR_H264_WEIGHTED_TABLE && (!p.weights || !p.bi_weights))
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
@@ -960,7 +964,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
                 const uint32_t src = r < p.ref_count[list] ? r : 0;
                 const mmr_recon_h264_reference& ref = p.refs[list][src];
                 for (int k = 0; k < 3; ++k) {
-                    const uint8_t* plane = p.ref_count[list] ? ref.planes[k] : p.refs[1 - list][0].planes[k];
+                    const uint8_t* plane = no_refs ? p.planes[k] : p.ref_count[list] ? ref.planes[k] : p.refs[1 - list][0].planes[k];
                     if (!plane) return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: reference plane is NULL", i);
                     j.ref_planes[list][r][k] = plane;
                 }

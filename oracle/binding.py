@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker. The product library never loads it.
+cpu_baseline leg of bench.py as the reference answer. The product library never loads it.
 Descriptors are the include/mmr_recon.h structs filled with HOST pointers.
 """
 import ctypes as C
@@ -30,6 +30,11 @@ def lib():
         L = C.CDLL(path)
         i32, u8, u32, dp = C.c_int32, C.c_uint8, C.c_uint32, C.c_void_p
         sig = {
+            "oracle_h264_decoder_new": (C.c_void_p, []),
+            "oracle_h264_decoder_free": (None, [dp]),
+            "oracle_h264_decode_access_unit": (i32, [dp, dp, C.c_size_t, dp]),
+            "oracle_h264_picture_records": (i32, [dp, dp, dp, dp, dp, dp, dp, dp]),
+            "oracle_h264_cavlc_block": (i32, [dp, C.c_size_t, i32, i32, dp, dp]),
             "oracle_linear_multiply_channel": (C.c_uint16, [C.c_uint16, C.c_uint16]),
             "oracle_srgb_to_linear_u16": (C.c_uint16, [u8]),
             "oracle_linear_u16_to_srgb": (u8, [C.c_uint16]),

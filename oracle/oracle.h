@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product library links, loads or
  * calls this code; tests, the smoke check and the CPU-baseline leg of the
- * benchmark use it as the checker (and as the timed CPU baseline).
+ * benchmark use it as the reference answer (and as the timed CPU baseline).
  *
  * Every function follows the operation order, integer widths, rounding and
  * non-fused f64 arithmetic of the Rust function it cites (paths relative to
@@ -133,6 +133,30 @@ void oracle_blend_rgba_pixel(uint8_t destination[4], const uint8_t source[4]);
 void oracle_premultiplied_over_pixel(uint8_t destination[4], const uint8_t source[4],
                                      uint32_t opacity);
 uint8_t oracle_blend_channel(uint8_t destination, uint8_t source, uint8_t alpha);
+/* H.264 CAVLC front end (h264_entropy.c): access units in, the library's records out. */
+typedef struct oracle_h264_decoder oracle_h264_decoder;
+typedef struct {
+    uint32_t width_mbs, height_mbs;
+    int32_t slot;               /* id of this picture; reference lists name pictures by it */
+    int32_t poc;
+    int32_t is_reference;
+    uint32_t slice_type;        /* first slice: 0 P, 1 B, 2 I */
+    uint32_t ref_count[2];
+    int32_t ref_slot[2][16];
+    int32_t ref_poc[2][16];
+    uint32_t coded_mbs;         /* level records */
+    uint32_t weighted_mode;
+    uint32_t deblock;
+    int32_t chroma_qp_offset[2];
+    uint32_t transform_bypass;
+} oracle_h264_picture_info;
+oracle_h264_decoder *oracle_h264_decoder_new(void);
+void oracle_h264_decoder_free(oracle_h264_decoder *d);
+int oracle_h264_cavlc_block(const uint8_t *bytes, size_t n, int nc, int max_coeff, int32_t *coef, size_t *bits_used);
+int oracle_h264_decode_access_unit(oracle_h264_decoder *d, const uint8_t *data, size_t size, oracle_h264_picture_info *info);
+int oracle_h264_picture_records(const oracle_h264_decoder *d, mmr_recon_h264_macroblock *mbs, mmr_recon_h264_block *blocks,
+                                int16_t *levels, mmr_recon_h264_intra *intra, mmr_recon_h264_weight *weights,
+                                mmr_recon_h264_bi_weight *bi_weights, int16_t *mvd);
 int oracle_composite(const mmr_recon_composite_job *job);
 uint16_t oracle_linear_multiply_channel(uint16_t left, uint16_t right);
 uint16_t oracle_srgb_to_linear_u16(uint8_t value);

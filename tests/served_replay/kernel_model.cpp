@@ -210,6 +210,7 @@ void h264_used_refs(const std::set<std::pair<int, int>>& pairs, const mmr::H264J
         if (r0 >= (int)job.ref_count[0] || r1 >= (int)job.ref_count[1] || (r0 < 0 && r1 < 0)) {
             r0 = job.ref_count[0] ? std::min(std::max(r0, 0), (int)job.ref_count[0] - 1) : -1;
             r1 = (r0 < 0 && job.ref_count[1]) ? std::min(std::max(r1, 0), (int)job.ref_count[1] - 1) : -1;
+            if (r0 < 0 && r1 < 0) r0 = 0;
         }
         if (r0 >= 0) used[0][r0] = true;
         if (r1 >= 0) used[1][r1] = true;
