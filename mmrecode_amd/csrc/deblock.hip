@@ -20,10 +20,11 @@
 // the reader has consumed it (progress of the row below >= x - ring + 2); waits between adjacent
 // rows cannot form a cycle (ring >= 4). Across bands the hand-over goes through the picture in
 // global memory: the band's last row writes its bottom rows as well and publishes its progress
-// with an agent-scope release every kDbPublish macroblocks; the next band's first row waits for
-// progress >= x+2 with relaxed loads, then an agent-scope acquire, and reads the four rows above
-// from the picture. Tickets are taken in order, so the band a workgroup waits for is held by a
-// running workgroup (or done), and no band waits for a later one.
+// with an agent-scope release every `publish` macroblocks; the next band's first row waits for
+// the progress that covers macroblock x + 2 with relaxed loads, then an agent-scope acquire, and
+// reads the four rows above that macroblock from the picture two steps before it filters it.
+// Tickets are taken in order, so the band a workgroup waits for is held by a running workgroup (or
+// done), and no band waits for a later one.
 //
 // Write-back. Each row writes its final samples once: luma rows y*16-3 .. y*16+12 (chroma
 // y*8-3 .. y*8+4) of columns x*16-4 .. x*16+11 after step x (the rows below are handed to and
@@ -87,52 +88,75 @@ MMR_DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on one line of samples held in
-// registers as packed bytes: sample i is byte i of the line, p_i is sample k - 1 - i and q_i sample
-// k + i (k is a compile-time index, so every extract / insert is one bitfield instruction).
-// Samples are only rewritten when the filter applies, which leaves the same values as the
-// reference's unconditional write-back of unchanged samples.
-template <int NW>
-MMR_DEV int get_byte(const uint32_t (&w)[NW], int i) { return (w[i >> 2] >> (8 * (i & 3))) & 0xff; }
-template <int NW>
-MMR_DEV void set_byte(uint32_t (&w)[NW], int i, int v) {
-    const uint32_t sh = 8 * (i & 3);
-    w[i >> 2] = (w[i >> 2] & ~(0xffu << sh)) | ((uint32_t)v << sh);
-}
+// Lane masks for the edge filter: 0 or -1 (all bits), built from sign bits so that the filter is
+// straight-line vector arithmetic (no per-lane branches or exec-mask changes).
+MMR_DEV int mask_lt(int a, int b) { return (a - b) >> 31; }                   // a < b, |a - b| < 2^31
+MMR_DEV int mask_sel(int m, int a, int b) { return (a & m) | (b & ~m); }      // m ? a : b
+MMR_DEV int absdiff(int a, int b) { return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }   // a, b in 0..65535
+MMR_DEV int clamp_sym(int v, int t) { return min(max(v, -t), t); }           // t >= 0
 
+// filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on one line of samples, one
+// sample per register: p_i is s[k - 1 - i] and q_i is s[k + i] (k is a compile-time index). The
+// line's edges run one after another (each reads samples the previous one wrote), which makes
+// this the wavefront's critical path, so it is straight-line code: both filters are evaluated and
+// the results selected with lane masks; samples keep their values where the filter does not apply
+// (the reference writes unchanged samples back, which leaves the same values). luma_m is -1 on
+// luma lines, 0 on chroma lines.
 template <int k>
-MMR_DEV void filter_line(uint32_t (&w)[5], EdgeParams ep, int bs, bool chroma) {
+MMR_DEV void filter_edge(int (&s)[20], EdgeParams ep, int bs, int luma_m) {
+    const int p0 = s[k - 1], p1 = s[k - 2], p2 = s[k - 3], p3 = s[k - 4];
+    const int q0 = s[k], q1 = s[k + 1], q2 = s[k + 2], q3 = s[k + 3];
     const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
-    const int p0 = get_byte(w, k - 1), p1 = get_byte(w, k - 2), p2 = get_byte(w, k - 3), p3 = get_byte(w, k - 4);
-    const int q0 = get_byte(w, k), q1 = get_byte(w, k + 1), q2 = get_byte(w, k + 2), q3 = get_byte(w, k + 3);
-    if (bs == 0 || iabs(p0 - q0) >= alpha || iabs(p1 - p0) >= beta || iabs(q1 - q0) >= beta) return;
-    // Both filters are evaluated and the results selected (no branches inside a wave).
-    const bool luma = !chroma;
-    const bool ap = iabs(p2 - p0) < beta, aq = iabs(q2 - q0) < beta;
+    const int d = absdiff(p0, q0);
+    // bS != 0 && |p0 - q0| < alpha && |p1 - p0| < beta && |q1 - q0| < beta
+    const int on = (-bs & (d - alpha) & (absdiff(p1, p0) - beta) & (absdiff(q1, q0) - beta)) >> 31;
+    const int ap = mask_lt(absdiff(p2, p0), beta), aq = mask_lt(absdiff(q2, q0), beta);
     // bS < 4
-    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);   // bs 1..3 (unused for 4)
-    const int tc = tc0 + (chroma ? 1 : ((int)ap + (int)aq));
-    const int delta = clampi((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);   // bS 1..3 (unused for 0 and 4)
+    const int tc = tc0 + mask_sel(luma_m, -(ap + aq), 1);
+    const int delta = clamp_sym((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, tc);
     const int avg = (p0 + q0 + 1) >> 1;
     const int np0 = clampi(p0 + delta, 0, 255), nq0 = clampi(q0 - delta, 0, 255);
-    const int np1 = (luma && ap) ? clampi(p1 + clampi((p2 + avg - (p1 << 1)) >> 1, -tc0, tc0), 0, 255) : p1;
-    const int nq1 = (luma && aq) ? clampi(q1 + clampi((q2 + avg - (q1 << 1)) >> 1, -tc0, tc0), 0, 255) : q1;
+    const int np1 = mask_sel(luma_m & ap, clampi(p1 + clamp_sym((p2 + avg - (p1 << 1)) >> 1, tc0), 0, 255), p1);
+    const int nq1 = mask_sel(luma_m & aq, clampi(q1 + clamp_sym((q2 + avg - (q1 << 1)) >> 1, tc0), 0, 255), q1);
     // bS == 4
-    const bool near = iabs(p0 - q0) < (alpha >> 2) + 2;
-    const bool sp = luma && ap && near, sq = luma && aq && near;
-    const int sp0 = sp ? (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3 : (2 * p1 + p0 + q1 + 2) >> 2;
-    const int sp1 = sp ? (p2 + p1 + p0 + q0 + 2) >> 2 : p1;
-    const int sp2 = sp ? (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3 : p2;
-    const int sq0 = sq ? (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3 : (2 * q1 + q0 + p1 + 2) >> 2;
-    const int sq1 = sq ? (p0 + q0 + q1 + q2 + 2) >> 2 : q1;
-    const int sq2 = sq ? (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3 : q2;
-    const bool strong = bs == 4;
-    set_byte(w, k - 3, strong ? sp2 : p2);
-    set_byte(w, k - 2, strong ? sp1 : np1);
-    set_byte(w, k - 1, strong ? sp0 : np0);
-    set_byte(w, k, strong ? sq0 : nq0);
-    set_byte(w, k + 1, strong ? sq1 : nq1);
-    set_byte(w, k + 2, strong ? sq2 : q2);
+    const int near = mask_lt(d, (alpha >> 2) + 2);
+    const int sp = luma_m & ap & near, sq = luma_m & aq & near;
+    const int sp0 = mask_sel(sp, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, (2 * p1 + p0 + q1 + 2) >> 2);
+    const int sp1 = mask_sel(sp, (p2 + p1 + p0 + q0 + 2) >> 2, p1);
+    const int sp2 = mask_sel(sp, (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3, p2);
+    const int sq0 = mask_sel(sq, (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3, (2 * q1 + q0 + p1 + 2) >> 2);
+    const int sq1 = mask_sel(sq, (p0 + q0 + q1 + q2 + 2) >> 2, q1);
+    const int sq2 = mask_sel(sq, (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3, q2);
+    const int strong = (3 - bs) >> 31;   // bS == 4 (bS is 0..4)
+    const int on_strong = on & strong;
+    s[k - 3] = mask_sel(on_strong, sp2, p2);
+    s[k - 2] = mask_sel(on, mask_sel(strong, sp1, np1), p1);
+    s[k - 1] = mask_sel(on, mask_sel(strong, sp0, np0), p0);
+    s[k] = mask_sel(on, mask_sel(strong, sq0, nq0), q0);
+    s[k + 1] = mask_sel(on, mask_sel(strong, sq1, nq1), q1);
+    s[k + 2] = mask_sel(on_strong, sq2, q2);
+}
+
+// filter_edge for an internal edge of a macroblock (luma edges 1-3, chroma edge 2), whose strength
+// is at most 3 (boundary_strength gives 4 only on macroblock edges): the normal filter alone.
+template <int k>
+MMR_DEV void filter_edge_inner(int (&s)[20], EdgeParams ep, int bs, int luma_m) {
+    const int p0 = s[k - 1], p1 = s[k - 2], p2 = s[k - 3];
+    const int q0 = s[k], q1 = s[k + 1], q2 = s[k + 2];
+    const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
+    const int on = (-bs & (absdiff(p0, q0) - alpha) & (absdiff(p1, p0) - beta) & (absdiff(q1, q0) - beta)) >> 31;
+    const int ap = mask_lt(absdiff(p2, p0), beta), aq = mask_lt(absdiff(q2, q0), beta);
+    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);
+    const int tc = tc0 + mask_sel(luma_m, -(ap + aq), 1);
+    const int delta = clamp_sym((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, tc);
+    const int avg = (p0 + q0 + 1) >> 1;
+    const int np1 = clampi(p1 + clamp_sym((p2 + avg - (p1 << 1)) >> 1, tc0), 0, 255);
+    const int nq1 = clampi(q1 + clamp_sym((q2 + avg - (q1 << 1)) >> 1, tc0), 0, 255);
+    s[k - 2] = mask_sel(on & luma_m & ap, np1, p1);
+    s[k - 1] = mask_sel(on, clampi(p0 + delta, 0, 255), p0);
+    s[k] = mask_sel(on, clampi(q0 - delta, 0, 255), q0);
+    s[k + 1] = mask_sel(on & luma_m & aq, nq1, q1);
 }
 
 struct RefMotion {
@@ -172,9 +196,13 @@ MMR_DEV int boundary_strength(const H264Job& job, uint8_t p_flags, uint8_t q_fla
 
 MMR_DEV int luma_block_index(int x, int y) { return (y / 2) * 8 + (x / 2) * 4 + (y % 2) * 2 + x % 2; }
 
-struct DbWave {
+// The samples one step filters: the macroblock with 4 rows of top and 4 columns of left context.
+struct DbWindow {
     uint8_t ly[20 * 20];               // rows y*16-4 .., cols x*16-4 ..
     uint8_t lc[2][20 * kLc];           // rows y*8-4 .., cols x*8-4 .. (12 x 12 used)
+};
+struct DbWave {
+    DbWindow win[2];                   // even and odd steps
     uint8_t ring_y[kDbRing][4 * 16];   // bottom 4 luma rows of a macroblock, for the row below
     uint8_t ring_c[kDbRing][2][4 * 8];
 };
@@ -204,16 +232,45 @@ MMR_DEV mmr_recon_h264_block db_block(const H264Job& job, size_t i) {
     return b;
 }
 
-MMR_DEV void db_prefetch(const H264Job& job, int MW, int x, int y, int l, DbPrefetch& f) {
+// The picture buffers a row reads and writes, taken from the job once into scalar registers. As
+// far as the compiler knows a store to the picture may change the job record, so a field read
+// through the record after a store is loaded again -- and that load's wait covers every store
+// issued before it, which would put a memory round trip between consecutive stores.
+struct DbPic {
+    const MMR_GLOBAL mmr_recon_h264_macroblock* mbs;
+    const MMR_GLOBAL uint32_t* bs;
+    MMR_GLOBAL uint8_t* py;
+    MMR_GLOBAL uint8_t* pcb;
+    MMR_GLOBAL uint8_t* pcr;
+    MMR_DEV MMR_GLOBAL uint8_t* chroma(int comp) const { return comp ? pcr : pcb; }
+};
+MMR_DEV DbPic db_pic(const H264Job& job) {
+    DbPic p;
+    p.mbs = uniform_ptr(as_global(job.mbs));
+    p.bs = uniform_ptr(as_global(static_cast<const uint32_t*>(job.bs)));
+    p.py = uniform_ptr(as_global(job.planes[0]));
+    p.pcb = uniform_ptr(as_global(job.planes[1]));
+    p.pcr = uniform_ptr(as_global(job.planes[2]));
+    return p;
+}
+
+MMR_DEV void db_prefetch(const DbPic& P, int MW, int x, int y, int l, DbPrefetch& f) {
     const int W = MW * 16, CW = MW * 8;
     const uint32_t a = (uint32_t)(y * MW + x);
-    f.m = db_mb(job, a);
-    f.qp_top = y > 0 ? as_global(job.mbs)[a - MW].qp : 0;
-    f.bs = load_global_int4(job.bs + (size_t)a * 4);
-    f.own_y = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+    {
+        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.mbs + a);
+        const uint32_t w[3] = {q[0], q[1], q[2]};
+        __builtin_memcpy(&f.m, w, sizeof f.m);
+    }
+    f.qp_top = y > 0 ? P.mbs[a - MW].qp : 0;
+    {
+        const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(P.bs + (size_t)a * 4);
+        f.bs = make_int4(v.x, v.y, v.z, v.w);
+    }
+    f.own_y = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.py + (size_t)(y * 16 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
     if (l < 32) {
         const int comp = l >> 4, row = (l >> 1) & 7;
-        f.own_c = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
+        f.own_c = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)(y * 8 + row) * CW + x * 8 + 4 * (l & 1));
     }
 }
 
@@ -310,26 +367,155 @@ MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint
     return true;
 }
 
+// The top context of macroblock x of row y (the four sample rows above it, luma and both chroma
+// components), one dword per lane (lanes 0-15 luma, 16-31 chroma), read from the picture.
+MMR_DEV uint32_t db_load_top(const DbPic& P, int MW, int x, int y, int l) {
+    const int W = MW * 16, CW = MW * 8;
+    if (l < 16)
+        return *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.py + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+    if (l < 32) {
+        const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
+        return *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
+    }
+    return 0u;
+}
+
 // One macroblock row. band_first: the top context comes from the picture (gprev: the previous
 // band's progress); band_last: no row below in this band (gself: progress for the next band, or
 // null for the picture's last row) -- the row writes its bottom sample rows itself.
+//
+// A step is the wavefront's unit of latency, so it is kept short and free of memory round trips:
+// the macroblock's own samples, record and strengths are loaded two steps ahead, and a band's
+// first row loads its top context two steps ahead as well (it waits for progress >= x + 4 in the
+// middle of step x, then reads the rows above macroblock x + 2). Even and odd steps filter in two
+// windows; step x takes its left context from the other window and writes the other window's final
+// samples (step x - 1's) back to the picture, so a step's stores have a whole step to complete
+// before anything waits for memory again. Agent-scope acquires and releases sit in the middle of a
+// step, after the filtering and before the step's prefetch is issued. Published progress p means
+// that steps 0 .. p-1 are complete and their samples written; the last row publishes p = x in the
+// middle of step x (every `publish` macroblocks) and p = MW after its last store. The vertical
+// edges do not read the rows above the macroblock, so the wait for the row above comes after them.
+// A wait that gives up (it reports a time-out in the status word) marks the wave failed: later
+// waits return at once and the row ends after the step.
 template <int ROWS>
 MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, DbWave* waves, uint32_t* prog, uint32_t* status,
                     bool band_first, bool band_last, uint32_t* gprev, uint32_t* gself, uint32_t publish) {
     DbWave& S = waves[w];
-    const DbWave& Above = waves[(w + ROWS - 1) % ROWS];
-    const int MW = (int)job.width_mbs, MH = (int)job.height_mbs;
+    DbWave& Above = waves[(w + ROWS - 1) % ROWS];
+    const int MW = (int)job.width_mbs;
     const int W = MW * 16, CW = MW * 8;
+    const DbPic P = db_pic(job);
+    const int chroma_off0 = job.chroma_qp_offset[0], chroma_off1 = job.chroma_qp_offset[1];
     const bool last_row = band_last;   // writes its own bottom rows
+    const bool global_top = y > 0 && band_first;
     uint32_t seen = 0;
-    DbPrefetch cur, nxt;
-    db_prefetch(job, MW, 0, y, l, cur);
-    // Lanes 0-15 filter luma lines, 16-23 Cb and 24-31 Cr lines (rows in the vertical pass, columns
-    // in the horizontal one); their QPs go through the chroma QP table.
+    bool failed = false;
+    // Two buffers, even and odd steps: step x reads its buffer, then refills it with step x + 2's
+    // loads, so a loaded value is never copied (a copy would wait for the load) before its step.
+    // (Named variables, not arrays: an array of them is placed in scratch memory.)
+    DbPrefetch pf_even, pf_odd;
+    uint32_t top_even = 0u, top_odd = 0u;   // top context of the buffer's macroblock (band_first rows)
+    db_prefetch(P, MW, 0, y, l, pf_even);
+    if (MW > 1) db_prefetch(P, MW, 1, y, l, pf_odd);
+    if (global_top) {
+        failed = !db_wait_band(gprev, (uint32_t)min(3, MW), seen, status);
+        top_even = db_load_top(P, MW, 0, y, l);
+        if (MW > 1) top_odd = db_load_top(P, MW, 1, y, l);
+    }
+    // Lane roles. Lanes 0-15 filter luma lines, 16-23 Cb and 24-31 Cr lines (rows in the vertical
+    // pass, columns in the horizontal one); their QPs go through the chroma QP table. The same 32
+    // lanes move one dword each of a macroblock's four-row top context (window rows 0-3, columns
+    // 4-19; ring slot rows 0-3) and of its bottom rows (window rows 16-19 luma / 8-11 chroma).
+    // Offsets are bytes from the start of a window (or of the wave's record for ring slots).
     const bool chroma_lane = l >= 16;
-    const int chroma_off = job.chroma_qp_offset[(l >> 3) & 1];
+    const int luma_m = chroma_lane ? 0 : -1;
+    const int chroma_off = (l >> 3) & 1 ? chroma_off1 : chroma_off0;
+    const int li = chroma_lane ? ((l - 16) & 7) : l;   // line index within its plane
+    constexpr int kLcOff = (int)offsetof(DbWindow, lc);
+    const int line_off = chroma_lane ? kLcOff + ((l - 16) >> 3) * 20 * kLc : 0;   // the lane's plane in a window
+    const int ck = l - 16, ccomp = (ck >> 3) & 1, crow = (ck >> 1) & 3;
+    const int ctx_off = chroma_lane ? kLcOff + ccomp * 20 * kLc + crow * kLc + 4 + 4 * (ck & 1) : (l >> 2) * 20 + 4 + 4 * (l & 3);
+    const int bot_off = ctx_off + (chroma_lane ? 8 * kLc : 16 * 20);
+    const int ring_off = chroma_lane ? (int)offsetof(DbWave, ring_c) + ccomp * 32 + crow * 8 + 4 * (ck & 1)
+                                     : (int)offsetof(DbWave, ring_y) + (l >> 2) * 16 + 4 * (l & 3);
+    static_assert(sizeof(DbWave::ring_y[0]) == 64 && sizeof(DbWave::ring_c[0]) == 64, "ring slots are 64 bytes");
+    // Right columns of the previous ring slot (the left edge of macroblock x changes them): lanes
+    // 32-35 luma rows, 40-47 chroma rows.
+    const int fk = l - 40, fcomp = (fk >> 2) & 1, frow = fk & 3;
+    const bool fix_lane = (l >= 32 && l < 36) || (l >= 40 && l < 48);
+    const int fix_win = l < 40 ? (16 + ((l - 32) & 3)) * 20 : kLcOff + fcomp * 20 * kLc + (8 + frow) * kLc;
+    const int fix_off = l < 40 ? (int)offsetof(DbWave, ring_y) + ((l - 32) & 3) * 16 + 12
+                               : (int)offsetof(DbWave, ring_c) + fcomp * 32 + frow * 8 + 4;
+    // Left context: lanes 0-19 the luma rows, 20-43 the chroma rows (the other window's right 4
+    // columns become this window's left 4).
+    const int sk = l - 20, scomp = sk >= 12 ? 1 : 0, srow = sk - 12 * scomp;
+    const int left_off = l < 20 ? l * 20 : kLcOff + scomp * 20 * kLc + srow * kLc;
+    const int left_from = l < 20 ? 16 : 8;
+    uint8_t* const wave_base = reinterpret_cast<uint8_t*>(&S);
+    const uint8_t* const above_base = reinterpret_cast<const uint8_t*>(&Above);
     int qp_left = 0;
-    for (int x = 0; x < MW; ++x) {
+
+    // Writes the samples step xs made final (in window wb) to the picture: luma rows y*16-3 ..
+    // y*16+12 of columns xs*16-4 .. xs*16+11 (chroma rows y*8-3 .. y*8+4 of columns xs*8-4 ..
+    // xs*8+3); the rows below are handed to and written by the next row, which modifies them, except
+    // on the last row, and the last column writes the right 4 columns as well.
+    auto write_back = [&](const int xs, const int wb) {
+        const DbWindow& Wn = S.win[wb];
+        const int r = 1 + (l >> 2), c4 = 4 * (l & 3);
+        const int gy = y * 16 - 4 + r, gx = xs * 16 - 4 + c4;
+        if (gy >= 0 && gx >= 0)
+            *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&Wn.ly[r * 20 + c4]);
+        if (l < 32) {   // chroma: window rows 1..8, cols 0..7
+            const int comp = l >> 4, rr = 1 + ((l >> 1) & 7), cc = 4 * (l & 1);
+            const int cy = y * 8 - 4 + rr, cx = xs * 8 - 4 + cc;
+            if (cy >= 0 && cx >= 0)
+                *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)cy * CW + cx) =
+                    *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]);
+        }
+        if (last_row || xs == MW - 1) {
+            if (last_row && l < 12) {   // rows y*16+13 .. +15 (window rows 17..19)
+                const int r2 = 17 + (l >> 2), gx2 = xs * 16 - 4 + c4;
+                if (gx2 >= 0)
+                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)(y * 16 - 4 + r2) * W + gx2) =
+                        *reinterpret_cast<const uint32_t*>(&Wn.ly[r2 * 20 + c4]);
+            }
+            if (xs == MW - 1 && l < 19) {   // last column: columns xs*16+12 .. +15 (window col 16)
+                const int r3 = 1 + l;
+                const int gy3 = y * 16 - 4 + r3;
+                if (gy3 >= 0 && (r3 <= 16 || last_row))
+                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)gy3 * W + xs * 16 + 12) =
+                        *reinterpret_cast<const uint32_t*>(&Wn.ly[r3 * 20 + 16]);
+            }
+            // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
+            // columns xs*8+4 .. +7 (window col 8, 2 x 11 rows): at most 34 dwords, one per lane
+            const int extra_rows = last_row ? 12 : 0, extra_cols = xs == MW - 1 ? 22 : 0;
+            if (l < extra_rows + extra_cols) {
+                const int i = l;
+                if (i < extra_rows) {
+                    const int comp = i / 6, rr = 9 + (i % 6) / 2, cc = 4 * (i & 1);
+                    const int cx = xs * 8 - 4 + cc;
+                    if (cx >= 0)
+                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)(y * 8 - 4 + rr) * CW + cx) =
+                            *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]);
+                } else {
+                    const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
+                    const int cy = y * 8 - 4 + rr;
+                    if (cy >= 0 && (rr <= 8 || last_row))
+                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)cy * CW + xs * 8 + 4) =
+                            *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + 8]);
+                }
+            }
+        }
+    };
+
+    // One step in window b = x & 1 (a constant after inlining: the loop below is unrolled by two).
+    // The top context travels by value (returned: the rows above macroblock x + 2, or the input):
+    // through a reference the compiler may load it through a pointer select with the ring slot.
+    auto step = [&](const int x, const int b, DbPrefetch& buf, const uint32_t top_in) -> uint32_t {
+        uint32_t top_next = top_in;
+        uint8_t* const wbase = reinterpret_cast<uint8_t*>(&S.win[b]);
+        const uint8_t* const wprev = reinterpret_cast<const uint8_t*>(&S.win[b ^ 1]);
+        const DbPrefetch cur = buf;
         const mmr_recon_h264_macroblock m = cur.m;
         const bool filter = !(m.flags & MMR_H264_MB_NO_DEBLOCK);
         // filter parameters of the left edge, the internal edges and the top edge
@@ -338,165 +524,100 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         const EdgeParams ev = edge_params(T, (lane_qp(qp_left) + qpc + 1) >> 1, m.offset_a, m.offset_b);
         const EdgeParams ei = edge_params(T, qpc, m.offset_a, m.offset_b);
         const EdgeParams eh = edge_params(T, (lane_qp(cur.qp_top) + qpc + 1) >> 1, m.offset_a, m.offset_b);
-        // own samples into the window (rows 4.., cols 4..)
-        *reinterpret_cast<uint32_t*>(&S.ly[(4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]) = cur.own_y;
-        if (l < 32) {
-            const int comp = l >> 4, row = (l >> 1) & 7;
-            *reinterpret_cast<uint32_t*>(&S.lc[comp][(4 + row) * kLc + 4 + 4 * (l & 1)]) = cur.own_c;
+        // Strengths from the macroblock's record: vertical edge e, segment g at nibble e*4+g of
+        // dwords x / y, horizontal at nibble e*4+g of dwords z / w. Edge slots k = 4, 8, 12, 16 are
+        // the luma edges 0..3 and the chroma edges 0 and 2 (slots 12, 16 have strength 0 on chroma
+        // lines).
+        const uint32_t sh = 4u * (uint32_t)(chroma_lane ? (li >> 1) : (li >> 2));
+        const int4 r = cur.bs;
+        auto nib = [&](int v, uint32_t shift) { return (int)(((uint32_t)v >> shift) & 0xfu); };
+        const int bv0 = nib(r.x, sh), bv1 = chroma_lane ? nib(r.y, sh) : nib(r.x, sh + 16);
+        const int bv2 = chroma_lane ? 0 : nib(r.y, sh), bv3 = chroma_lane ? 0 : nib(r.y, sh + 16);
+        const int bh0 = nib(r.z, sh), bh1 = chroma_lane ? nib(r.w, sh) : nib(r.z, sh + 16);
+        const int bh2 = chroma_lane ? 0 : nib(r.w, sh), bh3 = chroma_lane ? 0 : nib(r.w, sh + 16);
+        // own samples (rows 4.., cols 4..) and left context (cols 0..3) into the window
+        *reinterpret_cast<uint32_t*>(wbase + (4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)) = cur.own_y;
+        if (l < 32) *reinterpret_cast<uint32_t*>(wbase + kLcOff + (l >> 4) * 20 * kLc + (4 + ((l >> 1) & 7)) * kLc + 4 + 4 * (l & 1)) = cur.own_c;
+        if (l < 44) *reinterpret_cast<uint32_t*>(wbase + left_off) = *reinterpret_cast<const uint32_t*>(wprev + left_off + left_from);
+        // the previous step's final samples: its window is not written again before step x + 1
+        if (x > 0) write_back(x - 1, b ^ 1);
+        wave_sync();
+        if (filter && l < 32) {   // vertical edges: the lane's row, 5 dwords of the window
+            uint32_t* row = reinterpret_cast<uint32_t*>(wbase + line_off + (4 + li) * 20);
+            int sm[20];
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const uint32_t v = row[i];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) sm[4 * i + j] = (int)((v >> (8 * j)) & 0xffu);
+            }
+            filter_edge<4>(sm, ev, bv0, luma_m);
+            filter_edge_inner<8>(sm, ei, bv1, luma_m);
+            filter_edge_inner<12>(sm, ei, bv2, luma_m);
+            filter_edge_inner<16>(sm, ei, bv3, luma_m);
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                row[i] = (uint32_t)sm[4 * i] | ((uint32_t)sm[4 * i + 1] << 8) | ((uint32_t)sm[4 * i + 2] << 16) | ((uint32_t)sm[4 * i + 3] << 24);
         }
-        if (x + 1 < MW) db_prefetch(job, MW, x + 1, y, l, nxt);
-        // top context from the row above (slot x), after it finished macroblock x + 1
-        if (y > 0 && band_first) {
-            if (!db_wait_band(gprev, (uint32_t)min(x + 2, MW), seen, status)) return;
-            if (l < 16) {
-                *reinterpret_cast<uint32_t*>(&S.ly[(l >> 2) * 20 + 4 + 4 * (l & 3)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
-                    as_global(job.planes[0]) + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
-            } else if (l < 32) {
-                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc + 4 + 4 * (k & 1)]) = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(
-                    as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
+        // top context: from the picture (loaded two steps ago) or from the row above's ring slot x,
+        // after that row finished macroblock x + 1
+        if (y > 0) {
+            uint32_t top = top_in;
+            if (!band_first) {
+                if (!failed && !db_wait(&prog[w - 1], (uint32_t)min(x + 2, MW), status)) failed = true;
+                top = l < 32 ? *reinterpret_cast<const uint32_t*>(above_base + ring_off + 64 * (x % kDbRing)) : 0u;
             }
-        } else if (y > 0) {
-            if (!db_wait(&prog[w - 1], (uint32_t)min(x + 2, MW), status)) return;
-            const int slot = x % kDbRing;
-            if (l < 16) {
-                *reinterpret_cast<uint32_t*>(&S.ly[(l >> 2) * 20 + 4 + 4 * (l & 3)]) =
-                    *reinterpret_cast<const uint32_t*>(&Above.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]);
-            } else if (l < 32) {
-                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-                *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc + 4 + 4 * (k & 1)]) =
-                    *reinterpret_cast<const uint32_t*>(&Above.ring_c[slot][comp][row * 8 + 4 * (k & 1)]);
-            }
+            if (l < 32) *reinterpret_cast<uint32_t*>(wbase + ctx_off) = top;
         }
         wave_sync();
-        if (filter && l < 32) {
-            // One line per lane, in registers: lanes 0-15 luma (rows in the vertical pass,
-            // columns in the horizontal one: 4 samples of context + 16), 16-23 Cb and 24-31 Cr (4 +
-            // 8). Edge slots k = 4, 8, 12, 16 are the luma edges 0..3 and the chroma edges 0 and 2
-            // (slots 12, 16 have strength 0 on chroma lines). Strengths come from the macroblock's
-            // record: vertical edge e, segment g at nibble e*4+g of dwords x / y, horizontal at
-            // nibble e*4+g of dwords z / w.
-            const int li = chroma_lane ? ((l - 16) & 7) : l;
-            uint8_t* win = chroma_lane ? S.lc[(l - 16) >> 3] : S.ly;
-            const uint32_t sh = 4u * (uint32_t)(chroma_lane ? (li >> 1) : (li >> 2));
-            const int4 r = cur.bs;
-            auto nib = [&](int v, uint32_t shift) { return (int)(((uint32_t)v >> shift) & 0xfu); };
-            const int bv[4] = {nib(r.x, sh), chroma_lane ? nib(r.y, sh) : nib(r.x, sh + 16), chroma_lane ? 0 : nib(r.y, sh),
-                               chroma_lane ? 0 : nib(r.y, sh + 16)};
-            const int bh[4] = {nib(r.z, sh), chroma_lane ? nib(r.w, sh) : nib(r.z, sh + 16), chroma_lane ? 0 : nib(r.w, sh),
-                               chroma_lane ? 0 : nib(r.w, sh + 16)};
-            {   // vertical edges
-                uint32_t* row = reinterpret_cast<uint32_t*>(win + (4 + li) * 20);
-                uint32_t v[5];
+        if (filter && l < 32) {   // horizontal edges: the lane's column, 20 bytes of the window
+            uint8_t* col = wbase + line_off + 4 + li;
+            int sm[20];
 #pragma unroll
-                for (int i = 0; i < 5; ++i) v[i] = row[i];
-                filter_line<4>(v, ev, bv[0], chroma_lane);
-                filter_line<8>(v, ei, bv[1], chroma_lane);
-                filter_line<12>(v, ei, bv[2], chroma_lane);
-                filter_line<16>(v, ei, bv[3], chroma_lane);
+            for (int i = 0; i < 20; ++i) sm[i] = col[i * 20];
+            filter_edge<4>(sm, eh, bh0, luma_m);
+            filter_edge_inner<8>(sm, ei, bh1, luma_m);
+            filter_edge_inner<12>(sm, ei, bh2, luma_m);
+            filter_edge_inner<16>(sm, ei, bh3, luma_m);
 #pragma unroll
-                for (int i = 0; i < 5; ++i) row[i] = v[i];
-            }
-            wave_sync();
-            {   // horizontal edges
-                uint8_t* col = win + 4 + li;
-                uint32_t v[5];
-#pragma unroll
-                for (int i = 0; i < 5; ++i)
-                    v[i] = col[(4 * i) * 20] | (col[(4 * i + 1) * 20] << 8) | (col[(4 * i + 2) * 20] << 16) | ((uint32_t)col[(4 * i + 3) * 20] << 24);
-                filter_line<4>(v, eh, bh[0], chroma_lane);
-                filter_line<8>(v, ei, bh[1], chroma_lane);
-                filter_line<12>(v, ei, bh[2], chroma_lane);
-                filter_line<16>(v, ei, bh[3], chroma_lane);
-#pragma unroll
-                for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)get_byte(v, i);
-            }
+            for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)sm[i];
         }
         wave_sync();
+        // Mid-step: the agent-scope hand-over, then this step's loads for step x + 2.
+        if (gself && x > 0 && x % publish == 0)   // steps 0 .. x-1 are written (step x-1's stores were issued above)
+            __hip_atomic_store(gself, (uint32_t)x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (x + 2 < MW) {
+            if (global_top) {
+                if (!failed && !db_wait_band(gprev, (uint32_t)min(x + 4, MW), seen, status)) failed = true;
+                top_next = db_load_top(P, MW, x + 2, y, l);
+            }
+            db_prefetch(P, MW, x + 2, y, l, buf);
+        }
         // hand the bottom rows to the row below: slot x (all columns) and the right columns of
         // slot x - 1, which this step's left edge modified
         if (!band_last) {
             const int free_at = x - kDbRing + 2;
-            if (free_at > 0 && !db_wait(&prog[w + 1], (uint32_t)free_at, status)) return;
-            const int slot = x % kDbRing, prev = (x + kDbRing - 1) % kDbRing;
-            if (l < 16) {
-                *reinterpret_cast<uint32_t*>(&S.ring_y[slot][(l >> 2) * 16 + 4 * (l & 3)]) =
-                    *reinterpret_cast<const uint32_t*>(&S.ly[(16 + (l >> 2)) * 20 + 4 + 4 * (l & 3)]);
-            } else if (l < 32) {
-                const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-                *reinterpret_cast<uint32_t*>(&S.ring_c[slot][comp][row * 8 + 4 * (k & 1)]) =
-                    *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * kLc + 4 + 4 * (k & 1)]);
-            } else if (x > 0 && l < 36) {
-                const int row = l - 32;
-                *reinterpret_cast<uint32_t*>(&S.ring_y[prev][row * 16 + 12]) = *reinterpret_cast<const uint32_t*>(&S.ly[(16 + row) * 20]);
-            } else if (x > 0 && l >= 40 && l < 48) {
-                const int k = l - 40, comp = k >> 2, row = k & 3;
-                *reinterpret_cast<uint32_t*>(&S.ring_c[prev][comp][row * 8 + 4]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][(8 + row) * kLc]);
-            }
-        }
-        // write back the samples that are now final
-        {
-            // luma: rows y*16-3 .. y*16+12 (window rows 1..16), columns x*16-4 .. x*16+11 (window cols 0..15)
-            const int r = 1 + (l >> 2), c4 = 4 * (l & 3);
-            const int gy = y * 16 - 4 + r, gx = x * 16 - 4 + c4;
-            if (gy >= 0 && gx >= 0)
-                *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&S.ly[r * 20 + c4]);
-            if (last_row && l < 12) {   // rows y*16+13 .. +15 (window rows 17..19)
-                const int r2 = 17 + (l >> 2), gx2 = x * 16 - 4 + c4;
-                if (gx2 >= 0)
-                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y * 16 - 4 + r2) * W + gx2) =
-                        *reinterpret_cast<const uint32_t*>(&S.ly[r2 * 20 + c4]);
-            }
-            if (x == MW - 1 && l < 19) {   // last column: columns x*16+12 .. +15 (window col 16)
-                const int r3 = 1 + l;
-                const int gy3 = y * 16 - 4 + r3;
-                if (gy3 >= 0 && (r3 <= 16 || last_row))
-                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)gy3 * W + x * 16 + 12) =
-                        *reinterpret_cast<const uint32_t*>(&S.ly[r3 * 20 + 16]);
-            }
-            // chroma: rows y*8-3 .. y*8+4 (window rows 1..8), columns x*8-4 .. x*8+3 (window cols 0..7)
-            if (l < 32) {
-                const int comp = l >> 4, rr = 1 + ((l >> 1) & 7), cc = 4 * (l & 1);
-                const int cy = y * 8 - 4 + rr, cx = x * 8 - 4 + cc;
-                if (cy >= 0 && cx >= 0)
-                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + cx) =
-                        *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + cc]);
-            }
-            // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
-            // columns x*8+4 .. +7 (window col 8, 2 x 11 rows)
-            const int extra_rows = last_row ? 12 : 0, extra_cols = x == MW - 1 ? 22 : 0;
-            for (int i = l; i < extra_rows + extra_cols; i += 64) {
-                if (i < extra_rows) {
-                    const int comp = i / 6, rr = 9 + (i % 6) / 2, cc = 4 * (i & 1);
-                    const int cx = x * 8 - 4 + cc;
-                    if (cx >= 0)
-                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)(y * 8 - 4 + rr) * CW + cx) =
-                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + cc]);
-                } else {
-                    const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
-                    const int cy = y * 8 - 4 + rr;
-                    if (cy >= 0 && (rr <= 8 || last_row))
-                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[1 + comp]) + (size_t)cy * CW + x * 8 + 4) =
-                            *reinterpret_cast<const uint32_t*>(&S.lc[comp][rr * kLc + 8]);
-                }
-            }
+            if (free_at > 0 && !failed && !db_wait(&prog[w + 1], (uint32_t)free_at, status)) failed = true;
+            if (l < 32)
+                *reinterpret_cast<uint32_t*>(wave_base + ring_off + 64 * (x % kDbRing)) = *reinterpret_cast<const uint32_t*>(wbase + bot_off);
+            else if (x > 0 && fix_lane)
+                *reinterpret_cast<uint32_t*>(wave_base + fix_off + 64 * ((x + kDbRing - 1) % kDbRing)) = *reinterpret_cast<const uint32_t*>(wbase + fix_win);
         }
         wave_sync();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (gself && ((x + 1) % publish == 0 || x == MW - 1))
-            __hip_atomic_store(gself, (uint32_t)(x + 1), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        // shift the window left by one macroblock (its right 4 columns become the left context)
-        if (l < 20) {
-            *reinterpret_cast<uint32_t*>(&S.ly[l * 20]) = *reinterpret_cast<const uint32_t*>(&S.ly[l * 20 + 16]);
-        } else if (l < 44) {
-            const int k = l - 20, comp = k / 12, row = k % 12;
-            *reinterpret_cast<uint32_t*>(&S.lc[comp][row * kLc]) = *reinterpret_cast<const uint32_t*>(&S.lc[comp][row * kLc + 8]);
-        }
-        wave_sync();
         qp_left = m.qp;
-        cur = nxt;
+        return top_next;
+    };
+    for (int x = 0; x < MW && !failed; x += 2) {
+        top_even = step(x, 0, pf_even, top_even);
+        if (x + 1 < MW && !failed) top_odd = step(x + 1, 1, pf_odd, top_odd);
     }
+    if (!failed) {
+        wave_sync();
+        write_back(MW - 1, (MW - 1) & 1);
+    }
+    if (gself) __hip_atomic_store(gself, (uint32_t)MW, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // ROWS macroblock rows per band, one wave per row.
@@ -514,7 +635,7 @@ __global__ __launch_bounds__(ROWS * 64) void h264_deblock_kernel(const H264Job* 
         if (threadIdx.x == 0) s_ticket = fetch_add_relaxed(&counters[0], 1);
         if (threadIdx.x < ROWS) prog[threadIdx.x] = 0;
         __syncthreads();
-        const uint32_t t = s_ticket;
+        const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_ticket);
         if (t >= total_bands) break;
         const H264Job& job = jobs[find_job(jobs, job_count, t, [](const H264Job& j) { return j.band_begin; })];
         const uint32_t band = t - job.band_begin;

@@ -46,6 +46,14 @@ MMR_DEV T* pick3(T* const* p, int i) {
     return reinterpret_cast<T*>(i == 0 ? v[0] : (i == 1 ? v[1] : v[2]));
 }
 
+// A pointer that is the same in every lane, in scalar registers.
+template <typename T>
+MMR_DEV T* uniform_ptr(T* p) {
+    const uint64_t x = reinterpret_cast<uint64_t>(p);
+    return reinterpret_cast<T*>(((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+                                (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)x));
+}
+
 // Non-fused f64 arithmetic. The reference evaluates every product and sum with its own rounding
 // (Rust never contracts a*b+c into an FMA); these builtins keep the compiler from fusing too.
 MMR_DEV double dmul(double a, double b) { return __dmul_rn(a, b); }

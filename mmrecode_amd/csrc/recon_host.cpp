@@ -674,7 +674,7 @@ mmr_recon_tuning default_tuning() {
     t.mjpeg_group_blocks = 32;
     t.mpeg2_group_blocks = 32;
     t.dv_group_blocks = 32;
-    t.h264_deblock_band_rows = 16;
+    t.h264_deblock_band_rows = 8;
     t.h264_deblock_groups_per_cu = 1;
     t.h264_deblock_publish = 2;
     return t;
