@@ -535,6 +535,14 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         const int bv2 = chroma_lane ? 0 : nib(r.y, sh), bv3 = chroma_lane ? 0 : nib(r.y, sh + 16);
         const int bh0 = nib(r.z, sh), bh1 = chroma_lane ? nib(r.w, sh) : nib(r.z, sh + 16);
         const int bh2 = chroma_lane ? 0 : nib(r.w, sh), bh3 = chroma_lane ? 0 : nib(r.w, sh + 16);
+        // Edge slots whose strengths are all 0 (luma and chroma lines alike) change no sample; the
+        // record is the same in every lane, so the test is a scalar branch. Slot 4: dword bits 0-15
+        // of x (z); slot 8: bits 16-31 of x (z) and, for chroma, bits 0-15 of y (w); slot 12: bits
+        // 0-15 of y (w); slot 16: bits 16-31 of y (w).
+        const uint32_t vx = (uint32_t)__builtin_amdgcn_readfirstlane(r.x), vy = (uint32_t)__builtin_amdgcn_readfirstlane(r.y);
+        const uint32_t hz = (uint32_t)__builtin_amdgcn_readfirstlane(r.z), hw = (uint32_t)__builtin_amdgcn_readfirstlane(r.w);
+        const bool v4 = (vx & 0xffffu) != 0, v8 = (vx >> 16) != 0 || (vy & 0xffffu) != 0, v12 = (vy & 0xffffu) != 0, v16 = (vy >> 16) != 0;
+        const bool h4 = (hz & 0xffffu) != 0, h8 = (hz >> 16) != 0 || (hw & 0xffffu) != 0, h12 = (hw & 0xffffu) != 0, h16 = (hw >> 16) != 0;
         // own samples (rows 4.., cols 4..) and left context (cols 0..3) into the window
         *reinterpret_cast<uint32_t*>(wbase + (4 + (l >> 2)) * 20 + 4 + 4 * (l & 3)) = cur.own_y;
         if (l < 32) *reinterpret_cast<uint32_t*>(wbase + kLcOff + (l >> 4) * 20 * kLc + (4 + ((l >> 1) & 7)) * kLc + 4 + 4 * (l & 1)) = cur.own_c;
@@ -551,10 +559,10 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
 #pragma unroll
                 for (int j = 0; j < 4; ++j) sm[4 * i + j] = (int)((v >> (8 * j)) & 0xffu);
             }
-            filter_edge<4>(sm, ev, bv0, luma_m);
-            filter_edge_inner<8>(sm, ei, bv1, luma_m);
-            filter_edge_inner<12>(sm, ei, bv2, luma_m);
-            filter_edge_inner<16>(sm, ei, bv3, luma_m);
+            if (v4) filter_edge<4>(sm, ev, bv0, luma_m);
+            if (v8) filter_edge_inner<8>(sm, ei, bv1, luma_m);
+            if (v12) filter_edge_inner<12>(sm, ei, bv2, luma_m);
+            if (v16) filter_edge_inner<16>(sm, ei, bv3, luma_m);
 #pragma unroll
             for (int i = 0; i < 5; ++i)
                 row[i] = (uint32_t)sm[4 * i] | ((uint32_t)sm[4 * i + 1] << 8) | ((uint32_t)sm[4 * i + 2] << 16) | ((uint32_t)sm[4 * i + 3] << 24);
@@ -575,10 +583,10 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             int sm[20];
 #pragma unroll
             for (int i = 0; i < 20; ++i) sm[i] = col[i * 20];
-            filter_edge<4>(sm, eh, bh0, luma_m);
-            filter_edge_inner<8>(sm, ei, bh1, luma_m);
-            filter_edge_inner<12>(sm, ei, bh2, luma_m);
-            filter_edge_inner<16>(sm, ei, bh3, luma_m);
+            if (h4) filter_edge<4>(sm, eh, bh0, luma_m);
+            if (h8) filter_edge_inner<8>(sm, ei, bh1, luma_m);
+            if (h12) filter_edge_inner<12>(sm, ei, bh2, luma_m);
+            if (h16) filter_edge_inner<16>(sm, ei, bh3, luma_m);
 #pragma unroll
             for (int i = 1; i < 19; ++i) col[i * 20] = (uint8_t)sm[i];
         }
