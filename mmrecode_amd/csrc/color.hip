@@ -66,15 +66,13 @@ MMR_DEV void rgb_of(uint32_t y, const ChromaTerms& t, float out[3]) {
 // Byte k (0..3) of dword d replaced by the integer v in [0, 255] held as binary32.
 MMR_DEV uint32_t put_byte(uint32_t d, float v, uint32_t k) { return __builtin_amdgcn_cvt_pk_u8_f32(v, k, d); }
 
-// One wave converts 256 pixels of one row, 4 horizontally adjacent pixels per thread (one luma
-// dword, 12 RGB bytes / 16 RGBA bytes); four waves per workgroup tile. The host's tile -> job
-// table gives the tile's job with one scalar load; the row, its chroma row and the job's constants
-// are wave-uniform. The chroma sample of pixel x is floor(x cw / width) (video_frame_image): when
-// 2 cw == width that is x / 2 and pixels 2k, 2k + 1 share one set of chroma terms; otherwise the
-// first from div_floor, the next three by carrying the remainder (cw <= width, so each step moves
-// at most one chroma sample).
+// Four horizontally adjacent pixels (x0 .. x0 + 3 of row y, clipped to the width) per thread: one
+// luma dword, 12 RGB bytes / 16 RGBA bytes. The chroma row is floor(y ch / height) and the chroma
+// sample of pixel x floor(x cw / width) (video_frame_image): when 2 cw == width that is x / 2 and
+// pixels 2k, 2k + 1 share one set of chroma terms; otherwise the first from div_floor, the next
+// three by carrying the remainder (cw <= width, so each step moves at most one chroma sample).
 template <bool kLimited>
-MMR_DEV void yuv_to_rgb_wave(const RgbJob& j, uint32_t y, uint32_t x0) {
+MMR_DEV void yuv_to_rgb_px4(const RgbJob& j, uint32_t y, uint32_t x0) {
     const uint32_t cy = min(div_floor(y * j.ch, j.height, j.inv_height), j.ch - 1);
     const MMR_GLOBAL uint8_t* Y = as_global(j.planes[0]) + (size_t)y * j.luma_stride;
     const MMR_GLOBAL uint8_t* U = as_global(j.planes[1]) + (size_t)cy * j.chroma_stride;
@@ -93,8 +91,21 @@ MMR_DEV void yuv_to_rgb_wave(const RgbJob& j, uint32_t y, uint32_t x0) {
         // x0 + 2 exists, so it never passes cw - 1).
         const uint32_t c0 = x0 >> 1;
         const uint32_t c1 = n > 2 ? c0 + 1 : c0;
-        const ChromaTerms t0 = chroma_terms<kLimited>(j, U[c0], V[c0]);
-        const ChromaTerms t1 = chroma_terms<kLimited>(j, U[c1], V[c1]);
+        uint32_t u0, u1, v0, v1;
+        if (n > 2 && ((reinterpret_cast<uintptr_t>(U + c0) | reinterpret_cast<uintptr_t>(V + c0)) & 1) == 0) {   // both samples at once
+            const uint32_t uu = *reinterpret_cast<const MMR_GLOBAL uint16_t*>(U + c0), vv = *reinterpret_cast<const MMR_GLOBAL uint16_t*>(V + c0);
+            u0 = uu & 0xffu;
+            u1 = uu >> 8;
+            v0 = vv & 0xffu;
+            v1 = vv >> 8;
+        } else {
+            u0 = U[c0];
+            u1 = U[c1];
+            v0 = V[c0];
+            v1 = V[c1];
+        }
+        const ChromaTerms t0 = chroma_terms<kLimited>(j, u0, v0);
+        const ChromaTerms t1 = chroma_terms<kLimited>(j, u1, v1);
         rgb_of<kLimited>(ys & 0xffu, t0, px[0]);
         rgb_of<kLimited>((ys >> 8) & 0xffu, t0, px[1]);
         rgb_of<kLimited>((ys >> 16) & 0xffu, t1, px[2]);
@@ -127,10 +138,11 @@ MMR_DEV void yuv_to_rgb_wave(const RgbJob& j, uint32_t y, uint32_t x0) {
             d2 = put_byte(d2, px[3][0], 1);
             d2 = put_byte(d2, px[3][1], 2);
             d2 = put_byte(d2, px[3][2], 3);
-            MMR_GLOBAL uint32_t* o32 = reinterpret_cast<MMR_GLOBAL uint32_t*>(o);
-            o32[0] = d0;
-            o32[1] = d1;
-            o32[2] = d2;
+            mmr_v3u32 v;   // one 12-byte store: a wave's stores are contiguous
+            v.x = d0;
+            v.y = d1;
+            v.z = d2;
+            *reinterpret_cast<MMR_GLOBAL mmr_v3u32*>(o) = v;
         } else {
             for (uint32_t i = 0; i < n; ++i) {
                 o[i * 3 + 0] = (uint8_t)px[i][0];
@@ -167,12 +179,25 @@ __global__ __launch_bounds__(256) void yuv_to_rgb_kernel(const RgbJob* __restric
     const uint32_t tile = tile_base + local;
     const RgbJob& j = jobs[tile_job[tile]];
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
-    const uint32_t y = wave / j.waves_per_row;
-    if (y >= j.height) return;   // the job's last tile may hold fewer waves
-    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
-    if (x0 >= j.width) return;
-    if (j.limited) yuv_to_rgb_wave<true>(j, y, x0);
-    else yuv_to_rgb_wave<false>(j, y, x0);
+    const uint32_t lane = threadIdx.x & 63;
+    // The wave's segments (see set_rgb_layout); the job's last tile may hold fewer.
+    for (uint32_t k = 0; k < kRgbSegmentsPerWave; ++k) {
+        const uint32_t seg = wave * kRgbSegmentsPerWave + k;
+        if (seg >= j.segments) break;   // uniform over the wave
+        uint32_t y, x0;
+        if (j.flat) {   // raster order: a lane's 4 pixels lie in one row (width % 4 == 0)
+            const uint32_t p = seg * 256 + lane * 4;
+            if (p >= j.width * j.height) continue;
+            y = div_floor(p, j.width, j.inv_width);
+            x0 = p - y * j.width;
+        } else {
+            y = seg / j.waves_per_row;
+            x0 = (seg - y * j.waves_per_row) * 256 + lane * 4;
+            if (x0 >= j.width) continue;
+        }
+        if (j.limited) yuv_to_rgb_px4<true>(j, y, x0);
+        else yuv_to_rgb_px4<false>(j, y, x0);
+    }
 }
 
 // ---------------------------------------------------------------- MMFX linear light (u16)

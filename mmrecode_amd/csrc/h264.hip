@@ -255,8 +255,8 @@ __global__ __launch_bounds__(256) void h264_mb_kernel(const H264Job* __restrict_
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
-        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, t8, l, job, kInterLists, status);
-        else residual_phases<int64_t>(S, lv, qp, cqp, t8, l, job, kInterLists, status);
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, t8, l, as_global(job.scaling), kInterLists, status);
+        else residual_phases<int64_t>(S, lv, qp, cqp, t8, l, as_global(job.scaling), kInterLists, status);
         if (job.transform_bypass && (qp == 0 || cqp == 0)) {
             const Bypass by = {qp == 0 && !t8, cqp == 0, {0x02020202u, 0x02020202u, 0x02020202u, 0x02020202u}, 2};
             bypass_phases(S, lv, l, by);

@@ -205,8 +205,8 @@ MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon
         for (int d = 32; d >= 1; d >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, d, 64));
         const int qp = m.qp, cqp = chroma_qp(qp, job.chroma_qp_offset[0]);   // first offset for both
         const ResidualLists rl = {0, 0, 1, i16};
-        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, job, rl, status);
-        else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, job, rl, status);
+        if (mx < 250) residual_phases<int32_t>(S, lv, qp, cqp, i8, l, as_global(job.scaling), rl, status);
+        else residual_phases<int64_t>(S, lv, qp, cqp, i8, l, as_global(job.scaling), rl, status);
         if (job.transform_bypass && (qp == 0 || cqp == 0)) {
             Bypass by;
             by.luma = qp == 0 && type == MMR_H264_INTRA_4X4;

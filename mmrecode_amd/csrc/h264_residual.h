@@ -161,9 +161,8 @@ MMR_DEV void bypass_residual_4x4(const int32_t* lv, int mode, int32_t out[16]) {
 // Residual of one macroblock (phases 2-4 of the kernel) into S.res_y / S.res_c, in integer type T,
 // from its level record lv (global memory, or a copy in LDS).
 template <typename T, typename LV>
-MMR_DEV void residual_phases(MbShared& S, LV lv, int qp, int cqp, bool t8, int l, const H264Job& job,
+MMR_DEV void residual_phases(MbShared& S, LV lv, int qp, int cqp, bool t8, int l, const MMR_GLOBAL uint8_t* scaling,
                              ResidualLists rl, uint32_t* status) {
-    const MMR_GLOBAL uint8_t* scaling = as_global(job.scaling);
     // ---- phase 1 (Intra16x16): transform_intra16_luma_dc, lane l = output (row l >> 2, column l & 3)
     if (rl.i16) {
         if (l < 16) {

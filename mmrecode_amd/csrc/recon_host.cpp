@@ -641,9 +641,9 @@ mmr::RgbJob rgb_job_of(const mmr_recon_mjpeg_frame& f, const mmr::MjpegJob& j) {
     r.chroma_stride = j.plane_w[1];
     r.range = f.rgb_range ? f.rgb_range : MMR_RANGE_FULL;
     r.matrix = MMR_MATRIX_BT601;
-    r.waves_per_row = (f.width + 255) / 256;
     r.inv_width = 1.0 / f.width;
     r.inv_height = 1.0 / f.height;
+    mmr::set_rgb_layout(r);
     mmr::set_rgb_terms(r);
     return r;
 }
@@ -1083,9 +1083,9 @@ mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, cons
         r.chroma_stride = in[i].chroma_stride;
         r.range = in[i].range;
         r.matrix = in[i].matrix;
-        r.waves_per_row = (in[i].width + 255) / 256;
         r.inv_width = 1.0 / in[i].width;
         r.inv_height = 1.0 / in[i].height;
+        mmr::set_rgb_layout(r);
         mmr::set_rgb_terms(r);
         r.tile_begin = add_tiles(tiles, i, mmr::rgb_job_tiles(r), 1);
     }

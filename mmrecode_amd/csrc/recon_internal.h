@@ -111,10 +111,22 @@ struct RgbJob {
     float kcb[3], kcr[3], k0[3];
     uint32_t limited;             // range == MMR_RANGE_LIMITED
     uint32_t half_chroma;         // 2 cw == width: pixels 2k and 2k + 1 share chroma sample k
+    uint32_t flat;                // segments run through the pixels in raster order (width % 4 == 0)
+    uint32_t segments;            // 256-pixel segments of the job (one wave pass, 4 pixels per lane)
 };
-// Workgroup tiles of the YUV->RGB kernel: 4 waves (256 threads, 4 pixels each), never straddling
-// two jobs; job j has ceil(waves_per_row * height / 4) tiles.
-inline uint32_t rgb_job_tiles(const RgbJob& j) { return (j.waves_per_row * j.height + 3) / 4; }
+// Segments of the YUV->RGB kernel: 256 pixels, 4 per lane of a wave. When the width is a multiple
+// of 4 (and the picture below 2^31 pixels) they run through the pixels in raster order across rows
+// (every segment but the last full); otherwise each row has ceil(width / 256) segments. A wave
+// converts kRgbSegmentsPerWave consecutive segments and a workgroup tile holds 4 waves, never
+// straddling two jobs.
+constexpr uint32_t kRgbSegmentsPerWave = 4;
+inline void set_rgb_layout(RgbJob& j) {
+    j.waves_per_row = (j.width + 255) / 256;
+    const uint64_t px = (uint64_t)j.width * j.height;
+    j.flat = j.width % 4 == 0 && px < (1ull << 31) ? 1u : 0u;
+    j.segments = j.flat ? (uint32_t)((px + 255) / 256) : j.waves_per_row * j.height;
+}
+inline uint32_t rgb_job_tiles(const RgbJob& j) { return (j.segments + 4 * kRgbSegmentsPerWave - 1) / (4 * kRgbSegmentsPerWave); }
 
 // Fills the chroma-term constants, limited and half_chroma of j from range, matrix, width and cw.
 // Integer forms (b = Cb - 128, r = Cr - 128, yy = max(Y - 16, 0)), ycbcr_to_rgb

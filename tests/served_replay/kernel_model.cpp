@@ -279,8 +279,8 @@ hipError_t launch_mjpeg(const MjpegJob* d_jobs, const uint32_t* d_tile_job, uint
 }
 
 // color.hip: yuv_to_rgb_kernel. Tile t belongs to job tile_job[t]; its four waves are wave numbers
-// (t - tile_begin) * 4 .. + 3 of the job, wave w covers row w / waves_per_row, pixels
-// (w % waves_per_row) * 256 .. + 255 clipped to the width. Row y reads Y row y, chroma row
+// (t - tile_begin) * 4 .. + 3 of the job, wave w covers segments 4w .. 4w + 3 (set_rgb_layout:
+// raster-order runs of 256 pixels, or 256-pixel pieces of rows). Row y reads Y row y, chroma row
 // min(floor(y ch / height), ch - 1), chroma samples up to min(floor((width - 1) cw / width), cw - 1),
 // and writes packed RGB / RGBA row y.
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
@@ -292,8 +292,11 @@ hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, u
         for (auto& [i, v] : offs) {
             const RgbJob& j = recs[i];
             model::stats().jobs++;
-            if (j.waves_per_row != (j.width + 255) / 256)
-                violation("yuv_to_rgb: job %u has %u waves per row for width %u", i, j.waves_per_row, j.width);
+            RgbJob layout = j;
+            set_rgb_layout(layout);
+            if (j.waves_per_row != layout.waves_per_row || j.flat != layout.flat || j.segments != layout.segments)
+                violation("yuv_to_rgb: job %u segment layout (%u per row, flat %u, %u segments) does not match width %u height %u", i,
+                          j.waves_per_row, j.flat, j.segments, j.width, j.height);
             check_tiles("yuv_to_rgb", i, v, rgb_job_tiles(j));
             if (!j.width || !j.height || !j.cw || !j.ch) {
                 violation("yuv_to_rgb: job %u has a zero dimension", i);
