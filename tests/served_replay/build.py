@@ -18,12 +18,14 @@ def build(verbose=False):
         os.path.join(ROOT, "mmrecode_amd", "csrc", "recon_internal.h"), os.path.join(ROOT, "include", "mmr_recon.h")]
     if os.path.exists(LIB) and os.path.getmtime(LIB) >= max(os.path.getmtime(d) for d in deps):
         return LIB
+    # a per-process temporary name: parallel test workers may build at the same time
+    tmp = f"{LIB}.tmp{os.getpid()}"
     cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-g", "-fPIC", "-shared", "-fvisibility=hidden", "-Wall",
-           "-Wno-unused-function", "-I", HERE, "-o", LIB + ".tmp", *SOURCES, "-lpthread"]
+           "-Wno-unused-function", "-I", HERE, "-o", tmp, *SOURCES, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(tmp, LIB)
     return LIB
 
 
