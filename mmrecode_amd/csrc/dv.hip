@@ -67,13 +67,144 @@ MMR_DEV void dv_mb_coordinates(uint32_t system, int seq, int slot, int mb, int* 
     }
 }
 
-// BPG 8x8 blocks (8 lanes each) per workgroup.
+// Fused separable form of both transforms, used where it provably rounds like the reference.
+// inverse_88 is sum_v sum_u cu cv c[v][u] C16[x][u] C16[y][v] / 4; inverse_248, with its row-pair
+// butterfly expanded, is sum_r V[2y+f][r] (sum_u cu c[r][u] C16[x][u]) / 4 with V[2y+f][2v] =
+// cv C8[y][v] and V[2y+f][2v+1] = (f ? -1 : 1) cv C8[y][v]. So both are one row pass with
+// kDvRowA[x][u] = cu C16[x][u] (lane v: tmp[v][x]) and one column pass (lane x) with the mode's
+// vertical table, every step fused (tables rounded once on the host). Error: the reference's
+// evaluation is within gamma_66 A of the exact sum S (64 sequential sums of three-factor
+// products; inverse_248 less), the fused one within (2 gamma_8 + gamma_8^2 + 3u) A of S (tables
+// and two 8-term fused chains), A <= sum |c| <= 64 * 132092 (|c| < 2^17: the dequantised values
+// are int32 >> 14, the DC an int16 * 4 + 1024); divided by 4 both lie within 2.1e-8 of S / 4. A
+// fused value farther than kDvTieWindow = 2^-25 (3.0e-8) from every half-integer therefore
+// rounds (half away from zero) like the reference's; a block with a value inside the window is
+// recomputed in the reference's order.
+__constant__ double kDvRowA[64];     // [x*8+u] = cu * C16[x][u]
+__constant__ double kDvCol88[64];    // [y*8+v] = cv * C16[y][v]
+__constant__ double kDvCol248[64];   // [(2y+f)*8 + 2v+s] = (s && f ? -1 : 1) * cv * C8[y][v]
+constexpr double kDvTieWindow = 0x1p-25;
+
+MMR_DEV void dv_row_pass(const int32_t c[8], double* __restrict__ tmp_row) {
+    double cd[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cd[u] = (double)c[u];
+    double t[8];
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+        double s = dmul(kDvRowA[x * 8], cd[0]);
+#pragma unroll
+        for (int u = 1; u < 8; ++u) s = __builtin_fma(kDvRowA[x * 8 + u], cd[u], s);
+        t[x] = s;
+    }
+#pragma unroll
+    for (int x = 0; x < 8; x += 2) *reinterpret_cast<double2*>(tmp_row + x) = make_double2(t[x], t[x + 1]);
+}
+
+// Column pass of lane x over the rows in `rows` (wave-uniform, row 0 always included; rows that
+// are zero in every block of the wave have zero tmp and are skipped), vertical table B.
+MMR_DEV void dv_column_pass(const double* __restrict__ tmp, int x, uint32_t rows, const double* __restrict__ B, double out[8]) {
+    const double c0 = tmp[x];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) out[y] = dmul(B[y * 8], c0);
+    uint32_t rest = rows & ~1u;
+    while (rest) {
+        const int v = __builtin_ctz(rest);
+        rest &= rest - 1;
+        const double cv = tmp[v * 8 + x];
+#pragma unroll
+        for (int y = 0; y < 8; ++y) out[y] = __builtin_fma(B[y * 8 + v], cv, out[y]);
+    }
+}
+
+// Index of the k-th set bit (k from 0) of m; m has more than k set bits.
+MMR_DEV uint32_t kth_set_bit(uint64_t m, uint32_t k) {
+    uint32_t base = 0;
+#pragma unroll
+    for (uint32_t w = 32; w >= 1; w >>= 1) {
+        const uint64_t lo = m & ((1ull << w) - 1);
+        const uint32_t c = (uint32_t)__popcll(lo);
+        if (k >= c) {
+            k -= c;
+            m >>= w;
+            base += w;
+        } else {
+            m = lo;
+        }
+    }
+    return base;
+}
+
+// inverse_88 in the reference's order (decoder.rs:520-548) for lane x of a group: sum over
+// (v, u) in ascending natural order of cu*cv*c*cos((2x+1)u pi/16)*cos((2y+1)v pi/16), then
+// (sum / 4).round().clamp(0, 255). Zero coefficients contribute signed zeros and are skipped.
+MMR_DEV void dv_inverse_88_reference(const int32_t* __restrict__ cf, const double* __restrict__ cos16,
+                                     const double* __restrict__ cucv, int x, int lane, uint8_t out[8]) {
+    uint32_t rowbits = 0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) rowbits |= (uint32_t)(cf[lane * 8 + u] != 0) << u;
+    const uint32_t lo = group8_or(lane < 4 ? rowbits << (lane * 8) : 0u);
+    const uint32_t hi = group8_or(lane >= 4 ? rowbits << ((lane - 4) * 8) : 0u);
+    double sum[8];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) sum[y] = 0.0;
+    uint64_t mask = ((uint64_t)hi << 32) | lo;
+    while (mask) {
+        const int p = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int v = p >> 3, u = p & 7;
+        double t = dmul(cucv[p], (double)cf[p]);
+        t = dmul(t, cos16[x * 8 + u]);
+#pragma unroll
+        for (int y = 0; y < 8; ++y) sum[y] = dadd(sum[y], dmul(t, cos16[y * 8 + v]));
+    }
+#pragma unroll
+    for (int y = 0; y < 8; ++y) out[y] = clamp_u8_d(round_half_away(sum[y] / 4.0));
+}
+
+// inverse_248 in the reference's order (decoder.rs:555-602) for lane x: butterfly of row pairs,
+// horizontal 8-point pass (/2), per-field vertical 4-point pass (/2), round, clamp.
+MMR_DEV void dv_inverse_248_reference(const int32_t* __restrict__ cf, const double* __restrict__ cos16, int x, uint8_t out[8]) {
+    double h[8];
+#pragma unroll
+    for (int row = 0; row < 8; ++row) {
+        const int pair = row >> 1, second = row & 1;
+        double s = 0.0;
+        for (int u = 0; u < 8; ++u) {
+            const double first = (double)cf[pair * 16 + u];
+            const double other = (double)cf[pair * 16 + 8 + u];
+            const double bfly = second ? first - other : first + other;
+            if (bfly != 0.0) {
+                const double cu = u == 0 ? kFrac1Sqrt2 : 1.0;
+                s = dadd(s, dmul(dmul(cu, bfly), cos16[x * 8 + u]));
+            }
+        }
+        h[row] = s / 2.0;
+    }
+#pragma unroll
+    for (int field = 0; field < 2; ++field)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            double s = 0.0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double cv = v == 0 ? kFrac1Sqrt2 : 1.0;
+                s = dadd(s, dmul(dmul(cv, h[v * 2 + field]), kDvCos8[y * 4 + v]));
+            }
+            out[y * 2 + field] = clamp_u8_d(round_half_away(s / 2.0));
+        }
+}
+
+// BPG 8x8 blocks (8 lanes each) per workgroup, BPG <= 64. The tile's blocks are taken in
+// mode-sorted order (8-8 blocks first, then 2-4-8 blocks), so a wave's blocks share their
+// vertical table and only the wave on the boundary between the two runs both column passes.
 template <int BPG>
 __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ jobs,
                                                      const uint32_t* __restrict__ tile_job, uint32_t tile_base,
                                                      uint32_t tiles, uint32_t per_xcd) {
-    __shared__ __align__(16) int32_t coef[BPG][64];
-    __shared__ __align__(16) uint8_t pix[BPG][64];
+    static_assert(BPG <= 64, "one ballot covers the tile's blocks");
+    // Per block group: natural-order coefficients, then the row-pass output, then the samples.
+    __shared__ __align__(16) double s_area[BPG * kIdctTmpStride];
     __shared__ double s_cos16[64];
     __shared__ double s_cucv[64];
     const int g = threadIdx.x >> 3, lane = threadIdx.x & 7;
@@ -85,10 +216,26 @@ __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ j
     }
     const uint32_t tile = tile_base + local;
     const DvJob& job = jobs[tile_job[tile]];
-    const uint32_t blk = (tile - job.tile_begin) * BPG + g;
-    const bool active = blk < job.block_count;
+    const uint32_t first = (tile - job.tile_begin) * BPG;
+
+    // Mode-sorted order of the tile's blocks: every wave reads the tile's BPG mode bits (lane i:
+    // block first + i) and group g takes the g-th block of the order.
+    uint32_t k = (uint32_t)g;
+    uint64_t m88, m248;
+    {
+        const uint32_t i = threadIdx.x & 63;
+        const bool in = i < (uint32_t)BPG && first + i < job.block_count;
+        const uint8_t inf = in ? as_global(job.info)[first + i] : (uint8_t)0;
+        m248 = __ballot(in && (inf & 1));
+        m88 = __ballot(in && !(inf & 1));
+    }
+    const uint32_t n88 = (uint32_t)__popcll(m88), n248 = (uint32_t)__popcll(m248);
+    const bool active = k < n88 + n248;
+    const uint32_t blk = first + (k < n88 ? kth_set_bit(m88, k) : (active ? kth_set_bit(m248, k - n88) : 0u));
     const uint8_t info = active ? as_global(job.info)[blk] : 0;
     const int mode248 = info & 1, cls = (info >> 1) & 3, qno = (info >> 3) & 15;
+    int32_t* coef = reinterpret_cast<int32_t*>(s_area + g * kIdctTmpStride);
+    double* tmp = s_area + g * kIdctTmpStride;
 
     if (active) {
         const int4 raw = load_global_int4(job.levels + (size_t)blk * 64 + lane * 8);
@@ -111,78 +258,62 @@ __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ j
                 const int32_t factor = weight[p] << (kQuantShifts[q][area] + 1 + class_shift);
                 v = (level * factor + 8192) >> 14;
             }
-            coef[g][scan[p]] = v;
+            coef[scan[p]] = v;
         }
     }
-    __syncthreads();
+    __syncthreads();   // s_cos16 / s_cucv for the reference-order fallback; every wave passes
 
-    // Nonzero pattern of the block: bit v*8+u set when coefficient (v,u) != 0.
-    uint32_t rowbits = 0;
+    // Row v = lane of the block, in registers, and the wave's nonzero rows.
+    int32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (active) {
+        const int4 lo = *reinterpret_cast<const int4*>(coef + lane * 8);
+        const int4 hi = *reinterpret_cast<const int4*>(coef + lane * 8 + 4);
+        c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w; c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
+    }
+    const uint32_t rows = wave_nonzero_rows((c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) != 0);
+    wave_lds_sync();   // every row is in registers before the row pass overwrites the area
+    if (active) dv_row_pass(c, tmp + lane * 8);
+    wave_lds_sync();
+    double col[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) rowbits |= (uint32_t)(coef[g][lane * 8 + u] != 0) << u;
-    const uint32_t lo = group8_or(lane < 4 ? rowbits << (lane * 8) : 0u);
-    const uint32_t hi = group8_or(lane >= 4 ? rowbits << ((lane - 4) * 8) : 0u);
+    for (int y = 0; y < 8; ++y) col[y] = 0.0;
     const int x = lane;
+    if (__ballot(active && !mode248) != 0) {   // wave-uniform
+        if (active && !mode248) dv_column_pass(tmp, x, rows, kDvCol88, col);
+    }
+    if (__ballot(active && mode248) != 0) {
+        if (active && mode248) dv_column_pass(tmp, x, rows, kDvCol248, col);
+    }
     uint8_t out[8];
-
-    if (!mode248) {
-        // inverse_88: for each (y, x) sum over (v, u) in ascending natural order of
-        // cu*cv*c*cos((2x+1)u pi/16)*cos((2y+1)v pi/16), then (sum / 4).round().clamp(0, 255).
-        // Zero coefficients contribute signed zeros and are skipped (bit-exact, see idct8x8_column).
-        double sum[8];
+    bool near = false;
 #pragma unroll
-        for (int y = 0; y < 8; ++y) sum[y] = 0.0;
-        uint64_t mask = ((uint64_t)hi << 32) | lo;
-        while (mask) {
-            const int p = __builtin_ctzll(mask);
-            mask &= mask - 1;
-            const int v = p >> 3, u = p & 7;
-            double t = dmul(s_cucv[p], (double)coef[g][p]);
-            t = dmul(t, s_cos16[x * 8 + u]);
-#pragma unroll
-            for (int y = 0; y < 8; ++y) sum[y] = dadd(sum[y], dmul(t, s_cos16[y * 8 + v]));
+    for (int y = 0; y < 8; ++y) {
+        const double v = col[y] * 0.25;   // exact
+        near |= __builtin_fabs(__builtin_amdgcn_fract(__builtin_fabs(v)) - 0.5) <= kDvTieWindow;
+        out[y] = (uint8_t)clampi(round_half_away_i32(v), 0, 255);
+    }
+    const bool redo = active && group8_or(near ? 1u : 0u) != 0;
+    if (__ballot(redo) != 0) {   // wave-uniform: some block falls back to the reference's order
+        wave_lds_sync();   // the column reads are done before the coefficients come back
+        if (redo) {
+            *reinterpret_cast<int4*>(coef + lane * 8) = make_int4(c[0], c[1], c[2], c[3]);
+            *reinterpret_cast<int4*>(coef + lane * 8 + 4) = make_int4(c[4], c[5], c[6], c[7]);
         }
-#pragma unroll
-        for (int y = 0; y < 8; ++y) out[y] = clamp_u8_d(round_half_away(sum[y] / 4.0));
-    } else {
-        // inverse_248: butterfly of row pairs, horizontal 8-point pass (/2), per-field vertical
-        // 4-point pass (/2), round, clamp.
-        double h[8];
-#pragma unroll
-        for (int row = 0; row < 8; ++row) {
-            const int pair = row >> 1, second = row & 1;
-            double s = 0.0;
-            for (int u = 0; u < 8; ++u) {
-                const double first = (double)coef[g][pair * 16 + u];
-                const double other = (double)coef[g][pair * 16 + 8 + u];
-                const double bfly = second ? first - other : first + other;
-                if (bfly != 0.0) {
-                    const double cu = u == 0 ? kFrac1Sqrt2 : 1.0;
-                    s = dadd(s, dmul(dmul(cu, bfly), s_cos16[x * 8 + u]));
-                }
-            }
-            h[row] = s / 2.0;
+        wave_lds_sync();
+        if (redo) {
+            if (mode248) dv_inverse_248_reference(coef, s_cos16, x, out);
+            else dv_inverse_88_reference(coef, s_cos16, s_cucv, x, lane, out);
         }
-#pragma unroll
-        for (int field = 0; field < 2; ++field)
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                double s = 0.0;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const double cv = v == 0 ? kFrac1Sqrt2 : 1.0;
-                    s = dadd(s, dmul(dmul(cv, h[v * 2 + field]), kDvCos8[y * 4 + v]));
-                }
-                out[y * 2 + field] = clamp_u8_d(round_half_away(s / 2.0));
-            }
     }
 
     // Column x of the block -> LDS, then lane r stores row r (place_macroblock / put_split_chroma).
+    wave_lds_sync();   // the area's last reads are done
+    uint8_t* pix = reinterpret_cast<uint8_t*>(tmp);
 #pragma unroll
-    for (int r = 0; r < 8; ++r) pix[g][r * 8 + x] = out[r];
+    for (int r = 0; r < 8; ++r) pix[r * 8 + x] = out[r];
     wave_lds_sync();
     if (!active) return;
-    const uint64_t row = *reinterpret_cast<const uint64_t*>(&pix[g][lane * 8]);
+    const uint64_t row = *reinterpret_cast<const uint64_t*>(pix + lane * 8);
     const uint32_t mb_index = blk / 6;
     const int b = (int)(blk % 6);
     const int seq = (int)(mb_index / 135), slot = (int)((mb_index % 135) / 5), mbs = (int)(mb_index % 5);
@@ -216,7 +347,7 @@ __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ j
 }
 
 hipError_t dv_upload_tables(hipStream_t s) {
-    double cos16[64], cos8[32], cucv[64];
+    double cos16[64], cos8[32], cucv[64], rowa[64], col88[64], col248[64];
     for (int x = 0; x < 8; ++x)
         for (int u = 0; u < 8; ++u) cos16[x * 8 + u] = cos((double)(2 * x + 1) * (double)u * M_PI / 16.0);
     for (int y = 0; y < 8; ++y)
@@ -226,9 +357,27 @@ hipError_t dv_upload_tables(hipStream_t s) {
             const volatile double cu = u == 0 ? M_SQRT1_2 : 1.0, cv = v == 0 ? M_SQRT1_2 : 1.0;
             cucv[v * 8 + u] = cu * cv;
         }
+    // Fused-form tables: each entry one rounded product of the reference's factors.
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+            const volatile double cj = j == 0 ? M_SQRT1_2 : 1.0;
+            rowa[i * 8 + j] = cj * cos16[i * 8 + j];
+            col88[i * 8 + j] = cj * cos16[i * 8 + j];
+        }
+    for (int y = 0; y < 4; ++y)
+        for (int f = 0; f < 2; ++f)
+            for (int v = 0; v < 4; ++v) {
+                const volatile double cv = v == 0 ? M_SQRT1_2 : 1.0;
+                const double e = cv * cos8[y * 4 + v];
+                col248[(2 * y + f) * 8 + 2 * v] = e;
+                col248[(2 * y + f) * 8 + 2 * v + 1] = f ? -e : e;
+            }
     hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCos16), cos16, sizeof cos16, 0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCos8), cos8, sizeof cos8, 0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCuCv), cucv, sizeof cucv, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvRowA), rowa, sizeof rowa, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCol88), col88, sizeof col88, 0, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(kDvCol248), col248, sizeof col248, 0, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     return e;
 }

@@ -540,8 +540,9 @@ hipError_t launch_mpeg2(const Mpeg2Job* d_jobs, const uint32_t* d_tile_job, uint
     return hipSuccess;
 }
 
-// dv.hip: dv_kernel. Block blk = (tile - tile_begin) BPG + g (active when blk < block_count) reads
-// info[blk] and levels + 64 blk and stores at its macroblock_coordinates position (dv_extent).
+// dv.hip: dv_kernel. The blocks (tile - tile_begin) BPG + i, i < BPG, below block_count (taken by
+// the tile's groups in mode-sorted order) read info[blk] and levels + 64 blk and store at their
+// macroblock_coordinates position (dv_extent).
 hipError_t launch_dv(const DvJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, uint32_t group_blocks,
                      hipStream_t s) {
     if (tiles == 0) return hipSuccess;

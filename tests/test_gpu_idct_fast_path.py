@@ -120,3 +120,33 @@ def test_mjpeg_fast_idct_fallback_bit_exact(ctx, oracle):
     assert oracle.oracle_mjpeg_reconstruct(C.byref(u.host_struct(out))) == 0
     assert np.array_equal(d.download("plane0"), out["plane0"])
     d.free()
+
+
+def test_dv_fast_transform_fallback_bit_exact(ctx, oracle):
+    """DV: the committed blocks of tests/golden/dv_fallback_blocks.json, on which the fused
+    transform without its window rounds a sample differently from the reference (checked on the
+    CPU by tests/test_idct_fast_path.py), placed among seeded blocks of both modes so that the
+    kernel's mode-sorted order moves them between groups and waves, in a 625/50 and a 525/60 frame
+    (the 525/60 one with every third block in 2-4-8 mode), against the oracle bit for bit."""
+    fb = json.load(open(os.path.join(HERE, "golden", "dv_fallback_blocks.json")))
+    special = fb["dv88"] + fb["dv248"]
+    units = []
+    for i, system in enumerate((abi.DV_SYSTEM_625_50, abi.DV_SYSTEM_525_60)):
+        u = wl.dv_frame(seed=41, index=i, system=system, mode248_fraction=0.3 if i == 0 else 1.0 / 3.0)
+        levels, info = u.inputs["levels"], u.inputs["info"]
+        # fixtures at blocks 3, 10, 17, ...: spread over several 32-block tiles, mixed with both modes
+        for k, b in enumerate(special * 3):
+            pos = 3 + 7 * k + i
+            levels[pos] = 0
+            for p, v in b["levels"].items():
+                levels[pos, int(p)] = v
+            info[pos] = b["info"]
+        units.append(u)
+    dev = [wl.DeviceUnit(ctx, u) for u in units]
+    ctx.wait(ctx.submit_dv([d.struct for d in dev]))
+    for i, (u, d) in enumerate(zip(units, dev)):
+        out = u.host_outputs()
+        assert oracle.oracle_dv_reconstruct(C.byref(u.host_struct(out))) == 0
+        for n in ("y", "cb", "cr"):
+            assert np.array_equal(d.download(n), out[n]), (i, n)
+        d.free()

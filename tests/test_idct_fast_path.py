@@ -107,3 +107,89 @@ def test_planted_missing_window_fails_on_the_fixtures(tmp_path, mode, kinds):
     exe = _checker(tmp_path, "-1.0")
     count, flagged, bad = _run(exe, 0, 1, _fixtures(kinds), mode=mode)
     assert bad > 0
+
+
+# ---------------------------------------------------------------- DV (dv.hip: fused row / column passes)
+DV_WINDOW = 2.0 ** -25     # kDvTieWindow
+DV_BLOCKS = json.load(open(os.path.join(HERE, "golden", "dv_fallback_blocks.json")))
+
+
+def test_dv_window_covers_the_error_bound():
+    """Reference order: 64 sequential sums of three-factor products (gamma_66); fused form: two
+    8-term fused chains and tables rounded once (2 gamma_8 + gamma_8^2 + 3u); every factor of a term
+    is at most 1 in magnitude, and |c| <= 132092 (AC: int32 >> 14; DC: int16 * 4 + 1024)."""
+    u = 2.0 ** -53
+    g = lambda n: n * u / (1 - n * u)
+    a = 64 * (32767 * 4 + 1024)
+    assert (1 << 31) >> 14 <= 32767 * 4 + 1024
+    bound = (2 * g(8) + g(8) ** 2 + 3 * u + g(66)) * a / 4
+    assert bound < DV_WINDOW, (bound, DV_WINDOW)
+
+
+def _dv_checker(tmp_path):
+    exe = tmp_path / "dv_fast_check"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-o", str(exe),
+                    os.path.join(HERE, "fastpath", "dv_fast_check.c"), "-ldl", "-lm"], check=True)
+    return str(exe)
+
+
+def _dv_run(exe, n, seed, mode, blocks=(), window=None):
+    from oracle import binding
+
+    binding.lib()
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    stdin = "\n".join(" ".join(str(v) for v in b) for b in blocks)
+    args = [exe, so, str(n), str(seed), mode] + ([window] if window else [])
+    r = subprocess.run(args, input=stdin, capture_output=True, text=True, check=True, timeout=600)
+    return tuple(map(int, r.stdout.split()))
+
+
+def _dv_pins(kind):
+    out = []
+    for cat in ("tie", "fused"):
+        for d in PINS[kind][cat]:
+            c = [0] * 64
+            for k, v in d.items():
+                c[int(k)] = v
+            out.append(c)
+    return out
+
+
+def _dv_fallback_coefficients(oracle, kind):
+    import ctypes as C
+
+    out = []
+    for b in DV_BLOCKS[kind]:
+        lv = (C.c_int16 * 64)()
+        for p, v in b["levels"].items():
+            lv[int(p)] = v
+        c = (C.c_int32 * 64)()
+        oracle.oracle_dv_dequantize(lv, C.c_uint8(b["info"]), c)
+        assert (b["info"] & 1) == (kind == "dv248")
+        out.append(list(c))
+    return out
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", ["88", "248"])
+def test_dv_fast_path_rounds_like_the_reference_outside_the_window(tmp_path, mode):
+    exe = _dv_checker(tmp_path)
+    count, flagged, bad = _dv_run(exe, 1_000_000, 7, mode)
+    assert count == 1_000_000 and bad == 0
+    assert 0 < flagged < count // 100    # ties do occur (and fall back), rarely
+    fx = _dv_pins("dv" + mode)
+    count, flagged, bad = _dv_run(exe, 0, 1, mode, fx)
+    assert count == len(fx) and flagged == len(fx) and bad == 0
+
+
+@pytest.mark.parametrize("mode", ["88", "248"])
+def test_dv_planted_missing_window_fails(tmp_path, oracle, mode):
+    """Without the window the fused form rounds samples of the tie / FMA pins and of the committed
+    level blocks differently from the reference; with it, every such block is flagged."""
+    exe = _dv_checker(tmp_path)
+    for blocks in (_dv_pins("dv" + mode), _dv_fallback_coefficients(oracle, "dv" + mode)):
+        assert _dv_run(exe, 0, 1, mode, blocks, window="-1.0")[2] > 0
+        count, flagged, bad = _dv_run(exe, 0, 1, mode, blocks)
+        assert flagged == count == len(blocks) and bad == 0
+    for c in _dv_fallback_coefficients(oracle, "dv" + mode):   # each committed block on its own
+        assert _dv_run(exe, 0, 1, mode, [c], window="-1.0")[2] > 0
