@@ -225,17 +225,18 @@ MMR_DEV void idct8x8_row_pass_fast(const int32_t c[8], double* __restrict__ tmp_
     for (int x = 0; x < 8; x += 2) *reinterpret_cast<double2*>(tmp_row + x) = make_double2(t[x], t[x + 1]);
 }
 
-MMR_DEV void idct8x8_column_pass_fast(const double* __restrict__ tmp, int x, uint32_t rows, double out[8]) {
-    const double c0 = tmp[x];
+// Every row, zero or not (a zero row adds exact zeros): fully unrolled, the basis addresses are
+// static, so its scalar loads are batched and issued ahead instead of eight waits per row.
+MMR_DEV void idct8x8_column_pass_fast(const double* __restrict__ tmp, int x, double out[8]) {
+    double t[8];
 #pragma unroll
-    for (int y = 0; y < 8; ++y) out[y] = dmul(kIdctBasis[y * 8], c0);
-    uint32_t rest = rows & ~1u;
-    while (rest) {
-        const int v = __builtin_ctz(rest);
-        rest &= rest - 1;
-        const double cv = tmp[v * 8 + x];
+    for (int v = 0; v < 8; ++v) t[v] = tmp[v * 8 + x];
 #pragma unroll
-        for (int y = 0; y < 8; ++y) out[y] = __builtin_fma(kIdctBasis[y * 8 + v], cv, out[y]);
+    for (int y = 0; y < 8; ++y) {
+        double s = dmul(kIdctBasis[y * 8], t[0]);
+#pragma unroll
+        for (int v = 1; v < 8; ++v) s = __builtin_fma(kIdctBasis[y * 8 + v], t[v], s);
+        out[y] = s;
     }
 }
 

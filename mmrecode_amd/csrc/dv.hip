@@ -101,19 +101,18 @@ MMR_DEV void dv_row_pass(const int32_t c[8], double* __restrict__ tmp_row) {
     for (int x = 0; x < 8; x += 2) *reinterpret_cast<double2*>(tmp_row + x) = make_double2(t[x], t[x + 1]);
 }
 
-// Column pass of lane x over the rows in `rows` (wave-uniform, row 0 always included; rows that
-// are zero in every block of the wave have zero tmp and are skipped), vertical table B.
-MMR_DEV void dv_column_pass(const double* __restrict__ tmp, int x, uint32_t rows, const double* __restrict__ B, double out[8]) {
-    const double c0 = tmp[x];
+// Column pass of lane x with vertical table B, over every row (a zero row adds exact zeros), fully
+// unrolled so that the table's scalar loads have static addresses and are issued ahead.
+MMR_DEV void dv_column_pass(const double* __restrict__ tmp, int x, const double* __restrict__ B, double out[8]) {
+    double t[8];
 #pragma unroll
-    for (int y = 0; y < 8; ++y) out[y] = dmul(B[y * 8], c0);
-    uint32_t rest = rows & ~1u;
-    while (rest) {
-        const int v = __builtin_ctz(rest);
-        rest &= rest - 1;
-        const double cv = tmp[v * 8 + x];
+    for (int v = 0; v < 8; ++v) t[v] = tmp[v * 8 + x];
 #pragma unroll
-        for (int y = 0; y < 8; ++y) out[y] = __builtin_fma(B[y * 8 + v], cv, out[y]);
+    for (int y = 0; y < 8; ++y) {
+        double s = dmul(B[y * 8], t[0]);
+#pragma unroll
+        for (int v = 1; v < 8; ++v) s = __builtin_fma(B[y * 8 + v], t[v], s);
+        out[y] = s;
     }
 }
 
@@ -263,14 +262,13 @@ __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ j
     }
     __syncthreads();   // s_cos16 / s_cucv for the reference-order fallback; every wave passes
 
-    // Row v = lane of the block, in registers, and the wave's nonzero rows.
+    // Row v = lane of the block, in registers.
     int32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (active) {
         const int4 lo = *reinterpret_cast<const int4*>(coef + lane * 8);
         const int4 hi = *reinterpret_cast<const int4*>(coef + lane * 8 + 4);
         c[0] = lo.x; c[1] = lo.y; c[2] = lo.z; c[3] = lo.w; c[4] = hi.x; c[5] = hi.y; c[6] = hi.z; c[7] = hi.w;
     }
-    const uint32_t rows = wave_nonzero_rows((c[0] | c[1] | c[2] | c[3] | c[4] | c[5] | c[6] | c[7]) != 0);
     wave_lds_sync();   // every row is in registers before the row pass overwrites the area
     if (active) dv_row_pass(c, tmp + lane * 8);
     wave_lds_sync();
@@ -279,10 +277,10 @@ __global__ __launch_bounds__(BPG * 8) void dv_kernel(const DvJob* __restrict__ j
     for (int y = 0; y < 8; ++y) col[y] = 0.0;
     const int x = lane;
     if (__ballot(active && !mode248) != 0) {   // wave-uniform
-        if (active && !mode248) dv_column_pass(tmp, x, rows, kDvCol88, col);
+        if (active && !mode248) dv_column_pass(tmp, x, kDvCol88, col);
     }
     if (__ballot(active && mode248) != 0) {
-        if (active && mode248) dv_column_pass(tmp, x, rows, kDvCol248, col);
+        if (active && mode248) dv_column_pass(tmp, x, kDvCol248, col);
     }
     uint8_t out[8];
     bool near = false;

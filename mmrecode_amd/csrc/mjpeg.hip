@@ -63,7 +63,7 @@ __global__ __launch_bounds__(BPG * 8) void mjpeg_blocks_kernel(const MjpegJob* _
     double col[8];
     bool near = false;
     if (active) {
-        idct8x8_column_pass_fast(tmp, lane, rows, col);
+        idct8x8_column_pass_fast(tmp, lane, col);
 #pragma unroll
         for (int y = 0; y < 8; ++y) col[y] = dadd(col[y], 128.0);
         near = !idct_row_within_fast_bound(c) || idct_near_half(col);

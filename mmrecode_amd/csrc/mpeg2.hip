@@ -429,7 +429,7 @@ __global__ __launch_bounds__(BPG * 8, 7) void mpeg2_kernel(const Mpeg2Job* __res
     double col[8];
     bool near = false;
     if (coded) {
-        idct8x8_column_pass_fast(tmp, lane, rows, col);
+        idct8x8_column_pass_fast(tmp, lane, col);
         near = idct_near_half(col);
     }
     if (__ballot(near) != 0) {   // wave-uniform

@@ -48,16 +48,11 @@ static void tables(const double* cos16, const double* cos8) {
             }
 }
 
-/* The kernel's fused path for one block: row pass (lane v), column pass (lane x) over the
- * block's nonzero rows (the kernel skips rows zero in the whole wave, a superset), values / 4.
- * Returns 1 when a value lies within the window of a half-integer. */
+/* The kernel's fused path for one block: row pass (lane v), column pass (lane x) over all 8 rows,
+ * values / 4. Returns 1 when a value lies within the window of a half-integer. */
 static int fast(const int32_t c[64], int m248, double out[64]) {
     const double* B = m248 ? COL248 : COL88;
     double tmp[64];
-    int rows = 1;
-    for (int v = 0; v < 8; ++v)
-        for (int u = 0; u < 8; ++u)
-            if (c[v * 8 + u]) rows |= 1 << v;
     for (int v = 0; v < 8; ++v)
         for (int x = 0; x < 8; ++x) {
             double s = ROWA[x * 8] * (double)c[v * 8];
@@ -68,8 +63,7 @@ static int fast(const int32_t c[64], int m248, double out[64]) {
     for (int x = 0; x < 8; ++x)
         for (int y = 0; y < 8; ++y) {
             double s = B[y * 8] * tmp[x];
-            for (int v = 1; v < 8; ++v)
-                if (rows >> v & 1) s = fma(B[y * 8 + v], tmp[v * 8 + x], s);
+            for (int v = 1; v < 8; ++v) s = fma(B[y * 8 + v], tmp[v * 8 + x], s);
             s = s * 0.25;
             out[y * 8 + x] = s;
             double a = fabs(s), f = a - floor(a);

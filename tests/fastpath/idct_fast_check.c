@@ -31,14 +31,10 @@ static uint64_t next(void) {   /* splitmix64 */
     return z ^ (z >> 31);
 }
 
-/* The device's fast path: row pass lane v, column pass lane x, fused steps, rows skipped when zero
- * in the whole wave (here: the block), then the near-half test. Returns 1 when any sample is near. */
+/* The device's fast path: row pass lane v, column pass lane x over all 8 rows, fused steps, then
+ * the near-half test. Returns 1 when any sample is near. */
 static int fast_idct(const int32_t c[64], double out[64]) {
     double tmp[64];
-    int rows = 1;
-    for (int v = 0; v < 8; ++v)
-        for (int u = 0; u < 8; ++u)
-            if (c[v * 8 + u]) rows |= 1 << v;
     for (int v = 0; v < 8; ++v)
         for (int x = 0; x < 8; ++x) {
             double s = M[x][0] * (double)c[v * 8];
@@ -49,8 +45,7 @@ static int fast_idct(const int32_t c[64], double out[64]) {
     for (int x = 0; x < 8; ++x)
         for (int y = 0; y < 8; ++y) {
             double s = M[y][0] * tmp[x];
-            for (int v = 1; v < 8; ++v)
-                if (rows >> v & 1) s = fma(M[y][v], tmp[v * 8 + x], s);
+            for (int v = 1; v < 8; ++v) s = fma(M[y][v], tmp[v * 8 + x], s);
             if (mjpeg) s = s + 128.0;
             out[y * 8 + x] = s;
             double a = fabs(s), f = a - floor(a);
