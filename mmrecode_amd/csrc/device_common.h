@@ -21,7 +21,8 @@ MMR_DEV MMR_GLOBAL T* as_global(T* p) { return (MMR_GLOBAL T*)p; }
 // address-space-qualified lvalue).
 typedef int32_t mmr_v4i32 __attribute__((ext_vector_type(4)));
 typedef uint32_t mmr_v2u32 __attribute__((ext_vector_type(2)));
-typedef uint32_t mmr_v3u32 __attribute__((ext_vector_type(3)));
+// 12 bytes at a 4-byte aligned address (the vector type's own alignment would be 16).
+typedef uint32_t mmr_v3u32 __attribute__((ext_vector_type(3), aligned(4)));
 typedef int16_t mmr_v2i16 __attribute__((ext_vector_type(2)));
 MMR_DEV int4 load_global_int4(const void* p) {
     const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(as_global(p));

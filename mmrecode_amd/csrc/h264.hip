@@ -32,93 +32,162 @@ MMR_DEV int div8(int v) { return v >> 3; }
 
 MMR_DEV uint32_t avg2(uint32_t a, uint32_t b) { return (a + b + 1) >> 1; }
 
-// Samples x-2 .. x+6 of reference row y (reference_sample clamps both coordinates into the
-// plane). When the 9 samples lie inside the row and the plane is 4-byte aligned they come from the
-// three aligned dwords that hold them; otherwise from per-sample clamped loads.
-MMR_DEV void load_window_row(const MMR_GLOBAL uint8_t* __restrict__ p, int W, int rowoff, int x, bool fast, int win[9]) {
-    if (fast) {
-        const int c0 = x - 2;
-        const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(p + rowoff + (c0 & ~3));
-        const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], o = (uint32_t)(c0 & 3);
-        const uint32_t a0 = __builtin_amdgcn_alignbyte(w1, w0, o), a1 = __builtin_amdgcn_alignbyte(w2, w1, o);
-        const uint32_t a2 = __builtin_amdgcn_alignbyte(w2, w2, o);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            win[i] = (a0 >> (8 * i)) & 0xff;
-            win[4 + i] = (a1 >> (8 * i)) & 0xff;
-        }
-        win[8] = a2 & 0xff;
-    } else {
-#pragma unroll
-        for (int i = 0; i < 9; ++i) win[i] = p[rowoff + clampi(x - 2 + i, 0, W - 1)];
-    }
+// Sample pairs (w[k], w[k+1]) of a 9-sample row window held in a0 (samples 0-3), a1 (4-7) and a2
+// (sample 8, byte 0), zero-extended to two 16-bit halves: P[k] for k = 0..7.
+MMR_DEV void window_pairs(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t P[8]) {
+    P[0] = __builtin_amdgcn_perm(a1, a0, 0x0c010c00u);
+    P[1] = __builtin_amdgcn_perm(a1, a0, 0x0c020c01u);
+    P[2] = __builtin_amdgcn_perm(a1, a0, 0x0c030c02u);
+    P[3] = __builtin_amdgcn_perm(a1, a0, 0x0c040c03u);
+    P[4] = __builtin_amdgcn_perm(a2, a1, 0x0c010c00u);
+    P[5] = __builtin_amdgcn_perm(a2, a1, 0x0c020c01u);
+    P[6] = __builtin_amdgcn_perm(a2, a1, 0x0c030c02u);
+    P[7] = __builtin_amdgcn_perm(a2, a1, 0x0c040c03u);
 }
 
+MMR_DEV mmr_v2i16 as_v2i16(uint32_t v) {
+    mmr_v2i16 r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+MMR_DEV uint32_t as_u32(mmr_v2i16 v) {
+    uint32_t r;
+    __builtin_memcpy(&r, &v, 4);
+    return r;
+}
+
+// The six-tap sum a - 5b + 20c + 20d - 5e + f of luma_qpel's half-sample terms, on two 16-bit
+// halves at once (sample bytes in: every sum lies in [-2550, 10710], so the 16-bit results are
+// exact). Written as the packed instructions themselves: the compiler otherwise splits the packed
+// sums whose halves have different uses into 16-bit scalar mads, which measured wrong on gfx950.
+MMR_DEV mmr_v2i16 tap6_pairs(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t e, uint32_t f) {
+    uint32_t af, be, cd, r;
+    asm("v_pk_add_u16 %0, %1, %2" : "=v"(af) : "v"(a), "v"(f));
+    asm("v_pk_add_u16 %0, %1, %2" : "=v"(be) : "v"(b), "v"(e));
+    asm("v_pk_add_u16 %0, %1, %2" : "=v"(cd) : "v"(c), "v"(d));
+    asm("v_pk_mad_u16 %0, %1, 20, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(cd), "v"(af));
+    asm("v_pk_mad_u16 %0, %1, -5, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(be), "v"(r));
+    return as_v2i16(r);
+}
+
+// clip255((v + 16) >> 5) on both halves.
+MMR_DEV uint32_t half_clip_pairs(mmr_v2i16 v) {
+    v = (v + (mmr_v2i16){16, 16}) >> (mmr_v2i16){5, 5};
+    v = __builtin_elementwise_max(v, (mmr_v2i16){0, 0});
+    v = __builtin_elementwise_min(v, (mmr_v2i16){255, 255});
+    return as_u32(v);
+}
+
+// Low bytes of the halves of (c0, c1) and (c2, c3) as the bytes c0..c3 of a dword.
+MMR_DEV uint32_t pack_pairs(uint32_t lo, uint32_t hi) { return __builtin_amdgcn_perm(hi, lo, 0x06040200u); }
+
 // Four consecutive luma samples (x_q4 + 4c, y_q4), c = 0..3, of luma_qpel (decoder.rs:13770-13829)
-// with reference_sample's edge clamp. The 6x9 integer window around the row is loaded once and
-// the half-sample terms are formed from registers exactly as the reference forms them.
-MMR_DEV void luma_qpel_row4(const uint8_t* pg, int W, int H, int x_q4, int y_q4, bool aligned, uint32_t out[4]) {
+// with reference_sample's edge clamp, as the bytes of one dword. The 6x9 integer window around
+// the row is loaded in one batch (three aligned dwords per row inside the picture, clamped single
+// samples at its edges), every half-sample term the reference may use is formed from it without
+// branches -- b (rows y, y+1), h (columns x, x+1), j -- exactly as the reference forms it, and the
+// quarter position picks the pair that is averaged ((a + b + 1) >> 1; a lone term is averaged
+// with itself).
+MMR_DEV uint32_t luma_qpel_row4(const uint8_t* pg, int W, int H, int x_q4, int y_q4, bool aligned) {
     const MMR_GLOBAL uint8_t* p = as_global(pg);
     const int x = div4(x_q4), y = div4(y_q4), fx = x_q4 & 3, fy = y_q4 & 3;
     const bool fast = aligned && x >= 2 && x + 6 < W;
     int rows[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) rows[j] = clampi(y - 2 + j, 0, H - 1) * W;
-    int win[6][9];
-    if (fx == 0 && fy == 0) {
-        load_window_row(p, W, rows[2], x, fast, win[2]);
+    const int c0 = x - 2;
+    uint32_t a0[6], a1[6], a2[6];
+    if (fast) {
+        // The three dwords end inside the row: c0 & ~3 <= c0 and c0 + 8 < W, a multiple of 4.
+        uint32_t d[6][3];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) out[c] = win[2][c + 2];
-        return;
-    }
-    if (fy == 0) {
-        load_window_row(p, W, rows[2], x, fast, win[2]);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int raw = win[2][c] - 5 * win[2][c + 1] + 20 * win[2][c + 2] + 20 * win[2][c + 3] - 5 * win[2][c + 4] + win[2][c + 5];
-            const uint32_t b = (uint32_t)clip255((raw + 16) >> 5);
-            out[c] = fx == 2 ? b : (fx == 1 ? avg2((uint32_t)win[2][c + 2], b) : avg2(b, (uint32_t)win[2][c + 3]));
+        for (int j = 0; j < 6; ++j) {
+            const MMR_GLOBAL uint32_t* q = reinterpret_cast<const MMR_GLOBAL uint32_t*>(p + rows[j] + (c0 & ~3));
+            d[j][0] = q[0];
+            d[j][1] = q[1];
+            d[j][2] = q[2];
         }
-        return;
-    }
+        const uint32_t o = (uint32_t)(c0 & 3);
 #pragma unroll
-    for (int j = 0; j < 6; ++j) load_window_row(p, W, rows[j], x, fast, win[j]);
-    if (fx == 0) {
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int i = c + 2;
-            const int raw = win[0][i] - 5 * win[1][i] + 20 * win[2][i] + 20 * win[3][i] - 5 * win[4][i] + win[5][i];
-            const uint32_t h = (uint32_t)clip255((raw + 16) >> 5);
-            out[c] = fy == 2 ? h : (fy == 1 ? avg2((uint32_t)win[2][i], h) : avg2(h, (uint32_t)win[3][i]));
+        for (int j = 0; j < 6; ++j) {
+            a0[j] = __builtin_amdgcn_alignbyte(d[j][1], d[j][0], o);
+            a1[j] = __builtin_amdgcn_alignbyte(d[j][2], d[j][1], o);
+            a2[j] = __builtin_amdgcn_alignbyte(d[j][2], d[j][2], o);
         }
-        return;
-    }
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        int rawh[6];
+    } else {
+        uint32_t s[6][9];
 #pragma unroll
         for (int j = 0; j < 6; ++j)
-            rawh[j] = win[j][c] - 5 * win[j][c + 1] + 20 * win[j][c + 2] + 20 * win[j][c + 3] - 5 * win[j][c + 4] + win[j][c + 5];
-        const uint32_t b0 = (uint32_t)clip255((rawh[2] + 16) >> 5);   // half_horizontal(x, y)
-        const uint32_t b1 = (uint32_t)clip255((rawh[3] + 16) >> 5);   // half_horizontal(x, y + 1)
-        const int i0 = c + 2, i1 = c + 3;
-        const uint32_t h0 = (uint32_t)clip255((win[0][i0] - 5 * win[1][i0] + 20 * win[2][i0] + 20 * win[3][i0] - 5 * win[4][i0] + win[5][i0] + 16) >> 5);
-        const uint32_t h1 = (uint32_t)clip255((win[0][i1] - 5 * win[1][i1] + 20 * win[2][i1] + 20 * win[3][i1] - 5 * win[4][i1] + win[5][i1] + 16) >> 5);
-        const uint32_t j = (uint32_t)clip255((rawh[0] - 5 * rawh[1] + 20 * rawh[2] + 20 * rawh[3] - 5 * rawh[4] + rawh[5] + 512) >> 10);
-        uint32_t r;
-        switch (fy * 4 + fx) {
-        case 5: r = avg2(b0, h0); break;
-        case 6: r = avg2(b0, j); break;
-        case 7: r = avg2(b0, h1); break;
-        case 9: r = avg2(h0, j); break;
-        case 10: r = j; break;
-        case 11: r = avg2(j, h1); break;
-        case 13: r = avg2(h0, b1); break;
-        case 14: r = avg2(j, b1); break;
-        default: r = avg2(h1, b1); break;  // case 15
+#pragma unroll
+            for (int i = 0; i < 9; ++i) s[j][i] = p[rows[j] + clampi(c0 + i, 0, W - 1)];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            a0[j] = s[j][0] | (s[j][1] << 8) | (s[j][2] << 16) | (s[j][3] << 24);
+            a1[j] = s[j][4] | (s[j][5] << 8) | (s[j][6] << 16) | (s[j][7] << 24);
+            a2[j] = s[j][8];
         }
-        out[c] = r;
     }
+    uint32_t P[6][8];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) window_pairs(a0[j], a1[j], a2[j], P[j]);
+    // Horizontal sums of rows y and y + 1 (window rows 2, 3) for columns (0, 1) and (2, 3).
+    mmr_v2i16 rh[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            rh[j][q] = tap6_pairs(P[2 + j][2 * q], P[2 + j][2 * q + 1], P[2 + j][2 * q + 2], P[2 + j][2 * q + 3], P[2 + j][2 * q + 4],
+                                  P[2 + j][2 * q + 5]);
+    // Vertical sums of every window column: pairs (0, 1), (2, 3), (4, 5), (6, 7) and (7, 8).
+    mmr_v2i16 rv[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int k = q < 4 ? 2 * q : 7;
+        rv[q] = tap6_pairs(P[0][k], P[1][k], P[2][k], P[3][k], P[4][k], P[5][k]);
+    }
+    const uint32_t b0 = pack_pairs(half_clip_pairs(rh[0][0]), half_clip_pairs(rh[0][1]));   // half_horizontal(x + c, y)
+    const uint32_t b1 = pack_pairs(half_clip_pairs(rh[1][0]), half_clip_pairs(rh[1][1]));   // half_horizontal(x + c, y + 1)
+    const uint32_t v23 = half_clip_pairs(rv[1]), v45 = half_clip_pairs(rv[2]), v67 = half_clip_pairs(rv[3]);
+    const uint32_t h0 = pack_pairs(v23, v45);                                                // half_vertical(x + c, y)
+    const uint32_t h1 = pack_pairs(__builtin_amdgcn_alignbyte(v45, v23, 2), __builtin_amdgcn_alignbyte(v67, v45, 2));   // (x + c + 1, y)
+    // j: the two-dimensional six-tap sum, here as the horizontal six-tap over the vertical sums of
+    // window columns c .. c + 5 (the same integer sum as over the horizontal sums of rows y - 2 ..
+    // y + 3, in another order), 32-bit.
+    int col[9];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        col[2 * q] = rv[q].x;
+        col[2 * q + 1] = rv[q].y;
+    }
+    col[8] = rv[4].y;
+    uint32_t jb = 0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int v = clip255((col[c] - 5 * col[c + 1] + 20 * col[c + 2] + 20 * col[c + 3] - 5 * col[c + 4] + col[c + 5] + 512) >> 10);
+        jb |= (uint32_t)v << (8 * c);
+    }
+    const uint32_t G = __builtin_amdgcn_alignbyte(a1[2], a0[2], 2);    // integer samples x + c, row y
+    const uint32_t Gr = __builtin_amdgcn_alignbyte(a1[2], a0[2], 3);   // x + c + 1, row y
+    const uint32_t Gd = __builtin_amdgcn_alignbyte(a1[3], a0[3], 2);   // x + c, row y + 1
+    // The averaged pair of each quarter position (decoder.rs:13770-13829).
+    uint32_t A, B;
+    if (fy == 0) {
+        A = fx == 0 ? G : (fx == 1 ? G : b0);
+        B = fx == 0 ? G : (fx == 3 ? Gr : b0);
+    } else if (fx == 0) {
+        A = fy == 1 ? G : h0;
+        B = fy == 3 ? Gd : h0;
+    } else if (fy == 1) {
+        A = b0;
+        B = fx == 1 ? h0 : (fx == 2 ? jb : h1);
+    } else if (fy == 2) {
+        A = fx == 1 ? h0 : jb;
+        B = fx == 3 ? h1 : jb;
+    } else {
+        A = fx == 1 ? h0 : (fx == 2 ? jb : h1);
+        B = b1;
+    }
+    return __builtin_amdgcn_lerp(A, B, 0x01010101u);
 }
 
 // chroma_epel (decoder.rs:13851-13886).
@@ -209,12 +278,18 @@ __global__ __launch_bounds__(256, 5) void h264_mb_kernel(const H264Job* __restri
         const int mvx = blk.mv[la][0], mvy = blk.mv[la][1];
         uint32_t ys[4], a[4];
         uint32_t cs[2], sa[2];
-        luma_qpel_row4(job.ref_planes[la][ra][0], W, H, ox * 4 + mvx, oy * 4 + mvy, job.refs_aligned, a);
+        {
+            const uint32_t q = luma_qpel_row4(job.ref_planes[la][ra][0], W, H, ox * 4 + mvx, oy * 4 + mvy, job.refs_aligned);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) a[c] = (q >> (8 * c)) & 0xffu;
+        }
 #pragma unroll
         for (int cc = 0; cc < 2; ++cc) sa[cc] = chroma_epel(job.ref_planes[la][ra][1 + cc], CW, CH, cxs * 8 + mvx, cys * 8 + mvy);
         if (bi) {
             uint32_t b[4];
-            luma_qpel_row4(job.ref_planes[1][r1][0], W, H, ox * 4 + blk.mv[1][0], oy * 4 + blk.mv[1][1], job.refs_aligned, b);
+            const uint32_t q = luma_qpel_row4(job.ref_planes[1][r1][0], W, H, ox * 4 + blk.mv[1][0], oy * 4 + blk.mv[1][1], job.refs_aligned);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = (q >> (8 * c)) & 0xffu;
             const MMR_GLOBAL mmr_recon_h264_bi_weight* bw = weighted ? as_global(job.bi_weights) + ((size_t)r0 * MMR_H264_MAX_REFS + r1) * 3 : nullptr;
 #pragma unroll
             for (int c = 0; c < 4; ++c) ys[c] = weight2(a[c], b[c], bw, weighted);
