@@ -698,12 +698,13 @@ typedef struct mmr_recon_tuning {
     size_t struct_size;
     uint32_t mjpeg_group_blocks;         /* 8x8 blocks per workgroup: 16, 32 (default) or 64 */
     uint32_t mpeg2_group_blocks;         /* same for MPEG-2 (default 32) */
-    uint32_t dv_group_blocks;            /* same for DV (default 32) */
+    uint32_t dv_group_blocks;            /* same for DV (default 64) */
     uint32_t h264_deblock_band_rows;     /* macroblock rows per deblocking band, one wave each:
-                                            4, 8 or 16 */
-    uint32_t h264_deblock_groups_per_cu; /* persistent deblocking workgroups per compute unit: 1..4 */
+                                            4, 8 (default) or 16 */
+    uint32_t h264_deblock_groups_per_cu; /* persistent deblocking workgroups per compute unit: 1..4
+                                            (default 2) */
     uint32_t h264_deblock_publish;       /* macroblocks between two progress publications of a
-                                            band's last row to the next band: 1..32 */
+                                            band's last row to the next band: 1..32 (default 8) */
 } mmr_recon_tuning;
 
 MMR_RECON_API mmr_status mmr_recon_get_tuning(mmr_recon_context *context, mmr_recon_tuning *out_tuning);

@@ -673,10 +673,10 @@ mmr_recon_tuning default_tuning() {
     t.struct_size = sizeof t;
     t.mjpeg_group_blocks = 32;
     t.mpeg2_group_blocks = 32;
-    t.dv_group_blocks = 32;
+    t.dv_group_blocks = 64;
     t.h264_deblock_band_rows = 8;
-    t.h264_deblock_groups_per_cu = 1;
-    t.h264_deblock_publish = 2;
+    t.h264_deblock_groups_per_cu = 2;
+    t.h264_deblock_publish = 8;
     return t;
 }
 
