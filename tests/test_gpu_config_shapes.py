@@ -93,3 +93,64 @@ def test_h264_mixed_sizes_one_submission_bit_exact(ctx, oracle, band_rows):
         for r in devr:
             for b in r.values():
                 b.free()
+
+
+# Output geometries of the mixed workload's frames: (width, height, chroma width, chroma height,
+# luma stride, chroma stride) of MJPEG 640x480 4:2:0, MPEG-2 1920x1080 (coded-size planes), DV
+# 720x576 4:2:0 and 720x480 4:1:1, H.264 1280x720, 1920x1088 and 3840x2160.
+MIXED_GEOMETRIES = [(640, 480, 320, 240, 640, 320), (1920, 1080, 960, 540, 1920, 960), (720, 576, 360, 288, 720, 360),
+                    (720, 480, 180, 480, 720, 180), (1280, 720, 640, 360, 1280, 640), (1920, 1088, 960, 544, 1920, 960),
+                    (3840, 2160, 1920, 1080, 3840, 1920)]
+
+
+def test_mixed_workload_convert_and_composite_shapes_bit_exact(ctx, oracle):
+    """BT.709 limited-range YUV->RGBA of every output geometry of the mixed workload in one batched
+    submission, then a 3-layer premultiplied alpha-over composite onto each converted frame in one
+    batched submission, against the oracle's ycbcr_to_rgb and blend chain."""
+    from mmrecode_amd import abi
+
+    rng = np.random.default_rng(99)
+    host, dev, keep = [], [], []
+    conv_d, comp_d = [], []
+    for (w, h, cw, ch, ls, cs) in MIXED_GEOMETRIES:
+        planes = [rng.integers(0, 256, (h, ls), dtype=np.uint8), rng.integers(0, 256, (ch, cs), dtype=np.uint8),
+                  rng.integers(0, 256, (ch, cs), dtype=np.uint8)]
+        layers = []
+        for _ in range(3):
+            px = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+            px[..., :3] = (px[..., :3].astype(np.uint32) * px[..., 3:4] // 255).astype(np.uint8)
+            layers.append(px)
+        rgba = np.zeros((h, w, 4), np.uint8)
+        job = abi.new(abi.YuvToRgbJob, width=w, height=h, chroma_width=cw, chroma_height=ch, luma_stride=ls,
+                      chroma_stride=cs, range=abi.RANGE_LIMITED, matrix=abi.MATRIX_BT709, rgba=rgba.ctypes.data)
+        for k in range(3):
+            job.planes[k] = planes[k].ctypes.data
+        assert oracle.oracle_yuv_to_rgb(C.byref(job)) == 0
+        cj = abi.new(abi.CompositeJob, target=rgba.ctypes.data, width=w, height=h, stride=w * 4, layer_count=3)
+        for i, px in enumerate(layers):
+            cj.layers[i] = abi.Layer(rgba=px.ctypes.data, width=w, height=h, stride=w * 4, x=0, y=0,
+                                     op=abi.BLEND_PREMULTIPLIED_OVER, opacity=255)
+        assert oracle.oracle_composite(C.byref(cj)) == 0
+        host.append(rgba)
+        bufs = [ctx.upload(p) for p in planes]
+        lb = [ctx.upload(px) for px in layers]
+        out = ctx.alloc(w * h * 4)
+        keep += bufs + lb + [out]
+        dj = abi.new(abi.YuvToRgbJob, width=w, height=h, chroma_width=cw, chroma_height=ch, luma_stride=ls,
+                     chroma_stride=cs, range=abi.RANGE_LIMITED, matrix=abi.MATRIX_BT709, rgba=out.ptr)
+        for k in range(3):
+            dj.planes[k] = bufs[k].ptr
+        conv_d.append(dj)
+        dc = abi.new(abi.CompositeJob, target=out.ptr, width=w, height=h, stride=w * 4, layer_count=3)
+        for i, b in enumerate(lb):
+            dc.layers[i] = abi.Layer(rgba=b.ptr, width=w, height=h, stride=w * 4, x=0, y=0,
+                                     op=abi.BLEND_PREMULTIPLIED_OVER, opacity=255)
+        comp_d.append(dc)
+        dev.append(out)
+    print(f"mixed geometries: {len(dev)} frames", flush=True)
+    ctx.wait(ctx.submit_yuv_to_rgb(conv_d))
+    ctx.wait(ctx.submit_composite(comp_d))
+    for g, out, want in zip(MIXED_GEOMETRIES, dev, host):
+        _assert_same(f"{g[0]}x{g[1]}", ctx.download(out, np.uint8, want.shape), want)
+    for b in keep:
+        b.free()
