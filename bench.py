@@ -704,6 +704,181 @@ def run_served_sweep(ctx, rates, seconds, plain_limit):
             "seconds_per_rate": seconds, "rates": out}
 
 
+class MixedWorkload:
+    """The served mix's per-frame chain, batched on one stream: a fixed set of request windows -- 30
+    MJPEG 640x480 frames, one MPEG-2 1920x1080 closed GOP of 15 pictures, 30 DV frames (720x576
+    4:2:0 and 720x480 4:1:1 alternating), 12 H.264 pictures (4 each of 1280x720, 1920x1088 and
+    3840x2160, 4 resident references per size, weighted bi-prediction, deblocking) -- where every
+    frame is reconstructed, converted (BT.709 limited YUV->RGBA) and composited (3 premultiplied
+    RGBA8 layers, alpha-over, at its output size). A step is one batch per codec, one conversion
+    batch and one composite batch; the plain path runs the same chain one frame and one launch at a
+    time. Mpix count output frames."""
+    name = "mixed"
+    codec = None
+    data = "synthetic (seeded post-entropy buffers per codec, seeded premultiplied RGBA8 layers)"
+    metric = "reconstructed + composited Mpix/s, mixed codecs (batched vs plain one-frame-per-launch)"
+    metric_unit = "Mpix/s"
+    kernel = "all kernels of the step"
+    dtype = "u8/i16 in, int32 + f64 (IDCT) arithmetic"
+    H264_SIZES = ((80, 45), (120, 68), (240, 135))
+
+    def __init__(self, ctx, seed=2024):
+        self.ctx = ctx
+        t = time.time()
+        self.frames = []   # (codec, unit, device unit)
+        for i in range(30):
+            u = wl.mjpeg_frame(seed, 2000 + i, rgb=False)
+            self.frames.append((abi.CODEC_MJPEG, u, wl.DeviceUnit(ctx, u)))
+        gop = wl.mpeg2_gop(seed, 77)
+        dev = []
+        for i, u in enumerate(gop):
+            extra = {"forward_source": -1 if u.meta["fwd"] is None else u.meta["fwd"],
+                     "backward_source": -1 if u.meta["bwd"] is None else u.meta["bwd"]}
+            for tag in ("fwd", "bwd"):
+                if u.meta[tag] is not None:
+                    for n in ("y", "cb", "cr"):
+                        extra[f"{tag}_{n}"] = dev[u.meta[tag]].ptr(n)
+            dev.append(wl.DeviceUnit(ctx, u, extra))
+            self.frames.append((abi.CODEC_MPEG2, u, dev[-1]))
+        self.mpeg2_gop = gop
+        for i in range(30):
+            u = wl.dv_frame(seed, 2000 + i, system=i % 2)
+            self.frames.append((abi.CODEC_DV, u, wl.DeviceUnit(ctx, u)))
+        self.h264_host_refs = {}
+        self.h264_dev_refs = {}
+        for wmb, hmb in self.H264_SIZES:
+            host = [dict(zip(("y", "cb", "cr"), wl.h264_reference_planes(seed, wmb, hmb, 50 + i))) for i in range(4)]
+            devr = [{n: ctx.upload(a) for n, a in r.items()} for r in host]
+            self.h264_host_refs[(wmb, hmb)] = host
+            self.h264_dev_refs[(wmb, hmb)] = devr
+            ptrs = [{n: b.ptr for n, b in r.items()} for r in devr]
+            for i in range(4):
+                u = wl.h264_picture(seed, 200 + i, wmb, hmb)
+                self.frames.append((abi.CODEC_H264, u, wl.DeviceUnit(ctx, u, wl.h264_ref_ptrs(u, ptrs, "dev"))))
+        # conversion + composite of every frame: its RGBA target and 3 layers per output geometry
+        rng = np.random.Generator(np.random.PCG64([seed, 88]))
+        self.layers, self.rgba, self.convert, self.composite = {}, [], [], []
+        self.mpix_list = []
+        self.bytes = 0
+        for codec, u, d in self.frames:
+            w, h, cw, ch, ls, cs, names = ServedMix.geometry(codec, u)
+            if (w, h) not in self.layers:
+                bufs = []
+                for _ in range(3):
+                    px = rng.integers(0, 256, (h, w, 4), dtype=np.uint8)
+                    px[..., :3] = (px[..., :3].astype(np.uint32) * px[..., 3:4] // 255).astype(np.uint8)
+                    bufs.append((px, ctx.upload(px)))
+                self.layers[(w, h)] = bufs
+            out = ctx.alloc(w * h * 4)
+            self.rgba.append(out)
+            conv = abi.new(abi.YuvToRgbJob, width=w, height=h, chroma_width=cw, chroma_height=ch, luma_stride=ls,
+                           chroma_stride=cs, range=abi.RANGE_LIMITED, matrix=abi.MATRIX_BT709, rgba=out.ptr)
+            for k, n in enumerate(names):
+                conv.planes[k] = d.ptr(n)
+            comp = abi.new(abi.CompositeJob, target=out.ptr, width=w, height=h, stride=w * 4, layer_count=3)
+            for i, (_, b) in enumerate(self.layers[(w, h)]):
+                comp.layers[i] = abi.Layer(rgba=b.ptr, width=w, height=h, stride=w * 4, x=0, y=0,
+                                           op=abi.BLEND_PREMULTIPLIED_OVER, opacity=255)
+            self.convert.append(conv)
+            self.composite.append(comp)
+            self.mpix_list.append(metrics.mpix(w, h))
+            decode = {abi.CODEC_MJPEG: metrics.mjpeg_frame_bytes, abi.CODEC_MPEG2: metrics.mpeg2_picture_bytes,
+                      abi.CODEC_DV: metrics.dv_frame_bytes, abi.CODEC_H264: metrics.h264_picture_bytes}[codec](u)
+            # conversion: planes read once, RGBA written; composite: target read and written, 3 layers read
+            self.bytes += decode + (w * h + 2 * cw * ch + 4 * w * h) + 20 * w * h
+        self.by_codec = {c: [d.struct for cc, _, d in self.frames if cc == c]
+                         for c in (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)}
+        self.submitters = {abi.CODEC_MJPEG: ctx.submit_mjpeg, abi.CODEC_MPEG2: ctx.submit_mpeg2, abi.CODEC_DV: ctx.submit_dv,
+                           abi.CODEC_H264: ctx.submit_h264}
+        self.pictures = len(self.frames)
+        self.mpix = sum(self.mpix_list)
+        self.f64_ops = None
+        self.step_tickets = {}
+        log(f"mixed workload: {self.pictures} frames generated and uploaded in {time.time() - t:.1f}s")
+
+    STAGES = ("mjpeg", "mpeg2", "dv", "h264", "yuv_to_rgb", "composite")
+
+    def submit(self, flags=0):
+        ts = [self.submitters[c](self.by_codec[c], flags) for c in (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)]
+        ts.append(self.ctx.submit_yuv_to_rgb(self.convert, flags))
+        ts.append(self.ctx.submit_composite(self.composite, flags))
+        self.step_tickets[ts[-1]] = ts
+        return ts[-1]
+
+    def step_phases(self, tickets):
+        """Device ms of each stage of a step (the library's events around each submission's
+        kernels), averaged over the given steps."""
+        return {name: float(np.mean([self.ctx.kernel_ms(self.step_tickets[t][k]) for t in tickets]))
+                for k, name in enumerate(self.STAGES)}
+
+    def plain_items(self):
+        return list(range(len(self.frames)))
+
+    def submit_items(self, i, flags):
+        codec, _, d = self.frames[i]
+        self.ctx.wait(self.submitters[codec]([d.struct], flags))
+        self.ctx.wait(self.ctx.submit_yuv_to_rgb([self.convert[i]], flags))
+        return self.ctx.submit_composite([self.composite[i]], flags)
+
+    def config(self):
+        return {"workload": "mixed codecs per step: 30 MJPEG 640x480, one MPEG-2 1920x1080 GOP of 15, 30 DV "
+                            "720x576/720x480, 12 H.264 1280x720/1920x1088/3840x2160 pictures; every frame + BT.709 "
+                            "limited YUV->RGBA + 3-layer premultiplied alpha-over at its output size",
+                "pictures_per_step": self.pictures}
+
+    def oracle_chain(self, L, i, o, mpeg2_host):
+        """Frame i through the oracle: reconstruction into o (its host planes), BT.709 limited
+        YUV->RGBA, the 3-layer composite; returns the RGBA frame. MPEG-2 pictures must come in
+        decode order with one mpeg2_host dict (GOP index -> planes) for the GOP."""
+        codec, u, _ = self.frames[i]
+        if codec == abi.CODEC_MPEG2:
+            mpeg2_host[self.mpeg2_gop.index(u)] = o
+            assert L.oracle_mpeg2_reconstruct(C.byref(u.host_struct(o, wl.mpeg2_ref_ptrs(mpeg2_host, u, "host")))) == 0
+        elif codec == abi.CODEC_H264:
+            refs = self.h264_host_refs[(u.meta["width_mbs"], u.meta["height_mbs"])]
+            assert L.oracle_h264_reconstruct(C.byref(u.host_struct(o, wl.h264_ref_ptrs(u, refs, "host")))) == 0
+        else:
+            fn = L.oracle_mjpeg_reconstruct if codec == abi.CODEC_MJPEG else L.oracle_dv_reconstruct
+            assert fn(C.byref(u.host_struct(o))) == 0
+        w, h, cw, ch, ls, cs, names = ServedMix.geometry(codec, u)
+        rgba = np.empty((h, w, 4), np.uint8)
+        conv = abi.new(abi.YuvToRgbJob, width=w, height=h, chroma_width=cw, chroma_height=ch, luma_stride=ls,
+                       chroma_stride=cs, range=abi.RANGE_LIMITED, matrix=abi.MATRIX_BT709, rgba=rgba.ctypes.data)
+        for k, nm in enumerate(names):
+            conv.planes[k] = o[nm].ctypes.data
+        assert L.oracle_yuv_to_rgb(C.byref(conv)) == 0
+        comp = abi.new(abi.CompositeJob, target=rgba.ctypes.data, width=w, height=h, stride=w * 4, layer_count=3)
+        for k, (px, _) in enumerate(self.layers[(w, h)]):
+            comp.layers[k] = abi.Layer(rgba=px.ctypes.data, width=w, height=h, stride=w * 4, x=0, y=0,
+                                       op=abi.BLEND_PREMULTIPLIED_OVER, opacity=255)
+        assert L.oracle_composite(C.byref(comp)) == 0
+        return rgba
+
+    def cpu_sample(self, threads):
+        """The whole step's frames through the oracle chain (reconstruction, conversion, composite),
+        spread over the threads (the MPEG-2 GOP on one thread, in decode order)."""
+        from oracle import binding
+
+        L = binding.lib()
+        n = max(1, threads)
+        items = []
+        for i, (codec, u, _) in enumerate(self.frames):
+            if codec != abi.CODEC_MPEG2:
+                items.append([i])
+        items.insert(0, [i for i, (c, _, _) in enumerate(self.frames) if c == abi.CODEC_MPEG2])
+        groups = [[] for _ in range(n)]
+        for k, it in enumerate(items):
+            groups[k % n] += it
+        outs = {i: u.host_outputs() for i, (_, u, _) in enumerate(self.frames)}
+        mpeg2_host = {}
+        chain = lambda i: self.oracle_chain(L, i, outs[i], mpeg2_host)
+        jobs = [lambda g=g: [chain(i) for i in g] for g in groups if g]
+        passes, dt = cpu_passes(jobs)
+        return (passes * self.mpix / dt, len(jobs),
+                f"{passes} pass(es) over the step's {self.pictures} frames (reconstruction + YUV->RGBA + composite), "
+                f"spread over {len(jobs)} threads (the MPEG-2 GOP on one), {dt:.1f} s wall")
+
+
 class LinearWorkload:
     """MMFX linear-light compositing (LinearPixel::over, crates/mmfx/src/render.rs:1826-1854): 16
     targets of 1920x1080 premultiplied linear u16 RGBA per launch, each with a full-frame surface at
@@ -779,7 +954,7 @@ class LinearWorkload:
                 f"{passes} pass(es) over {n} targets 1920x1080 x 3 layers, one target per thread, {dt:.1f} s wall on {n} threads")
 
 
-WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload,
+WORKLOADS = {"mpeg2": Mpeg2Workload, "mjpeg": MjpegWorkload, "dv": DvWorkload, "h264": H264Workload, "mixed": MixedWorkload,
              "h264i": H264IntraWorkload, "scale": ScaleWorkload, "overlay": OverlayWorkload, "linear": LinearWorkload}
 
 
@@ -877,10 +1052,14 @@ def main():
     elapsed = time.perf_counter() - t0
     # Device time of each launch phase (HIP events the library records on the submission stream
     # between phases; they add up to the submission's kernel span).
-    span_ms = float(np.mean([ctx.kernel_ms(t) for t in tickets]))
-    phase_lists = [ctx.kernel_phases_ms(t) for t in tickets]
-    names = phase_names(w, len(phase_lists[0]))
-    phase_ms = {n: float(np.mean([p[i] for p in phase_lists])) for i, n in enumerate(names)}
+    if hasattr(w, "step_phases"):   # several submissions per step: the device time of each
+        phase_ms = w.step_phases(tickets)
+        span_ms = float(sum(phase_ms.values()))
+    else:
+        span_ms = float(np.mean([ctx.kernel_ms(t) for t in tickets]))
+        phase_lists = [ctx.kernel_phases_ms(t) for t in tickets]
+        names = phase_names(w, len(phase_lists[0]))
+        phase_ms = {n: float(np.mean([p[i] for p in phase_lists])) for i, n in enumerate(names)}
     k_ms = phase_ms.get(w.kernel, span_ms)
     ms_per_step = elapsed * 1e3 / args.steps
     value = w.mpix * args.steps / elapsed

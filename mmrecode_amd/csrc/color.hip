@@ -335,37 +335,84 @@ MMR_DEV void blend_rgba_pixel(uint32_t d[4], const uint32_t s[4]) {
     d[3] = oa;
 }
 
-MMR_DEV uint32_t mul255(uint32_t a, uint32_t b) { return (a * b + 127) / 255; }
+// (a b + 127) / 255 for a, b <= 255: t / 255 == (t + 1 + (t >> 8)) >> 8 for every t <= 65534.
+MMR_DEV uint32_t mul255(uint32_t a, uint32_t b) {
+    const uint32_t t = a * b + 127;
+    return (t + 1 + (t >> 8)) >> 8;
+}
 MMR_DEV uint32_t sat255(uint32_t v) { return v > 255 ? 255 : v; }
 
-MMR_DEV void premultiplied_over(uint32_t d[4], const uint32_t s[4], uint32_t opacity) {
-    const uint32_t sa = mul255(s[3], opacity), inverse = 255 - sa;
+// `opaque`: opacity 255, where mul255(v, 255) == v.
+MMR_DEV void premultiplied_over(uint32_t d[4], const uint32_t s[4], uint32_t opacity, bool opaque) {
+    const uint32_t sa = opaque ? s[3] : mul255(s[3], opacity), inverse = 255 - sa;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) d[c] = sat255(mul255(s[c], opacity) + mul255(d[c], inverse));
+    for (int c = 0; c < 3; ++c) d[c] = sat255((opaque ? s[c] : mul255(s[c], opacity)) + mul255(d[c], inverse));
     d[3] = sat255(sa + mul255(d[3], inverse));
 }
 
-// One thread per target pixel; the layers covering the pixel are applied in order.
-__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, uint32_t job_count, uint32_t total_px,
-                                                        uint32_t per_xcd) {
-    const uint32_t gp = xcd_tile(blockIdx.x, per_xcd) * 256 + threadIdx.x;
-    if (gp >= total_px) return;
-    const uint32_t ji = find_job(jobs, job_count, gp, [](const CompositeJobDev& j) { return j.px_begin; });
-    const CompositeJobDev& j = jobs[ji];
-    const uint32_t p = gp - j.px_begin;
-    const uint32_t y = p / j.width, x = p % j.width;
-    uint8_t* t = j.target + (size_t)y * j.stride + (size_t)x * 4;
-    const uint32_t tw = *reinterpret_cast<const uint32_t*>(t);
-    uint32_t d[4] = {tw & 255, (tw >> 8) & 255, (tw >> 16) & 255, tw >> 24};
+// One lane per 4 consecutive target pixels, a wave per 256 pixels of one row (tile -> job table);
+// the layers covering each pixel are applied in order. A group of 4 pixels inside a layer, at
+// 16-byte aligned addresses, moves as one 16-byte load; the target as one 16-byte load and store.
+__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                                        uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const CompositeJobDev& j = jobs[tile_job[tile]];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
+    const uint32_t y = wave / j.waves_per_row;
+    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
+    if (y >= j.height || x0 >= j.width) return;
+    const uint32_t n = min(4u, j.width - x0);
+    MMR_GLOBAL uint32_t* t = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(j.target) + (size_t)y * j.stride + (size_t)x0 * 4);
+    const bool tvec = n == 4 && ((uintptr_t)t & 15) == 0;
+    uint32_t px[4] = {0, 0, 0, 0};
+    if (tvec) {
+        const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) px[i] = (uint32_t)v[i];
+    } else {
+        for (uint32_t i = 0; i < n; ++i) px[i] = t[i];
+    }
     for (uint32_t li = 0; li < j.layer_count; ++li) {
         const mmr_recon_layer& L = j.layers[li];
-        if (x < L.x || y < L.y || x >= L.x + L.width || y >= L.y + L.height) continue;
-        const uint32_t sw = *reinterpret_cast<const uint32_t*>(L.rgba + (size_t)(y - L.y) * L.stride + (size_t)(x - L.x) * 4);
-        const uint32_t s[4] = {sw & 255, (sw >> 8) & 255, (sw >> 16) & 255, sw >> 24};
-        if (L.op == MMR_BLEND_PREMULTIPLIED_OVER) premultiplied_over(d, s, L.opacity);
-        else blend_rgba_pixel(d, s);
+        if (y < L.y || y >= L.y + L.height) continue;   // uniform over the wave (one row)
+        const MMR_GLOBAL uint8_t* row = as_global(L.rgba) + (size_t)(y - L.y) * L.stride;
+        uint32_t sp[4] = {0, 0, 0, 0};
+        uint32_t covered = 0;
+        if (n == 4 && x0 >= L.x && x0 + 4 <= L.x + L.width && (((uintptr_t)(row + (size_t)(x0 - L.x) * 4)) & 15) == 0) {
+            const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(row + (size_t)(x0 - L.x) * 4);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sp[i] = (uint32_t)v[i];
+            covered = 15;
+        } else {
+            for (uint32_t i = 0; i < n; ++i) {
+                const uint32_t x = x0 + i;
+                if (x >= L.x && x < L.x + L.width) {
+                    sp[i] = *reinterpret_cast<const MMR_GLOBAL uint32_t*>(row + (size_t)(x - L.x) * 4);
+                    covered |= 1u << i;
+                }
+            }
+        }
+        const bool premultiplied = L.op == MMR_BLEND_PREMULTIPLIED_OVER, opaque = L.opacity == 255;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (!((covered >> i) & 1u)) continue;
+            uint32_t d[4] = {px[i] & 255, (px[i] >> 8) & 255, (px[i] >> 16) & 255, px[i] >> 24};
+            const uint32_t s[4] = {sp[i] & 255, (sp[i] >> 8) & 255, (sp[i] >> 16) & 255, sp[i] >> 24};
+            if (premultiplied) premultiplied_over(d, s, L.opacity, opaque);
+            else blend_rgba_pixel(d, s);
+            px[i] = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
+        }
     }
-    *reinterpret_cast<uint32_t*>(t) = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
+    if (tvec) {
+        mmr_v4i32 v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = (int)px[i];
+        *reinterpret_cast<MMR_GLOBAL mmr_v4i32*>(t) = v;
+    } else {
+        for (uint32_t i = 0; i < n; ++i) t[i] = px[i];
+    }
 }
 
 MMR_DEV uint8_t blend_channel(uint32_t d, uint32_t s, uint32_t a) {
@@ -598,10 +645,10 @@ hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t*
     return hipGetLastError();
 }
 
-hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s) {
-    if (total_px == 0) return hipSuccess;
-    const uint32_t per = xcd_per((total_px + 255) / 256);
-    hipLaunchKernelGGL(composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, job_count, total_px, per);
+hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
     return hipGetLastError();
 }
 

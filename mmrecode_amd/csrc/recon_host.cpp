@@ -1114,11 +1114,12 @@ mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
     if (!in || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
     const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
     std::vector<mmr::CompositeJobDev> jobs(count);
-    uint32_t px = 0;
+    std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
         const mmr_recon_composite_job& c = in[i];
         if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
-        if (!c.target || !c.width || !c.height || c.stride < c.width * 4 || c.layer_count > MMR_RECON_MAX_LAYERS)
+        if (!c.target || !c.width || !c.height || c.width > 65535 || c.height > 65535 || c.stride < c.width * 4 ||
+            c.layer_count > MMR_RECON_MAX_LAYERS)
             return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
         if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
         for (uint32_t l = 0; l < c.layer_count; ++l) {
@@ -1139,18 +1140,23 @@ mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
         j.stride = c.stride;
         j.layer_count = c.layer_count;
         for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
-        j.px_begin = per_frame ? 0 : px;
-        px += c.width * c.height;
+        j.waves_per_row = (c.width + 255) / 256;
+        j.tile_begin = add_tiles(tiles, i, mmr::composite_job_tiles(j.waves_per_row, j.height), 1);
     }
+    std::vector<char> blob;
+    const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::CompositeJobDev));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
     Submission sub(ctx, stream);
-    mmr_status s = sub.begin(jobs.data(), jobs.size() * sizeof(mmr::CompositeJobDev), 0);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
     if (s) return s;
-    auto* dj = reinterpret_cast<const mmr::CompositeJobDev*>(sub.jobs_ptr());
+    auto* dj = reinterpret_cast<const mmr::CompositeJobDev*>((const char*)sub.jobs_ptr() + jobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
     hipError_t e = hipSuccess;
     if (per_frame) {
-        for (uint32_t i = 0; i < count && e == hipSuccess; ++i) e = mmr::launch_composite(dj + i, 1, jobs[i].width * jobs[i].height, stream);
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_composite(dj, dt, jobs[i].tile_begin, mmr::composite_job_tiles(jobs[i].waves_per_row, jobs[i].height), stream);
     } else {
-        e = mmr::launch_composite(dj, count, px, stream);
+        e = mmr::launch_composite(dj, dt, 0, (uint32_t)tiles.size(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "composite launch");
     return sub.end(out_ticket);

@@ -181,10 +181,11 @@ constexpr size_t kLinearTableBytes = 256 * 2 + 65536;
 struct CompositeJobDev {
     uint8_t* target;
     uint32_t width, height, stride, layer_count;
-    uint32_t px_begin;
-    uint32_t pad;
+    uint32_t tile_begin;          // first workgroup tile of this job in the launch's tile -> job table
+    uint32_t waves_per_row;       // ceil(width / 256): a wave covers 256 pixels of one row
     mmr_recon_layer layers[MMR_RECON_MAX_LAYERS];
 };
+inline uint32_t composite_job_tiles(uint32_t waves_per_row, uint32_t height) { return (waves_per_row * height + 3) / 4; }
 
 // Kernel launchers (stream-ordered). Each returns hipSuccess or the launch error.
 // The block kernels (MJPEG, MPEG-2, DV) run one workgroup per tile of group_blocks blocks (16, 32
@@ -217,7 +218,7 @@ hipError_t launch_linear_composite(const LinearCompositeDev* d_jobs, const uint3
 hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles,
                                 const uint8_t* d_tables, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
-hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s);
+hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
 
 // One plane of a scale job on the device: the canvas window (copy_w x copy_h at dst_x, dst_y)

@@ -378,28 +378,26 @@ hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t*
     return hipSuccess;
 }
 
-// color.hip: composite_kernel. Pixel gp maps to the last job with px_begin <= gp; row
-// y = p / width of the target (stride bytes apart) and of every layer that covers it.
-hipError_t launch_composite(const CompositeJobDev* d_jobs, uint32_t job_count, uint32_t total_px, hipStream_t s) {
-    if (job_count == 0 || total_px == 0) return hipSuccess;
+// color.hip: composite_kernel. Tile t belongs to job tile_job[t]; wave w of the job covers pixels
+// (w % waves_per_row) 256 .. +255 of row w / waves_per_row (rows below height, pixels below width):
+// the target's rows and each layer's rows and columns that overlap them.
+hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
     model::enqueue(s, "composite", [=] {
-        if (!check(d_jobs, sizeof(CompositeJobDev) * job_count, "composite job records")) return;
-        for (uint32_t i = 0; i < job_count; ++i) {
-            const CompositeJobDev& j = d_jobs[i];
+        std::map<uint32_t, std::vector<uint32_t>> offs;
+        std::map<uint32_t, CompositeJobDev> recs;
+        if (!walk_tiles("composite job / tile table", d_jobs, d_tile_job, tile_base, tiles, offs, recs)) return;
+        for (auto& [i, v] : offs) {
+            const CompositeJobDev& j = recs[i];
             model::stats().jobs++;
-            const uint64_t begin = j.px_begin, end = i + 1 < job_count ? d_jobs[i + 1].px_begin : total_px;
-            if (end < begin || end - begin != (uint64_t)j.width * j.height) {
-                violation("composite: job %u gets %lld pixels, its target has %llu", i, (long long)(end - begin),
-                          (unsigned long long)j.width * j.height);
-                if (end < begin) continue;
-            }
-            if (end == begin || !j.width) continue;
-            const uint64_t ymax = (end - 1 - begin) / j.width;
-            check(j.target, ymax * j.stride + (uint64_t)j.width * 4, "composite target");
+            if (j.waves_per_row != (j.width + 255) / 256) violation("composite: job %u waves per row", i);
+            check_tiles("composite", i, v, composite_job_tiles(j.waves_per_row, j.height));
+            if (!j.width || !j.height) continue;
+            check(j.target, (uint64_t)(j.height - 1) * j.stride + (uint64_t)j.width * 4, "composite target");
             for (uint32_t l = 0; l < j.layer_count && l < MMR_RECON_MAX_LAYERS; ++l) {
                 const mmr_recon_layer& L = j.layers[l];
-                if (ymax < L.y || L.height == 0 || L.width == 0 || L.x >= j.width) continue;
-                const uint64_t rows = std::min<uint64_t>(L.height, ymax + 1 - L.y);
+                if (L.y >= j.height || L.height == 0 || L.width == 0 || L.x >= j.width) continue;
+                const uint64_t rows = std::min<uint64_t>(L.height, j.height - L.y);
                 const uint64_t cols = std::min<uint64_t>(L.width, j.width - L.x);
                 check(L.rgba, (rows - 1) * L.stride + cols * 4, "composite layer");
             }

@@ -154,3 +154,28 @@ def test_mixed_workload_convert_and_composite_shapes_bit_exact(ctx, oracle):
         _assert_same(f"{g[0]}x{g[1]}", ctx.download(out, np.uint8, want.shape), want)
     for b in keep:
         b.free()
+
+
+def test_mixed_workload_step_bit_exact(ctx, oracle):
+    """One step of bench.py's mixed workload (30 MJPEG, a 15-picture MPEG-2 GOP, 30 DV, 12 H.264
+    pictures up to 3840x2160; one batch per codec, then one conversion and one composite batch)
+    against the oracle chain of every frame, RGBA bit for bit."""
+    import bench
+
+    w = bench.MixedWorkload(ctx)
+    print(f"mixed step: {w.pictures} frames", flush=True)
+    ctx.wait(w.submit())
+    outs = {i: u.host_outputs() for i, (_, u, _) in enumerate(w.frames)}
+    want = [None] * w.pictures
+    mpeg2 = [i for i, (c, _, _) in enumerate(w.frames) if c == bench.abi.CODEC_MPEG2]
+    rest = [i for i in range(w.pictures) if i not in mpeg2]
+
+    def run(idx):
+        host = {}
+        for i in idx:
+            want[i] = w.oracle_chain(oracle, i, outs[i], host)
+
+    _threads([lambda: run(mpeg2)] + [lambda k=k: run(rest[k::15]) for k in range(15)])
+    for i in range(w.pictures):
+        got = ctx.download(w.rgba[i], np.uint8, want[i].shape)
+        _assert_same(f"frame {i} (codec {w.frames[i][0]})", got, want[i])
