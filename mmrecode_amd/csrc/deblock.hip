@@ -9,16 +9,19 @@
 // samples of (x-1, y) and (x, y-1), and (x+1, y-1) writes samples that (x, y) reads, so (x, y) may
 // run once (x-1, y) and (x+1, y-1) are done: a wavefront over t = x + 2y.
 //
-// Mapping. Pictures are cut into bands of 16 macroblock rows; a 1024-thread workgroup filters one
-// band at a time (band tickets from a device counter, in picture and band order) and wave w owns
-// row w of the band, walking it left to right with a private LDS window (20x20 luma, 2 x 12x12
-// chroma: 4 samples of left / top context). The left context stays in the window (shifted by one
-// macroblock per step). Inside a band the top context comes from the wave of the row above
-// through an LDS ring: after its step x that wave stores the bottom four sample rows of macroblock
-// x (and completes those of x-1, whose right columns step x modified) and publishes progress x+1;
-// the row below reads slot x once progress >= x+2. A writer waits before reusing a ring slot until
-// the reader has consumed it (progress of the row below >= x - ring + 2); waits between adjacent
-// rows cannot form a cycle (ring >= 4). Across bands the hand-over goes through the picture in
+// Mapping. Pictures are cut into bands of ROWS (4, 8 or 16) macroblock rows; a workgroup of ROWS
+// waves filters one band at a time (band tickets from a device counter, in picture and band order)
+// and wave w owns row w of the band, walking it left to right with a private LDS window (20x20
+// luma, 2 x 12x12 chroma: 4 samples of left / top context). The left context stays in the window
+// (shifted by one macroblock per step). Inside a band the top context comes from the wave of the
+// row above through an LDS ring: the bottom four sample rows of macroblock x are final for that
+// row once its step x + 1 has filtered the vertical edges (the left edge of x + 1 changes their
+// right columns), so right there, before its horizontal edges, the row stores them as slot x and
+// publishes progress x + 1; the row below waits for progress >= x + 1 before the horizontal edges
+// of its step x. Rows therefore trail each other by one step plus the hand-over, not by a whole
+// step plus the horizontal half. A writer waits before reusing a ring slot until the reader has
+// consumed it (progress of the row below >= x - ring + 1); waits between adjacent rows cannot
+// form a cycle (ring >= 2). Across bands the hand-over goes through the picture in
 // global memory: the band's last row writes its bottom rows as well and publishes its progress
 // with an agent-scope release every `publish` macroblocks; the next band's first row waits for
 // the progress that covers macroblock x + 2 with relaxed loads, then an agent-scope acquire, and
@@ -439,13 +442,11 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
     const int ring_off = chroma_lane ? (int)offsetof(DbWave, ring_c) + ccomp * 32 + crow * 8 + 4 * (ck & 1)
                                      : (int)offsetof(DbWave, ring_y) + (l >> 2) * 16 + 4 * (l & 3);
     static_assert(sizeof(DbWave::ring_y[0]) == 64 && sizeof(DbWave::ring_c[0]) == 64, "ring slots are 64 bytes");
-    // Right columns of the previous ring slot (the left edge of macroblock x changes them): lanes
-    // 32-35 luma rows, 40-47 chroma rows.
-    const int fk = l - 40, fcomp = (fk >> 2) & 1, frow = fk & 3;
-    const bool fix_lane = (l >= 32 && l < 36) || (l >= 40 && l < 48);
-    const int fix_win = l < 40 ? (16 + ((l - 32) & 3)) * 20 : kLcOff + fcomp * 20 * kLc + (8 + frow) * kLc;
-    const int fix_off = l < 40 ? (int)offsetof(DbWave, ring_y) + ((l - 32) & 3) * 16 + 12
-                               : (int)offsetof(DbWave, ring_c) + fcomp * 32 + frow * 8 + 4;
+    // The right four columns of a macroblock's bottom rows (luma dword 3 of a row, chroma dword 1):
+    // once the next macroblock's left edge has filtered them they are that step's window's left
+    // context (columns 0-3), `right_shift` bytes before bot_off.
+    const bool right_lane = chroma_lane ? (ck & 1) != 0 : (l & 3) == 3;
+    const int right_shift = chroma_lane ? 8 : 16;
     // Left context: lanes 0-19 the luma rows, 20-43 the chroma rows (the other window's right 4
     // columns become this window's left 4).
     const int sk = l - 20, scomp = sk >= 12 ? 1 : 0, srow = sk - 12 * scomp;
@@ -508,6 +509,26 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         }
     };
 
+    // Hands the bottom four rows of macroblock xs (window wmb; its right columns in window wright
+    // at columns 0-3 unless `last`, when no later left edge changes them) to the row below as ring
+    // slot xs, then publishes progress xs + 1: slots 0 .. xs are handed over. Every row publishes,
+    // the band's last one too: a row's progress p also tells the row above that slots 0 .. p - 1
+    // were read (slot j is read in step j, before progress j + 1 is published in step j + 1), so a
+    // row writes slot xs over slot xs - kDbRing once the row below has progress >= xs - kDbRing + 1.
+    auto hand_over = [&](const int xs, const uint8_t* wmb, const uint8_t* wright, const bool last) {
+        if (!band_last) {
+            const int free_at = xs + 1 - kDbRing;
+            if (free_at > 0 && !failed && !db_wait(&prog[w + 1], (uint32_t)free_at, status)) failed = true;
+            if (l < 32) {
+                const uint8_t* src = !last && right_lane ? wright + bot_off - right_shift : wmb + bot_off;
+                *reinterpret_cast<uint32_t*>(wave_base + ring_off + 64 * (xs % kDbRing)) = *reinterpret_cast<const uint32_t*>(src);
+            }
+        }
+        wave_sync();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(xs + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+
     // One step in window b = x & 1 (a constant after inlining: the loop below is unrolled by two).
     // The top context travels by value (returned: the rows above macroblock x + 2, or the input):
     // through a reference the compiler may load it through a pointer select with the ring slot.
@@ -567,12 +588,16 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             for (int i = 0; i < 5; ++i)
                 row[i] = (uint32_t)sm[4 * i] | ((uint32_t)sm[4 * i + 1] << 8) | ((uint32_t)sm[4 * i + 2] << 16) | ((uint32_t)sm[4 * i + 3] << 24);
         }
+        // macroblock x - 1's bottom rows are final now (this step's left edge filtered their right
+        // columns): hand them to the row below
+        wave_sync();
+        if (x > 0) hand_over(x - 1, wprev, wbase, false);
         // top context: from the picture (loaded two steps ago) or from the row above's ring slot x,
-        // after that row finished macroblock x + 1
+        // once that row has handed it over (after the left edge of its macroblock x + 1)
         if (y > 0) {
             uint32_t top = top_in;
             if (!band_first) {
-                if (!failed && !db_wait(&prog[w - 1], (uint32_t)min(x + 2, MW), status)) failed = true;
+                if (!failed && !db_wait(&prog[w - 1], (uint32_t)(x + 1), status)) failed = true;
                 top = l < 32 ? *reinterpret_cast<const uint32_t*>(above_base + ring_off + 64 * (x % kDbRing)) : 0u;
             }
             if (l < 32) *reinterpret_cast<uint32_t*>(wbase + ctx_off) = top;
@@ -601,19 +626,6 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
             }
             db_prefetch(P, MW, x + 2, y, l, buf);
         }
-        // hand the bottom rows to the row below: slot x (all columns) and the right columns of
-        // slot x - 1, which this step's left edge modified
-        if (!band_last) {
-            const int free_at = x - kDbRing + 2;
-            if (free_at > 0 && !failed && !db_wait(&prog[w + 1], (uint32_t)free_at, status)) failed = true;
-            if (l < 32)
-                *reinterpret_cast<uint32_t*>(wave_base + ring_off + 64 * (x % kDbRing)) = *reinterpret_cast<const uint32_t*>(wbase + bot_off);
-            else if (x > 0 && fix_lane)
-                *reinterpret_cast<uint32_t*>(wave_base + fix_off + 64 * ((x + kDbRing - 1) % kDbRing)) = *reinterpret_cast<const uint32_t*>(wbase + fix_win);
-        }
-        wave_sync();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-        if (l == 0) __hip_atomic_store(&prog[w], (uint32_t)(x + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         qp_left = m.qp;
         return top_next;
     };
@@ -623,6 +635,8 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
     }
     if (!failed) {
         wave_sync();
+        const uint8_t* const wlast = reinterpret_cast<const uint8_t*>(&S.win[(MW - 1) & 1]);
+        hand_over(MW - 1, wlast, wlast, true);
         write_back(MW - 1, (MW - 1) & 1);
     }
     if (gself) __hip_atomic_store(gself, (uint32_t)MW, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);

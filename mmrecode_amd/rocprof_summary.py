@@ -103,8 +103,10 @@ def summarize(workload, root, bench_path, out_path, traffic_path):
     # ---- traffic per kernel
     fetch, nf = counters(os.path.join(root, f"pmc_{workload}_FETCH_SIZE"))
     write, nw = counters(os.path.join(root, f"pmc_{workload}_WRITE_SIZE"))
-    steps_f, steps_w = nf.get("fill_zero_kernel", 1), nw.get("fill_zero_kernel", 1)
-    lines += ["HBM traffic, two counter passes (each over every submission of the run; per step = total / submissions):",
+    subs = bench["config"].get("submissions_per_step", 1)
+    steps_f, steps_w = nf.get("fill_zero_kernel", subs) / subs, nw.get("fill_zero_kernel", subs) / subs
+    lines += [f"HBM traffic, two counter passes (each over every submission of the run; {subs} submission(s) per step, "
+              "per step = total / steps):",
               "", "```", PREFIX + "rocprofv3 --pmc FETCH_SIZE --output-format csv -d <out> -o run -- " + bench_cmd(workload, 2, 1),
               PREFIX + "rocprofv3 --pmc WRITE_SIZE --output-format csv -d <out> -o run -- " + bench_cmd(workload, 2, 1), "```", "",
               "| kernel | dispatches / step | FETCH KiB / step (raw) | WRITE KiB / step (raw) | HBM bytes / step | algorithmic | ratio |",
@@ -123,7 +125,15 @@ def summarize(workload, root, bench_path, out_path, traffic_path):
         lines.append(f"| `{k}` | {nf[k] / steps_f:g} | {f_kib:.0f} | {w_kib:.0f} | {fb + wb:.4g} | "
                      f"{a:.4g} | {(fb + wb) / a:.2f} |" if a else
                      f"| `{k}` | {nf[k] / steps_f:g} | {f_kib:.0f} | {w_kib:.0f} | {fb + wb:.4g} | — | — |")
-    if dominant in kernels:
+    if dominant not in kernels and roof.get("phases"):
+        # the roofline covers every kernel of the step: its traffic is the sum over them
+        tot = {c: sum(v[c] for v in kernels.values()) for c in ("fetch_bytes_per_step", "write_bytes_per_step", "hbm_bytes_per_step")}
+        tot["dispatches_per_step"] = round(sum(v["dispatches_per_step"] for v in kernels.values()), 2)
+        kernels[dominant] = tot
+        a = roof["algorithmic_bytes_per_launch"]
+        lines.append(f"| all kernels | {tot['dispatches_per_step']:g} | | | {tot['hbm_bytes_per_step']:.4g} | {a:.4g} | "
+                     f"{tot['hbm_bytes_per_step'] / a:.2f} |")
+    if dominant in kernels and "avg_dispatch_us" in kernels[dominant]:
         d = kernels[dominant]
         ev = roof["kernel_ms"] * 1e3 / d["dispatches_per_step"]
         lines += ["", f"`{dominant}`: {d['dispatches_per_step']:g} dispatch(es) per step; the bench's event time per step / "
