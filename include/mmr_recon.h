@@ -704,7 +704,7 @@ typedef struct mmr_recon_tuning {
     uint32_t h264_deblock_groups_per_cu; /* persistent deblocking workgroups per compute unit: 1..4
                                             (default 2) */
     uint32_t h264_deblock_publish;       /* macroblocks between two progress publications of a
-                                            band's last row to the next band: 1..32 (default 8) */
+                                            band's last row to the next band: 1..32 (default 2) */
 } mmr_recon_tuning;
 
 MMR_RECON_API mmr_status mmr_recon_get_tuning(mmr_recon_context *context, mmr_recon_tuning *out_tuning);

@@ -22,10 +22,11 @@
 // step plus the horizontal half. A writer waits before reusing a ring slot until the reader has
 // consumed it (progress of the row below >= x - ring + 1); waits between adjacent rows cannot
 // form a cycle (ring >= 2). Across bands the hand-over goes through the picture in
-// global memory: the band's last row writes its bottom rows as well and publishes its progress
-// with an agent-scope release every `publish` macroblocks; the next band's first row waits for
-// the progress that covers macroblock x + 2 with relaxed loads, then an agent-scope acquire, and
-// reads the four rows above that macroblock from the picture two steps before it filters it.
+// global memory: the band's last row writes its bottom rows as well, with stores coherent across
+// the XCDs, and publishes its progress every `publish` macroblocks once they have completed; the
+// next band's first row waits for the progress that covers macroblock x + 2 and reads the four
+// rows above that macroblock from the picture, with coherent loads, two steps before it filters
+// it (db_put / publish_agent / db_wait_band: no L2 write-back or invalidate).
 // Tickets are taken in order, so the band a workgroup waits for is held by a running workgroup (or
 // done), and no band waits for a later one.
 //
@@ -351,9 +352,22 @@ MMR_DEV bool db_wait(uint32_t* counter, uint32_t need, uint32_t* status) {
     return true;
 }
 
-// Wait for the previous band's published progress (another workgroup): relaxed agent-scope loads,
-// then one agent-scope acquire, so the rows that band wrote before its release are visible. `seen`
-// keeps the last value observed (and acquired).
+// Hand-over between bands: the row that hands samples to the next band stores all of them with
+// agent_store and publishes with publish_agent; the next band's first row reads them with
+// agent_load (coherent across the XCDs' L2 caches, no write-back or invalidate of the L2).
+MMR_DEV void db_put(MMR_GLOBAL uint8_t* p, uint32_t v, bool agent) {
+    MMR_GLOBAL uint32_t* q = reinterpret_cast<MMR_GLOBAL uint32_t*>(p);
+    if (agent)
+        agent_store(q, v);
+    else
+        *q = v;
+}
+MMR_DEV uint32_t db_get_coherent(const MMR_GLOBAL uint8_t* p) { return agent_load(reinterpret_cast<const MMR_GLOBAL uint32_t*>(p)); }
+
+// Wait for the previous band's published progress (another workgroup) with relaxed agent-scope
+// loads. The samples it covers are read afterwards with coherent loads (db_load_top), issued only
+// once the progress load has returned (the compiler barrier keeps them after it). `seen` keeps the
+// last value observed.
 MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint32_t* status) {
     if (seen >= need) return true;
     uint32_t spins = 0;
@@ -366,19 +380,19 @@ MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint
         }
         __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("" ::: "memory");
     return true;
 }
 
 // The top context of macroblock x of row y (the four sample rows above it, luma and both chroma
-// components), one dword per lane (lanes 0-15 luma, 16-31 chroma), read from the picture.
+// components), one dword per lane (lanes 0-15 luma, 16-31 chroma), read from the picture with
+// coherent loads (the previous band's last row wrote them).
 MMR_DEV uint32_t db_load_top(const DbPic& P, int MW, int x, int y, int l) {
     const int W = MW * 16, CW = MW * 8;
-    if (l < 16)
-        return *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.py + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
+    if (l < 16) return db_get_coherent(P.py + (size_t)(y * 16 - 4 + (l >> 2)) * W + x * 16 + 4 * (l & 3));
     if (l < 32) {
         const int k = l - 16, comp = k >> 3, row = (k >> 1) & 3;
-        return *reinterpret_cast<const MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
+        return db_get_coherent(P.chroma(comp) + (size_t)(y * 8 - 4 + row) * CW + x * 8 + 4 * (k & 1));
     }
     return 0u;
 }
@@ -393,8 +407,8 @@ MMR_DEV uint32_t db_load_top(const DbPic& P, int MW, int x, int y, int l) {
 // middle of step x, then reads the rows above macroblock x + 2). Even and odd steps filter in two
 // windows; step x takes its left context from the other window and writes the other window's final
 // samples (step x - 1's) back to the picture, so a step's stores have a whole step to complete
-// before anything waits for memory again. Agent-scope acquires and releases sit in the middle of a
-// step, after the filtering and before the step's prefetch is issued. Published progress p means
+// before anything waits for memory again. The band hand-over (publish and wait) sits in the middle
+// of a step, after the filtering and before the step's prefetch is issued. Published progress p means
 // that steps 0 .. p-1 are complete and their samples written; the last row publishes p = x in the
 // middle of step x (every `publish` macroblocks) and p = MW after its last store. The vertical
 // edges do not read the rows above the macroblock, so the wait for the row above comes after them.
@@ -411,6 +425,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
     const int chroma_off0 = job.chroma_qp_offset[0], chroma_off1 = job.chroma_qp_offset[1];
     const bool last_row = band_last;   // writes its own bottom rows
     const bool global_top = y > 0 && band_first;
+    const bool agent_out = gself != nullptr;   // the next band reads this row's samples
     uint32_t seen = 0;
     bool failed = false;
     // Two buffers, even and odd steps: step x reads its buffer, then refills it with step x + 2's
@@ -465,27 +480,24 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         const int r = 1 + (l >> 2), c4 = 4 * (l & 3);
         const int gy = y * 16 - 4 + r, gx = xs * 16 - 4 + c4;
         if (gy >= 0 && gx >= 0)
-            *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)gy * W + gx) = *reinterpret_cast<const uint32_t*>(&Wn.ly[r * 20 + c4]);
+            db_put(P.py + (size_t)gy * W + gx, *reinterpret_cast<const uint32_t*>(&Wn.ly[r * 20 + c4]), agent_out);
         if (l < 32) {   // chroma: window rows 1..8, cols 0..7
             const int comp = l >> 4, rr = 1 + ((l >> 1) & 7), cc = 4 * (l & 1);
             const int cy = y * 8 - 4 + rr, cx = xs * 8 - 4 + cc;
             if (cy >= 0 && cx >= 0)
-                *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)cy * CW + cx) =
-                    *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]);
+                db_put(P.chroma(comp) + (size_t)cy * CW + cx, *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]), agent_out);
         }
         if (last_row || xs == MW - 1) {
             if (last_row && l < 12) {   // rows y*16+13 .. +15 (window rows 17..19)
                 const int r2 = 17 + (l >> 2), gx2 = xs * 16 - 4 + c4;
                 if (gx2 >= 0)
-                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)(y * 16 - 4 + r2) * W + gx2) =
-                        *reinterpret_cast<const uint32_t*>(&Wn.ly[r2 * 20 + c4]);
+                    db_put(P.py + (size_t)(y * 16 - 4 + r2) * W + gx2, *reinterpret_cast<const uint32_t*>(&Wn.ly[r2 * 20 + c4]), agent_out);
             }
             if (xs == MW - 1 && l < 19) {   // last column: columns xs*16+12 .. +15 (window col 16)
                 const int r3 = 1 + l;
                 const int gy3 = y * 16 - 4 + r3;
                 if (gy3 >= 0 && (r3 <= 16 || last_row))
-                    *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.py + (size_t)gy3 * W + xs * 16 + 12) =
-                        *reinterpret_cast<const uint32_t*>(&Wn.ly[r3 * 20 + 16]);
+                    db_put(P.py + (size_t)gy3 * W + xs * 16 + 12, *reinterpret_cast<const uint32_t*>(&Wn.ly[r3 * 20 + 16]), agent_out);
             }
             // chroma extras: last row, rows y*8+5 .. +7 (window rows 9..11, 12 dwords); last column,
             // columns xs*8+4 .. +7 (window col 8, 2 x 11 rows): at most 34 dwords, one per lane
@@ -496,14 +508,12 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
                     const int comp = i / 6, rr = 9 + (i % 6) / 2, cc = 4 * (i & 1);
                     const int cx = xs * 8 - 4 + cc;
                     if (cx >= 0)
-                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)(y * 8 - 4 + rr) * CW + cx) =
-                            *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]);
+                        db_put(P.chroma(comp) + (size_t)(y * 8 - 4 + rr) * CW + cx, *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + cc]), agent_out);
                 } else {
                     const int k = i - extra_rows, comp = k / 11, rr = 1 + k % 11;
                     const int cy = y * 8 - 4 + rr;
                     if (cy >= 0 && (rr <= 8 || last_row))
-                        *reinterpret_cast<MMR_GLOBAL uint32_t*>(P.chroma(comp) + (size_t)cy * CW + xs * 8 + 4) =
-                            *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + 8]);
+                        db_put(P.chroma(comp) + (size_t)cy * CW + xs * 8 + 4, *reinterpret_cast<const uint32_t*>(&Wn.lc[comp][rr * kLc + 8]), agent_out);
                 }
             }
         }
@@ -618,7 +628,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         wave_sync();
         // Mid-step: the agent-scope hand-over, then this step's loads for step x + 2.
         if (gself && x > 0 && x % publish == 0)   // steps 0 .. x-1 are written (step x-1's stores were issued above)
-            __hip_atomic_store(gself, (uint32_t)x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            publish_agent(gself, (uint32_t)x);
         if (x + 2 < MW) {
             if (global_top) {
                 if (!failed && !db_wait_band(gprev, (uint32_t)min(x + 4, MW), seen, status)) failed = true;
@@ -639,7 +649,7 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         hand_over(MW - 1, wlast, wlast, true);
         write_back(MW - 1, (MW - 1) & 1);
     }
-    if (gself) __hip_atomic_store(gself, (uint32_t)MW, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (gself) publish_agent(gself, (uint32_t)MW);
 }
 
 // ROWS macroblock rows per band, one wave per row.

@@ -297,3 +297,24 @@ MMR_DEV void add_release(uint32_t* p, uint32_t v) {
 MMR_DEV uint32_t fetch_add_relaxed(uint32_t* p, uint32_t v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// Data handed between workgroups of one kernel (possibly on different XCDs) without cache
+// maintenance. Agent-scope relaxed atomic stores and loads are coherent across the XCDs' L2
+// caches (the sc1 forms of the memory instructions): a producer that writes the data with
+// agent_store and then publishes a counter with publish_agent (its earlier memory operations
+// complete before the counter store is issued), and a consumer that polls the counter with
+// agent-scope loads and only then reads the data with agent_load (behind a compiler barrier), see
+// the data without the write-back of the XCD's whole L2 that an agent-scope release costs, or the
+// invalidate of an agent-scope acquire.
+template <typename T>
+MMR_DEV void agent_store(MMR_GLOBAL T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+MMR_DEV T agent_load(const MMR_GLOBAL T* p) {
+    return __hip_atomic_load(const_cast<MMR_GLOBAL T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+MMR_DEV void publish_agent(uint32_t* counter, uint32_t v) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __hip_atomic_store(counter, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}

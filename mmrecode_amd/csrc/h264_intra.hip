@@ -17,10 +17,12 @@
 // waits for is held by a running workgroup or done) and wave w walks row w of the band left to
 // right. Before an intra macroblock x a row publishes progress x (every macroblock before x is
 // final) and waits until the row above has published min(x + 2, width): inside the band through an
-// LDS counter with workgroup-scope release / acquire; across bands the band's last row publishes to
-// global memory with an agent-scope release every `publish` macroblocks and at its end, and the next
-// band's first row polls with relaxed agent-scope loads, then acquires at agent scope. A wave keeps
-// its own row's previous intra macroblock in LDS for the left context.
+// LDS counter with workgroup-scope release / acquire; across bands the band's last row stores its
+// samples coherently across the XCDs (agent_store) and publishes to global memory every `publish`
+// macroblocks and at its end once they have completed (publish_agent), and the next band's first
+// row polls with relaxed agent-scope loads and reads the row above with coherent loads
+// (agent_load): no L2 write-back or invalidate. A wave keeps its own row's previous intra
+// macroblock in LDS for the left context.
 //
 // Inside a macroblock the residuals of all blocks are computed first (in parallel, shared with
 // the inter kernel); then Intra4x4 walks its 16 blocks and Intra8x8 its four in block order, each
@@ -151,8 +153,8 @@ MMR_DEV uint32_t plane_sample(const uint8_t (*tile)[STRIDE], int n, int x, int y
 // band: the LDS counter of the row above, then an acquire of LDS only -- everything a row hands to
 // the row below inside the band is in LDS (its ring of bottom rows); the global samples it reads
 // there were written by an earlier kernel (inter macroblocks). Across bands: relaxed
-// agent-scope polls of the previous band's published progress, then an agent-scope acquire so the
-// samples it wrote before its release are visible (and this wave's stale cached lines dropped).
+// agent-scope polls of the previous band's published progress; the samples it covers are then read
+// with coherent loads (the compiler barrier keeps them after the poll).
 MMR_DEV void intra_wait_lds(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
     while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
@@ -177,7 +179,7 @@ MMR_DEV void intra_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, u
         }
         __builtin_amdgcn_s_sleep(2);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    asm volatile("" ::: "memory");
 }
 
 // Residuals of every block of an intra macroblock (luma 4x4 / 8x8 / Intra16x16, chroma) into
@@ -229,7 +231,7 @@ MMR_DEV void intra_residual(IntraShared& sh, const H264Job& job, const mmr_recon
 // is intra, i.e. its bottom row is in above's ring (inter macroblocks: global memory).
 MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int mby, const mmr_recon_h264_macroblock& m,
                               const mmr_recon_h264_intra& in, bool left_mb, bool top_mb, bool left_in_lds,
-                              const IntraShared* above, uint32_t above_intra, int l, uint32_t* status) {
+                              const IntraShared* above, uint32_t above_intra, bool agent_out, int l, uint32_t* status) {
     MbShared& S = sh.S;
     const int W = (int)job.width_mbs * 16, CW = W / 2;
     const MMR_GLOBAL uint8_t* py = as_global(job.planes[0]);
@@ -241,15 +243,19 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
     // ---- context tile
     const int x0 = mbx * 16, y0 = mby * 16, cx0 = mbx * 8, cy0 = mby * 8;
     const uint32_t ring_mid = above && (above_intra & 2u) ? (uint32_t)(mbx % kIntraRing) : ~0u;
+    // A band's first row reads the row above from the picture, written by the previous band (another
+    // workgroup) with coherent stores: coherent loads (the row above in the band is in LDS, or an
+    // inter macroblock an earlier kernel wrote).
+    auto top_sample = [&](const MMR_GLOBAL uint8_t* p) -> uint8_t { return above ? *p : agent_load(p); };
     if (top_mb) {
         if (l < 16) {
-            sh.ty[0][1 + l] = ring_mid != ~0u ? above->ring_y[ring_mid][l] : py[(size_t)(y0 - 1) * W + x0 + l];
+            sh.ty[0][1 + l] = ring_mid != ~0u ? above->ring_y[ring_mid][l] : top_sample(py + (size_t)(y0 - 1) * W + x0 + l);
         } else if (l < 24 && x0 + l < W) {
             sh.ty[0][1 + l] = above && (above_intra & 4u) ? above->ring_y[(mbx + 1) % kIntraRing][l - 16]
-                                                          : py[(size_t)(y0 - 1) * W + x0 + l];
+                                                          : top_sample(py + (size_t)(y0 - 1) * W + x0 + l);
         } else if (l >= 48) {
             const int comp = (l - 48) >> 3, i = (l - 48) & 7;
-            sh.tc[comp][0][1 + i] = ring_mid != ~0u ? above->ring_c[ring_mid][comp][i] : pc[comp][(size_t)(cy0 - 1) * CW + cx0 + i];
+            sh.tc[comp][0][1 + i] = ring_mid != ~0u ? above->ring_c[ring_mid][comp][i] : top_sample(pc[comp] + (size_t)(cy0 - 1) * CW + cx0 + i);
         }
     }
     if (left_mb) {
@@ -264,8 +270,8 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
     if (top_mb && left_mb && l >= 24 && l < 27) {
         const bool ring = above && (above_intra & 1u);
         const int slot = (mbx - 1) % kIntraRing;
-        if (l == 24) sh.ty[0][0] = ring ? above->ring_y[slot][15] : py[(size_t)(y0 - 1) * W + x0 - 1];
-        else sh.tc[l - 25][0][0] = ring ? above->ring_c[slot][l - 25][7] : pc[l - 25][(size_t)(cy0 - 1) * CW + cx0 - 1];
+        if (l == 24) sh.ty[0][0] = ring ? above->ring_y[slot][15] : top_sample(py + (size_t)(y0 - 1) * W + x0 - 1);
+        else sh.tc[l - 25][0][0] = ring ? above->ring_c[slot][l - 25][7] : top_sample(pc[l - 25] + (size_t)(cy0 - 1) * CW + cx0 - 1);
     }
 
     wave_lds_sync();   // context tile
@@ -467,10 +473,18 @@ MMR_DEV void intra_macroblock(IntraShared& sh, const H264Job& job, int mbx, int 
         uint32_t word = 0;
 #pragma unroll
         for (int c = 0; c < 4; ++c) word |= (uint32_t)sh.ty[1 + r][1 + c0 + c] << (8 * c);
-        *reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y0 + r) * W + x0 + c0) = word;
+        // (a band's last row: coherent stores, read by the next band's first row)
+        MMR_GLOBAL uint32_t* dy = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(job.planes[0]) + (size_t)(y0 + r) * W + x0 + c0);
         const int comp = l >> 5, cr = (l >> 2) & 7, cc = (l & 3) * 2;
-        const uint32_t half = (uint32_t)sh.tc[comp][1 + cr][1 + cc] | (uint32_t)sh.tc[comp][1 + cr][2 + cc] << 8;
-        *reinterpret_cast<MMR_GLOBAL uint16_t*>(as_global(job.planes[1 + comp]) + (size_t)(cy0 + cr) * CW + cx0 + cc) = (uint16_t)half;
+        const uint16_t half = (uint16_t)((uint32_t)sh.tc[comp][1 + cr][1 + cc] | (uint32_t)sh.tc[comp][1 + cr][2 + cc] << 8);
+        MMR_GLOBAL uint16_t* dc = reinterpret_cast<MMR_GLOBAL uint16_t*>(as_global(job.planes[1 + comp]) + (size_t)(cy0 + cr) * CW + cx0 + cc);
+        if (agent_out) {
+            agent_store(dy, word);
+            agent_store(dc, half);
+        } else {
+            *dy = word;
+            *dc = half;
+        }
     }
     if (l < 16) sh.left_y[l] = sh.ty[1 + l][16];
     else if (l < 32) sh.left_c[(l - 16) >> 3][(l - 16) & 7] = sh.tc[(l - 16) >> 3][1 + ((l - 16) & 7)][8];
@@ -508,7 +522,7 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
                 }
                 if (gself && x > gpublished && (force || x - gpublished >= publish)) {
                     gpublished = x;
-                    __hip_atomic_store(gself, x, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                    publish_agent(gself, x);
                 }
             };
             // Records of 64 macroblocks at a time, lane i holding macroblock chunk + i: its record
@@ -606,7 +620,7 @@ __global__ __launch_bounds__(kIntraRows * 64) void h264_intra_kernel(const H264J
                 const bool left_mb = mbx > 0 && prev_slice == slice;
                 const bool top_mb = mby > 0 && __builtin_amdgcn_readlane(above_slice, lane) == slice;
                 intra_macroblock(sh, job, mbx, mby, m, in, left_mb, top_mb, left_in_lds, above,
-                                 __builtin_amdgcn_readlane(above3, lane), l, status);
+                                 __builtin_amdgcn_readlane(above3, lane), gself != nullptr, l, status);
                 left_in_lds = true;
                 if (!last) {
                     // Hand the bottom row to the row below through the ring; its slot was last read

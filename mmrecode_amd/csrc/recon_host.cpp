@@ -676,7 +676,7 @@ mmr_recon_tuning default_tuning() {
     t.dv_group_blocks = 64;
     t.h264_deblock_band_rows = 8;
     t.h264_deblock_groups_per_cu = 2;
-    t.h264_deblock_publish = 8;
+    t.h264_deblock_publish = 2;
     return t;
 }
 
@@ -2298,7 +2298,7 @@ mmr_status mmr_recon_autotune(mmr_recon_context* ctx, uint32_t codec, const void
         case MMR_CODEC_H264:
             for (uint32_t rows : {4u, 8u, 16u})
                 for (uint32_t per_cu : {1u, 2u})
-                    for (uint32_t publish : {2u, 8u}) {
+                    for (uint32_t publish : {1u, 2u, 4u, 8u}) {
                         mmr_recon_tuning t = base;
                         t.h264_deblock_band_rows = rows;
                         t.h264_deblock_groups_per_cu = per_cu;

@@ -108,7 +108,10 @@ def test_autotune_h264_candidates(tuned, oracle):
     u = wl.h264_picture(seed=53, index=0, width_mbs=wmb, height_mbs=hmb, weighted=False, deblock=True)
     d = wl.DeviceUnit(ctx, u, wl.h264_ref_ptrs(u, [{n: b.ptr for n, b in r.items()} for r in dev_refs], "dev"))
     best, trials = ctx.autotune(abi.CODEC_H264, [d.struct], repeats=1)
-    assert len(trials) == 12 and all(ms > 0 for _, ms in trials)
+    # band rows {4, 8, 16} x groups per compute unit {1, 2} x publish interval {1, 2, 4, 8}
+    assert len(trials) == 24 and all(ms > 0 for _, ms in trials)
+    assert {(t["h264_deblock_band_rows"], t["h264_deblock_groups_per_cu"], t["h264_deblock_publish"]) for t, _ in trials} == \
+        {(r, g, p) for r in (4, 8, 16) for g in (1, 2) for p in (1, 2, 4, 8)}
     assert ctx.get_tuning() == best
     want = _oracle_run(oracle, "oracle_h264_reconstruct", u, wl.h264_ref_ptrs(u, host_refs, "host"))
     for n in ("y", "cb", "cr"):
