@@ -705,25 +705,30 @@ def run_served_sweep(ctx, rates, seconds, plain_limit):
 
 
 class MixedWorkload:
-    """The served mix's per-frame chain, batched on one stream: a fixed set of request windows -- 30
+    """The served mix's per-frame chain, batched on two streams: a fixed set of request windows -- 30
     MJPEG 640x480 frames, one MPEG-2 1920x1080 closed GOP of 15 pictures, 30 DV frames (720x576
     4:2:0 and 720x480 4:1:1 alternating), 12 H.264 pictures (4 each of 1280x720, 1920x1088 and
     3840x2160, 4 resident references per size, weighted bi-prediction, deblocking) -- where every
     frame is reconstructed, converted (BT.709 limited YUV->RGBA) and composited (3 premultiplied
-    RGBA8 layers, alpha-over, at its output size). A step is one batch per codec, one conversion
-    batch and one composite batch; the plain path runs the same chain one frame and one launch at a
-    time. Mpix count output frames."""
+    RGBA8 layers, alpha-over, at its output size). A step is one batch per codec, then a conversion
+    batches: the chain of the 3840x2160 H.264 pictures on the context's stream, every other chain
+    (MJPEG, MPEG-2, DV, the smaller H.264 pictures) on a second context's stream beside it (the
+    deblocking wavefront of the 3840x2160 pictures is bound by its dependency chain and leaves most
+    compute units idle). The plain path runs
+    the same chain one frame and one launch at a time on one stream. Mpix count output frames."""
     name = "mixed"
     codec = None
     data = "synthetic (seeded post-entropy buffers per codec, seeded premultiplied RGBA8 layers)"
     metric = "reconstructed + composited Mpix/s, mixed codecs (batched vs plain one-frame-per-launch)"
     metric_unit = "Mpix/s"
     kernel = "all kernels of the step"
+    streams = 2
     dtype = "u8/i16 in, int32 + f64 (IDCT) arithmetic"
     H264_SIZES = ((80, 45), (120, 68), (240, 135))
 
     def __init__(self, ctx, seed=2024):
         self.ctx = ctx
+        self.side = mmrecode_amd.Context(device=0)   # second stream; buffers are the device's, shared
         t = time.time()
         self.frames = []   # (codec, unit, device unit)
         for i in range(30):
@@ -790,6 +795,18 @@ class MixedWorkload:
                          for c in (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)}
         self.submitters = {abi.CODEC_MJPEG: ctx.submit_mjpeg, abi.CODEC_MPEG2: ctx.submit_mpeg2, abi.CODEC_DV: ctx.submit_dv,
                            abi.CODEC_H264: ctx.submit_h264}
+        # The 3840x2160 H.264 pictures (the longest deblocking chains) on the context's stream,
+        # every other frame on the second stream.
+        big = self.H264_SIZES[-1]
+
+        def on_main(c, u):
+            return c == abi.CODEC_H264 and (u.meta["width_mbs"], u.meta["height_mbs"]) == big
+        main = [i for i, (c, u, _) in enumerate(self.frames) if on_main(c, u)]
+        side = [i for i, (c, u, _) in enumerate(self.frames) if not on_main(c, u)]
+        self.h264_main = [self.frames[i][2].struct for i in main]
+        self.h264_side = [d.struct for c, u, d in self.frames if c == abi.CODEC_H264 and not on_main(c, u)]
+        self.chains = {"side": ([self.convert[i] for i in side], [self.composite[i] for i in side]),
+                       "main": ([self.convert[i] for i in main], [self.composite[i] for i in main])}
         self.pictures = len(self.frames)
         self.mpix = sum(self.mpix_list)
         self.f64_ops = None
@@ -799,17 +816,41 @@ class MixedWorkload:
     STAGES = ("mjpeg", "mpeg2", "dv", "h264", "yuv_to_rgb", "composite")
 
     def submit(self, flags=0):
-        ts = [self.submitters[c](self.by_codec[c], flags) for c in (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)]
-        ts.append(self.ctx.submit_yuv_to_rgb(self.convert, flags))
-        ts.append(self.ctx.submit_composite(self.composite, flags))
-        self.step_tickets[ts[-1]] = ts
-        return ts[-1]
+        """One step; returns the ticket of the H.264 chain's last submission (wait() waits for the
+        other stream's chain as well)."""
+        c = self.ctx
+        h = [c.submit_h264(self.h264_main, flags)]
+        h.append(c.submit_yuv_to_rgb(self.chains["main"][0], flags))
+        h.append(c.submit_composite(self.chains["main"][1], flags))
+        s = self.side
+        o = [s.submit_mjpeg(self.by_codec[abi.CODEC_MJPEG], flags), s.submit_mpeg2(self.by_codec[abi.CODEC_MPEG2], flags),
+             s.submit_dv(self.by_codec[abi.CODEC_DV], flags), s.submit_h264(self.h264_side, flags)]
+        o.append(s.submit_yuv_to_rgb(self.chains["side"][0], flags))
+        o.append(s.submit_composite(self.chains["side"][1], flags))
+        self.step_tickets[h[-1]] = (h, o)
+        return h[-1]
+
+    def close(self):
+        self.side.close()
+
+    def wait(self, ticket):
+        self.ctx.wait(ticket)
+        self.side.wait(self.step_tickets[ticket][1][-1])
+
+    def synchronize(self):
+        self.ctx.synchronize()
+        self.side.synchronize()
 
     def step_phases(self, tickets):
         """Device ms of each stage of a step (the library's events around each submission's
-        kernels), averaged over the given steps."""
-        return {name: float(np.mean([self.ctx.kernel_ms(self.step_tickets[t][k]) for t in tickets]))
-                for k, name in enumerate(self.STAGES)}
+        kernels, on either stream; the streams overlap), averaged over the given steps."""
+        def ms(t, stage):
+            h, o = self.step_tickets[t]
+            pick = {"mjpeg": [(self.side, o[0])], "mpeg2": [(self.side, o[1])], "dv": [(self.side, o[2])],
+                    "h264": [(self.ctx, h[0]), (self.side, o[3])], "yuv_to_rgb": [(self.ctx, h[1]), (self.side, o[4])],
+                    "composite": [(self.ctx, h[2]), (self.side, o[5])]}[stage]
+            return sum(cx.kernel_ms(k) for cx, k in pick)
+        return {name: float(np.mean([ms(t, name) for t in tickets])) for name in self.STAGES}
 
     def plain_items(self):
         return list(range(len(self.frames)))
@@ -1029,9 +1070,11 @@ def main():
         tuning = {"chosen": best, "trials": [{"tuning": t, "kernel_ms": round(ms, 4)} for t, ms in trials],
                   "seconds": round(time.perf_counter() - t0, 2)}
         log(f"autotuned in {tuning['seconds']} s: {best}")
+    wait = getattr(w, "wait", ctx.wait)                  # a workload on two streams waits for both
+    sync = getattr(w, "synchronize", ctx.synchronize)
     for _ in range(args.warmup):
-        ctx.wait(w.submit())
-    ctx.synchronize()
+        wait(w.submit())
+    sync()
     # Frame latency of the batched path: one batch in flight, submit -> completion observed on the
     # host; every picture of a batch completes with it.
     batch_lat, idle_submit = [], []
@@ -1039,7 +1082,7 @@ def main():
         ts = time.perf_counter()
         t = w.submit()
         idle_submit.append(time.perf_counter() - ts)
-        ctx.wait(t)
+        wait(t)
         batch_lat.append((time.perf_counter() - ts) * 1e3)
     t0 = time.perf_counter()
     tickets, submit_s = [], []
@@ -1048,7 +1091,7 @@ def main():
         tickets.append(w.submit())
         submit_s.append(time.perf_counter() - ts)
     for t in tickets:
-        ctx.wait(t)
+        wait(t)
     elapsed = time.perf_counter() - t0
     # Device time of each launch phase (HIP events the library records on the submission stream
     # between phases; they add up to the submission's kernel span).
@@ -1062,6 +1105,8 @@ def main():
         phase_ms = {n: float(np.mean([p[i] for p in phase_lists])) for i, n in enumerate(names)}
     k_ms = phase_ms.get(w.kernel, span_ms)
     ms_per_step = elapsed * 1e3 / args.steps
+    if getattr(w, "streams", 1) > 1 and w.kernel not in phase_ms:
+        k_ms = ms_per_step   # overlapping streams: the phases do not add up; the step's wall time bounds the span
     value = w.mpix * args.steps / elapsed
     log(f"batched: {ms_per_step:.3f} ms/step, {value:.1f} Mpix/s, kernels {span_ms:.4f} ms "
         f"({', '.join(f'{n} {v:.4f}' for n, v in phase_ms.items())})")
@@ -1154,7 +1199,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
                      "traffic": None, "kernel": w.kernel, "kernel_ms": round(k_ms, 4),
                      "algorithmic_bytes_per_launch": int(kernel_bytes[w.kernel]),
-                     "per": "step: every launch of the kernel in one step (bytes and device time summed over them)",
+                     "per": ("step: every launch of the step on both streams (bytes summed; time = the step's wall time, "
+                             "the streams overlap)" if getattr(w, "streams", 1) > 1 and w.kernel not in phase_ms else
+                             "step: every launch of the kernel in one step (bytes and device time summed over them)"),
                      "phases": phases, "kernels_ms_per_step": round(span_ms, 4),
                      "timing": "HIP events recorded by the library on the submission stream around each launch phase"},
         "cpu_baseline": cpu,
@@ -1169,6 +1216,8 @@ def main():
         result["roofline"]["fp64_dense_tflops_equiv"] = round(w.f64_ops / (k_ms * 1e-3) / 1e12, 3)
         result["roofline"]["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
     print(json.dumps(result), flush=True)
+    if hasattr(w, "close"):
+        w.close()
     ctx.close()
 
 

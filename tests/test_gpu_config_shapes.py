@@ -158,13 +158,13 @@ def test_mixed_workload_convert_and_composite_shapes_bit_exact(ctx, oracle):
 
 def test_mixed_workload_step_bit_exact(ctx, oracle):
     """One step of bench.py's mixed workload (30 MJPEG, a 15-picture MPEG-2 GOP, 30 DV, 12 H.264
-    pictures up to 3840x2160; one batch per codec, then one conversion and one composite batch)
-    against the oracle chain of every frame, RGBA bit for bit."""
+    pictures up to 3840x2160; one batch per codec, then conversion and composite batches, the
+    3840x2160 H.264 chain and the others on two streams) against the oracle chain of every frame, RGBA bit for bit."""
     import bench
 
     w = bench.MixedWorkload(ctx)
     print(f"mixed step: {w.pictures} frames", flush=True)
-    ctx.wait(w.submit())
+    w.wait(w.submit())   # the H.264 chain on ctx's stream, the others on a second context's
     outs = {i: u.host_outputs() for i, (_, u, _) in enumerate(w.frames)}
     want = [None] * w.pictures
     mpeg2 = [i for i, (c, _, _) in enumerate(w.frames) if c == bench.abi.CODEC_MPEG2]
@@ -179,3 +179,4 @@ def test_mixed_workload_step_bit_exact(ctx, oracle):
     for i in range(w.pictures):
         got = ctx.download(w.rgba[i], np.uint8, want[i].shape)
         _assert_same(f"frame {i} (codec {w.frames[i][0]})", got, want[i])
+    w.close()
