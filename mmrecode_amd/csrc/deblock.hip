@@ -341,7 +341,7 @@ hipError_t launch_h264_bs(const H264Job* d_jobs, const uint32_t* d_tile_job, uin
 constexpr uint32_t kDbSpinLimit = 1u << 25;
 MMR_DEV bool db_wait(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+    while (poll_workgroup(counter) < need) {
         if (++spins > kDbSpinLimit) {
             atomicOr(status, kDevStatusTimeout);
             return false;
@@ -372,7 +372,7 @@ MMR_DEV bool db_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, uint
     if (seen >= need) return true;
     uint32_t spins = 0;
     for (;;) {
-        seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        seen = poll_agent(counter);
         if (seen >= need) break;
         if (++spins > kDbSpinLimit) {
             atomicOr(status, kDevStatusTimeout);

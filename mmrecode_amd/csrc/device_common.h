@@ -318,3 +318,13 @@ MMR_DEV void publish_agent(uint32_t* counter, uint32_t v) {
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __hip_atomic_store(counter, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+// A counter polled by a whole wave, read once per lane and made wave-uniform: the polling loop is
+// then a scalar loop (a per-lane compare would make it a divergent loop, with exec-mask
+// bookkeeping on every iteration that takes issue slots from the other waves of the SIMD).
+MMR_DEV uint32_t poll_workgroup(uint32_t* counter) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+}
+MMR_DEV uint32_t poll_agent(uint32_t* counter) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}

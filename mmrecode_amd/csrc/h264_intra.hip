@@ -157,7 +157,7 @@ MMR_DEV uint32_t plane_sample(const uint8_t (*tile)[STRIDE], int n, int x, int y
 // with coherent loads (the compiler barrier keeps them after the poll).
 MMR_DEV void intra_wait_lds(uint32_t* counter, uint32_t need, uint32_t* status) {
     uint32_t spins = 0;
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+    while (poll_workgroup(counter) < need) {
         if (++spins > kIntraSpinLimit) {
             atomicOr(status, kDevStatusTimeout);
             break;
@@ -170,7 +170,7 @@ MMR_DEV void intra_wait_band(uint32_t* counter, uint32_t need, uint32_t& seen, u
     if (seen >= need) return;
     uint32_t spins = 0;
     for (;;) {
-        seen = __hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        seen = poll_agent(counter);
         if (seen >= need) break;
         if (++spins > kIntraSpinLimit) {
             atomicOr(status, kDevStatusTimeout);

@@ -927,7 +927,17 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     uint32_t bands = 0, intra_bands = 0;
     const uint32_t intra_rows_per_band = mmr::h264_intra_band_rows();
     const uint32_t band_rows = ctx->tuning.h264_deblock_band_rows, publish = ctx->tuning.h264_deblock_publish;
-    for (uint32_t i = 0; i < count; ++i) {
+    // Jobs are built longest wavefront first (width + 2 x height macroblocks, descending; ties in
+    // submission order): deblocking and intra band tickets follow job order, so the pictures whose
+    // dependency chains set the launch's length get workgroups first. The pictures of a batch are
+    // independent, so their order changes no result. Job k is picture order[k].
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        return pics[a].width_mbs + 2 * pics[a].height_mbs > pics[b].width_mbs + 2 * pics[b].height_mbs;
+    });
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t i = order[k];
         const mmr_recon_h264_picture& p = pics[i];
         // descriptors that end before intra (built without intra records) stay valid
         constexpr size_t kH264Base = offsetof(mmr_recon_h264_picture, intra);
@@ -949,7 +959,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 picture %u: weighted mode needs weight tables", i);
         if (!aligned(p.planes[0], 4) || !aligned(p.planes[1], 4) || !aligned(p.planes[2], 4))
             return fail(MMR_STATUS_INVALID_ARGUMENT, "H.264 planes must be 4-byte aligned");
-        mmr::H264Job& j = jobs[i];
+        mmr::H264Job& j = jobs[k];
         memset(&j, 0, sizeof j);
         j.mbs = p.macroblocks;
         j.blocks = p.blocks;
@@ -977,7 +987,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
         j.deblock = p.deblock ? 1u : 0u;
         j.chroma_qp_offset[0] = p.chroma_qp_offset[0];
         j.chroma_qp_offset[1] = p.chroma_qp_offset[1];
-        j.tile_begin = add_tiles(tiles, i, (p.width_mbs * p.height_mbs) * 8);   // 4 macroblocks per 32-entry tile unit
+        j.tile_begin = add_tiles(tiles, k, (p.width_mbs * p.height_mbs) * 8);   // 4 macroblocks per 32-entry tile unit
         j.refs_aligned = 1;
         for (int list = 0; list < 2; ++list)
             for (uint32_t r = 0; r < MMR_H264_MAX_REFS; ++r)
@@ -994,11 +1004,11 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     Submission sub(ctx, stream, cap);
     // Counters: deblocking band tickets and hand-over progress, then intra band tickets and
     // hand-over progress (per picture with one launch per picture).
-    auto pic_intra_bands = [&](uint32_t i) { return (pics[i].height_mbs + intra_rows_per_band - 1) / intra_rows_per_band; };
+    auto pic_intra_bands = [&](uint32_t i) { return (pics[order[i]].height_mbs + intra_rows_per_band - 1) / intra_rows_per_band; };
     size_t counters = 0;
     if (per_frame) {
         for (uint32_t i = 0; i < count; ++i)
-            counters += 1 + (jobs[i].deblock ? pics[i].height_mbs : 0) + (jobs[i].intra ? 1 + pic_intra_bands(i) : 0);
+            counters += 1 + (jobs[i].deblock ? pics[order[i]].height_mbs : 0) + (jobs[i].intra ? 1 + pic_intra_bands(i) : 0);
     } else {
         counters = 1 + bands + (intra_bands ? 1 + intra_bands : 0);
     }
@@ -1007,7 +1017,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     size_t scratch_total = 0;
     for (uint32_t i = 0; i < count; ++i) {
         bs_off[i] = scratch_total;
-        if (jobs[i].deblock) scratch_total += ((size_t)pics[i].width_mbs * pics[i].height_mbs * 16 + 255) & ~size_t(255);
+        if (jobs[i].deblock) scratch_total += ((size_t)pics[order[i]].width_mbs * pics[order[i]].height_mbs * 16 + 255) & ~size_t(255);
     }
     std::vector<char> blob;
     const size_t jobs_off = blob_append(blob, jobs.data(), jobs.size() * sizeof(mmr::H264Job));
@@ -1025,7 +1035,7 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     if (per_frame) {
         uint32_t* c = sub.counters_ptr();
         for (uint32_t i = 0; i < count && e == hipSuccess; ++i) {
-            const uint32_t mc_tiles = (pics[i].width_mbs * pics[i].height_mbs + 3) / 4;
+            const uint32_t mc_tiles = (pics[order[i]].width_mbs * pics[order[i]].height_mbs + 3) / 4;
             if (jobs[i].deblock) e = mmr::launch_h264_bs(dj, dt, jobs[i].tile_begin, mc_tiles, stream);
             if (e == hipSuccess) e = mmr::launch_h264_mc(dj, dt, jobs[i].tile_begin, mc_tiles, sub.status_ptr(), stream);
             if (e == hipSuccess && jobs[i].intra) {
@@ -1033,9 +1043,9 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
                 c += 1 + pic_intra_bands(i);
             }
             if (e == hipSuccess && jobs[i].deblock) {
-                const uint32_t nb = (pics[i].height_mbs + band_rows - 1) / band_rows;
+                const uint32_t nb = (pics[order[i]].height_mbs + band_rows - 1) / band_rows;
                 e = mmr::launch_h264_deblock(dj + i, 1, nb, c, groups, band_rows, publish, sub.status_ptr(), stream);
-                c += 1 + pics[i].height_mbs;
+                c += 1 + pics[order[i]].height_mbs;
             } else {
                 c += 1;
             }
