@@ -865,7 +865,7 @@ class MixedWorkload:
         return {"workload": "mixed codecs per step: 30 MJPEG 640x480, one MPEG-2 1920x1080 GOP of 15, 30 DV "
                             "720x576/720x480, 12 H.264 1280x720/1920x1088/3840x2160 pictures; every frame + BT.709 "
                             "limited YUV->RGBA + 3-layer premultiplied alpha-over at its output size",
-                "pictures_per_step": self.pictures, "submissions_per_step": 6}
+                "pictures_per_step": self.pictures, "submissions_per_step": 9}
 
     def oracle_chain(self, L, i, o, mpeg2_host):
         """Frame i through the oracle: reconstruction into o (its host planes), BT.709 limited
