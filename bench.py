@@ -710,12 +710,13 @@ class MixedWorkload:
     4:2:0 and 720x480 4:1:1 alternating), 12 H.264 pictures (4 each of 1280x720, 1920x1088 and
     3840x2160, 4 resident references per size, weighted bi-prediction, deblocking) -- where every
     frame is reconstructed, converted (BT.709 limited YUV->RGBA) and composited (3 premultiplied
-    RGBA8 layers, alpha-over, at its output size). A step is one batch per codec, then a conversion
-    batches: the chain of the 3840x2160 H.264 pictures on the context's stream, every other chain
+    RGBA8 layers, alpha-over, at its output size; conversion and composite as one pass,
+    mmr_recon_yuv_composite_submit). A step is one batch per codec, then a conversion + composite
+    batch: the chain of the 3840x2160 H.264 pictures on the context's stream, every other chain
     (MJPEG, MPEG-2, DV, the smaller H.264 pictures) on a second context's stream beside it (the
     deblocking wavefront of the 3840x2160 pictures is bound by its dependency chain and leaves most
-    compute units idle). The plain path runs
-    the same chain one frame and one launch at a time on one stream. Mpix count output frames."""
+    compute units idle). The plain path runs the same chain one frame and one launch at a time on
+    one stream. Mpix count output frames."""
     name = "mixed"
     codec = None
     data = "synthetic (seeded post-entropy buffers per codec, seeded premultiplied RGBA8 layers)"
@@ -789,8 +790,8 @@ class MixedWorkload:
             self.mpix_list.append(metrics.mpix(w, h))
             decode = {abi.CODEC_MJPEG: metrics.mjpeg_frame_bytes, abi.CODEC_MPEG2: metrics.mpeg2_picture_bytes,
                       abi.CODEC_DV: metrics.dv_frame_bytes, abi.CODEC_H264: metrics.h264_picture_bytes}[codec](u)
-            # conversion: planes read once, RGBA written; composite: target read and written, 3 layers read
-            self.bytes += decode + (w * h + 2 * cw * ch + 4 * w * h) + 20 * w * h
+            # conversion + composite in one pass: planes read, 3 layers read, the target written once
+            self.bytes += decode + (w * h + 2 * cw * ch) + 16 * w * h
         self.by_codec = {c: [d.struct for cc, _, d in self.frames if cc == c]
                          for c in (abi.CODEC_MJPEG, abi.CODEC_MPEG2, abi.CODEC_DV, abi.CODEC_H264)}
         self.submitters = {abi.CODEC_MJPEG: ctx.submit_mjpeg, abi.CODEC_MPEG2: ctx.submit_mpeg2, abi.CODEC_DV: ctx.submit_dv,
@@ -813,25 +814,34 @@ class MixedWorkload:
         self.step_tickets = {}
         log(f"mixed workload: {self.pictures} frames generated and uploaded in {time.time() - t:.1f}s")
 
-    STAGES = ("mjpeg", "mpeg2", "dv", "h264", "yuv_to_rgb", "composite")
+    STAGES = ("mjpeg", "mpeg2", "dv", "h264", "yuv_composite")
 
     def submit(self, flags=0):
         """One step; returns the ticket of the H.264 chain's last submission (wait() waits for the
         other stream's chain as well)."""
         c = self.ctx
         h = [c.submit_h264(self.h264_main, flags)]
-        h.append(c.submit_yuv_to_rgb(self.chains["main"][0], flags))
-        h.append(c.submit_composite(self.chains["main"][1], flags))
+        h.append(c.submit_yuv_composite(*self.chains["main"], flags))
         s = self.side
         o = [s.submit_mjpeg(self.by_codec[abi.CODEC_MJPEG], flags), s.submit_mpeg2(self.by_codec[abi.CODEC_MPEG2], flags),
              s.submit_dv(self.by_codec[abi.CODEC_DV], flags), s.submit_h264(self.h264_side, flags)]
-        o.append(s.submit_yuv_to_rgb(self.chains["side"][0], flags))
-        o.append(s.submit_composite(self.chains["side"][1], flags))
+        o.append(s.submit_yuv_composite(*self.chains["side"], flags))
         self.step_tickets[h[-1]] = (h, o)
         return h[-1]
 
     def close(self):
         self.side.close()
+
+    def autotune(self):
+        """Each context's launch shapes tuned on its own batches of the step: H.264 (the 3840x2160
+        pictures) on the main context; MJPEG, MPEG-2, DV and the smaller H.264 pictures on the
+        second. Returns {"main": tuning, "side": tuning}."""
+        main, _ = self.ctx.autotune(abi.CODEC_H264, self.h264_main)
+        side = None
+        for codec, items in ((abi.CODEC_MJPEG, self.by_codec[abi.CODEC_MJPEG]), (abi.CODEC_MPEG2, self.by_codec[abi.CODEC_MPEG2]),
+                             (abi.CODEC_DV, self.by_codec[abi.CODEC_DV]), (abi.CODEC_H264, self.h264_side)):
+            side, _ = self.side.autotune(codec, items)
+        return {"main": main, "side": side}
 
     def wait(self, ticket):
         self.ctx.wait(ticket)
@@ -847,8 +857,7 @@ class MixedWorkload:
         def ms(t, stage):
             h, o = self.step_tickets[t]
             pick = {"mjpeg": [(self.side, o[0])], "mpeg2": [(self.side, o[1])], "dv": [(self.side, o[2])],
-                    "h264": [(self.ctx, h[0]), (self.side, o[3])], "yuv_to_rgb": [(self.ctx, h[1]), (self.side, o[4])],
-                    "composite": [(self.ctx, h[2]), (self.side, o[5])]}[stage]
+                    "h264": [(self.ctx, h[0]), (self.side, o[3])], "yuv_composite": [(self.ctx, h[1]), (self.side, o[4])]}[stage]
             return sum(cx.kernel_ms(k) for cx, k in pick)
         return {name: float(np.mean([ms(t, name) for t in tickets])) for name in self.STAGES}
 
@@ -858,14 +867,13 @@ class MixedWorkload:
     def submit_items(self, i, flags):
         codec, _, d = self.frames[i]
         self.ctx.wait(self.submitters[codec]([d.struct], flags))
-        self.ctx.wait(self.ctx.submit_yuv_to_rgb([self.convert[i]], flags))
-        return self.ctx.submit_composite([self.composite[i]], flags)
+        return self.ctx.submit_yuv_composite([self.convert[i]], [self.composite[i]], flags)
 
     def config(self):
         return {"workload": "mixed codecs per step: 30 MJPEG 640x480, one MPEG-2 1920x1080 GOP of 15, 30 DV "
                             "720x576/720x480, 12 H.264 1280x720/1920x1088/3840x2160 pictures; every frame + BT.709 "
-                            "limited YUV->RGBA + 3-layer premultiplied alpha-over at its output size",
-                "pictures_per_step": self.pictures, "submissions_per_step": 9}
+                            "limited YUV->RGBA + 3-layer premultiplied alpha-over at its output size (one pass)",
+                "pictures_per_step": self.pictures, "submissions_per_step": 7}
 
     def oracle_chain(self, L, i, o, mpeg2_host):
         """Frame i through the oracle: reconstruction into o (its host planes), BT.709 limited
@@ -1070,6 +1078,11 @@ def main():
         tuning = {"chosen": best, "trials": [{"tuning": t, "kernel_ms": round(ms, 4)} for t, ms in trials],
                   "seconds": round(time.perf_counter() - t0, 2)}
         log(f"autotuned in {tuning['seconds']} s: {best}")
+    elif not args.no_autotune and hasattr(w, "autotune"):   # several codecs / contexts: the workload tunes each
+        t0 = time.perf_counter()
+        tuning = {"chosen": w.autotune(), "trials": None}
+        tuning["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"autotuned in {tuning['seconds']} s: {tuning['chosen']}")
     wait = getattr(w, "wait", ctx.wait)                  # a workload on two streams waits for both
     sync = getattr(w, "synchronize", ctx.synchronize)
     for _ in range(args.warmup):

@@ -495,6 +495,21 @@ MMR_RECON_API mmr_status mmr_recon_composite_submit(mmr_recon_context *context,
                                                     uint32_t job_count, uint32_t flags,
                                                     uint64_t *out_ticket);
 
+/*
+ * Conversion conversions[i] followed by composite composites[i], in one pass: each target pixel is
+ * converted, composited and stored once (the converted frame is not written and read back).
+ * composites[i].target must be conversions[i].rgba with the same width and height and stride
+ * 4 x width, and conversions[i].rgb NULL; the result is that of mmr_recon_yuv_to_rgb_submit then
+ * mmr_recon_composite_submit on the same jobs. No counterpart in the reference, which converts
+ * a frame (ycbcr_to_rgb, crates/cli/src/terminal_preview.rs:7159-7238) and composites over it
+ * (crates/render/src/frame_graph.rs:307-355) as two passes: for callers that do both per frame.
+ */
+MMR_RECON_API mmr_status mmr_recon_yuv_composite_submit(mmr_recon_context *context,
+                                                        const mmr_recon_yuv_to_rgb_job *conversions,
+                                                        const mmr_recon_composite_job *composites,
+                                                        uint32_t job_count, uint32_t flags,
+                                                        uint64_t *out_ticket);
+
 /* -------------------------------------------- MMFX linear light (u16 per channel) */
 
 /*

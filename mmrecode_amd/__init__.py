@@ -157,6 +157,16 @@ class Context:
     def submit_composite(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_composite_submit, abi.CompositeJob, jobs, flags)
 
+    def submit_yuv_composite(self, conversions, composites, flags=0):
+        """mmr_recon_yuv_composite_submit: conversion i then composite i (over conversion i's RGBA
+        output) in one pass."""
+        assert len(conversions) == len(composites)
+        conv = (abi.YuvToRgbJob * len(conversions))(*conversions)
+        comp = (abi.CompositeJob * len(composites))(*composites)
+        ticket = C.c_uint64()
+        check(self.lib.mmr_recon_yuv_composite_submit(self.handle, conv, comp, len(conversions), flags, C.byref(ticket)))
+        return ticket.value
+
     def submit_linear_composite(self, jobs, flags=0):
         return self._submit(self.lib.mmr_recon_linear_composite_submit, abi.LinearCompositeJob, jobs, flags)
 

@@ -495,6 +495,8 @@ ENTRY_POINTS = [
      [C.c_void_p, C.POINTER(YuvToRgbJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_composite_submit", C.c_int32,
      [C.c_void_p, C.POINTER(CompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    ("mmr_recon_yuv_composite_submit", C.c_int32,
+     [C.c_void_p, C.POINTER(YuvToRgbJob), C.POINTER(CompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_linear_composite_submit", C.c_int32,
      [C.c_void_p, C.POINTER(LinearCompositeJob), C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mmr_recon_linear_convert_submit", C.c_int32,

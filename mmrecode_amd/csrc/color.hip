@@ -66,13 +66,13 @@ MMR_DEV void rgb_of(uint32_t y, const ChromaTerms& t, float out[3]) {
 // Byte k (0..3) of dword d replaced by the integer v in [0, 255] held as binary32.
 MMR_DEV uint32_t put_byte(uint32_t d, float v, uint32_t k) { return __builtin_amdgcn_cvt_pk_u8_f32(v, k, d); }
 
-// Four horizontally adjacent pixels (x0 .. x0 + 3 of row y, clipped to the width) per thread: one
-// luma dword, 12 RGB bytes / 16 RGBA bytes. The chroma row is floor(y ch / height) and the chroma
+// Four horizontally adjacent pixels (x0 .. x0 + 3 of row y, clipped to the width; returns how many)
+// per thread as R, G, B in [0, 255] held as binary32, from one luma dword. The chroma row is floor(y ch / height) and the chroma
 // sample of pixel x floor(x cw / width) (video_frame_image): when 2 cw == width that is x / 2 and
 // pixels 2k, 2k + 1 share one set of chroma terms; otherwise the first from div_floor, the next
 // three by carrying the remainder (cw <= width, so each step moves at most one chroma sample).
 template <bool kLimited>
-MMR_DEV void yuv_to_rgb_px4(const RgbJob& j, uint32_t y, uint32_t x0) {
+MMR_DEV uint32_t yuv_px4(const RgbJob& j, uint32_t y, uint32_t x0, float px[4][3]) {
     const uint32_t cy = min(div_floor(y * j.ch, j.height, j.inv_height), j.ch - 1);
     const MMR_GLOBAL uint8_t* Y = as_global(j.planes[0]) + (size_t)y * j.luma_stride;
     const MMR_GLOBAL uint8_t* U = as_global(j.planes[1]) + (size_t)cy * j.chroma_stride;
@@ -85,7 +85,6 @@ MMR_DEV void yuv_to_rgb_px4(const RgbJob& j, uint32_t y, uint32_t x0) {
         ys = 0;
         for (uint32_t i = 0; i < n; ++i) ys |= (uint32_t)Y[x0 + i] << (8 * i);
     }
-    float px[4][3];
     if (j.half_chroma) {
         // x0 is a multiple of 4: chroma samples x0 / 2 and x0 / 2 + 1 (the second only when pixel
         // x0 + 2 exists, so it never passes cw - 1).
@@ -124,6 +123,14 @@ MMR_DEV void yuv_to_rgb_px4(const RgbJob& j, uint32_t y, uint32_t x0) {
             }
         }
     }
+    return n;
+}
+
+// One lane's four pixels (yuv_px4) stored as RGB8 and / or RGBA8 (alpha 255), rows packed.
+template <bool kLimited>
+MMR_DEV void yuv_to_rgb_px4(const RgbJob& j, uint32_t y, uint32_t x0) {
+    float px[4][3];
+    const uint32_t n = yuv_px4<kLimited>(j, y, x0, px);
     const size_t p = (size_t)y * j.width + x0;
     if (j.rgb) {
         MMR_GLOBAL uint8_t* o = as_global(j.rgb) + p * 3;
@@ -350,30 +357,10 @@ MMR_DEV void premultiplied_over(uint32_t d[4], const uint32_t s[4], uint32_t opa
     d[3] = sat255(sa + mul255(d[3], inverse));
 }
 
-// One lane per 4 consecutive target pixels, a wave per 256 pixels of one row (tile -> job table);
-// the layers covering each pixel are applied in order. A group of 4 pixels inside a layer, at
-// 16-byte aligned addresses, moves as one 16-byte load; the target as one 16-byte load and store.
-__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
-                                                        uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
-    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
-    if (local >= tiles) return;
-    const uint32_t tile = tile_base + local;
-    const CompositeJobDev& j = jobs[tile_job[tile]];
-    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
-    const uint32_t y = wave / j.waves_per_row;
-    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
-    if (y >= j.height || x0 >= j.width) return;
-    const uint32_t n = min(4u, j.width - x0);
-    MMR_GLOBAL uint32_t* t = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(j.target) + (size_t)y * j.stride + (size_t)x0 * 4);
-    const bool tvec = n == 4 && ((uintptr_t)t & 15) == 0;
-    uint32_t px[4] = {0, 0, 0, 0};
-    if (tvec) {
-        const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(t);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) px[i] = (uint32_t)v[i];
-    } else {
-        for (uint32_t i = 0; i < n; ++i) px[i] = t[i];
-    }
+// The layers of job j applied in order to this lane's pixels x0 .. x0 + n - 1 of row y (px, RGBA8
+// dwords): a layer covers a pixel inside its placement; four covered pixels at a 16-byte aligned
+// layer address move as one 16-byte load.
+MMR_DEV void composite_layers(const CompositeJobDev& j, uint32_t y, uint32_t x0, uint32_t n, uint32_t px[4]) {
     for (uint32_t li = 0; li < j.layer_count; ++li) {
         const mmr_recon_layer& L = j.layers[li];
         if (y < L.y || y >= L.y + L.height) continue;   // uniform over the wave (one row)
@@ -405,7 +392,69 @@ __global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* _
             px[i] = d[0] | (d[1] << 8) | (d[2] << 16) | (d[3] << 24);
         }
     }
+}
+
+// One lane per 4 consecutive target pixels, a wave per 256 pixels of one row (tile -> job table);
+// the layers covering each pixel are applied in order. A group of 4 pixels inside a layer, at
+// 16-byte aligned addresses, moves as one 16-byte load; the target as one 16-byte load and store.
+__global__ __launch_bounds__(256) void composite_kernel(const CompositeJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+                                                        uint32_t tile_base, uint32_t tiles, uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const CompositeJobDev& j = jobs[tile_job[tile]];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
+    const uint32_t y = wave / j.waves_per_row;
+    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
+    if (y >= j.height || x0 >= j.width) return;
+    const uint32_t n = min(4u, j.width - x0);
+    MMR_GLOBAL uint32_t* t = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(j.target) + (size_t)y * j.stride + (size_t)x0 * 4);
+    const bool tvec = n == 4 && ((uintptr_t)t & 15) == 0;
+    uint32_t px[4] = {0, 0, 0, 0};
     if (tvec) {
+        const mmr_v4i32 v = *reinterpret_cast<const MMR_GLOBAL mmr_v4i32*>(t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) px[i] = (uint32_t)v[i];
+    } else {
+        for (uint32_t i = 0; i < n; ++i) px[i] = t[i];
+    }
+    composite_layers(j, y, x0, n, px);
+    if (tvec) {
+        mmr_v4i32 v;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = (int)px[i];
+        *reinterpret_cast<MMR_GLOBAL mmr_v4i32*>(t) = v;
+    } else {
+        for (uint32_t i = 0; i < n; ++i) t[i] = px[i];
+    }
+}
+
+// Conversion and composite in one pass: job i converts rjobs[i]'s planes (yuv_px4, alpha 255)
+// into the pixels of cjobs[i]'s target (the conversion's RGBA output, rows packed), applies the
+// layers (composite_layers) and stores each target pixel once -- the result of the conversion
+// followed by the composite, without writing the converted frame and reading it back. Launch
+// shape of composite_kernel (a lane per 4 pixels, a wave per 256 pixels of one row).
+__global__ __launch_bounds__(256) void yuv_composite_kernel(const RgbJob* __restrict__ rjobs, const CompositeJobDev* __restrict__ cjobs,
+                                                            const uint32_t* __restrict__ tile_job, uint32_t tile_base, uint32_t tiles,
+                                                            uint32_t per_xcd) {
+    const uint32_t local = xcd_tile(blockIdx.x, per_xcd);
+    if (local >= tiles) return;
+    const uint32_t tile = tile_base + local;
+    const uint32_t ji = tile_job[tile];
+    const CompositeJobDev& j = cjobs[ji];
+    const RgbJob& r = rjobs[ji];
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)((tile - j.tile_begin) * 4 + (threadIdx.x >> 6)));
+    const uint32_t y = wave / j.waves_per_row;
+    const uint32_t x0 = (wave - y * j.waves_per_row) * 256 + (threadIdx.x & 63) * 4;
+    if (y >= j.height || x0 >= j.width) return;
+    float f[4][3];
+    const uint32_t n = r.limited ? yuv_px4<true>(r, y, x0, f) : yuv_px4<false>(r, y, x0, f);
+    uint32_t px[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) px[i] = put_byte(put_byte(put_byte(0xff000000u, f[i][0], 0), f[i][1], 1), f[i][2], 2);
+    composite_layers(j, y, x0, n, px);
+    MMR_GLOBAL uint32_t* t = reinterpret_cast<MMR_GLOBAL uint32_t*>(as_global(j.target) + (size_t)y * j.stride + (size_t)x0 * 4);
+    if (n == 4 && ((uintptr_t)t & 15) == 0) {
         mmr_v4i32 v;
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = (int)px[i];
@@ -649,6 +698,14 @@ hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_til
     if (tiles == 0) return hipSuccess;
     const uint32_t per = xcd_per(tiles);
     hipLaunchKernelGGL(composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_jobs, d_tile_job, tile_base, tiles, per);
+    return hipGetLastError();
+}
+
+hipError_t launch_yuv_composite(const RgbJob* d_rjobs, const CompositeJobDev* d_cjobs, const uint32_t* d_tile_job, uint32_t tile_base,
+                                uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    const uint32_t per = xcd_per(tiles);
+    hipLaunchKernelGGL(yuv_composite_kernel, dim3(per * kXcds), dim3(256), 0, s, d_rjobs, d_cjobs, d_tile_job, tile_base, tiles, per);
     return hipGetLastError();
 }
 

@@ -1070,6 +1070,29 @@ mmr_status h264_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_
     return sub.end(out_ticket);
 }
 
+// The device job of a checked conversion job (its tile range is the caller's).
+mmr_status build_rgb_job(const mmr_recon_yuv_to_rgb_job& in, mmr::RgbJob& r) {
+    mmr_status s = check_rgb_job(in);
+    if (s) return s;
+    memset(&r, 0, sizeof r);
+    for (int k = 0; k < 3; ++k) r.planes[k] = in.planes[k];
+    r.rgb = in.rgb;
+    r.rgba = in.rgba;
+    r.width = in.width;
+    r.height = in.height;
+    r.cw = in.chroma_width;
+    r.ch = in.chroma_height;
+    r.luma_stride = in.luma_stride;
+    r.chroma_stride = in.chroma_stride;
+    r.range = in.range;
+    r.matrix = in.matrix;
+    r.inv_width = 1.0 / in.width;
+    r.inv_height = 1.0 / in.height;
+    mmr::set_rgb_layout(r);
+    mmr::set_rgb_terms(r);
+    return MMR_STATUS_OK;
+}
+
 mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_yuv_to_rgb_job* in, uint32_t count, uint32_t flags,
                                        uint64_t* out_ticket) {
     CHECK_CTX(ctx);
@@ -1078,25 +1101,9 @@ mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, cons
     std::vector<mmr::RgbJob> jobs(count);
     std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
-        mmr_status s = check_rgb_job(in[i]);
+        mmr_status s = build_rgb_job(in[i], jobs[i]);
         if (s) return s;
         mmr::RgbJob& r = jobs[i];
-        memset(&r, 0, sizeof r);
-        for (int k = 0; k < 3; ++k) r.planes[k] = in[i].planes[k];
-        r.rgb = in[i].rgb;
-        r.rgba = in[i].rgba;
-        r.width = in[i].width;
-        r.height = in[i].height;
-        r.cw = in[i].chroma_width;
-        r.ch = in[i].chroma_height;
-        r.luma_stride = in[i].luma_stride;
-        r.chroma_stride = in[i].chroma_stride;
-        r.range = in[i].range;
-        r.matrix = in[i].matrix;
-        r.inv_width = 1.0 / in[i].width;
-        r.inv_height = 1.0 / in[i].height;
-        mmr::set_rgb_layout(r);
-        mmr::set_rgb_terms(r);
         r.tile_begin = add_tiles(tiles, i, mmr::rgb_job_tiles(r), 1);
     }
     std::vector<char> blob;
@@ -1118,6 +1125,34 @@ mmr_status yuv_to_rgb_submit_on(mmr_recon_context* ctx, hipStream_t stream, cons
     return sub.end(out_ticket);
 }
 
+// The device job of composite job c (index i for messages), checked; its tile range is the caller's.
+mmr_status build_composite_job(const mmr_recon_composite_job& c, uint32_t i, mmr::CompositeJobDev& j) {
+    if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
+    if (!c.target || !c.width || !c.height || c.width > 65535 || c.height > 65535 || c.stride < c.width * 4 ||
+        c.layer_count > MMR_RECON_MAX_LAYERS)
+        return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
+    if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
+    for (uint32_t l = 0; l < c.layer_count; ++l) {
+        const mmr_recon_layer& L = c.layers[l];
+        if (!L.rgba || L.stride < L.width * 4 || L.stride % 4 || !aligned(L.rgba, 4))
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u layer %u storage is invalid", i, l);
+        if ((uint64_t)L.x + L.width > c.width || (uint64_t)L.y + L.height > c.height)
+            return fail(MMR_STATUS_INVALID_DATA, "RGBA overlay storage or placement is invalid (job %u layer %u)", i, l);
+        if (L.op != MMR_BLEND_SOURCE_OVER && L.op != MMR_BLEND_PREMULTIPLIED_OVER)
+            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite operator %u invalid", L.op);
+        if (L.opacity > 255) return fail(MMR_STATUS_INVALID_ARGUMENT, "opacity must be 0..255");
+    }
+    memset(&j, 0, sizeof j);
+    j.target = c.target;
+    j.width = c.width;
+    j.height = c.height;
+    j.stride = c.stride;
+    j.layer_count = c.layer_count;
+    for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
+    j.waves_per_row = (c.width + 255) / 256;
+    return MMR_STATUS_OK;
+}
+
 mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_composite_job* in, uint32_t count, uint32_t flags,
                                       uint64_t* out_ticket) {
     CHECK_CTX(ctx);
@@ -1126,31 +1161,9 @@ mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
     std::vector<mmr::CompositeJobDev> jobs(count);
     std::vector<uint32_t> tiles;
     for (uint32_t i = 0; i < count; ++i) {
-        const mmr_recon_composite_job& c = in[i];
-        if (c.struct_size < sizeof(mmr_recon_composite_job)) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite struct_size too small");
-        if (!c.target || !c.width || !c.height || c.width > 65535 || c.height > 65535 || c.stride < c.width * 4 ||
-            c.layer_count > MMR_RECON_MAX_LAYERS)
-            return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u is invalid", i);
-        if (!aligned(c.target, 4) || c.stride % 4) return fail(MMR_STATUS_INVALID_ARGUMENT, "composite target must be 4-byte aligned");
-        for (uint32_t l = 0; l < c.layer_count; ++l) {
-            const mmr_recon_layer& L = c.layers[l];
-            if (!L.rgba || L.stride < L.width * 4 || L.stride % 4 || !aligned(L.rgba, 4))
-                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite job %u layer %u storage is invalid", i, l);
-            if ((uint64_t)L.x + L.width > c.width || (uint64_t)L.y + L.height > c.height)
-                return fail(MMR_STATUS_INVALID_DATA, "RGBA overlay storage or placement is invalid (job %u layer %u)", i, l);
-            if (L.op != MMR_BLEND_SOURCE_OVER && L.op != MMR_BLEND_PREMULTIPLIED_OVER)
-                return fail(MMR_STATUS_INVALID_ARGUMENT, "composite operator %u invalid", L.op);
-            if (L.opacity > 255) return fail(MMR_STATUS_INVALID_ARGUMENT, "opacity must be 0..255");
-        }
+        mmr_status s = build_composite_job(in[i], i, jobs[i]);
+        if (s) return s;
         mmr::CompositeJobDev& j = jobs[i];
-        memset(&j, 0, sizeof j);
-        j.target = c.target;
-        j.width = c.width;
-        j.height = c.height;
-        j.stride = c.stride;
-        j.layer_count = c.layer_count;
-        for (uint32_t l = 0; l < c.layer_count; ++l) j.layers[l] = c.layers[l];
-        j.waves_per_row = (c.width + 255) / 256;
         j.tile_begin = add_tiles(tiles, i, mmr::composite_job_tiles(j.waves_per_row, j.height), 1);
     }
     std::vector<char> blob;
@@ -1169,6 +1182,48 @@ mmr_status composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const
         e = mmr::launch_composite(dj, dt, 0, (uint32_t)tiles.size(), stream);
     }
     if (e != hipSuccess) return hip_fail(e, "composite launch");
+    return sub.end(out_ticket);
+}
+
+// Conversion conv[i] followed by composite comp[i] in one pass: comp[i]'s target must be conv[i]'s
+// RGBA output (rows packed: stride 4 x width, the same size, no RGB output).
+mmr_status yuv_composite_submit_on(mmr_recon_context* ctx, hipStream_t stream, const mmr_recon_yuv_to_rgb_job* conv,
+                                   const mmr_recon_composite_job* comp, uint32_t count, uint32_t flags, uint64_t* out_ticket) {
+    CHECK_CTX(ctx);
+    if (!conv || !comp || count == 0) return fail(MMR_STATUS_INVALID_ARGUMENT, "no jobs");
+    const bool per_frame = (flags & MMR_SUBMIT_PER_FRAME_LAUNCH) != 0;
+    std::vector<mmr::RgbJob> rjobs(count);
+    std::vector<mmr::CompositeJobDev> cjobs(count);
+    std::vector<uint32_t> tiles;
+    for (uint32_t i = 0; i < count; ++i) {
+        mmr_status s = build_rgb_job(conv[i], rjobs[i]);
+        if (s) return s;
+        if ((s = build_composite_job(comp[i], i, cjobs[i]))) return s;
+        if (conv[i].rgb || conv[i].rgba != comp[i].target || conv[i].width != comp[i].width || conv[i].height != comp[i].height ||
+            comp[i].stride != comp[i].width * 4)
+            return fail(MMR_STATUS_INVALID_ARGUMENT,
+                        "conversion + composite job %u: the composite's target must be the conversion's packed RGBA output", i);
+        cjobs[i].tile_begin = add_tiles(tiles, i, mmr::composite_job_tiles(cjobs[i].waves_per_row, cjobs[i].height), 1);
+    }
+    std::vector<char> blob;
+    const size_t rjobs_off = blob_append(blob, rjobs.data(), rjobs.size() * sizeof(mmr::RgbJob));
+    const size_t cjobs_off = blob_append(blob, cjobs.data(), cjobs.size() * sizeof(mmr::CompositeJobDev));
+    const size_t tiles_off = blob_append(blob, tiles.data(), tiles.size() * sizeof(uint32_t));
+    Submission sub(ctx, stream);
+    mmr_status s = sub.begin(blob.data(), blob.size(), 0);
+    if (s) return s;
+    auto* drj = reinterpret_cast<const mmr::RgbJob*>((const char*)sub.jobs_ptr() + rjobs_off);
+    auto* dcj = reinterpret_cast<const mmr::CompositeJobDev*>((const char*)sub.jobs_ptr() + cjobs_off);
+    auto* dt = reinterpret_cast<const uint32_t*>((const char*)sub.jobs_ptr() + tiles_off);
+    hipError_t e = hipSuccess;
+    if (per_frame) {
+        for (uint32_t i = 0; i < count && e == hipSuccess; ++i)
+            e = mmr::launch_yuv_composite(drj, dcj, dt, cjobs[i].tile_begin, mmr::composite_job_tiles(cjobs[i].waves_per_row, cjobs[i].height),
+                                          stream);
+    } else {
+        e = mmr::launch_yuv_composite(drj, dcj, dt, 0, (uint32_t)tiles.size(), stream);
+    }
+    if (e != hipSuccess) return hip_fail(e, "conversion + composite launch");
     return sub.end(out_ticket);
 }
 
@@ -2092,6 +2147,15 @@ mmr_status mmr_recon_composite_submit(mmr_recon_context* ctx, const mmr_recon_co
         CHECK_CTX(ctx);
         LOCK_AND_SET_DEVICE(ctx);
         return mmr_host::composite_submit_on(ctx, ctx->stream, in, count, flags, out_ticket);
+    });
+}
+
+mmr_status mmr_recon_yuv_composite_submit(mmr_recon_context* ctx, const mmr_recon_yuv_to_rgb_job* conversions,
+                                          const mmr_recon_composite_job* composites, uint32_t count, uint32_t flags, uint64_t* out_ticket) {
+    return guarded("mmr_recon_yuv_composite_submit", [&]() -> mmr_status {
+        CHECK_CTX(ctx);
+        LOCK_AND_SET_DEVICE(ctx);
+        return mmr_host::yuv_composite_submit_on(ctx, ctx->stream, conversions, composites, count, flags, out_ticket);
     });
 }
 

@@ -219,6 +219,11 @@ hipError_t launch_linear_convert(const LinearConvertDev* d_jobs, const uint32_t*
                                 const uint8_t* d_tables, hipStream_t s);
 hipError_t launch_yuv_to_rgb(const RgbJob* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
 hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_tile_job, uint32_t tile_base, uint32_t tiles, hipStream_t s);
+// Conversion rjobs[i] and composite cjobs[i] in one pass over the composite's tiles (job i's
+// target is conversion i's RGBA output, rows packed): the result of launch_yuv_to_rgb followed by
+// launch_composite.
+hipError_t launch_yuv_composite(const RgbJob* d_rjobs, const CompositeJobDev* d_cjobs, const uint32_t* d_tile_job, uint32_t tile_base,
+                                uint32_t tiles, hipStream_t s);
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t max_px, hipStream_t s);
 
 // One plane of a scale job on the device: the canvas window (copy_w x copy_h at dst_x, dst_y)

@@ -406,6 +406,49 @@ hipError_t launch_composite(const CompositeJobDev* d_jobs, const uint32_t* d_til
     return hipSuccess;
 }
 
+// color.hip: yuv_composite_kernel, composite_kernel's tiles over the composite jobs; job i also
+// reads conversion record i and its planes (yuv_to_rgb's reads) and writes the target only.
+hipError_t launch_yuv_composite(const RgbJob* d_rjobs, const CompositeJobDev* d_cjobs, const uint32_t* d_tile_job, uint32_t tile_base,
+                                uint32_t tiles, hipStream_t s) {
+    if (tiles == 0) return hipSuccess;
+    model::enqueue(s, "yuv_composite", [=] {
+        std::map<uint32_t, std::vector<uint32_t>> offs;
+        std::map<uint32_t, CompositeJobDev> recs;
+        if (!walk_tiles("yuv_composite job / tile table", d_cjobs, d_tile_job, tile_base, tiles, offs, recs)) return;
+        for (auto& [i, v] : offs) {
+            const CompositeJobDev& j = recs[i];
+            model::stats().jobs++;
+            if (j.waves_per_row != (j.width + 255) / 256) violation("yuv_composite: job %u waves per row", i);
+            check_tiles("yuv_composite", i, v, composite_job_tiles(j.waves_per_row, j.height));
+            if (!check(d_rjobs + i, sizeof(RgbJob), "yuv_composite conversion record")) continue;
+            const RgbJob& r = d_rjobs[i];
+            if (!j.width || !j.height || !r.cw || !r.ch || r.width != j.width || r.height != j.height || j.stride != j.width * 4 ||
+                r.rgba != j.target || r.rgb) {
+                violation("yuv_composite: job %u conversion and composite do not describe one packed RGBA target", i);
+                continue;
+            }
+            if (r.inv_width != 1.0 / r.width || r.inv_height != 1.0 / r.height || r.cw > r.width ||
+                r.half_chroma != ((uint64_t)r.cw * 2 == r.width ? 1u : 0u) || r.limited != (r.range == MMR_RANGE_LIMITED ? 1u : 0u))
+                violation("yuv_composite: job %u conversion record does not match its sizes and range", i);
+            const uint64_t ymax = r.height - 1;
+            const uint64_t cy = std::min<uint64_t>(ymax * r.ch / r.height, r.ch - 1);
+            const uint64_t cx = std::min<uint64_t>((uint64_t)(r.width - 1) * r.cw / r.width, r.cw - 1);
+            check(r.planes[0], ymax * r.luma_stride + r.width, "yuv_composite luma plane");
+            check(r.planes[1], cy * r.chroma_stride + cx + 1, "yuv_composite Cb plane");
+            check(r.planes[2], cy * r.chroma_stride + cx + 1, "yuv_composite Cr plane");
+            check(j.target, (uint64_t)(j.height - 1) * j.stride + (uint64_t)j.width * 4, "yuv_composite target");
+            for (uint32_t l = 0; l < j.layer_count && l < MMR_RECON_MAX_LAYERS; ++l) {
+                const mmr_recon_layer& L = j.layers[l];
+                if (L.y >= j.height || L.height == 0 || L.width == 0 || L.x >= j.width) continue;
+                const uint64_t rows = std::min<uint64_t>(L.height, j.height - L.y);
+                const uint64_t cols = std::min<uint64_t>(L.width, j.width - L.x);
+                check(L.rgba, (rows - 1) * L.stride + cols * 4, "yuv_composite layer");
+            }
+        }
+    });
+    return hipSuccess;
+}
+
 // color.hip: yuv_blend_kernel (job blockIdx.y, pixel p < width * height).
 hipError_t launch_yuv_blend(const mmr_recon_yuv_blend_job* d_jobs, uint32_t job_count, uint32_t, hipStream_t s) {
     if (job_count == 0) return hipSuccess;
