@@ -71,15 +71,26 @@ MMR_DEV void db_load_tables(DbTables& t) {
 }
 MMR_DEV int db_chroma_qp(const DbTables& t, int luma_qp, int offset) { return t.cqp[clampi(luma_qp + offset, 0, 51)]; }
 
-// Filter parameters of one edge class: alpha and TC0 from index A = clamp(qp + offset_a), beta
-// from index B = clamp(qp + offset_b).
+// Filter parameters of one edge class for this lane's line: alpha and TC0 from index A =
+// clamp(qp + offset_a), beta from index B = clamp(qp + offset_b). The thresholds are held negated,
+// as the bias an absolute difference is tested with (sad_bias): na = -alpha, nb = -beta, nnear =
+// -((alpha >> 2) + 2). nb_ap / nb_aq are -beta on luma lines and -256 / 0 on chroma lines, where
+// the p2 / q2 tests then read always / never true: tc0 + [ap] + [aq] is then chroma's tc0 + 1,
+// and the p1 / q1 updates and the strong filter, luma only, are masked by luma_m.
 struct EdgeParams {
-    uint32_t a, b;   // tab[A], tab[B]
+    uint32_t a;                        // tab[A]: TC0 for bS 1, 2, 3 at bits 16, 21, 26
+    int na, nb, nb_ap, nb_aq, nnear;
 };
-MMR_DEV EdgeParams edge_params(const DbTables& t, int qp, int offset_a, int offset_b) {
+MMR_DEV EdgeParams edge_params(const DbTables& t, int qp, int offset_a, int offset_b, int luma_m) {
     EdgeParams e;
     e.a = t.tab[clampi(qp + offset_a, 0, 51)];
-    e.b = t.tab[clampi(qp + offset_b, 0, 51)];
+    const uint32_t b = t.tab[clampi(qp + offset_b, 0, 51)];
+    const int alpha = (int)(e.a & 0xffu), beta = (int)((b >> 8) & 0x1fu);
+    e.na = -alpha;
+    e.nb = -beta;
+    e.nb_ap = (-beta & luma_m) | (-256 & ~luma_m);
+    e.nb_aq = -beta & luma_m;
+    e.nnear = -((alpha >> 2) + 2);
     return e;
 }
 MMR_DEV int iabs(int v) { return v < 0 ? -v : v; }
@@ -94,10 +105,18 @@ MMR_DEV void wave_sync() {
 
 // Lane masks for the edge filter: 0 or -1 (all bits), built from sign bits so that the filter is
 // straight-line vector arithmetic (no per-lane branches or exec-mask changes).
-MMR_DEV int mask_lt(int a, int b) { return (a - b) >> 31; }                   // a < b, |a - b| < 2^31
 MMR_DEV int mask_sel(int m, int a, int b) { return (a & m) | (b & ~m); }      // m ? a : b
-MMR_DEV int absdiff(int a, int b) { return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, 0u); }   // a, b in 0..65535
 MMR_DEV int clamp_sym(int v, int t) { return min(max(v, -t), t); }           // t >= 0
+// |a - b| + bias as a signed value, one v_sad_u16 (its 32-bit accumulator takes the bias): for a, b
+// in 0..255 and bias in -256..0 it is negative exactly when |a - b| < -bias.
+MMR_DEV int sad_bias(int a, int b, int bias) { return (int)__builtin_amdgcn_sad_u16((uint32_t)a, (uint32_t)b, (uint32_t)bias); }
+// bS != 0 && |p0 - q0| < alpha && |p1 - p0| < beta && |q1 - q0| < beta as 0 / -1: the three tests'
+// maximum is negative when all hold, and -bS is negative when bS != 0.
+MMR_DEV int edge_on(int p1, int p0, int q0, int q1, const EdgeParams& e, int bs) {
+    return (max(max(sad_bias(p0, q0, e.na), sad_bias(p1, p0, e.nb)), sad_bias(q1, q0, e.nb)) & -bs) >> 31;
+}
+// (p0 + q0 + 1) >> 1 for p0, q0 in 0..255: one v_lerp_u8 (byte 0 with its rounding bit set).
+MMR_DEV int avg_round(int p0, int q0) { return (int)__builtin_amdgcn_lerp((uint32_t)p0, (uint32_t)q0, 1u); }
 
 // filter_samples + strong_filter + normal_filter (deblock.rs:469-569) on one line of samples, one
 // sample per register: p_i is s[k - 1 - i] and q_i is s[k + i] (k is a compile-time index). The
@@ -107,24 +126,22 @@ MMR_DEV int clamp_sym(int v, int t) { return min(max(v, -t), t); }           // 
 // (the reference writes unchanged samples back, which leaves the same values). luma_m is -1 on
 // luma lines, 0 on chroma lines.
 template <int k>
-MMR_DEV void filter_edge(int (&s)[20], EdgeParams ep, int bs, int luma_m) {
+MMR_DEV void filter_edge(int (&s)[20], const EdgeParams& e, int bs, int luma_m) {
     const int p0 = s[k - 1], p1 = s[k - 2], p2 = s[k - 3], p3 = s[k - 4];
     const int q0 = s[k], q1 = s[k + 1], q2 = s[k + 2], q3 = s[k + 3];
-    const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
-    const int d = absdiff(p0, q0);
-    // bS != 0 && |p0 - q0| < alpha && |p1 - p0| < beta && |q1 - q0| < beta
-    const int on = (-bs & (d - alpha) & (absdiff(p1, p0) - beta) & (absdiff(q1, q0) - beta)) >> 31;
-    const int ap = mask_lt(absdiff(p2, p0), beta), aq = mask_lt(absdiff(q2, q0), beta);
+    const int on = edge_on(p1, p0, q0, q1, e, bs);
+    // |p2 - p0| < beta, |q2 - q0| < beta (luma lines; chroma: -1 and 0)
+    const int ap = sad_bias(p2, p0, e.nb_ap) >> 31, aq = sad_bias(q2, q0, e.nb_aq) >> 31;
     // bS < 4
-    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);   // bS 1..3 (unused for 0 and 4)
-    const int tc = tc0 + mask_sel(luma_m, -(ap + aq), 1);
+    const int tc0 = (int)((e.a >> (16 + 5 * (bs - 1))) & 0x1fu);   // bS 1..3 (unused for 0 and 4)
+    const int tc = tc0 - ap - aq;
     const int delta = clamp_sym((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, tc);
-    const int avg = (p0 + q0 + 1) >> 1;
+    const int avg = avg_round(p0, q0);
     const int np0 = clampi(p0 + delta, 0, 255), nq0 = clampi(q0 - delta, 0, 255);
     const int np1 = mask_sel(luma_m & ap, clampi(p1 + clamp_sym((p2 + avg - (p1 << 1)) >> 1, tc0), 0, 255), p1);
     const int nq1 = mask_sel(luma_m & aq, clampi(q1 + clamp_sym((q2 + avg - (q1 << 1)) >> 1, tc0), 0, 255), q1);
-    // bS == 4
-    const int near = mask_lt(d, (alpha >> 2) + 2);
+    // bS == 4: |p0 - q0| < (alpha >> 2) + 2
+    const int near = sad_bias(p0, q0, e.nnear) >> 31;
     const int sp = luma_m & ap & near, sq = luma_m & aq & near;
     const int sp0 = mask_sel(sp, (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3, (2 * p1 + p0 + q1 + 2) >> 2);
     const int sp1 = mask_sel(sp, (p2 + p1 + p0 + q0 + 2) >> 2, p1);
@@ -145,16 +162,15 @@ MMR_DEV void filter_edge(int (&s)[20], EdgeParams ep, int bs, int luma_m) {
 // filter_edge for an internal edge of a macroblock (luma edges 1-3, chroma edge 2), whose strength
 // is at most 3 (boundary_strength gives 4 only on macroblock edges): the normal filter alone.
 template <int k>
-MMR_DEV void filter_edge_inner(int (&s)[20], EdgeParams ep, int bs, int luma_m) {
+MMR_DEV void filter_edge_inner(int (&s)[20], const EdgeParams& e, int bs, int luma_m) {
     const int p0 = s[k - 1], p1 = s[k - 2], p2 = s[k - 3];
     const int q0 = s[k], q1 = s[k + 1], q2 = s[k + 2];
-    const int alpha = (int)(ep.a & 0xffu), beta = (int)((ep.b >> 8) & 0x1fu);
-    const int on = (-bs & (absdiff(p0, q0) - alpha) & (absdiff(p1, p0) - beta) & (absdiff(q1, q0) - beta)) >> 31;
-    const int ap = mask_lt(absdiff(p2, p0), beta), aq = mask_lt(absdiff(q2, q0), beta);
-    const int tc0 = (int)((ep.a >> (16 + 5 * (bs - 1))) & 0x1fu);
-    const int tc = tc0 + mask_sel(luma_m, -(ap + aq), 1);
+    const int on = edge_on(p1, p0, q0, q1, e, bs);
+    const int ap = sad_bias(p2, p0, e.nb_ap) >> 31, aq = sad_bias(q2, q0, e.nb_aq) >> 31;
+    const int tc0 = (int)((e.a >> (16 + 5 * (bs - 1))) & 0x1fu);
+    const int tc = tc0 - ap - aq;
     const int delta = clamp_sym((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, tc);
-    const int avg = (p0 + q0 + 1) >> 1;
+    const int avg = avg_round(p0, q0);
     const int np1 = clampi(p1 + clamp_sym((p2 + avg - (p1 << 1)) >> 1, tc0), 0, 255);
     const int nq1 = clampi(q1 + clamp_sym((q2 + avg - (q1 << 1)) >> 1, tc0), 0, 255);
     s[k - 2] = mask_sel(on & luma_m & ap, np1, p1);
@@ -552,9 +568,9 @@ MMR_DEV void db_row(const H264Job& job, const DbTables& T, int y, int w, int l, 
         // filter parameters of the left edge, the internal edges and the top edge
         auto lane_qp = [&](int qp) { return chroma_lane ? db_chroma_qp(T, qp, chroma_off) : qp; };
         const int qpc = lane_qp(m.qp);
-        const EdgeParams ev = edge_params(T, (lane_qp(qp_left) + qpc + 1) >> 1, m.offset_a, m.offset_b);
-        const EdgeParams ei = edge_params(T, qpc, m.offset_a, m.offset_b);
-        const EdgeParams eh = edge_params(T, (lane_qp(cur.qp_top) + qpc + 1) >> 1, m.offset_a, m.offset_b);
+        const EdgeParams ev = edge_params(T, (lane_qp(qp_left) + qpc + 1) >> 1, m.offset_a, m.offset_b, luma_m);
+        const EdgeParams ei = edge_params(T, qpc, m.offset_a, m.offset_b, luma_m);
+        const EdgeParams eh = edge_params(T, (lane_qp(cur.qp_top) + qpc + 1) >> 1, m.offset_a, m.offset_b, luma_m);
         // Strengths from the macroblock's record: vertical edge e, segment g at nibble e*4+g of
         // dwords x / y, horizontal at nibble e*4+g of dwords z / w. Edge slots k = 4, 8, 12, 16 are
         // the luma edges 0..3 and the chroma edges 0 and 2 (slots 12, 16 have strength 0 on chroma
